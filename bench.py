@@ -1,0 +1,239 @@
+"""Benchmark of the hot path on one MI355X.
+
+Workload (one step = one frame of rays, inputs resident in HBM):
+  world 384x128x384 uint8 terrain (seeded value noise, heights 20-90,
+  grass/dirt/stone layering, 1% of solid voxels given random ids 1-100),
+  distance field built on the GPU; at 1920x1080, 1 spp: one primary ray per
+  pixel, one sun-shadow ray per lit pixel, one cosine-weighted diffuse GI
+  ray per hit pixel plus a sun-shadow ray at the GI hit.  Grays/s counts
+  every ray whose traversal loop is entered.  SVGF ms/frame (pre-spatial,
+  temporal, variance, 5 a-trous, decode at 1920x1080) and DF-build ms are
+  reported beside it.
+
+Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+NX, NY, NZ = 384, 128, 384
+CAM_POS = (192.5, 100.5, 192.5)
+CAM_YAW, CAM_PITCH = 45.0, -20.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--svgf-frames", type=int, default=30)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1 "
+                          "(the path shards by pixel tiles / requests with no collective, see DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+    from voxelpathtracer_amd import SVGF, Context, World, generate_terrain, gi_params, make_camera, viewproj
+    from voxelpathtracer_amd.world import DeviceGBuffer
+
+    W, H = args.width, args.height
+    ctx = Context(0)
+    grid = generate_terrain(NX, NY, NZ)
+    world = World(ctx, NX, NY, NZ)
+    world.m_WorldData[...] = grid
+    world.Buffer()
+    stream = torch.cuda.current_stream()
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    # ---- DF build --------------------------------------------------------
+    for _ in range(3):
+        world.GenerateDistanceField()
+    torch.cuda.synchronize()
+    df_reps = 20
+    e0, e1 = ev(), ev()
+    e0.record(stream)
+    for _ in range(df_reps):
+        world.GenerateDistanceField()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    df_ms = e0.elapsed_time(e1) / df_reps
+
+    # ---- rays --------------------------------------------------------------
+    cam, view, proj = make_camera(CAM_POS, CAM_YAW, CAM_PITCH, W, H)
+    prm = gi_params(50.0, 0)
+    sun = list(prm.sun_dir)
+    gb = DeviceGBuffer(W, H)
+    mask = torch.empty((H, W), dtype=torch.uint8, device="cuda")
+    rgb = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    tex = torch.empty((H, W, 8), dtype=torch.int16, device="cuda")
+
+    def step(events=None):
+        if events is not None:
+            events[0].record(stream)
+        world.trace_primary(cam, gb, 350)
+        if events is not None:
+            events[1].record(stream)
+        world.trace_shadow(cam, gb, sun, mask, 350)
+        if events is not None:
+            events[2].record(stream)
+        world.trace_gi(cam, gb, prm, rgb, tex)
+        if events is not None:
+            events[3].record(stream)
+
+    # work counts (untimed): DF samples and traced rays per pass
+    counts = {}
+    counter = torch.zeros(2, dtype=torch.int64, device="cuda")
+    world.set_sample_counter(counter)
+    for name, fn in (("primary", lambda: world.trace_primary(cam, gb, 350)),
+                     ("shadow", lambda: world.trace_shadow(cam, gb, sun, mask, 350)),
+                     ("gi", lambda: world.trace_gi(cam, gb, prm, rgb, tex))):
+        counter.zero_()
+        fn()
+        torch.cuda.synchronize()
+        c = counter.cpu().numpy()
+        counts[name] = dict(samples=int(c[0]), rays=int(c[1]))
+    world.set_sample_counter(None)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[ev() for _ in range(4)] for _ in range(args.steps)]
+    t_start, t_end = ev(), ev()
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for k in range(args.steps):
+        step(evs[k])
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_end)
+    ms_per_step = total_ms / args.steps
+    pass_ms = {
+        "primary": float(np.mean([e[0].elapsed_time(e[1]) for e in evs])),
+        "shadow": float(np.mean([e[1].elapsed_time(e[2]) for e in evs])),
+        "gi": float(np.mean([e[2].elapsed_time(e[3]) for e in evs])),
+    }
+    rays_per_step = sum(c["rays"] for c in counts.values())
+    grays = rays_per_step / (ms_per_step * 1e-3) / 1e9
+
+    # algorithmic bytes per launch: 1 B per DF sample + 1 B final voxel per
+    # ray + ray I/O (primary out 10 B/px; shadow in 5 + out 1 B/px; GI in
+    # 5 B/px + out 12 B RGB + 16 B SVGF texel per px)
+    px = W * H
+    io = {"primary": 10 * px, "shadow": 6 * px, "gi": (5 + 12 + 16) * px}
+    alg = {k: counts[k]["samples"] + counts[k]["rays"] + io[k] for k in counts}
+    dom = max(pass_ms, key=lambda k: pass_ms[k])
+    achieved = alg[dom] / (pass_ms[dom] * 1e-3) / 1e9
+
+    # ---- SVGF --------------------------------------------------------------
+    sv = SVGF(ctx, W, H)
+    gb_prev = DeviceGBuffer(W, H)
+    world.trace_primary(cam, gb_prev, 350)
+    den = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    vp = viewproj(view, proj)
+    for f in range(3):
+        sv.frame(cam, vp, gb, gb_prev, tex, den, time=f / 60.0)
+    torch.cuda.synchronize()
+    s0, s1 = ev(), ev()
+    s0.record(stream)
+    for f in range(args.svgf_frames):
+        sv.frame(cam, vp, gb, gb_prev, tex, den, time=f / 60.0)
+    s1.record(stream)
+    torch.cuda.synchronize()
+    svgf_ms = s0.elapsed_time(s1) / args.svgf_frames
+
+    result = {
+        "metric": "Grays/s DDA+GI @1080p (bit-exact hits)",
+        "value": round(grays, 4),
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"GPU plain: {NX}x{NY}x{NZ} uint8 terrain + DF, {W}x{H} 1 spp primary + sun shadow "
+                               f"+ 1 diffuse GI bounce + GI-hit shadow",
+                   "world": [NX, NY, NZ], "image": [W, H], "rays_per_step": rays_per_step,
+                   "parallelism": "single GPU"},
+        "df_build_ms": round(df_ms, 4),
+        "svgf_ms_per_frame": round(svgf_ms, 4),
+        "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()},
+        "work": counts,
+        "roofline": {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "algorithmic_bytes": alg[dom]},
+    }
+
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(grid, gb, mask, rgb, cam, prm, args.cpu_reps)
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(grid, gb, mask, rgb, cam, prm, reps):
+    """The oracle (C++ transcription, OpenMP over rows) on the same 1080p
+    DDA+GI frame on this host's cores: DF build, primary, shadow and GI,
+    median of `reps` after a warm-up.  Its outputs are also compared with
+    the GPU's (bit-exact hits)."""
+    import oracle
+    import torch
+    iv = np.array(cam.inv_view, np.float32)
+    ip = np.array(cam.inv_proj, np.float32)
+    W, H = cam.width, cam.height
+    sun = np.array(prm.sun_dir, np.float32)
+    oprm = oracle.gi_params_from(prm)
+    times = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        df = oracle.df_build(grid)
+        t1 = time.perf_counter()
+        p = oracle.trace_primary(grid, df, iv, ip, W, H, 350)
+        s = oracle.trace_shadow(grid, df, iv, ip, p["t"], p["face"], sun, 350)
+        g = oracle.trace_gi(grid, df, iv, ip, p["t"], p["face"], oprm)
+        t2 = time.perf_counter()
+        if r > 0:
+            times.append((t1 - t0, t2 - t1))
+    df_s = float(np.median([a for a, _ in times]))
+    ray_s = float(np.median([b for _, b in times]))
+    rays = int(W * H + s["rays"].sum() + g["rays"].sum())
+    host = gb.numpy()
+    parity = bool(np.array_equal(host["face"], p["face"]) and np.array_equal(host["voxel"], p["voxel"])
+                  and np.array_equal(host["t"], p["t"]) and np.array_equal(mask.cpu().numpy(), s["mask"])
+                  and np.array_equal(rgb.cpu().numpy(), g["rgb"]))
+    threads = oracle.threads()
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(rays / ray_s / 1e9, 6), "unit": "Grays/s", "cores": threads, "kind": "port",
+            "sample": f"one {W}x{H} frame (primary+shadow+GI, {rays} rays), median of {reps} after 1 warm-up; "
+                      f"DF build {df_s * 1e3:.1f} ms",
+            "df_build_ms": round(df_s * 1e3, 2), "cpu_model": model, "gpu_matches_oracle": parity}
+
+
+if __name__ == "__main__":
+    main()

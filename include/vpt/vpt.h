@@ -1,0 +1,251 @@
+/*
+ * vpt.h — C ABI of the MI355X (gfx950) voxel path tracer hot path.
+ *
+ * Replaces the reference's GL-level boundary for three pieces:
+ *   - the distance-field build     (World::InitializeDistanceGenerator /
+ *                                    World::GenerateDistanceField,
+ *                                    Core/World.cpp:48-113, World.h:173-174)
+ *   - the DF-accelerated traversal (INITIAL_TRACE / SHADOW_TRACE /
+ *                                    DIFFUSE_TRACE passes, selected in
+ *                                    Core/ShaderManager.cpp:8-44 and drawn in
+ *                                    Core/Pipeline.cpp:1963-2016, 2171-2281,
+ *                                    2790-2852)
+ *   - the SVGF chain               (SPATIAL_INITIAL + SVGF_TEMPORAL +
+ *                                    SVGF_VARIANCE + 5x SVGF_SPATIAL,
+ *                                    Core/Pipeline.cpp:2284-2611)
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative VPT_E* code;
+ *     vpt_last_error() returns a message for the calling thread.  (The
+ *     reference reports errors by printf / throw const char*,
+ *     ShaderManager.cpp:96,111,126.)
+ *   - Pointers named d_* are device pointers; all others are host pointers
+ *     borrowed for the duration of the call.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Work is enqueued asynchronously on it; host-pointer outputs are
+ *     complete when the call returns.
+ *   - One host thread per object at a time; concurrency comes from streams.
+ *   - Matrices are 4x4 float, column-major (GLSL / glm layout).
+ *   - Images are row-major W x H; row 0 is the bottom row (v_TexCoords.y
+ *     smallest), pixel centre uv = ((px + 0.5) / W, (py + 0.5) / H).
+ *   - World grids are uint8 block ids, 0 = air, linear index
+ *     x + y*nx + z*nx*ny (World.h:46-70).
+ */
+#ifndef VPT_H
+#define VPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VPT_API __attribute__((visibility("default")))
+
+enum {
+    VPT_OK = 0,
+    VPT_EINVAL = -1,     /* bad argument (dims, null pointer, sizes) */
+    VPT_ENOMEM = -2,     /* device allocation failed / budget exceeded */
+    VPT_EDEVICE = -3,    /* HIP runtime error */
+    VPT_ESTATE = -4,     /* object used in the wrong state */
+};
+
+typedef struct vpt_ctx vpt_ctx;
+typedef struct vpt_world vpt_world;
+typedef struct vpt_svgf vpt_svgf;
+
+/* Camera as the passes see it: inverse view / inverse projection
+ * (u_InverseView, u_InverseProjection; ray origin = inv_view[12..14]). */
+typedef struct vpt_camera {
+    float inv_view[16];
+    float inv_proj[16];
+    int32_t width, height;
+} vpt_camera;
+
+/* G-buffer written by vpt_trace_primary (InitialTraceFBO, Pipeline.cpp:1094):
+ *   t      hit distance (o_HitDistance; -1 on a miss)                 fp32
+ *   face   normal id 0..5 (+z,-z,+y,-y,-x,+x), 255 on a miss (o_Normal) u8
+ *   block  hit block id, 0 on a miss (o_BlockID)                       u8
+ *   voxel  hit cell x | y<<10 | z<<20, 0xffffffff on a miss            u32
+ * All four are device pointers of width*height elements. */
+typedef struct vpt_gbuffer {
+    float* d_t;
+    uint8_t* d_face;
+    uint8_t* d_block;
+    uint32_t* d_voxel;
+    int32_t width, height;
+} vpt_gbuffer;
+
+/* Frame constants of the diffuse GI pass (DiffuseRayTraceFrag.glsl:822-840)
+ * with the workload's substitutions: PCG randoms seeded by
+ * pcg_hash(px + W*py + W*H*frame), Lambert sun term, per-id albedo and a
+ * constant sky. */
+typedef struct vpt_gi_params {
+    float sun_dir[3];            /* StrongerLightDirection, normalised */
+    float light_color[3];        /* LIGHT_COLOR */
+    float sky_radiance[3];       /* radiance of a GI ray that escapes */
+    float sky_pixel_radiance[3]; /* radiance written for primary-miss pixels */
+    float albedo[128 * 3];       /* Lambert albedo per block id */
+    uint32_t frame;
+    int32_t gi_iter;             /* traversal cap of the GI ray (48) */
+    int32_t shadow_iter;         /* traversal cap of the GI-hit shadow ray (128) */
+} vpt_gi_params;
+
+/* Per-pixel SVGF input planes written by vpt_trace_gi when non-NULL
+ * (DiffuseRawTraceFBO, Pipeline.cpp:1102): one 16-byte texel per pixel,
+ * uint16 halves [SH0 SH1 SH2 SH3 Co Cg luma aosky], where the last half
+ * holds the two unorm8 bytes (AO, sky) as (sky << 8) | ao. */
+typedef struct vpt_svgf_texel {
+    uint16_t h[8];
+} vpt_svgf_texel;
+
+/* An arbitrary ray (ray-query API): origin and unit direction. */
+typedef struct vpt_ray {
+    float origin[3];
+    float dir[3];
+} vpt_ray;
+
+/* Result of one ray query. */
+typedef struct vpt_hit {
+    float t;          /* -1 on a miss */
+    uint32_t voxel;   /* packed cell as in vpt_gbuffer, ~0 on a miss */
+    uint8_t face;     /* 0..5, 255 on a miss */
+    uint8_t block;    /* 0 on a miss */
+    uint16_t samples; /* DF samples taken (loop iterations inside the volume) */
+} vpt_hit;
+
+/* ---- diagnostics ------------------------------------------------------ */
+
+VPT_API const char* vpt_last_error(void);
+VPT_API const char* vpt_version(void);
+
+/* ---- context ----------------------------------------------------------- */
+
+/* Binds `device` (HIP ordinal) for the lifetime of the context. */
+VPT_API int vpt_ctx_create(int device, vpt_ctx** out);
+VPT_API int vpt_ctx_destroy(vpt_ctx* ctx);
+VPT_API int vpt_ctx_device(const vpt_ctx* ctx, int* device);
+
+/* ---- world (World.h:35-222) ----------------------------------------------
+ * Device-resident block grid plus distance field, both stored in 8x8x8
+ * bricks with Morton order inside a brick.  Dims must be multiples of 8,
+ * nx, ny, nz in [8, 1024]. */
+VPT_API int vpt_world_create(vpt_ctx* ctx, int nx, int ny, int nz, vpt_world** out);
+VPT_API int vpt_world_destroy(vpt_world* w);
+VPT_API int vpt_world_dims(const vpt_world* w, int* nx, int* ny, int* nz);
+/* Device bytes held by the world (grid + DF + staging). */
+VPT_API int vpt_world_bytes(const vpt_world* w, size_t* bytes);
+
+/* World::Buffer (World.h:167-171, Texture3D.cpp:27): upload a linear grid
+ * from host memory. */
+VPT_API int vpt_world_upload(vpt_world* w, const uint8_t* host_grid, void* stream);
+/* Same from a linear grid already in device memory. */
+VPT_API int vpt_world_upload_device(vpt_world* w, const uint8_t* d_grid, void* stream);
+
+/* Batched World::SetBlock (World.h:51-70) followed by the single-voxel
+ * glTexSubImage3D of World.cpp:367-374.  edits[i] = x | y<<10 | z<<20 and
+ * ids[i] the new block id.  Does not rebuild the DF. */
+VPT_API int vpt_world_set_blocks(vpt_world* w, const uint32_t* edits, const uint8_t* ids, int n, void* stream);
+
+/* World::GenerateDistanceField (World.cpp:69-113):
+ * DF = min(254, L1 distance to the nearest block with id > 0). */
+VPT_API int vpt_df_build(vpt_world* w, void* stream);
+
+/* Read back (parity tests / World::GetBlock): linear x-fastest layout. */
+VPT_API int vpt_df_download(vpt_world* w, uint8_t* host_df, void* stream);
+VPT_API int vpt_grid_download(vpt_world* w, uint8_t* host_grid, void* stream);
+
+/* ---- rays ---------------------------------------------------------------- */
+
+/* INITIAL_TRACE (InitialRayTraceFrag.glsl:417-468): one primary ray per
+ * pixel, at most max_iter traversal iterations (u_RenderDistance). */
+VPT_API int vpt_trace_primary(vpt_world* w, const vpt_camera* cam, int max_iter, const vpt_gbuffer* out, void* stream);
+
+/* SHADOW_TRACE (ShadowRayTraceFrag.glsl:414-513) at the G-buffer's
+ * resolution, unjittered: d_mask[i] = 1 in shadow, 0 lit or sky. */
+VPT_API int vpt_trace_shadow(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const float sun_dir[3],
+                             int max_iter, uint8_t* d_mask, void* stream);
+
+/* DIFFUSE_TRACE (DiffuseRayTraceFrag.glsl:822-935), 1 spp, 1 bounce with a
+ * shadow ray at the bounce hit.  d_rgb: 3 floats per pixel (clamped to
+ * [0, 8]).  d_svgf_in (optional): SVGF input texels. */
+VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
+                         float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream);
+
+/* Ray queries (serving API): n rays, traversal cap max_iter. */
+VPT_API int vpt_trace_rays(vpt_world* w, const vpt_ray* d_rays, int n, int max_iter, vpt_hit* d_hits, void* stream);
+
+/* Work counters for bandwidth accounting: when d_counter is non-NULL (two
+ * uint64 in device memory), every later trace call on this world adds its
+ * DF samples to d_counter[0] and its traced rays to d_counter[1].  Pass
+ * NULL to disable. */
+VPT_API int vpt_world_set_sample_counter(vpt_world* w, unsigned long long* d_counter);
+
+/* ---- SVGF ---------------------------------------------------------------
+ * Owns the history and ping-pong planes of one W x H view. */
+typedef struct vpt_svgf_params {
+    float color_phi_bias;   /* u_ColorPhiBias, 3.325 (Pipeline.cpp:85) */
+    float resolution_scale; /* u_ResolutionScale, 0.25 (Pipeline.cpp:78) */
+    int32_t pre_spatial;    /* PreTemporalSpatialPass (Pipeline.cpp:239) */
+    int32_t steps[5];       /* a-trous steps, 16 8 4 2 1 (Pipeline.cpp:2478-2497) */
+} vpt_svgf_params;
+
+VPT_API int vpt_svgf_create(vpt_ctx* ctx, int width, int height, const vpt_svgf_params* prm, vpt_svgf** out);
+VPT_API int vpt_svgf_destroy(vpt_svgf* s);
+/* Clears the temporal history (first frame after a cut). */
+VPT_API int vpt_svgf_reset(vpt_svgf* s, void* stream);
+
+/* One frame of the chain.  cur/prev: this and the previous frame's
+ * G-buffer; cam: this frame's camera; prev_viewproj: previous projection *
+ * previous view; d_in: this frame's raw GI texels; time: the value used for
+ * u_Time (a-trous jitter).  d_rgb_out: decoded irradiance, 3 floats per
+ * pixel (SHToIrridiance with the G-buffer normal, ColorPassFrag.glsl:356).
+ * d_texel_out (optional): the final SH/CoCg/variance/AO texels. */
+VPT_API int vpt_svgf_frame(vpt_svgf* s, const vpt_camera* cam, const float prev_viewproj[16],
+                           const vpt_gbuffer* cur, const vpt_gbuffer* prev, const vpt_svgf_texel* d_in, float time,
+                           float* d_rgb_out, vpt_svgf_texel* d_texel_out, void* stream);
+
+/* Debug access to the planes after the last frame: which = 0 pre-spatial,
+ * 1 temporal (history), 2 variance, 3 final a-trous output.  Copies
+ * width*height texels to device memory d_dst. */
+VPT_API int vpt_svgf_read_plane(vpt_svgf* s, int which, vpt_svgf_texel* d_dst, void* stream);
+/* The temporal utility plane of the last frame (o_Utility of
+ * TemporalFilter.glsl: accumulated frames, moment, luma as binary16, plus
+ * one unused half): 4 halves per pixel. */
+VPT_API int vpt_svgf_read_utility(vpt_svgf* s, uint16_t* d_dst, void* stream);
+
+/* ---- host utilities ------------------------------------------------------ */
+
+/* Synthetic terrain: seeded 2D value-noise heightfield in [h_min, h_max],
+ * grass (1) on top, 4 dirt (2), stone (3) below (WorldGenerator.cpp:28-47
+ * layering), then `random_permille` / 1000 of the solid voxels replaced by
+ * ids 1..100 (seed_random).  Host memory, linear layout. */
+VPT_API int vpt_generate_terrain(int nx, int ny, int nz, uint32_t seed, uint32_t seed_random, int h_min, int h_max,
+                                 int random_permille, uint8_t* host_grid);
+
+/* glm::lookAt / glm::perspective camera (FpsCamera.cpp:23-24,66-72,150-163):
+ * front = (cos p cos y, sin p, cos p sin y), up = +y. view/proj/inverses
+ * are written when non-NULL. */
+VPT_API int vpt_camera_make(const float pos[3], float yaw_deg, float pitch_deg, float fov_deg, int width, int height,
+                            float znear, float zfar, vpt_camera* cam, float view[16], float proj[16]);
+
+/* Frame constants of the GI pass for a sun tick (Pipeline.cpp:1652-1671,
+ * 1828; DiffuseRayTraceFrag.glsl:181-186, 829-840): sun direction, light
+ * colour, sky radiance from a constant sky colour; albedo table seeded. */
+VPT_API int vpt_gi_params_default(float sun_tick, uint32_t frame, uint32_t albedo_seed, vpt_gi_params* prm);
+
+VPT_API int vpt_svgf_params_default(vpt_svgf_params* prm);
+
+/* World::RaycastDetect (World.cpp:496-546): CPU block picking over a host
+ * grid, 48 epsilon steps of 0.001 past each cell boundary; cells with any
+ * coordinate <= 0 count as outside.  out = (x, y, z, id), or all -1 when
+ * nothing is hit (the reference falls off the end of the function there). */
+VPT_API int vpt_raycast_detect(const uint8_t* host_grid, int nx, int ny, int nz, const float pos[3],
+                               const float dir[3], int32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VPT_H */

@@ -1,0 +1,226 @@
+"""CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper of ``oracle/_build/libvpt_oracle.so`` (built from
+``oracle/vpt_oracle.cpp`` by ``oracle/Makefile``), a line-for-line C++
+restatement of the reference shaders.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` may use
+it, and only as the checker: the product path never calls it.
+
+All arrays are numpy; images are row-major (H, W, ...).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_DIR = Path(__file__).resolve().parent
+LIB_PATH = _DIR / "_build" / "libvpt_oracle.so"
+
+
+class OGiParams(C.Structure):
+    _fields_ = [("sun_dir", C.c_float * 3), ("light_color", C.c_float * 3), ("sky_radiance", C.c_float * 3),
+                ("sky_pixel_radiance", C.c_float * 3), ("albedo", C.c_float * 384), ("frame", C.c_uint32),
+                ("gi_iter", C.c_int32), ("shadow_iter", C.c_int32)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            import subprocess
+            subprocess.run(["make", "-s", "-C", str(_DIR)], check=True)
+        _lib = C.CDLL(str(LIB_PATH))
+    return _lib
+
+
+def _p(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def df_build(grid: np.ndarray) -> np.ndarray:
+    """grid: (nz, ny, nx) uint8 -> DF (nz, ny, nx) uint8."""
+    grid = np.ascontiguousarray(grid, dtype=np.uint8)
+    nz, ny, nx = grid.shape
+    df = np.empty_like(grid)
+    lib().oracle_df_build(_p(grid), nx, ny, nz, _p(df))
+    return df
+
+
+def trace_rays(grid, df, rays: np.ndarray, max_iter: int):
+    """rays: (n, 6) float32 origin+dir.  Returns dict of t, cell (n,3), face, block, samples."""
+    nz, ny, nx = grid.shape
+    rays = _f32(rays)
+    n = rays.shape[0]
+    t = np.empty(n, np.float32)
+    cell = np.empty((n, 3), np.int32)
+    face = np.empty(n, np.uint8)
+    block = np.empty(n, np.uint8)
+    samples = np.empty(n, np.int32)
+    lib().oracle_trace_rays(_p(grid), _p(df), nx, ny, nz, _p(rays), C.c_int(n), C.c_int(max_iter), _p(t), _p(cell),
+                            _p(face), _p(block), _p(samples))
+    return dict(t=t, cell=cell, face=face, block=block, samples=samples)
+
+
+def trace_primary(grid, df, inv_view, inv_proj, W: int, H: int, max_iter: int = 350):
+    nz, ny, nx = grid.shape
+    iv, ip = _f32(inv_view), _f32(inv_proj)
+    t = np.empty((H, W), np.float32)
+    face = np.empty((H, W), np.uint8)
+    block = np.empty((H, W), np.uint8)
+    voxel = np.empty((H, W), np.uint32)
+    samples = np.empty((H, W), np.int32)
+    lib().oracle_trace_primary(_p(grid), _p(df), nx, ny, nz, _p(iv), _p(ip), W, H, max_iter, _p(t), _p(face),
+                               _p(block), _p(voxel), _p(samples))
+    return dict(t=t, face=face, block=block, voxel=voxel, samples=samples)
+
+
+def trace_shadow(grid, df, inv_view, inv_proj, gb_t, gb_face, sun_dir, max_iter: int = 350):
+    nz, ny, nx = grid.shape
+    H, W = gb_t.shape
+    mask = np.empty((H, W), np.uint8)
+    samples = np.empty((H, W), np.int32)
+    rays = np.empty((H, W), np.int32)
+    sun = _f32(sun_dir)
+    lib().oracle_trace_shadow(_p(grid), _p(df), nx, ny, nz, _p(_f32(inv_view)), _p(_f32(inv_proj)), W, H,
+                              _p(_f32(gb_t)), _p(np.ascontiguousarray(gb_face, np.uint8)), _p(sun), max_iter,
+                              _p(mask), _p(samples), _p(rays))
+    return dict(mask=mask, samples=samples, rays=rays)
+
+
+def gi_params_from(prm) -> OGiParams:
+    """Copies a product-side GiParams (same layout) into the oracle struct."""
+    o = OGiParams()
+    C.memmove(C.addressof(o), C.addressof(prm), C.sizeof(OGiParams))
+    return o
+
+
+def trace_gi(grid, df, inv_view, inv_proj, gb_t, gb_face, prm: OGiParams):
+    nz, ny, nx = grid.shape
+    H, W = gb_t.shape
+    rgb = np.empty((H, W, 3), np.float32)
+    sh = np.empty((H, W, 6), np.float32)
+    luma = np.empty((H, W), np.float32)
+    aosky = np.empty((H, W, 2), np.float32)
+    samples = np.empty((H, W), np.int32)
+    rays = np.empty((H, W), np.int32)
+    lib().oracle_trace_gi(_p(grid), _p(df), nx, ny, nz, _p(_f32(inv_view)), _p(_f32(inv_proj)), W, H,
+                          _p(_f32(gb_t)), _p(np.ascontiguousarray(gb_face, np.uint8)), C.byref(prm), _p(rgb), _p(sh),
+                          _p(luma), _p(aosky), _p(samples), _p(rays))
+    return dict(rgb=rgb, sh=sh, luma=luma, aosky=aosky, samples=samples, rays=rays)
+
+
+# ---- SVGF passes (planes are float arrays of storage-rounded values) -------
+
+def svgf_prespatial(inv_view, inv_proj, gb_t, gb_face, sh, cocg, util, ao):
+    H, W = gb_t.shape
+    o_sh = np.empty((H, W, 4), np.float32)
+    o_cc = np.empty((H, W, 2), np.float32)
+    o_ut = np.empty((H, W), np.float32)
+    o_ao = np.empty((H, W, 2), np.float32)
+    lib().oracle_svgf_prespatial(_p(_f32(inv_view)), _p(_f32(inv_proj)), W, H, _p(_f32(gb_t)),
+                                 _p(np.ascontiguousarray(gb_face, np.uint8)), _p(_f32(sh)), _p(_f32(cocg)),
+                                 _p(_f32(util)), _p(_f32(ao)), _p(o_sh), _p(o_cc), _p(o_ut), _p(o_ao))
+    return dict(sh=o_sh, cocg=o_cc, util=o_ut, ao=o_ao)
+
+
+def svgf_temporal(inv_view, inv_proj, prev_vp, cur_gb, prev_gb, cur, prev):
+    """cur_gb / prev_gb: dicts with t, face, block.  cur: dict sh, cocg, luma, ao.
+    prev: dict sh, cocg, util (H,W,3), ao."""
+    H, W = cur_gb["t"].shape
+    o_sh = np.empty((H, W, 4), np.float32)
+    o_cc = np.empty((H, W, 2), np.float32)
+    o_ut = np.empty((H, W, 3), np.float32)
+    o_ao = np.empty((H, W, 2), np.float32)
+    u8 = lambda a: np.ascontiguousarray(a, np.uint8)  # noqa: E731
+    lib().oracle_svgf_temporal(_p(_f32(inv_view)), _p(_f32(inv_proj)), _p(_f32(prev_vp)), W, H,
+                               _p(_f32(cur_gb["t"])), _p(u8(cur_gb["face"])), _p(u8(cur_gb["block"])),
+                               _p(_f32(prev_gb["t"])), _p(u8(prev_gb["face"])), _p(u8(prev_gb["block"])),
+                               _p(_f32(cur["sh"])), _p(_f32(cur["cocg"])), _p(_f32(cur["luma"])), _p(_f32(cur["ao"])),
+                               _p(_f32(prev["sh"])), _p(_f32(prev["cocg"])), _p(_f32(prev["util"])),
+                               _p(_f32(prev["ao"])), _p(o_sh), _p(o_cc), _p(o_ut), _p(o_ao))
+    return dict(sh=o_sh, cocg=o_cc, util=o_ut, ao=o_ao)
+
+
+def svgf_variance(inv_view, inv_proj, gb_t, gb_face, sh, cocg, util):
+    H, W = gb_t.shape
+    o_sh = np.empty((H, W, 4), np.float32)
+    o_cc = np.empty((H, W, 2), np.float32)
+    o_var = np.empty((H, W), np.float32)
+    lib().oracle_svgf_variance(_p(_f32(inv_view)), _p(_f32(inv_proj)), W, H, _p(_f32(gb_t)),
+                               _p(np.ascontiguousarray(gb_face, np.uint8)), _p(_f32(sh)), _p(_f32(cocg)),
+                               _p(_f32(util)), _p(o_sh), _p(o_cc), _p(o_var))
+    return dict(sh=o_sh, cocg=o_cc, var=o_var)
+
+
+def svgf_atrous(gb_t, gb_face, sh, cocg, var, ao, temporal_util, step: int, scale: float = 1.35,
+                color_phi_bias: float = 3.325, time: float = 0.0):
+    H, W = gb_t.shape
+    o_sh = np.empty((H, W, 4), np.float32)
+    o_cc = np.empty((H, W, 2), np.float32)
+    o_var = np.empty((H, W), np.float32)
+    o_ao = np.empty((H, W, 2), np.float32)
+    lib().oracle_svgf_atrous(W, H, _p(_f32(gb_t)), _p(np.ascontiguousarray(gb_face, np.uint8)), _p(_f32(sh)),
+                             _p(_f32(cocg)), _p(_f32(var)), _p(_f32(ao)), _p(_f32(temporal_util)), C.c_int(step),
+                             C.c_float(scale), C.c_float(color_phi_bias), C.c_float(time), _p(o_sh), _p(o_cc),
+                             _p(o_var), _p(o_ao))
+    return dict(sh=o_sh, cocg=o_cc, var=o_var, ao=o_ao)
+
+
+def sh_decode(sh, cocg, face):
+    H, W = face.shape
+    rgb = np.empty((H, W, 3), np.float32)
+    lib().oracle_sh_decode(C.c_int(H * W), _p(_f32(sh)), _p(_f32(cocg)), _p(np.ascontiguousarray(face, np.uint8)),
+                           _p(rgb))
+    return rgb
+
+
+def raycast_detect(grid, pos, direction):
+    nz, ny, nx = grid.shape
+    out = np.empty(4, np.int32)
+    lib().oracle_raycast_detect(_p(grid), nx, ny, nz, _p(_f32(pos)), _p(_f32(direction)), _p(out))
+    return out
+
+
+def dda_textbook(grid, origin, direction, max_t: float):
+    nz, ny, nx = grid.shape
+    out = np.empty(4, np.int32)
+    o = np.ascontiguousarray(origin, np.float64)
+    d = np.ascontiguousarray(direction, np.float64)
+    lib().oracle_dda_textbook(_p(grid), nx, ny, nz, _p(o), _p(d), C.c_double(max_t), _p(out))
+    return out
+
+
+def camera(pos, yaw, pitch, fov=60.0, aspect=16 / 9, znear=0.1, zfar=1000.0):
+    view = np.empty(16, np.float32)
+    proj = np.empty(16, np.float32)
+    iv = np.empty(16, np.float32)
+    ip = np.empty(16, np.float32)
+    lib().oracle_camera(_p(_f32(pos)), C.c_float(yaw), C.c_float(pitch), C.c_float(fov), C.c_float(aspect),
+                        C.c_float(znear), C.c_float(zfar), _p(view), _p(proj), _p(iv), _p(ip))
+    return dict(view=view, proj=proj, inv_view=iv, inv_proj=ip)
+
+
+def time_plain(grid, inv_view, inv_proj, W, H, max_iter=350, reps=5):
+    """CPU baseline: DF build + primary trace, median of `reps` after a warm-up.
+    Returns dict(df_ms, primary_ms, samples, threads)."""
+    nz, ny, nx = grid.shape
+    out = np.empty(4, np.float64)
+    lib().oracle_time_plain(_p(grid), nx, ny, nz, _p(_f32(inv_view)), _p(_f32(inv_proj)), W, H, max_iter, reps,
+                            _p(out))
+    return dict(df_ms=float(out[0]), primary_ms=float(out[1]), samples=float(out[2]), threads=int(out[3]))
+
+
+def threads() -> int:
+    return int(lib().oracle_threads())

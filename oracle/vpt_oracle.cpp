@@ -1,0 +1,1218 @@
+/*
+ * vpt_oracle.cpp — CPU oracle for the voxel path tracer hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is a line-for-line C++17 restatement
+ * of the reference's GLSL passes, used by tests/, by smoke() and by the
+ * CPU-baseline leg of bench.py as the checker.  It is never linked into,
+ * loaded by, or called from the product library (voxelpathtracer_amd/).
+ *
+ * Build: oracle/Makefile (g++ -O3 -fopenmp -ffp-contract=off, no fast-math).
+ *
+ * Reference files followed (paths relative to the reference repository):
+ *   DF build        Core/Shaders/ManhattanDistance{X,Y,Z}.comp, Core/World.cpp:69-113
+ *   traversal       Core/Shaders/InitialRayTraceFrag.glsl:68-102, :307-374
+ *                   (identical copies: ShadowRayTraceFrag.glsl:222-289,
+ *                    DiffuseRayTraceFrag.glsl:1043-1110)
+ *   primary rays    InitialRayTraceFrag.glsl:143-185, :398-468; FBOVert.glsl:13-22
+ *   shadow rays     ShadowRayTraceFrag.glsl:303-331, :414-513
+ *   diffuse GI      DiffuseRayTraceFrag.glsl:535-664, :766-784, :822-935,
+ *                   :945-967, :984-988, :1202-1222
+ *   SVGF            Core/Shaders/Spatial3x3Initial.glsl, SVGF/TemporalFilter.glsl,
+ *                   SVGF/VarianceEstimate.glsl, SVGF/SpatialFilter.glsl,
+ *                   wiring Core/Pipeline.cpp:2284-2611
+ *   SH decode       Core/Shaders/ColorPassFrag.glsl:356-368
+ *   CPU picking     Core/World.cpp:496-546 (RaycastDetect)
+ *
+ * Parity status per function (what pins it besides this transcription):
+ *   oracle_df_build        closed form min(254, L1) + brute force   -> pinned
+ *   oracle_trace_*         textbook DDA agreement, RaycastDetect agreement,
+ *                          axis-aligned known answers, invariants  -> pinned
+ *                          (the reference's quirk cases are pinned by the
+ *                          transcription only; see the traversal tests)
+ *   oracle_gi              closed forms (SH round trip, luminance,
+ *                          cosine-lobe statistics)                 -> pinned
+ *   oracle_svgf_*          constant-image identity, weight
+ *                          normalisation, running-mean closed form,
+ *                          disocclusion passthrough                -> pinned
+ *   oracle_camera          closed forms of glm lookAt/perspective  -> pinned
+ */
+#include <stdint.h>
+#include <string.h>
+#include <math.h>
+#include <algorithm>
+#include <chrono>
+#include <vector>
+#include <omp.h>
+
+#include "../include/vpt/vpt_numerics.h"
+
+#define OAPI extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+struct V3 { float x, y, z; };
+static inline V3 v3(float x, float y, float z) { V3 r = {x, y, z}; return r; }
+static inline float comp(const V3& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+static inline void set_comp(V3& v, int i, float f) { if (i == 0) v.x = f; else if (i == 1) v.y = f; else v.z = f; }
+
+struct Dims { int nx, ny, nz; };
+
+/* IsInVolume(vec3(ivec3(floor(p)))) on the float value of floor(p)
+ * (InitialRayTraceFrag.glsl:68-77).  A NaN coordinate counts as outside. */
+static inline bool in_volume_f(float fx, float fy, float fz, const Dims& d)
+{
+    return fx >= 0.0f && fy >= 0.0f && fz >= 0.0f &&
+           fx <= (float)(d.nx - 1) && fy <= (float)(d.ny - 1) && fz <= (float)(d.nz - 1);
+}
+
+static inline size_t lin(int x, int y, int z, const Dims& d)
+{
+    return (size_t)x + (size_t)y * d.nx + (size_t)z * (size_t)d.nx * d.ny;
+}
+
+/* GetVoxel(ivec3(floor(p))) (InitialRayTraceFrag.glsl:79-87) as a byte id. */
+static inline int get_voxel_at(const uint8_t* grid, const Dims& d, V3 p)
+{
+    float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
+    if (!in_volume_f(fx, fy, fz, d)) return 0;
+    return grid[lin((int)fx, (int)fy, (int)fz, d)];
+}
+
+struct TraceResult {
+    float t;          /* distance(origin, initial_origin) or -1 */
+    int face;         /* 0..5, 255 on miss */
+    int block;        /* hit block id, 0 on miss */
+    int cx, cy, cz;   /* hit cell, -1 on miss */
+    int samples;      /* loop iterations that read the DF */
+};
+
+/* Face id of the normal with component -s on axis `axis`
+ * (GetNormalID, InitialRayTraceFrag.glsl:143-185: a zero normal matches no
+ * entry and yields id 0). */
+static inline int face_from_axis(int axis, int s)
+{
+    if (s == 0) return 0;
+    int n = -s;
+    if (axis == 2) return n > 0 ? 0 : 1;
+    if (axis == 1) return n > 0 ? 2 : 3;
+    return n < 0 ? 4 : 5;
+}
+
+/* VoxelTraversalDF (InitialRayTraceFrag.glsl:307-374). */
+static TraceResult voxel_traversal_df(const uint8_t* grid, const uint8_t* df, const Dims& d,
+                                      V3 origin, V3 direction, int max_iter)
+{
+    V3 initial_origin = origin;
+    bool intersection = false;
+    int min_idx = 0;
+    int rs[3];
+    for (int i = 0; i < 3; i++) {
+        float c = comp(direction, i);
+        rs[i] = c > 0.0f ? 1 : (c < 0.0f ? -1 : 0);      /* ivec3(sign(direction)) */
+    }
+    V3 inv = v3(1.0f / direction.x, 1.0f / direction.y, 1.0f / direction.z);
+    int samples = 0;
+
+    for (int itr = 0; itr < max_iter; itr++) {
+        float fx = floorf(origin.x), fy = floorf(origin.y), fz = floorf(origin.z);
+        if (!in_volume_f(fx, fy, fz, d)) {
+            intersection = false;
+            break;
+        }
+        int dist = df[lin((int)fx, (int)fy, (int)fz, d)];
+        samples++;
+        int euclidean = vpt_df_step(dist);
+        if (euclidean == 0) break;
+        if (euclidean == 1) {
+            /* one DDA step (lines 339-356) */
+            int g[3] = {(int)origin.x, (int)origin.y, (int)origin.z};
+            float w[3] = {origin.x - (float)g[0], origin.y - (float)g[1], origin.z - (float)g[2]};
+            int h[3] = {(1 + rs[0]) >> 1, (1 + rs[1]) >> 1, (1 + rs[2]) >> 1};
+            float f[3] = {((float)h[0] - w[0]) * inv.x, ((float)h[1] - w[1]) * inv.y,
+                          ((float)h[2] - w[2]) * inv.z};
+            min_idx = (f[0] < f[1] && rs[0] != 0)
+                          ? ((f[0] < f[2] || rs[2] == 0) ? 0 : 2)
+                          : ((f[1] < f[2] || rs[2] == 0) ? 1 : 2);
+            g[min_idx] += rs[min_idx];
+            float fm = f[min_idx];
+            w[0] = w[0] + direction.x * fm;
+            w[1] = w[1] + direction.y * fm;
+            w[2] = w[2] + direction.z * fm;
+            w[min_idx] = (float)(1 - h[min_idx]);
+            origin = v3((float)g[0] + w[0], (float)g[1] + w[1], (float)g[2] + w[2]);
+            set_comp(origin, min_idx, comp(origin, min_idx) + (float)rs[min_idx] * 0.0001f);
+            intersection = true;
+        } else {
+            float k = (float)(euclidean - 1);
+            origin = v3(origin.x + k * direction.x, origin.y + k * direction.y, origin.z + k * direction.z);
+        }
+    }
+
+    TraceResult r;
+    r.t = -1.0f; r.face = 255; r.block = 0; r.cx = r.cy = r.cz = -1; r.samples = samples;
+    if (intersection) {
+        int b = get_voxel_at(grid, d, origin);
+        if (b > 0) {
+            float dx = origin.x - initial_origin.x;
+            float dy = origin.y - initial_origin.y;
+            float dz = origin.z - initial_origin.z;
+            r.t = sqrtf((dx * dx + dy * dy) + dz * dz);
+            r.face = face_from_axis(min_idx, rs[min_idx]);
+            r.block = b;
+            r.cx = (int)floorf(origin.x);
+            r.cy = (int)floorf(origin.y);
+            r.cz = (int)floorf(origin.z);
+        }
+    }
+    return r;
+}
+
+/* mat4 * vec4 with GLSL column-major storage m[col*4 + row]; the sum is taken
+ * in column order ((c0 + c1) + c2) + c3. */
+static inline void mat_vec4(const float* m, const float v[4], float out[4])
+{
+    for (int r = 0; r < 4; r++) {
+        float acc = m[0 * 4 + r] * v[0];
+        acc = acc + m[1 * 4 + r] * v[1];
+        acc = acc + m[2 * 4 + r] * v[2];
+        acc = acc + m[3 * 4 + r] * v[3];
+        out[r] = acc;
+    }
+}
+
+static inline V3 normalize3(V3 v)
+{
+    float len = sqrtf((v.x * v.x + v.y * v.y) + v.z * v.z);
+    return v3(v.x / len, v.y / len, v.z / len);
+}
+
+/* GetRayDirectionAt (ShadowRayTraceFrag.glsl:292-297; GetRayStuff
+ * InitialRayTraceFrag.glsl:398-414 without TAA jitter), not normalised. */
+static inline V3 ray_direction_at(const float* inv_view, const float* inv_proj, float sx, float sy)
+{
+    float clip[4] = {sx * 2.0f - 1.0f, sy * 2.0f - 1.0f, -1.0f, 1.0f};
+    float e[4];
+    mat_vec4(inv_proj, clip, e);
+    float eye[4] = {e[0], e[1], -1.0f, 0.0f};
+    float rd[4];
+    mat_vec4(inv_view, eye, rd);
+    return v3(rd[0], rd[1], rd[2]);
+}
+
+/* v_TexCoords at the centre of pixel (px, py) of a W x H target. */
+static inline float texcoord(int p, int n) { return ((float)p + 0.5f) / (float)n; }
+
+static inline V3 face_normal(int face, float miss_value)
+{
+    switch (face) {
+    case 0: return v3(0.0f, 0.0f, 1.0f);
+    case 1: return v3(0.0f, 0.0f, -1.0f);
+    case 2: return v3(0.0f, 1.0f, 0.0f);
+    case 3: return v3(0.0f, -1.0f, 0.0f);
+    case 4: return v3(-1.0f, 0.0f, 0.0f);
+    case 5: return v3(1.0f, 0.0f, 0.0f);
+    default: return v3(miss_value, miss_value, miss_value);
+    }
+}
+
+static inline float dot3(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline V3 cross3(V3 a, V3 b)
+{
+    return v3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+static inline float fclamp(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+} /* namespace */
+
+/* ======================================================================= */
+/* Distance field                                                           */
+/* ======================================================================= */
+
+/* GenerateDistanceField (World.cpp:69-113): X, Y, Z passes in that order.
+ * The DF image is R8 unorm: every store is unorm8(val / 255), every load is
+ * floor(unorm * 255) (ManhattanDistanceX.comp:25-38). */
+OAPI void oracle_df_build(const uint8_t* grid, int nx, int ny, int nz, uint8_t* df)
+{
+    Dims d = {nx, ny, nz};
+    const float max_distance = (float)std::min(254, nx + ny + nz);
+    auto store = [&](size_t i, float val) { df[i] = vpt_unorm8_encode(val / 255.0f); };
+    auto load = [&](size_t i) { return floorf(vpt_unorm8_decode(df[i]) * 255.0f); };
+
+    /* X pass (ManhattanDistanceX.comp:45-68) */
+#pragma omp parallel for schedule(static)
+    for (int zy = 0; zy < ny * nz; zy++) {
+        int y = zy % ny, z = zy / ny;
+        size_t i0 = lin(0, y, z, d);
+        store(i0, grid[i0] > 0 ? 0.0f : max_distance);
+        for (int x = 1; x < nx; x++) {
+            size_t i = lin(x, y, z, d);
+            store(i, grid[i] > 0 ? 0.0f : fminf(max_distance, 1.0f + load(i - 1)));
+        }
+        for (int x = nx - 2; x >= 0; x--) {
+            size_t i = lin(x, y, z, d);
+            if (load(i + 1) < load(i)) store(i, 1.0f + load(i + 1));
+        }
+    }
+    /* Y pass (ManhattanDistanceY.comp:94-118) */
+#pragma omp parallel for schedule(static)
+    for (int zx = 0; zx < nx * nz; zx++) {
+        int x = zx % nx, z = zx / nx;
+        for (int y = 1; y < ny; y++) {
+            size_t i = lin(x, y, z, d), p = lin(x, y - 1, z, d);
+            if (load(p) < load(i)) store(i, 1.0f + load(p));
+        }
+        for (int y = ny - 2; y >= 0; y--) {
+            size_t i = lin(x, y, z, d), p = lin(x, y + 1, z, d);
+            if (load(p) < load(i)) store(i, 1.0f + load(p));
+        }
+    }
+    /* Z pass (ManhattanDistanceZ.comp:142-165) */
+#pragma omp parallel for schedule(static)
+    for (int yx = 0; yx < nx * ny; yx++) {
+        int x = yx % nx, y = yx / nx;
+        for (int z = 1; z < nz; z++) {
+            size_t i = lin(x, y, z, d), p = lin(x, y, z - 1, d);
+            if (load(p) < load(i)) store(i, 1.0f + load(p));
+        }
+        for (int z = nz - 2; z >= 0; z--) {
+            size_t i = lin(x, y, z, d), p = lin(x, y, z + 1, d);
+            if (load(p) < load(i)) store(i, 1.0f + load(p));
+        }
+    }
+}
+
+/* ======================================================================= */
+/* Rays                                                                     */
+/* ======================================================================= */
+
+/* Arbitrary ray list: rays[6*i] = origin xyz, direction xyz (already
+ * normalised by the caller).  cell[3*i] = hit cell or -1. */
+OAPI void oracle_trace_rays(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+                            const float* rays, int n, int max_iter,
+                            float* t, int32_t* cell, uint8_t* face, uint8_t* block, int32_t* samples)
+{
+    Dims d = {nx, ny, nz};
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; i++) {
+        const float* r = rays + 6 * (size_t)i;
+        TraceResult tr = voxel_traversal_df(grid, df, d, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), max_iter);
+        t[i] = tr.t;
+        if (cell) { cell[3 * i] = tr.cx; cell[3 * i + 1] = tr.cy; cell[3 * i + 2] = tr.cz; }
+        if (face) face[i] = (uint8_t)tr.face;
+        if (block) block[i] = (uint8_t)tr.block;
+        if (samples) samples[i] = tr.samples;
+    }
+}
+
+static inline uint32_t pack_cell(int x, int y, int z)
+{
+    if (x < 0) return 0xffffffffu;
+    return (uint32_t)x | ((uint32_t)y << 10) | ((uint32_t)z << 20);
+}
+
+/* Primary trace (InitialRayTraceFrag.glsl:417-468).  Outputs per pixel
+ * (row-major, row 0 = v_TexCoords.y smallest): t (o_HitDistance), face
+ * (o_Normal id; 255 = miss, the unorm 1.0 of the reference), block
+ * (o_BlockID), voxel (packed hit cell x | y<<10 | z<<20, or ~0). */
+OAPI void oracle_trace_primary(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+                               const float* inv_view, const float* inv_proj, int W, int H, int max_iter,
+                               float* t, uint8_t* face, uint8_t* block, uint32_t* voxel, int32_t* samples)
+{
+    Dims d = {nx, ny, nz};
+    V3 org = v3(inv_view[12], inv_view[13], inv_view[14]);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int py = 0; py < H; py++) {
+        for (int px = 0; px < W; px++) {
+            size_t i = (size_t)py * W + px;
+            V3 rd = ray_direction_at(inv_view, inv_proj, texcoord(px, W), texcoord(py, H));
+            V3 dir = normalize3(rd);
+            TraceResult tr = voxel_traversal_df(grid, df, d, org, dir, max_iter);
+            bool intersect = tr.t > 0.0f && tr.block > 0;
+            t[i] = tr.t;
+            face[i] = (uint8_t)(intersect ? tr.face : 255);
+            block[i] = (uint8_t)(intersect ? tr.block : 0);
+            if (voxel) voxel[i] = intersect ? pack_cell(tr.cx, tr.cy, tr.cz) : 0xffffffffu;
+            if (samples) samples[i] = tr.samples;
+        }
+    }
+}
+
+/* World-space position of a G-buffer pixel: GetPositionAt
+ * (ShadowRayTraceFrag.glsl:299-303) at the pixel centre. */
+static inline V3 gbuffer_position(const float* inv_view, const float* inv_proj, int px, int py, int W, int H, float dist)
+{
+    V3 org = v3(inv_view[12], inv_view[13], inv_view[14]);
+    V3 dir = normalize3(ray_direction_at(inv_view, inv_proj, texcoord(px, W), texcoord(py, H)));
+    return v3(org.x + dir.x * dist, org.y + dir.y * dist, org.z + dir.z * dist);
+}
+
+/* Sun shadow (ShadowRayTraceFrag.glsl:414-513, soft-shadow cone off,
+ * alpha test off).  mask: 1 = in shadow, 0 = lit or sky. */
+OAPI void oracle_trace_shadow(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+                              const float* inv_view, const float* inv_proj, int W, int H,
+                              const float* gb_t, const uint8_t* gb_face, const float* sun_dir, int max_iter,
+                              uint8_t* mask, int32_t* samples, int32_t* rays)
+{
+    Dims d = {nx, ny, nz};
+    V3 L = v3(sun_dir[0], sun_dir[1], sun_dir[2]);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int py = 0; py < H; py++) {
+        for (int px = 0; px < W; px++) {
+            size_t i = (size_t)py * W + px;
+            int smp = 0, nr = 0;
+            float dist = gb_t[i];
+            if (dist < 0.0f) { mask[i] = 0; if (samples) samples[i] = 0; if (rays) rays[i] = 0; continue; }
+            V3 P = gbuffer_position(inv_view, inv_proj, px, py, W, H, dist);
+            V3 N = face_normal(gb_face[i], 1.0f);
+            float ndotl = dot3(N, L);
+            if (ndotl <= 0.01f) { mask[i] = 1; if (samples) samples[i] = 0; if (rays) rays[i] = 0; continue; }
+            V3 bias = v3(N.x * 0.06f, N.y * 0.06f, N.z * 0.06f);
+            V3 o = v3(P.x + bias.x, P.y + bias.y, P.z + bias.z);
+            float T = -1.0f;
+            int block_at = get_voxel_at(grid, d, o);
+            if (dist > 0.0f) {
+                TraceResult tr = voxel_traversal_df(grid, df, d, o, L, max_iter);
+                T = tr.t;
+                smp = tr.samples;
+                nr = 1;
+            }
+            mask[i] = (uint8_t)((T > 0.0f || block_at > 0) ? 1 : 0);
+            if (samples) samples[i] = smp;
+            if (rays) rays[i] = nr;
+        }
+    }
+}
+
+/* Parameters of the diffuse GI pass.  Host-side constants of the frame
+ * (DiffuseRayTraceFrag.glsl:822-840) are precomputed by the caller. */
+struct OGiParams {
+    float sun_dir[3];          /* StrongerLightDirection */
+    float light_color[3];      /* LIGHT_COLOR (already * 0.4 * GISunStrength) */
+    float sky_radiance[3];     /* GetSkyColorAt(dir) * x for a constant sky */
+    float sky_pixel_radiance[3]; /* texture(u_Skymap, v_RayDirection) * 2.66 */
+    float albedo[128 * 3];     /* Lambert albedo per block id */
+    uint32_t frame;
+    int gi_iter;               /* u_DiffuseTraceLength (48) */
+    int shadow_iter;           /* GetShadowAt cap (128) */
+};
+
+/* IrridianceToSH (DiffuseRayTraceFrag.glsl:766-784). */
+static inline void irradiance_to_sh(V3 radiance, V3 dir, float sh[6])
+{
+    float co = radiance.x - radiance.z;
+    float t = radiance.z + co * 0.5f;
+    float cg = radiance.y - t;
+    float y = fmaxf(t + cg * 0.5f, 0.0f);
+    float l00 = 0.282095f;
+    float l1_1 = 0.488603f * dir.y;
+    float l10 = 0.488603f * dir.z;
+    float l11 = 0.488603f * dir.x;
+    sh[0] = fmaxf(l11 * y, -100.0f);
+    sh[1] = fmaxf(l1_1 * y, -100.0f);
+    sh[2] = fmaxf(l10 * y, -100.0f);
+    sh[3] = fmaxf(l00 * y, -100.0f);
+    sh[4] = co;
+    sh[5] = cg;
+}
+
+static inline float luminance(V3 c) { return (c.x * 0.299f + c.y * 0.587f) + c.z * 0.114f; }
+
+/* cosWeightedRandomHemisphereDirection (DiffuseRayTraceFrag.glsl:945-967)
+ * with the two uniforms r drawn from PCG instead of hash2(). */
+static inline V3 cos_weighted_dir(V3 n, float r0, float r1)
+{
+    V3 uu = normalize3(cross3(n, v3(0.0f, 1.0f, 1.0f)));
+    V3 vv = cross3(uu, n);
+    float ra = sqrtf(r1);
+    float s, c;
+    vpt_sincos_2pi(r0, &s, &c);
+    float rx = ra * c;
+    float ry = ra * s;
+    float rz = sqrtf(1.0f - r1);
+    V3 rr = v3((rx * uu.x + ry * vv.x) + rz * n.x, (rx * uu.y + ry * vv.y) + rz * n.y,
+               (rx * uu.z + ry * vv.z) + rz * n.z);
+    return normalize3(rr);
+}
+
+/* The GI pixel seed: pcg_hash(px + W*py + W*H*frame), chained for r.y. */
+static inline void gi_random(int px, int py, int W, int H, uint32_t frame, float* r0, float* r1)
+{
+    uint32_t seed = (uint32_t)px + (uint32_t)W * (uint32_t)py + (uint32_t)W * (uint32_t)H * frame;
+    uint32_t h0 = vpt_pcg_hash(seed);
+    uint32_t h1 = vpt_pcg_hash(h0);
+    *r0 = vpt_u32_to_unit(h0);
+    *r1 = vpt_u32_to_unit(h1);
+}
+
+/* Diffuse GI, 1 spp, 1 bounce (DiffuseRayTraceFrag.glsl:822-935 with
+ * CalculateDiffuse :535-664, MAX_BOUNCE_LIMIT = 1).  Substitutions fixed by
+ * the workload: PCG randoms, Lambert sun BRDF (DiffuseHammon -> N.L / pi),
+ * per-id albedo, constant sky.  Outputs:
+ *   rgb[3*i]      clamp(radiance, 0, 8)   (sky pixels: sky_pixel_radiance)
+ *   sh[6*i]       IrridianceToSH(radiance, first bounce dir), clamped +-100
+ *   luma[i]       o_Utility, aosky[2*i] o_AOAndSkyLighting (clamped to [0,1])
+ * The SVGF input planes are stored in binary16 / unorm8 by the caller. */
+OAPI void oracle_trace_gi(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+                          const float* inv_view, const float* inv_proj, int W, int H,
+                          const float* gb_t, const uint8_t* gb_face, const OGiParams* prm,
+                          float* rgb, float* sh_out, float* luma, float* aosky, int32_t* samples,
+                          int32_t* rays)
+{
+    Dims d = {nx, ny, nz};
+    V3 L = v3(prm->sun_dir[0], prm->sun_dir[1], prm->sun_dir[2]);
+    V3 light = v3(prm->light_color[0], prm->light_color[1], prm->light_color[2]);
+    const float PI = 3.14159265359f;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int py = 0; py < H; py++) {
+        for (int px = 0; px < W; px++) {
+            size_t i = (size_t)py * W + px;
+            float dist = gb_t[i];
+            V3 N = face_normal(gb_face[i], 0.5f);
+            int smp = 0;
+            if (dist < 0.0f) {
+                V3 sky = v3(prm->sky_pixel_radiance[0], prm->sky_pixel_radiance[1], prm->sky_pixel_radiance[2]);
+                float sh[6];
+                irradiance_to_sh(sky, N, sh);
+                for (int k = 0; k < 6; k++) sh_out[6 * i + k] = sh[k];
+                rgb[3 * i] = sky.x; rgb[3 * i + 1] = sky.y; rgb[3 * i + 2] = sky.z;
+                luma[i] = 0.0f;                  /* GetLuminance(vec3(0)) */
+                aosky[2 * i] = 1.0f; aosky[2 * i + 1] = 0.0f;
+                if (samples) samples[i] = 0;
+                if (rays) rays[i] = 0;
+                continue;
+            }
+            V3 P = gbuffer_position(inv_view, inv_proj, px, py, W, H, dist);
+            float r0, r1;
+            gi_random(px, py, W, H, prm->frame, &r0, &r1);
+
+            /* CalculateDiffuse, bounce 0 */
+            const float bias = 0.06f;
+            V3 o = v3(P.x + N.x * bias, P.y + N.y * bias, P.z + N.z * bias);
+            V3 dir = cos_weighted_dir(N, r0, r1);
+            float ao = 1.0f;
+            bool skyhit = false;
+            V3 contrib = v3(0.0f, 0.0f, 0.0f);
+            TraceResult tr = voxel_traversal_df(grid, df, d, o, dir, prm->gi_iter);
+            smp += tr.samples;
+            int nr = 1;
+            float T = tr.t;
+            if (T > 0.0f && tr.block > 0) {
+                V3 ip = v3(o.x + dir.x * T, o.y + dir.y * T, o.z + dir.z * T);
+                V3 hn = face_normal(tr.face, 0.0f);
+                int tex_ref = std::min(std::max(tr.block, 0), 127);
+                V3 albedo = v3(prm->albedo[3 * tex_ref], prm->albedo[3 * tex_ref + 1], prm->albedo[3 * tex_ref + 2]);
+                float ndotl = fmaxf(dot3(hn, L), 0.0f);
+                float shadow_at;
+                if (ndotl < 0.001f) {
+                    shadow_at = 0.0f;
+                } else {
+                    V3 so = v3(ip.x + hn.x * 0.045f, ip.y + hn.y * 0.045f, ip.z + hn.z * 0.045f);
+                    TraceResult st = voxel_traversal_df(grid, df, d, so, L, prm->shadow_iter);
+                    smp += st.samples;
+                    nr += 1;
+                    shadow_at = st.t > 0.0f ? 1.0f : 0.0f;
+                }
+                /* SunBRDF (:499-508) with DiffuseHammon replaced by Lambert N.L/pi */
+                float brdf = ndotl * (1.0f / PI);
+                V3 rad35 = v3(light.x * 3.5f, light.y * 3.5f, light.z * 3.5f);
+                float vis = 1.0f - shadow_at;
+                V3 sun = v3(((albedo.x * brdf) * rad35.x) * vis * PI,
+                            ((albedo.y * brdf) * rad35.y) * vis * PI,
+                            ((albedo.z * brdf) * rad35.z) * vis * PI);
+                contrib = sun;
+            } else {
+                V3 sky = v3(prm->sky_radiance[0], prm->sky_radiance[1], prm->sky_radiance[2]);
+                contrib = sky;
+                skyhit = true;
+            }
+            if (T < 2.0f && T > 0.0f) ao = fmaxf(T / 2.0f, 0.0f);
+
+            V3 x = v3(fclamp(contrib.x, 0.0f, 8.0f), fclamp(contrib.y, 0.0f, 8.0f), fclamp(contrib.z, 0.0f, 8.0f));
+            float sh[6];
+            irradiance_to_sh(x, dir, sh);
+            for (int k = 0; k < 4; k++) sh_out[6 * i + k] = fclamp(sh[k], -100.0f, 100.0f);
+            sh_out[6 * i + 4] = fclamp(sh[4], -100.0f, 100.0f);
+            sh_out[6 * i + 5] = fclamp(sh[5], -100.0f, 100.0f);
+            rgb[3 * i] = x.x; rgb[3 * i + 1] = x.y; rgb[3 * i + 2] = x.z;
+            float u = fmaxf(luminance(x), 0.01f);
+            luma[i] = fclamp(u, 0.001f, 64.0f);
+            aosky[2 * i] = fclamp(ao, 0.0f, 1.0f);
+            aosky[2 * i + 1] = fclamp(skyhit ? 1.0f : 0.0f, 0.0f, 1.0f);
+            if (samples) samples[i] = smp;
+            if (rays) rays[i] = nr;
+        }
+    }
+}
+
+/* ======================================================================= */
+/* SVGF                                                                     */
+/* ======================================================================= */
+/*
+ * Textures are float arrays holding values already rounded to their GL
+ * storage format; channels are interleaved (SH: 4, CoCg: 2, ...).  Sampling
+ * follows GL: LINEAR = bilinear with REPEAT wrap at fractional coordinates,
+ * NEAREST for the face/block planes (Framebuffer.cpp:55-73,
+ * Pipeline.cpp:1094-1108).  A tap at an integer pixel offset reads the
+ * texel directly (LINEAR at a texel centre returns the texel).
+ */
+
+namespace {
+
+struct Tex {
+    const float* p;
+    int W, H, C;
+    float at(int x, int y, int c) const { return p[((size_t)y * W + x) * C + c]; }
+};
+
+static inline int wrap(int i, int n) { int r = i % n; return r < 0 ? r + n : r; }
+
+/* texture(sampler2D LINEAR REPEAT, (s, t)), channel c. */
+static inline float bilinear(const Tex& T, float s, float t, int c)
+{
+    float u = s * (float)T.W - 0.5f;
+    float v = t * (float)T.H - 0.5f;
+    float fu = floorf(u), fv = floorf(v);
+    float a = u - fu, b = v - fv;
+    int i0 = wrap((int)fu, T.W), i1 = wrap((int)fu + 1, T.W);
+    int j0 = wrap((int)fv, T.H), j1 = wrap((int)fv + 1, T.H);
+    float top = T.at(i0, j0, c) * (1.0f - a) + T.at(i1, j0, c) * a;
+    float bot = T.at(i0, j1, c) * (1.0f - a) + T.at(i1, j1, c) * a;
+    return top * (1.0f - b) + bot * b;
+}
+
+/* texture(sampler2D NEAREST REPEAT) of a byte plane. */
+static inline int nearest_u8(const uint8_t* p, int W, int H, float s, float t)
+{
+    int i = wrap((int)floorf(s * (float)W), W);
+    int j = wrap((int)floorf(t * (float)H), H);
+    return p[(size_t)j * W + i];
+}
+
+static inline bool in_screen(float x, float y) { return x > 0.0f && x < 1.0f && y > 0.0f && y < 1.0f; }
+
+static inline float sh_to_y(float w) { return fmaxf(0.0f, 3.544905f * w); }
+
+struct Frame {
+    const float* inv_view;
+    const float* inv_proj;
+    int W, H;
+    V3 origin() const { return v3(inv_view[12], inv_view[13], inv_view[14]); }
+};
+
+/* GetPositionAt(tex, txc) with Dist already fetched. */
+static inline V3 position_from(const Frame& f, float sx, float sy, float dist)
+{
+    V3 o = f.origin();
+    V3 dir = normalize3(ray_direction_at(f.inv_view, f.inv_proj, sx, sy));
+    return v3(o.x + dir.x * dist, o.y + dir.y * dist, o.z + dir.z * dist);
+}
+
+static inline float r16(float x) { return vpt_round_f16(x); }
+static inline float r8(float x) { return vpt_unorm8_decode(vpt_unorm8_encode(x)); }
+
+} /* namespace */
+
+/* Spatial3x3Initial.main (Spatial3x3Initial.glsl:96-176).
+ * in:  sh[4], cocg[2], utility[1], ao[2]  (raw GI planes)
+ * out: sh[4], cocg[2], utility[1], ao[2]  (DiffusePreTemporal_SpatialFBO) */
+OAPI void oracle_svgf_prespatial(const float* inv_view, const float* inv_proj, int W, int H,
+                                 const float* gb_t, const uint8_t* gb_face,
+                                 const float* sh, const float* cocg, const float* util, const float* ao,
+                                 float* o_sh, float* o_cocg, float* o_util, float* o_ao)
+{
+    Frame fr = {inv_view, inv_proj, W, H};
+    const float atrous[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+#pragma omp parallel for schedule(static)
+    for (int py = 0; py < H; py++) {
+        for (int px = 0; px < W; px++) {
+            size_t i = (size_t)py * W + px;
+            float uvx = texcoord(px, W), uvy = texcoord(py, H);
+            V3 base_pos = position_from(fr, uvx, uvy, gb_t[i]);
+            V3 base_n = face_normal(gb_face[i], 1.0f);
+            float base_sh[4] = {sh[4 * i], sh[4 * i + 1], sh[4 * i + 2], sh[4 * i + 3]};
+            float base_luma = sh_to_y(base_sh[3]);
+            float tsh[4] = {base_sh[0], base_sh[1], base_sh[2], base_sh[3]};
+            float tcocg[2] = {cocg[2 * i], cocg[2 * i + 1]};
+            float tw = 1.0f;
+            float tao[2] = {ao[2 * i], ao[2 * i + 1]};
+            float taow = 1.0f;
+            const float phi_color = 4.0f;
+            for (int x = -1; x <= 1; x++) {
+                for (int y = -1; y <= 1; y++) {
+                    if (x == 0 && y == 0) continue;
+                    float sx = uvx + ((float)x * 1.0f) * tex_w;
+                    float sy = uvy + ((float)y * 1.0f) * tex_h;
+                    if (!in_screen(sx, sy)) continue;
+                    size_t j = (size_t)(py + y) * W + (px + x);
+                    V3 sp = position_from(fr, sx, sy, gb_t[j]);
+                    V3 pd = v3(fabsf(sp.x - base_pos.x), fabsf(sp.y - base_pos.y), fabsf(sp.z - base_pos.z));
+                    float dist_sqr = dot3(pd, pd);
+                    if (dist_sqr < 1.0f) {
+                        float ssh[4] = {sh[4 * j], sh[4 * j + 1], sh[4 * j + 2], sh[4 * j + 3]};
+                        V3 sn = face_normal(gb_face[j], 1.0f);
+                        float sluma = sh_to_y(ssh[3]);
+                        float nw = vpt_powi(fmaxf(dot3(base_n, sn), 0.0f), 16);
+                        float lw = fabsf(sluma - base_luma) / phi_color;
+                        float w = vpt_expf(-lw - nw);
+                        w = fmaxf(w, 0.01f);
+                        w = (atrous[abs(x)] * atrous[abs(y)]) * w;
+                        w = fmaxf(w, 0.01f);
+                        w = fclamp(w, 0.0f, 1.0f);
+                        for (int k = 0; k < 4; k++) tsh[k] = tsh[k] + ssh[k] * w;
+                        tcocg[0] = tcocg[0] + cocg[2 * j] * w;
+                        tcocg[1] = tcocg[1] + cocg[2 * j + 1] * w;
+                        tw = tw + w;
+                        tao[0] = tao[0] + ao[2 * j] * w;
+                        tao[1] = tao[1] + ao[2 * j + 1] * w;
+                        taow = taow + w;
+                    }
+                }
+            }
+            tw = fmaxf(tw, 0.01f);
+            for (int k = 0; k < 4; k++) o_sh[4 * i + k] = r16(tsh[k] / tw);
+            o_cocg[2 * i] = r16(tcocg[0] / tw);
+            o_cocg[2 * i + 1] = r16(tcocg[1] / tw);
+            float aw = fmaxf(taow, 0.01f);
+            o_ao[2 * i] = r8(tao[0] / aw);
+            o_ao[2 * i + 1] = r8(tao[1] / aw);
+            o_util[i] = r16(util[i]);
+        }
+    }
+}
+
+/* TemporalFilter.main (SVGF/TemporalFilter.glsl:132-360), u_BeUseful = true.
+ * cur_*: pre-spatial planes of this frame; prev_*: temporal output of the
+ * previous frame (sh 4, cocg 2, util 3 = spp/moment/luma, ao 2).
+ * Previous positions are rebuilt with the *current* camera origin and ray
+ * directions, as GetPositionAt does in the reference (v_RayOrigin and
+ * u_InverseView are the current frame's). */
+OAPI void oracle_svgf_temporal(const float* inv_view, const float* inv_proj, const float* prev_viewproj,
+                               int W, int H,
+                               const float* cur_t, const uint8_t* cur_face, const uint8_t* cur_block,
+                               const float* prev_t, const uint8_t* prev_face, const uint8_t* prev_block,
+                               const float* cur_sh, const float* cur_cocg, const float* cur_luma, const float* cur_ao,
+                               const float* prev_sh, const float* prev_cocg, const float* prev_util, const float* prev_ao,
+                               float* o_sh, float* o_cocg, float* o_util, float* o_ao)
+{
+    Frame fr = {inv_view, inv_proj, W, H};
+    Tex Tpt = {prev_t, W, H, 1};
+    Tex Tsh = {prev_sh, W, H, 4}, Tcc = {prev_cocg, W, H, 2}, Tut = {prev_util, W, H, 3}, Tao = {prev_ao, W, H, 2};
+    const float weights[5] = {3.0f / 32.0f, 3.0f / 32.0f, 9.0f / 64.0f, 3.0f / 32.0f, 3.0f / 32.0f};
+    const float offs[5][2] = {{1, 0}, {0, 1}, {0, 0}, {-1, 0}, {0, -1}};
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    const float b = 0.0035f;
+#pragma omp parallel for schedule(static)
+    for (int py = 0; py < H; py++) {
+        for (int px = 0; px < W; px++) {
+            size_t i = (size_t)py * W + px;
+            float uvx = texcoord(px, W), uvy = texcoord(py, H);
+            float base_w = cur_t[i];
+            V3 base_pos = position_from(fr, uvx, uvy, base_w);
+            V3 base_n = face_normal(cur_face[i], 1.0f);
+            float base_sh[4] = {cur_sh[4 * i], cur_sh[4 * i + 1], cur_sh[4 * i + 2], cur_sh[4 * i + 3]};
+            float base_cocg[2] = {cur_cocg[2 * i], cur_cocg[2 * i + 1]};
+            float base_ao[2] = {cur_ao[2 * i], cur_ao[2 * i + 1]};
+
+            /* Reprojection (:61-73) */
+            float p4[4] = {base_pos.x, base_pos.y, base_pos.z, 1.0f};
+            float q[4];
+            mat_vec4(prev_viewproj, p4, q);
+            float rx = (q[0] / q[3]) * 0.5f + 0.5f;
+            float ry = (q[1] / q[3]) * 0.5f + 0.5f;
+            float base_lum = cur_luma[i];
+
+            int base_block = std::min(std::max((int)cur_block[i], 0), 127);
+            /* Jitter = ivec2((GradientNoise() - 0.5) * 1.0) is always 0. */
+
+            bool do_block = true, do_normal = true;
+            float tol = 0.75f;
+            V3 cam = fr.origin();
+            V3 dv = v3(base_pos.x - cam.x, base_pos.y - cam.y, base_pos.z - cam.z);
+            float dtp = sqrtf(dot3(dv, dv));
+            if (dtp < 4.0f) { tol = 0.3f; do_block = true; do_normal = true; }
+            else if (dtp < 6.0f) { tol = 0.65f; do_block = true; do_normal = true; }
+            else if (dtp < 8.0f) { tol = 0.85f; do_block = true; do_normal = true; }
+            else if (dtp < 16.0f) { tol = 1.414f; do_block = true; do_normal = true; }
+            else if (dtp < 32.0f) { tol = 2.4f; do_block = true; do_normal = true; }
+            else if (dtp < 48.0f) { tol = 3.5f; do_block = false; do_normal = true; }
+            else if (dtp < 64.0f) { tol = 4.2f; do_block = false; do_normal = true; }
+            else if (dtp < 96.0f) { tol = 6.25f; do_block = false; do_normal = false; }
+            else if (dtp < 128.0f) { tol = 9.0f; do_block = false; do_normal = false; }
+            else if (dtp < 200.0f) { tol = 14.0f; do_block = false; do_normal = false; }
+
+            float tw = 0.0f;
+            float ssh[4] = {0, 0, 0, 0}, scc[2] = {0, 0}, sao[2] = {0, 0};
+            float slum = 0.0f, sspp = 0.0f, smom = 0.0f;
+            int successful = 0;
+            for (int k = 0; k < 5; k++) {
+                float sx = rx + (offs[k][0] + 0.0f) * tex_w;
+                float sy = ry + (offs[k][1] + 0.0f) * tex_h;
+                if (!(sx < 1.0f - b && sx > b && sy < 1.0f - b && sy > b)) continue;
+                float pw = bilinear(Tpt, sx, sy, 0);
+                V3 pp = position_from(fr, sx, sy, pw);
+                V3 pn = face_normal(nearest_u8(prev_face, W, H, sx, sy), 1.0f);
+                V3 pd = v3(fabsf(base_pos.x - pp.x), fabsf(base_pos.y - pp.y), fabsf(base_pos.z - pp.z));
+                float perr = dot3(pd, pd);
+                int sblock = std::min(std::max(nearest_u8(prev_block, W, H, sx, sy), 0), 127);
+                bool valid = false;
+                if (perr < tol && ((pw < 0.0f) == (base_w < 0.0f))) {
+                    valid = true;
+                    if (do_normal && !(pn.x == base_n.x && pn.y == base_n.y && pn.z == base_n.z)) valid = false;
+                    if (do_block && base_block != sblock) valid = false;
+                }
+                if (valid) {
+                    float cw = weights[k];
+                    for (int c = 0; c < 4; c++) ssh[c] = ssh[c] + bilinear(Tsh, sx, sy, c) * cw;
+                    scc[0] = scc[0] + bilinear(Tcc, sx, sy, 0) * cw;
+                    scc[1] = scc[1] + bilinear(Tcc, sx, sy, 1) * cw;
+                    sspp = sspp + bilinear(Tut, sx, sy, 0) * cw;
+                    smom = smom + bilinear(Tut, sx, sy, 1) * cw;
+                    slum = slum + bilinear(Tut, sx, sy, 2) * cw;
+                    sao[0] = sao[0] + bilinear(Tao, sx, sy, 0) * cw;
+                    sao[1] = sao[1] + bilinear(Tao, sx, sy, 1) * cw;
+                    tw = tw + cw;
+                    successful++;
+                }
+            }
+            if (tw > 0.001f) {
+                for (int c = 0; c < 4; c++) ssh[c] = ssh[c] / tw;
+                scc[0] = scc[0] / tw; scc[1] = scc[1] / tw;
+                smom = smom / tw;
+                sspp = sspp / tw;
+                slum = slum / tw;
+                sao[0] = sao[0] / tw; sao[1] = sao[1] / tw;
+            } else {
+                successful = 0;
+            }
+            float increment = 1.0f;
+            float spp_inc = sspp + increment;
+            if (successful <= 0) spp_inc = 0.01f;
+            float blend = fmaxf(1.0f / spp_inc, 0.05f);
+            float moment_factor = fmaxf(1.0f / spp_inc, 0.05f);
+            float util_spp = spp_inc;
+            if (successful <= 0) util_spp = 0.0f;
+            float util_moment = (1.0f - moment_factor) * smom + moment_factor * (base_lum * base_lum);
+            float store_luma = slum * (1.0f - blend) + base_lum * blend;   /* mix(x, y, a) */
+            float osh[4], occ[2], oao[2];
+            for (int c = 0; c < 4; c++) osh[c] = ssh[c] * (1.0f - blend) + base_sh[c] * blend;
+            for (int c = 0; c < 2; c++) occ[c] = scc[c] * (1.0f - blend) + base_cocg[c] * blend;
+            for (int c = 0; c < 2; c++) oao[c] = sao[c] * (1.0f - blend) + base_ao[c] * blend;
+            if (successful <= 0) {
+                for (int c = 0; c < 4; c++) osh[c] = base_sh[c];
+                for (int c = 0; c < 2; c++) { occ[c] = base_cocg[c]; oao[c] = base_ao[c]; }
+            }
+            for (int c = 0; c < 4; c++) o_sh[4 * i + c] = r16(fclamp(osh[c], -100.0f, 100.0f));
+            for (int c = 0; c < 2; c++) o_cocg[2 * i + c] = r16(fclamp(occ[c], -10.0f, 100.0f));
+            for (int c = 0; c < 2; c++) o_ao[2 * i + c] = r8(fclamp(oao[c], 0.0f, 1.0f));
+            o_util[3 * i] = r16(fclamp(util_spp, -150.0f, 150.0f));
+            o_util[3 * i + 1] = r16(fclamp(util_moment, -150.0f, 150.0f));
+            o_util[3 * i + 2] = r16(fclamp(store_luma, -150.0f, 150.0f));
+        }
+    }
+}
+
+/* VarianceEstimate.main (SVGF/VarianceEstimate.glsl:73-192), DO_SPATIAL and
+ * AGGRESSIVE_DISOCCLUSION_HANDLING true (Pipeline.cpp:83,87). */
+OAPI void oracle_svgf_variance(const float* inv_view, const float* inv_proj, int W, int H,
+                               const float* gb_t, const uint8_t* gb_face,
+                               const float* sh, const float* cocg, const float* util,
+                               float* o_sh, float* o_cocg, float* o_var)
+{
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    const float frame_bias = 4.0f;
+    const float thresh = 4.0f + 4.0f + frame_bias;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int py = 0; py < H; py++) {
+        for (int px = 0; px < W; px++) {
+            size_t i = (size_t)py * W + px;
+            float uvx = texcoord(px, W), uvy = texcoord(py, H);
+            float base_w = gb_t[i];
+            V3 base_n = face_normal(gb_face[i], 1.0f);
+            float frames = util[3 * i];
+            float base_moment = util[3 * i + 1];
+            float base_sh[4] = {sh[4 * i], sh[4 * i + 1], sh[4 * i + 2], sh[4 * i + 3]};
+            float base_cocg[2] = {cocg[2 * i], cocg[2 * i + 1]};
+            float base_lum = sh_to_y(base_sh[3]);
+            float osh[4], occ[2];
+            float variance;
+            if (frames < thresh) {
+                const float color_phi = 5.0f;
+                float tw = 0.0f, tmom = 0.0f, tlum = 0.0f, tw2 = 0.0f;
+                float tsh[4] = {0, 0, 0, 0}, tcc[2] = {0, 0};
+                for (int x = -4; x <= 4; x++) {
+                    for (int y = -4; y <= 4; y++) {
+                        float sx = uvx + (float)x * tex_w;
+                        float sy = uvy + (float)y * tex_h;
+                        if (!in_screen(sx, sy)) continue;
+                        size_t j = (size_t)(py + y) * W + (px + x);
+                        float sw = gb_t[j];
+                        V3 sn = face_normal(gb_face[j], 1.0f);
+                        float smom = util[3 * j + 1];
+                        float s_sh[4] = {sh[4 * j], sh[4 * j + 1], sh[4 * j + 2], sh[4 * j + 3]};
+                        float slum = sh_to_y(s_sh[3]);
+                        float nw = vpt_powi(fmaxf(dot3(base_n, sn), 0.0f), 16);
+                        float e = vpt_expf(-fabsf(sw - base_w));
+                        float dw = e * e;
+                        float lw = fabsf(slum - base_lum) / color_phi;
+                        float w = (vpt_expf(-lw) * nw) * dw;
+                        float w2 = w;
+                        w = fmaxf(w, 0.000000015f);
+                        w2 = fmaxf(w2, 0.0000000015f);
+                        tw = tw + w;
+                        tmom = tmom + smom * w2;
+                        for (int k = 0; k < 4; k++) tsh[k] = tsh[k] + s_sh[k] * w;
+                        tcc[0] = tcc[0] + cocg[2 * j] * w;
+                        tcc[1] = tcc[1] + cocg[2 * j + 1] * w;
+                        tlum = tlum + slum * w2;
+                        tw2 = tw2 + w2;
+                    }
+                }
+                if (tw > 0.0f) {
+                    tmom = tmom / tw2;
+                    tlum = tlum / tw2;
+                    tcc[0] = tcc[0] / tw; tcc[1] = tcc[1] / tw;
+                    for (int k = 0; k < 4; k++) tsh[k] = tsh[k] / tw;
+                }
+                for (int k = 0; k < 4; k++) osh[k] = tsh[k];
+                occ[0] = tcc[0]; occ[1] = tcc[1];
+                float acc = tlum * tlum;
+                variance = tmom - acc;
+                variance = variance * 3.0f;
+            } else {
+                for (int k = 0; k < 4; k++) osh[k] = base_sh[k];
+                occ[0] = base_cocg[0]; occ[1] = base_cocg[1];
+                variance = base_moment - base_lum * base_lum;
+            }
+            variance = variance * (thresh / frames);
+            for (int k = 0; k < 4; k++) o_sh[4 * i + k] = r16(fclamp(osh[k], -100.0f, 100.0f));
+            o_cocg[2 * i] = r16(fclamp(occ[0], -10.0f, 100.0f));
+            o_cocg[2 * i + 1] = r16(fclamp(occ[1], -10.0f, 100.0f));
+            o_var[i] = r16(fclamp(variance, -1.0f, 50.0f));
+        }
+    }
+}
+
+/* GradientNoise (SpatialFilter.glsl:160-165) with u_Time = time. */
+static inline float gradient_noise(int px, int py, float time)
+{
+    float t = time * 100.493850275f;
+    float m = t - 500.0f * floorf(t / 500.0f);          /* mod(x, 500) */
+    float cx = ((float)px + 0.5f) + m;
+    float cy = ((float)py + 0.5f) + m;
+    float inner = 0.06711056f * cx + 0.00583715f * cy;
+    inner = inner - floorf(inner);
+    float n = 52.9829189f * inner;
+    return n - floorf(n);
+}
+
+/* SpatialFilter.main (SVGF/SpatialFilter.glsl:175-355) for one a-trous pass.
+ * sh/cocg/var/ao: previous pass output; frames: temporal utility .x
+ * (u_TemporalMoment).  color_phi_bias = 3.325 (Pipeline.cpp:85),
+ * scale = mix(1, 2.4, u_ResolutionScale). */
+OAPI void oracle_svgf_atrous(int W, int H, const float* gb_t, const uint8_t* gb_face,
+                             const float* sh, const float* cocg, const float* var, const float* ao,
+                             const float* temporal_util, int step, float scale, float color_phi_bias, float time,
+                             float* o_sh, float* o_cocg, float* o_var, float* o_ao)
+{
+    Tex Tt = {gb_t, W, H, 1}, Tsh = {sh, W, H, 4}, Tcc = {cocg, W, H, 2}, Tv = {var, W, H, 1}, Tao = {ao, W, H, 2};
+    const float atrous[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+    const float gauss[2] = {0.60283f, 0.198585f};
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    bool filter_ao = step <= 4;
+    bool filter_sky = step <= 6;
+    filter_sky = filter_sky || filter_ao;
+#pragma omp parallel for schedule(static)
+    for (int py = 0; py < H; py++) {
+        for (int px = 0; px < W; px++) {
+            size_t i = (size_t)py * W + px;
+            float uvx = texcoord(px, W), uvy = texcoord(py, H);
+            int jit = (int)((gradient_noise(px, py, time) - 0.5f) * ((float)step * 0.8f));
+            float base_depth = gb_t[i];
+            V3 base_n = face_normal(gb_face[i], 1.0f);
+            float base_sh[4] = {sh[4 * i], sh[4 * i + 1], sh[4 * i + 2], sh[4 * i + 3]};
+            float base_cocg[2] = {cocg[2 * i], cocg[2 * i + 1]};
+            float base_lum = sh_to_y(base_sh[3]);
+            /* GaussianVariance (:94-129) */
+            float base_var = 0.0f, vsum = 0.0f, ksum = 0.0f;
+            for (int x = -1; x <= 1; x++) {
+                for (int y = -1; y <= 1; y++) {
+                    float sx = uvx + (float)x * tex_w;
+                    float sy = uvy + (float)y * tex_h;
+                    if (!in_screen(sx, sy)) continue;
+                    float kv = gauss[abs(x)] * gauss[abs(y)];
+                    float v = var[(size_t)(py + y) * W + (px + x)];
+                    if (x == 0 && y == 0) base_var = v;
+                    vsum = vsum + v * kv;
+                    ksum = ksum + kv;
+                }
+            }
+            float var_est = vsum / fmaxf(ksum, 0.01f);
+            float base_ao[2] = {ao[2 * i], ao[2 * i + 1]};
+
+            float tsh[4] = {base_sh[0], base_sh[1], base_sh[2], base_sh[3]};
+            float tcc[2] = {base_cocg[0], base_cocg[1]};
+            float tw = 1.0f;
+            float tvar = base_var;
+            float tao[2] = {base_ao[0], base_ao[1]};
+            float taow = 1.0f;
+
+            float frames = temporal_util[3 * i];
+            bool strong = frames <= 8.0f && step <= 8;
+
+            float curve = 0.0f;
+            if (var_est < 0.01f) curve = 128.0f;
+            else if (var_est < 0.025f) curve = 112.0f;
+            else if (var_est < 0.05f) curve = 96.0f;
+            else if (var_est < 0.075f) curve = 84.0f;
+            else if (var_est < 0.1f) curve = 70.0f;
+            float tweaked;
+            if (var_est < 0.1f) {
+                /* TweakVariance (:167-173): V * pow(1 - V, E + 6), E integral */
+                float fv = fclamp(var_est, 0.0f, 1.0f);
+                float tt = 1.0f - fv;
+                tweaked = fv * vpt_powi(tt, (int)(curve + 6.0f));
+            } else {
+                tweaked = var_est;
+            }
+            float phi = sqrtf(fmaxf(0.0f, 0.000001f + tweaked));
+            phi = phi / fmaxf(color_phi_bias, 0.1f);
+
+            if (!(base_depth < 0.0f)) {   /* (BaseDepth < 0) == (DepthDiff < 0), DepthDiff = abs(..) >= 0 */
+                for (int x = -1; x <= 1; x++) {
+                    for (int y = -1; y <= 1; y++) {
+                        if (x == 0 && y == 0) continue;
+                        float ox = ((float)x * (float)step) * scale + (float)jit * 0.5f;
+                        float oy = ((float)y * (float)step) * scale + (float)jit * 0.5f;
+                        float sx = uvx + ox * tex_w;
+                        float sy = uvy + oy * tex_h;
+                        if (!in_screen(sx, sy)) continue;
+                        float sdepth = bilinear(Tt, sx, sy, 0);
+                        float ddiff = fabsf(sdepth - base_depth);
+                        V3 sn = face_normal(nearest_u8(gb_face, W, H, sx, sy), 1.0f);
+                        float s_sh[4];
+                        for (int c = 0; c < 4; c++) s_sh[c] = bilinear(Tsh, sx, sy, c);
+                        float s_cc0 = bilinear(Tcc, sx, sy, 0), s_cc1 = bilinear(Tcc, sx, sy, 1);
+                        float slum = sh_to_y(s_sh[3]);
+                        float svar = bilinear(Tv, sx, sy, 0);
+                        float nw = vpt_powi(fmaxf(dot3(base_n, sn), 0.0f), 32);
+                        nw = fclamp(nw, 0.001f, 1.0f);
+                        float lw = fabsf(slum - base_lum) / phi;
+                        float e = vpt_expf(-fmaxf(ddiff, 0.00001f));
+                        float dw = fclamp(e * e, 0.0001f, 1.0f);
+                        float w = strong ? (nw * dw) : ((vpt_expf(-lw) * nw) * dw);
+                        w = fclamp(w, 0.001f, 1.0f);
+                        float xw = atrous[abs(x)], yw = atrous[abs(y)];
+                        w = (xw * yw) * w;
+                        w = fmaxf(w, 0.00000001f);
+                        for (int c = 0; c < 4; c++) tsh[c] = tsh[c] + s_sh[c] * w;
+                        tcc[0] = tcc[0] + s_cc0 * w;
+                        tcc[1] = tcc[1] + s_cc1 * w;
+                        tvar = tvar + (w * w) * svar;
+                        tw = tw + w;
+                        if (filter_sky || filter_ao) {
+                            float aow = fclamp(((xw * yw) * nw) * dw, 0.000001f, 1.0f);
+                            tao[0] = tao[0] + bilinear(Tao, sx, sy, 0) * aow;
+                            tao[1] = tao[1] + bilinear(Tao, sx, sy, 1) * aow;
+                            taow = taow + aow;
+                        }
+                    }
+                }
+            }
+            float osh[4], occ[2], ovar, oao[2];
+            for (int c = 0; c < 4; c++) osh[c] = tsh[c] / tw;
+            occ[0] = tcc[0] / tw; occ[1] = tcc[1] / tw;
+            ovar = tvar / (tw * tw);
+            oao[0] = tao[0] / taow; oao[1] = tao[1] / taow;
+            if (!filter_ao) oao[0] = base_ao[0];
+            for (int c = 0; c < 4; c++) o_sh[4 * i + c] = r16(fclamp(osh[c], -100.0f, 100.0f));
+            o_cocg[2 * i] = r16(fclamp(occ[0], -10.0f, 100.0f));
+            o_cocg[2 * i + 1] = r16(fclamp(occ[1], -10.0f, 100.0f));
+            o_var[i] = r16(fclamp(ovar, -1.0f, 50.0f));
+            o_ao[2 * i] = r8(fclamp(oao[0], 0.0f, 1.0f));
+            o_ao[2 * i + 1] = r8(fclamp(oao[1], 0.0f, 1.0f));
+        }
+    }
+}
+
+/* SHToIrridiance(shY, CoCg, v) (ColorPassFrag.glsl:356-368) with the
+ * G-buffer face normal as v. */
+OAPI void oracle_sh_decode(int n, const float* sh, const float* cocg, const uint8_t* face, float* rgb)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        V3 v = face_normal(face[i], 1.0f);
+        float x = (sh[4 * i] * v.x + sh[4 * i + 1] * v.y) + sh[4 * i + 2] * v.z;
+        float Y = 2.0f * (1.023326f * x + 0.886226f * sh[4 * i + 3]);
+        Y = fmaxf(Y, 0.0f);
+        float k = (Y * 0.282095f) / (sh[4 * i + 3] + 1e-6f);
+        float co = cocg[2 * i] * k, cg = cocg[2 * i + 1] * k;
+        float T = Y - cg * 0.5f;
+        float G = cg + T;
+        float B = T - co * 0.5f;
+        float R = B + co;
+        rgb[3 * i] = fmaxf(R, 0.0f);
+        rgb[3 * i + 1] = fmaxf(G, 0.0f);
+        rgb[3 * i + 2] = fmaxf(B, 0.0f);
+    }
+}
+
+/* ======================================================================= */
+/* Cross-checks                                                             */
+/* ======================================================================= */
+
+/* World::RaycastDetect (World.cpp:496-546): epsilon stepping, 48 steps,
+ * cells with any coordinate <= 0 treated as outside.  Falling off the end
+ * (undefined in the reference) returns -1.  out = (x, y, z, id). */
+OAPI void oracle_raycast_detect(const uint8_t* grid, int nx, int ny, int nz, const float* pos, const float* dir,
+                                int32_t* out)
+{
+    Dims d = {nx, ny, nz};
+    float p[3] = {pos[0], pos[1], pos[2]};
+    float sgn[3];
+    for (int i = 0; i < 3; i++) sgn[i] = dir[i] > 0 ? 1.0f : 0.0f;
+    out[0] = out[1] = out[2] = out[3] = -1;
+    for (int it = 0; it < 48; it++) {
+        float tv[3];
+        for (int i = 0; i < 3; i++) tv[i] = (floorf(p[i] + sgn[i]) - p[i]) / dir[i];
+        float t = std::min(tv[0], std::min(tv[1], tv[2]));
+        for (int i = 0; i < 3; i++) p[i] = p[i] + dir[i] * (t + 0.001f);
+        int fx = (int)floorf(p[0]), fy = (int)floorf(p[1]), fz = (int)floorf(p[2]);
+        if (!(fx >= nx || fy >= ny || fz >= nz || fx <= 0 || fy <= 0 || fz <= 0)) {
+            int b = grid[lin((int)p[0], (int)p[1], (int)p[2], d)];
+            if (b != 0) {
+                out[0] = fx; out[1] = fy; out[2] = fz; out[3] = b;
+                return;
+            }
+        }
+    }
+}
+
+/* Textbook Amanatides & Woo (1987) traversal in double precision: first
+ * solid cell along the ray within max_t, and the axis/sign of the face
+ * crossed.  out = (x, y, z, face) or -1. */
+OAPI void oracle_dda_textbook(const uint8_t* grid, int nx, int ny, int nz, const double* o, const double* dir,
+                              double max_t, int32_t* out)
+{
+    Dims d = {nx, ny, nz};
+    out[0] = out[1] = out[2] = out[3] = -1;
+    int c[3], step[3];
+    double tmax[3], tdelta[3];
+    int n[3] = {nx, ny, nz};
+    for (int i = 0; i < 3; i++) {
+        c[i] = (int)floor(o[i]);
+        if (dir[i] > 0) { step[i] = 1; tmax[i] = (floor(o[i]) + 1.0 - o[i]) / dir[i]; tdelta[i] = 1.0 / dir[i]; }
+        else if (dir[i] < 0) { step[i] = -1; tmax[i] = (o[i] - floor(o[i])) / -dir[i]; tdelta[i] = -1.0 / dir[i]; }
+        else { step[i] = 0; tmax[i] = INFINITY; tdelta[i] = INFINITY; }
+    }
+    int axis = -1;
+    double t = 0.0;
+    while (t <= max_t) {
+        if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= n[0] || c[1] >= n[1] || c[2] >= n[2]) return;
+        if (grid[lin(c[0], c[1], c[2], d)] > 0) {
+            out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+            out[3] = axis < 0 ? -1 : face_from_axis(axis, step[axis]);
+            return;
+        }
+        axis = (tmax[0] < tmax[1]) ? (tmax[0] < tmax[2] ? 0 : 2) : (tmax[1] < tmax[2] ? 1 : 2);
+        t = tmax[axis];
+        c[axis] += step[axis];
+        tmax[axis] += tdelta[axis];
+    }
+}
+
+/* ======================================================================= */
+/* Camera (FpsCamera.cpp:23-24,150-163; Player.cpp:10)                      */
+/* ======================================================================= */
+
+/* glm::perspective / glm::lookAt (right-handed, clip z in [-1, 1]) and
+ * their inverses, evaluated in double and rounded to float. */
+OAPI void oracle_camera(const float* pos, float yaw_deg, float pitch_deg, float fov_deg, float aspect, float znear,
+                        float zfar, float* view, float* proj, float* inv_view, float* inv_proj)
+{
+    const double PI = 3.14159265358979323846;
+    double yaw = yaw_deg * PI / 180.0, pitch = pitch_deg * PI / 180.0;
+    double f[3] = {cos(pitch) * cos(yaw), sin(pitch), cos(pitch) * sin(yaw)};
+    double fl = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (int i = 0; i < 3; i++) f[i] /= fl;
+    double up[3] = {0, 1, 0};
+    double s[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
+    double sl = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    for (int i = 0; i < 3; i++) s[i] /= sl;
+    double u[3] = {s[1] * f[2] - s[2] * f[1], s[2] * f[0] - s[0] * f[2], s[0] * f[1] - s[1] * f[0]};
+    double e[3] = {pos[0], pos[1], pos[2]};
+    double V[16] = {0};
+    V[0] = s[0]; V[4] = s[1]; V[8] = s[2];
+    V[1] = u[0]; V[5] = u[1]; V[9] = u[2];
+    V[2] = -f[0]; V[6] = -f[1]; V[10] = -f[2];
+    V[12] = -(s[0] * e[0] + s[1] * e[1] + s[2] * e[2]);
+    V[13] = -(u[0] * e[0] + u[1] * e[1] + u[2] * e[2]);
+    V[14] = (f[0] * e[0] + f[1] * e[1] + f[2] * e[2]);
+    V[15] = 1.0;
+    double th = tan(fov_deg * PI / 180.0 / 2.0);
+    double P[16] = {0};
+    P[0] = 1.0 / (aspect * th);
+    P[5] = 1.0 / th;
+    P[10] = -((double)zfar + znear) / ((double)zfar - znear);
+    P[11] = -1.0;
+    P[14] = -(2.0 * zfar * znear) / ((double)zfar - znear);
+    /* inverses: view is rigid, projection has a closed form */
+    double IV[16] = {0};
+    IV[0] = s[0]; IV[1] = s[1]; IV[2] = s[2];
+    IV[4] = u[0]; IV[5] = u[1]; IV[6] = u[2];
+    IV[8] = -f[0]; IV[9] = -f[1]; IV[10] = -f[2];
+    IV[12] = e[0]; IV[13] = e[1]; IV[14] = e[2]; IV[15] = 1.0;
+    double IP[16] = {0};
+    IP[0] = 1.0 / P[0];
+    IP[5] = 1.0 / P[5];
+    IP[11] = 1.0 / P[14];
+    IP[14] = -1.0;
+    IP[15] = P[10] / P[14];
+    for (int i = 0; i < 16; i++) {
+        view[i] = (float)V[i];
+        proj[i] = (float)P[i];
+        inv_view[i] = (float)IV[i];
+        inv_proj[i] = (float)IP[i];
+    }
+}
+
+/* ======================================================================= */
+/* CPU baseline timing                                                      */
+/* ======================================================================= */
+
+/* Times DF build and primary tracing on the host cores: one warm-up, then
+ * the median of `reps` runs.  out = {df_ms, primary_ms, total_samples,
+ * threads}. */
+OAPI void oracle_time_plain(const uint8_t* grid, int nx, int ny, int nz, const float* inv_view, const float* inv_proj,
+                            int W, int H, int max_iter, int reps, double* out)
+{
+    std::vector<uint8_t> df((size_t)nx * ny * nz);
+    std::vector<float> t((size_t)W * H);
+    std::vector<uint8_t> face((size_t)W * H), block((size_t)W * H);
+    std::vector<uint32_t> vox((size_t)W * H);
+    std::vector<int32_t> smp((size_t)W * H);
+    std::vector<double> dms, pms;
+    for (int r = 0; r <= reps; r++) {
+        auto t0 = std::chrono::steady_clock::now();
+        oracle_df_build(grid, nx, ny, nz, df.data());
+        auto t1 = std::chrono::steady_clock::now();
+        oracle_trace_primary(grid, df.data(), nx, ny, nz, inv_view, inv_proj, W, H, max_iter, t.data(), face.data(),
+                             block.data(), vox.data(), smp.data());
+        auto t2 = std::chrono::steady_clock::now();
+        if (r > 0) {
+            dms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+            pms.push_back(std::chrono::duration<double, std::milli>(t2 - t1).count());
+        }
+    }
+    std::sort(dms.begin(), dms.end());
+    std::sort(pms.begin(), pms.end());
+    double total = 0;
+    for (auto v : smp) total += v;
+    out[0] = dms.empty() ? 0 : dms[dms.size() / 2];
+    out[1] = pms.empty() ? 0 : pms[pms.size() / 2];
+    out[2] = total;
+    out[3] = (double)omp_get_max_threads();
+}
+
+/* OpenMP threads the oracle runs with (cores used by the CPU baseline). */
+OAPI int oracle_threads(void) { return omp_get_max_threads(); }

@@ -1,0 +1,10 @@
+"""MI355X-native (gfx950) hot path of a distance-field voxel path tracer:
+Manhattan distance-field build, DF-accelerated voxel traversal for primary,
+sun-shadow and diffuse-GI rays, and the SVGF denoiser, behind a C ABI
+(include/vpt/vpt.h) implemented by hand-written HIP kernels in libvpt.so.
+"""
+from ._lib import LIB_PATH, VptError, lib  # noqa: F401
+from .world import Context, DeviceGBuffer, World, generate_terrain, gi_params, make_camera  # noqa: F401
+from .pipeline import SVGF, Pipeline, decode_texels, svgf_params_default, viewproj  # noqa: F401
+
+__version__ = "0.1.0"

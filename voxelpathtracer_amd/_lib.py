@@ -1,0 +1,127 @@
+"""ctypes binding of libvpt.so (include/vpt/vpt.h).
+
+The library is loaded from ``voxelpathtracer_amd/_build/libvpt.so``; there
+is no fallback implementation: if the library is missing, every entry point
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("VPT_LIB", _PKG / "_build" / "libvpt.so"))
+
+VPT_OK = 0
+VPT_EINVAL = -1
+VPT_ENOMEM = -2
+VPT_EDEVICE = -3
+VPT_ESTATE = -4
+
+
+class VptError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"vpt error {code}: {msg}")
+        self.code = code
+
+
+class Camera(C.Structure):
+    _fields_ = [("inv_view", C.c_float * 16), ("inv_proj", C.c_float * 16), ("width", C.c_int32),
+                ("height", C.c_int32)]
+
+
+class GBuffer(C.Structure):
+    _fields_ = [("d_t", C.c_void_p), ("d_face", C.c_void_p), ("d_block", C.c_void_p), ("d_voxel", C.c_void_p),
+                ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class GiParams(C.Structure):
+    _fields_ = [("sun_dir", C.c_float * 3), ("light_color", C.c_float * 3), ("sky_radiance", C.c_float * 3),
+                ("sky_pixel_radiance", C.c_float * 3), ("albedo", C.c_float * 384), ("frame", C.c_uint32),
+                ("gi_iter", C.c_int32), ("shadow_iter", C.c_int32)]
+
+
+class SvgfParams(C.Structure):
+    _fields_ = [("color_phi_bias", C.c_float), ("resolution_scale", C.c_float), ("pre_spatial", C.c_int32),
+                ("steps", C.c_int32 * 5)]
+
+
+class Ray(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("dir", C.c_float * 3)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("voxel", C.c_uint32), ("face", C.c_uint8), ("block", C.c_uint8),
+                ("samples", C.c_uint16)]
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_f = C.c_float
+_u32 = C.c_uint32
+_PROTOS = {
+    "vpt_last_error": (C.c_char_p, []),
+    "vpt_version": (C.c_char_p, []),
+    "vpt_ctx_create": (_i, [_i, C.POINTER(_vp)]),
+    "vpt_ctx_destroy": (_i, [_vp]),
+    "vpt_ctx_device": (_i, [_vp, C.POINTER(_i)]),
+    "vpt_world_create": (_i, [_vp, _i, _i, _i, C.POINTER(_vp)]),
+    "vpt_world_destroy": (_i, [_vp]),
+    "vpt_world_dims": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "vpt_world_bytes": (_i, [_vp, C.POINTER(C.c_size_t)]),
+    "vpt_world_upload": (_i, [_vp, _vp, _vp]),
+    "vpt_world_upload_device": (_i, [_vp, _vp, _vp]),
+    "vpt_world_set_blocks": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "vpt_df_build": (_i, [_vp, _vp]),
+    "vpt_df_download": (_i, [_vp, _vp, _vp]),
+    "vpt_grid_download": (_i, [_vp, _vp, _vp]),
+    "vpt_trace_primary": (_i, [_vp, C.POINTER(Camera), _i, C.POINTER(GBuffer), _vp]),
+    "vpt_trace_shadow": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _vp, _vp]),
+    "vpt_trace_gi": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(GiParams), _vp, _vp, _vp]),
+    "vpt_trace_rays": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
+    "vpt_world_set_sample_counter": (_i, [_vp, _vp]),
+    "vpt_svgf_create": (_i, [_vp, _i, _i, C.POINTER(SvgfParams), C.POINTER(_vp)]),
+    "vpt_svgf_destroy": (_i, [_vp]),
+    "vpt_svgf_reset": (_i, [_vp, _vp]),
+    "vpt_svgf_frame": (_i, [_vp, C.POINTER(Camera), C.POINTER(_f), C.POINTER(GBuffer), C.POINTER(GBuffer), _vp, _f,
+                            _vp, _vp, _vp]),
+    "vpt_svgf_read_plane": (_i, [_vp, _i, _vp, _vp]),
+    "vpt_svgf_read_utility": (_i, [_vp, _vp, _vp]),
+    "vpt_generate_terrain": (_i, [_i, _i, _i, _u32, _u32, _i, _i, _i, _vp]),
+    "vpt_camera_make": (_i, [C.POINTER(_f), _f, _f, _f, _i, _i, _f, _f, C.POINTER(Camera), C.POINTER(_f),
+                             C.POINTER(_f)]),
+    "vpt_gi_params_default": (_i, [_f, _u32, _u32, C.POINTER(GiParams)]),
+    "vpt_svgf_params_default": (_i, [C.POINTER(SvgfParams)]),
+    "vpt_raycast_detect": (_i, [_vp, _i, _i, _i, C.POINTER(_f), C.POINTER(_f), C.POINTER(C.c_int32)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libvpt.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} not found: build it with `python -m voxelpathtracer_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_PROTOS)
+
+
+def check(rc: int) -> None:
+    if rc != VPT_OK:
+        raise VptError(rc, lib().vpt_last_error().decode(errors="replace"))
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))

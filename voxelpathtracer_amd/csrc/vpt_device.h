@@ -1,0 +1,245 @@
+// Device-side building blocks shared by the gfx950 kernels: brick
+// addressing, the DF traversal, ray generation.  Every floating-point
+// expression here is the same IEEE operation sequence as the reference
+// shader it cites (compiled with -ffp-contract=off), which is what makes hit
+// cells, faces and distances reproducible bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vpt/vpt_numerics.h"
+
+namespace vpt {
+
+constexpr int kBrick = 8;
+constexpr int kBrickBytes = 512;
+
+// Spread the three bits of v (0..7) to positions 0, 3, 6.
+__host__ __device__ inline uint32_t spread3(uint32_t v) { return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4); }
+
+// Read-only view of a world's bricked planes, passed to kernels by value.
+struct WorldView {
+    const uint8_t* grid;
+    const uint8_t* df;
+    int nx, ny, nz;
+    uint32_t row_stride;    // nbx * 512
+    uint32_t slab_stride;   // nbx * nby * 512
+    float xmax, ymax, zmax; // dims - 1 as float (IsInVolume bounds)
+};
+
+// Byte offset of cell (x, y, z) in a bricked plane: the three axes
+// contribute independent terms.
+__host__ __device__ inline uint32_t brick_offset(uint32_t x, uint32_t y, uint32_t z, uint32_t row_stride,
+                                                 uint32_t slab_stride)
+{
+    return ((x >> 3) * kBrickBytes + spread3(x & 7u)) + ((y >> 3) * row_stride + (spread3(y & 7u) << 1)) +
+           ((z >> 3) * slab_stride + (spread3(z & 7u) << 2));
+}
+
+struct Vec3 {
+    float x, y, z;
+};
+
+__device__ inline Vec3 mk3(float x, float y, float z)
+{
+    Vec3 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
+}
+
+__device__ inline float dot3(Vec3 a, Vec3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+__device__ inline Vec3 normalize3(Vec3 v)
+{
+    float len = sqrtf((v.x * v.x + v.y * v.y) + v.z * v.z);
+    return mk3(v.x / len, v.y / len, v.z / len);
+}
+
+__device__ inline Vec3 cross3(Vec3 a, Vec3 b)
+{
+    return mk3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+
+// mat4 * vec4 rows 0..n-1, GLSL column-major, summed in column order.
+__device__ inline float mat_row(const float* m, int r, float v0, float v1, float v2, float v3)
+{
+    float acc = m[r] * v0;
+    acc = acc + m[4 + r] * v1;
+    acc = acc + m[8 + r] * v2;
+    acc = acc + m[12 + r] * v3;
+    return acc;
+}
+
+// GetRayStuff / GetRayDirectionAt (InitialRayTraceFrag.glsl:398-414,
+// ShadowRayTraceFrag.glsl:292-297), normalised as in main() (:427).
+__device__ inline Vec3 ray_dir_at(const float* inv_view, const float* inv_proj, float sx, float sy)
+{
+    float cx = sx * 2.0f - 1.0f;
+    float cy = sy * 2.0f - 1.0f;
+    float ex = mat_row(inv_proj, 0, cx, cy, -1.0f, 1.0f);
+    float ey = mat_row(inv_proj, 1, cx, cy, -1.0f, 1.0f);
+    Vec3 rd;
+    rd.x = mat_row(inv_view, 0, ex, ey, -1.0f, 0.0f);
+    rd.y = mat_row(inv_view, 1, ex, ey, -1.0f, 0.0f);
+    rd.z = mat_row(inv_view, 2, ex, ey, -1.0f, 0.0f);
+    return rd;
+}
+
+__device__ inline float texcoord(int p, int n) { return ((float)p + 0.5f) / (float)n; }
+
+// GetNormalFromID for a face byte; `miss` is the value of the reference's
+// out-of-range branch (1.0 in most passes, 0.5 in the diffuse pass).
+__device__ inline Vec3 face_normal(int face, float miss)
+{
+    switch (face) {
+    case 0: return mk3(0.0f, 0.0f, 1.0f);
+    case 1: return mk3(0.0f, 0.0f, -1.0f);
+    case 2: return mk3(0.0f, 1.0f, 0.0f);
+    case 3: return mk3(0.0f, -1.0f, 0.0f);
+    case 4: return mk3(-1.0f, 0.0f, 0.0f);
+    case 5: return mk3(1.0f, 0.0f, 0.0f);
+    default: return mk3(miss, miss, miss);
+    }
+}
+
+// GetNormalID of the normal with component -s on `axis`
+// (InitialRayTraceFrag.glsl:143-185).
+__device__ inline int face_from_axis(int axis, int s)
+{
+    if (s == 0) return 0;
+    int n = -s;
+    if (axis == 2) return n > 0 ? 0 : 1;
+    if (axis == 1) return n > 0 ? 2 : 3;
+    return n < 0 ? 4 : 5;
+}
+
+// The DF step E of the reference for a DF byte: floor(M * 0.57735026918f),
+// except M == 1 -> 1.  The unorm8 round trip (M/255)*255 is exact for every
+// byte, so M is the byte itself (the host tests check this against
+// vpt_df_step for all 256 values).
+__device__ inline int df_step(int m)
+{
+    return m == 1 ? 1 : (int)((float)m * 0.57735026918f);
+}
+
+struct Hit {
+    float t;
+    int face;
+    int block;
+    uint32_t voxel;
+    int samples;
+};
+
+__device__ inline uint32_t pack_cell(int x, int y, int z)
+{
+    return (uint32_t)x | ((uint32_t)y << 10) | ((uint32_t)z << 20);
+}
+
+// VoxelTraversalDF (InitialRayTraceFrag.glsl:307-374; identical copies in
+// ShadowRayTraceFrag.glsl:222-289 and DiffuseRayTraceFrag.glsl:1043-1110).
+__device__ inline Hit traverse(const WorldView& w, Vec3 o, Vec3 d, int max_iter)
+{
+    const float ox0 = o.x, oy0 = o.y, oz0 = o.z;
+    const int sx = d.x > 0.0f ? 1 : (d.x < 0.0f ? -1 : 0);
+    const int sy = d.y > 0.0f ? 1 : (d.y < 0.0f ? -1 : 0);
+    const int sz = d.z > 0.0f ? 1 : (d.z < 0.0f ? -1 : 0);
+    const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+    const int hx = (1 + sx) >> 1, hy = (1 + sy) >> 1, hz = (1 + sz) >> 1;
+    bool inter = false;
+    int min_idx = 0;
+    int samples = 0;
+    for (int itr = 0; itr < max_iter; itr++) {
+        float fx = floorf(o.x), fy = floorf(o.y), fz = floorf(o.z);
+        if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax)) {
+            inter = false;
+            break;
+        }
+        uint32_t off = brick_offset((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, w.row_stride,
+                                    w.slab_stride);
+        int m = w.df[off];
+        samples++;
+        int e = df_step(m);
+        if (e == 0) break;
+        if (e == 1) {
+            int gx = (int)o.x, gy = (int)o.y, gz = (int)o.z;
+            float wx = o.x - (float)gx, wy = o.y - (float)gy, wz = o.z - (float)gz;
+            float fxx = ((float)hx - wx) * ix;
+            float fyy = ((float)hy - wy) * iy;
+            float fzz = ((float)hz - wz) * iz;
+            min_idx = (fxx < fyy && sx != 0) ? ((fxx < fzz || sz == 0) ? 0 : 2)
+                                             : ((fyy < fzz || sz == 0) ? 1 : 2);
+            float fm = min_idx == 0 ? fxx : (min_idx == 1 ? fyy : fzz);
+            wx = wx + d.x * fm;
+            wy = wy + d.y * fm;
+            wz = wz + d.z * fm;
+            if (min_idx == 0) {
+                gx += sx;
+                wx = (float)(1 - hx);
+            } else if (min_idx == 1) {
+                gy += sy;
+                wy = (float)(1 - hy);
+            } else {
+                gz += sz;
+                wz = (float)(1 - hz);
+            }
+            o.x = (float)gx + wx;
+            o.y = (float)gy + wy;
+            o.z = (float)gz + wz;
+            if (min_idx == 0) o.x = o.x + (float)sx * 0.0001f;
+            else if (min_idx == 1) o.y = o.y + (float)sy * 0.0001f;
+            else o.z = o.z + (float)sz * 0.0001f;
+            inter = true;
+        } else {
+            float k = (float)(e - 1);
+            o.x = o.x + k * d.x;
+            o.y = o.y + k * d.y;
+            o.z = o.z + k * d.z;
+        }
+    }
+    Hit h;
+    h.t = -1.0f;
+    h.face = 255;
+    h.block = 0;
+    h.voxel = 0xffffffffu;
+    h.samples = samples;
+    if (inter) {
+        float fx = floorf(o.x), fy = floorf(o.y), fz = floorf(o.z);
+        if (fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax) {
+            int cx = (int)fx, cy = (int)fy, cz = (int)fz;
+            int b = w.grid[brick_offset(cx, cy, cz, w.row_stride, w.slab_stride)];
+            if (b > 0) {
+                float dx = o.x - ox0, dy = o.y - oy0, dz = o.z - oz0;
+                h.t = sqrtf((dx * dx + dy * dy) + dz * dz);
+                int s = min_idx == 0 ? sx : (min_idx == 1 ? sy : sz);
+                h.face = face_from_axis(min_idx, s);
+                h.block = b;
+                h.voxel = pack_cell(cx, cy, cz);
+            }
+        }
+    }
+    return h;
+}
+
+// GetVoxel(ivec3(floor(p))) (InitialRayTraceFrag.glsl:79-87).
+__device__ inline int voxel_at(const WorldView& w, Vec3 p)
+{
+    float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax)) return 0;
+    return w.grid[brick_offset((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, w.row_stride,
+                               w.slab_stride)];
+}
+
+// Blocks are dealt to the 8 XCDs round-robin by hardware; remap the block
+// index so that XCD k receives a contiguous 1/8 of the tiles, keeping
+// neighbouring tiles (which touch the same bricks) in one L2.
+__device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks)
+{
+    uint32_t per = nblocks / 8u;
+    if (per == 0u || bid >= per * 8u) return bid;
+    return (bid & 7u) * per + (bid >> 3);
+}
+
+}  // namespace vpt

@@ -1,0 +1,768 @@
+// SVGF chain on gfx950: pre-temporal 3x3 spatial, temporal reprojection,
+// variance estimate, a-trous x5, SH decode (Core/Pipeline.cpp:2284-2611).
+//
+// Storage follows the reference's render targets (Pipeline.cpp:1102-1108):
+// SH RGBA16F + CoCg RG16F + one R16F channel + AO/sky RG8, packed as one
+// 16-byte texel per pixel [SH0..3, Co, Cg, X, (sky<<8)|ao] so that a tap is a
+// single 128-bit load; the temporal utility (spp, moment, luma) is a second
+// 8-byte plane.  Every pass computes in fp32 with the same IEEE operation
+// sequence as the shader it cites and rounds to binary16 / unorm8 on store.
+// Fractional taps (temporal reprojection, a-trous at step * 1.35 + jitter)
+// are GL LINEAR + REPEAT bilinear fetches; taps at integer pixel offsets
+// read the texel directly.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "vpt_device.h"
+#include "vpt_internal.h"
+
+using vpt::set_error;
+using vpt::Vec3;
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_error(VPT_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));      \
+    } while (0)
+
+struct vpt_svgf {
+    int device = 0;
+    int W = 0, H = 0;
+    vpt_svgf_params prm{};
+    vpt_svgf_texel* d_pre = nullptr;        // pre-spatial output
+    vpt_svgf_texel* d_temporal[2] = {};     // temporal output, ping-pong (history)
+    uint2* d_util[2] = {};                  // temporal utility (spp, moment, luma, 0)
+    vpt_svgf_texel* d_var = nullptr;        // variance-pass output
+    vpt_svgf_texel* d_atrous[2] = {};       // a-trous ping-pong
+    int cur = 0;                            // index of the newest temporal plane
+    int last_final = 0;                     // a-trous plane holding the last output
+    bool have_history = false;
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+struct CamView {
+    float iv[16];
+    float ip[16];
+};
+
+struct Mat16 {
+    float m[16];
+};
+
+struct Tx {
+    float sh[4];
+    float co, cg;
+    float x;
+    float ao, sky;
+};
+
+__device__ inline float h2f(uint32_t h) { return vpt_f16_to_f32((uint16_t)h); }
+
+__device__ inline Tx load_tx(const vpt_svgf_texel* p, size_t i)
+{
+    uint4 v = *reinterpret_cast<const uint4*>(p + i);
+    Tx t;
+    t.sh[0] = h2f(v.x & 0xffffu);
+    t.sh[1] = h2f(v.x >> 16);
+    t.sh[2] = h2f(v.y & 0xffffu);
+    t.sh[3] = h2f(v.y >> 16);
+    t.co = h2f(v.z & 0xffffu);
+    t.cg = h2f(v.z >> 16);
+    t.x = h2f(v.w & 0xffffu);
+    t.ao = vpt_unorm8_decode((uint8_t)((v.w >> 16) & 0xffu));
+    t.sky = vpt_unorm8_decode((uint8_t)(v.w >> 24));
+    return t;
+}
+
+__device__ inline void store_tx(vpt_svgf_texel* p, size_t i, const float sh[4], float co, float cg, float x, float ao,
+                                float sky)
+{
+    uint32_t w0 = (uint32_t)vpt_f32_to_f16(sh[0]) | ((uint32_t)vpt_f32_to_f16(sh[1]) << 16);
+    uint32_t w1 = (uint32_t)vpt_f32_to_f16(sh[2]) | ((uint32_t)vpt_f32_to_f16(sh[3]) << 16);
+    uint32_t w2 = (uint32_t)vpt_f32_to_f16(co) | ((uint32_t)vpt_f32_to_f16(cg) << 16);
+    uint32_t w3 = (uint32_t)vpt_f32_to_f16(x) | ((uint32_t)vpt_unorm8_encode(ao) << 16) |
+                  ((uint32_t)vpt_unorm8_encode(sky) << 24);
+    *reinterpret_cast<uint4*>(p + i) = make_uint4(w0, w1, w2, w3);
+}
+
+__device__ inline float fclamp(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ inline float sh_to_y(float w) { return fmaxf(0.0f, 3.544905f * w); }
+__device__ inline bool in_screen(float x, float y) { return x > 0.0f && x < 1.0f && y > 0.0f && y < 1.0f; }
+
+__device__ inline Vec3 position_from(const CamView& c, float sx, float sy, float dist)
+{
+    Vec3 dir = vpt::normalize3(vpt::ray_dir_at(c.iv, c.ip, sx, sy));
+    return vpt::mk3(c.iv[12] + dir.x * dist, c.iv[13] + dir.y * dist, c.iv[14] + dir.z * dist);
+}
+
+// GL LINEAR + REPEAT footprint at normalised (s, t); valid for coordinates
+// inside (0, 1), where the taps lie in [-1, n].
+struct Bil {
+    size_t i00, i10, i01, i11;
+    float a, b;
+};
+
+__device__ inline int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+__device__ inline Bil bil_setup(float s, float t, int W, int H)
+{
+    float u = s * (float)W - 0.5f;
+    float v = t * (float)H - 0.5f;
+    float fu = floorf(u), fv = floorf(v);
+    Bil b;
+    b.a = u - fu;
+    b.b = v - fv;
+    int x0 = wrap1((int)fu, W), x1 = wrap1((int)fu + 1, W);
+    int y0 = wrap1((int)fv, H), y1 = wrap1((int)fv + 1, H);
+    b.i00 = (size_t)y0 * W + x0;
+    b.i10 = (size_t)y0 * W + x1;
+    b.i01 = (size_t)y1 * W + x0;
+    b.i11 = (size_t)y1 * W + x1;
+    return b;
+}
+
+__device__ inline float bil(float t00, float t10, float t01, float t11, float a, float b)
+{
+    float top = t00 * (1.0f - a) + t10 * a;
+    float bot = t01 * (1.0f - a) + t11 * a;
+    return top * (1.0f - b) + bot * b;
+}
+
+__device__ inline Tx bil_tx(const vpt_svgf_texel* p, const Bil& f)
+{
+    Tx q00 = load_tx(p, f.i00), q10 = load_tx(p, f.i10), q01 = load_tx(p, f.i01), q11 = load_tx(p, f.i11);
+    Tx r;
+    for (int c = 0; c < 4; c++) r.sh[c] = bil(q00.sh[c], q10.sh[c], q01.sh[c], q11.sh[c], f.a, f.b);
+    r.co = bil(q00.co, q10.co, q01.co, q11.co, f.a, f.b);
+    r.cg = bil(q00.cg, q10.cg, q01.cg, q11.cg, f.a, f.b);
+    r.x = bil(q00.x, q10.x, q01.x, q11.x, f.a, f.b);
+    r.ao = bil(q00.ao, q10.ao, q01.ao, q11.ao, f.a, f.b);
+    r.sky = bil(q00.sky, q10.sky, q01.sky, q11.sky, f.a, f.b);
+    return r;
+}
+
+__device__ inline int nearest_idx(float s, float t, int W, int H)
+{
+    int i = (int)floorf(s * (float)W), j = (int)floorf(t * (float)H);
+    i = ((i % W) + W) % W;
+    j = ((j % H) + H) % H;
+    return j * W + i;
+}
+
+// Pixel of this lane: 32x8 tiles (a wavefront covers 32x2 pixels, i.e.
+// 512-byte texel rows), XCD-banded.
+__device__ inline bool pixel_of(int W, int H, int& px, int& py)
+{
+    const uint32_t tiles_x = (uint32_t)(W + 31) / 32u;
+    const uint32_t tiles_y = (uint32_t)(H + 7) / 8u;
+    const uint32_t tile = vpt::xcd_remap(blockIdx.x, tiles_x * tiles_y);
+    px = (int)(tile % tiles_x) * 32 + (int)(threadIdx.x & 31);
+    py = (int)(tile / tiles_x) * 8 + (int)(threadIdx.x >> 5);
+    return px < W && py < H;
+}
+
+dim3 pass_grid(int W, int H) { return dim3((unsigned)(((W + 31) / 32) * ((H + 7) / 8))); }
+
+__device__ inline float texc(int p, int n) { return ((float)p + 0.5f) / (float)n; }
+
+// ---- Spatial3x3Initial.main (Spatial3x3Initial.glsl:96-176) --------------
+__global__ void __launch_bounds__(256) prespatial_kernel(CamView cam, int W, int H, const float* __restrict__ gb_t,
+                                                         const uint8_t* __restrict__ gb_face,
+                                                         const vpt_svgf_texel* __restrict__ in,
+                                                         vpt_svgf_texel* __restrict__ out)
+{
+    int px, py;
+    if (!pixel_of(W, H, px, py)) return;
+    const size_t i = (size_t)py * W + px;
+    const float atrous[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    float uvx = texc(px, W), uvy = texc(py, H);
+    Vec3 base_pos = position_from(cam, uvx, uvy, gb_t[i]);
+    Vec3 base_n = vpt::face_normal(gb_face[i], 1.0f);
+    Tx base = load_tx(in, i);
+    float base_luma = sh_to_y(base.sh[3]);
+    float tsh[4] = {base.sh[0], base.sh[1], base.sh[2], base.sh[3]};
+    float tco = base.co, tcg = base.cg, tw = 1.0f;
+    float tao = base.ao, tsky = base.sky, taow = 1.0f;
+    const float phi_color = 4.0f;
+    for (int x = -1; x <= 1; x++) {
+        for (int y = -1; y <= 1; y++) {
+            if (x == 0 && y == 0) continue;
+            float sx = uvx + ((float)x * 1.0f) * tex_w;
+            float sy = uvy + ((float)y * 1.0f) * tex_h;
+            if (!in_screen(sx, sy)) continue;
+            size_t j = (size_t)(py + y) * W + (px + x);
+            Vec3 sp = position_from(cam, sx, sy, gb_t[j]);
+            Vec3 pd = vpt::mk3(fabsf(sp.x - base_pos.x), fabsf(sp.y - base_pos.y), fabsf(sp.z - base_pos.z));
+            float dist_sqr = vpt::dot3(pd, pd);
+            if (dist_sqr < 1.0f) {
+                Tx s = load_tx(in, j);
+                Vec3 sn = vpt::face_normal(gb_face[j], 1.0f);
+                float sluma = sh_to_y(s.sh[3]);
+                float nw = vpt_powi(fmaxf(vpt::dot3(base_n, sn), 0.0f), 16);
+                float lw = fabsf(sluma - base_luma) / phi_color;
+                float w = vpt_expf(-lw - nw);
+                w = fmaxf(w, 0.01f);
+                w = (atrous[abs(x)] * atrous[abs(y)]) * w;
+                w = fmaxf(w, 0.01f);
+                w = fclamp(w, 0.0f, 1.0f);
+                for (int k = 0; k < 4; k++) tsh[k] = tsh[k] + s.sh[k] * w;
+                tco = tco + s.co * w;
+                tcg = tcg + s.cg * w;
+                tw = tw + w;
+                tao = tao + s.ao * w;
+                tsky = tsky + s.sky * w;
+                taow = taow + w;
+            }
+        }
+    }
+    tw = fmaxf(tw, 0.01f);
+    float osh[4];
+    for (int k = 0; k < 4; k++) osh[k] = tsh[k] / tw;
+    float aw = fmaxf(taow, 0.01f);
+    store_tx(out, i, osh, tco / tw, tcg / tw, base.x, tao / aw, tsky / aw);
+}
+
+// ---- TemporalFilter.main (SVGF/TemporalFilter.glsl:132-360) --------------
+__global__ void __launch_bounds__(256) temporal_kernel(CamView cam, Mat16 prev_vp, int W, int H,
+                                                       const float* __restrict__ cur_t,
+                                                       const uint8_t* __restrict__ cur_face,
+                                                       const uint8_t* __restrict__ cur_block,
+                                                       const float* __restrict__ prev_t,
+                                                       const uint8_t* __restrict__ prev_face,
+                                                       const uint8_t* __restrict__ prev_block,
+                                                       const vpt_svgf_texel* __restrict__ cur,
+                                                       const vpt_svgf_texel* __restrict__ hist,
+                                                       const uint2* __restrict__ hist_util,
+                                                       vpt_svgf_texel* __restrict__ out, uint2* __restrict__ out_util)
+{
+    int px, py;
+    if (!pixel_of(W, H, px, py)) return;
+    const size_t i = (size_t)py * W + px;
+    const float weights[5] = {3.0f / 32.0f, 3.0f / 32.0f, 9.0f / 64.0f, 3.0f / 32.0f, 3.0f / 32.0f};
+    const float offx[5] = {1.0f, 0.0f, 0.0f, -1.0f, 0.0f};
+    const float offy[5] = {0.0f, 1.0f, 0.0f, 0.0f, -1.0f};
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    const float bth = 0.0035f;
+    float uvx = texc(px, W), uvy = texc(py, H);
+    float base_w = cur_t[i];
+    Vec3 base_pos = position_from(cam, uvx, uvy, base_w);
+    Vec3 base_n = vpt::face_normal(cur_face[i], 1.0f);
+    Tx base = load_tx(cur, i);
+
+    float q0 = vpt::mat_row(prev_vp.m, 0, base_pos.x, base_pos.y, base_pos.z, 1.0f);
+    float q1 = vpt::mat_row(prev_vp.m, 1, base_pos.x, base_pos.y, base_pos.z, 1.0f);
+    float q3 = vpt::mat_row(prev_vp.m, 3, base_pos.x, base_pos.y, base_pos.z, 1.0f);
+    float rx = (q0 / q3) * 0.5f + 0.5f;
+    float ry = (q1 / q3) * 0.5f + 0.5f;
+    float base_lum = base.x;
+    int base_block = min(max((int)cur_block[i], 0), 127);
+
+    bool do_block = true, do_normal = true;
+    float tol = 0.75f;
+    Vec3 dv = vpt::mk3(base_pos.x - cam.iv[12], base_pos.y - cam.iv[13], base_pos.z - cam.iv[14]);
+    float dtp = sqrtf(vpt::dot3(dv, dv));
+    if (dtp < 4.0f) { tol = 0.3f; }
+    else if (dtp < 6.0f) { tol = 0.65f; }
+    else if (dtp < 8.0f) { tol = 0.85f; }
+    else if (dtp < 16.0f) { tol = 1.414f; }
+    else if (dtp < 32.0f) { tol = 2.4f; }
+    else if (dtp < 48.0f) { tol = 3.5f; do_block = false; }
+    else if (dtp < 64.0f) { tol = 4.2f; do_block = false; }
+    else if (dtp < 96.0f) { tol = 6.25f; do_block = false; do_normal = false; }
+    else if (dtp < 128.0f) { tol = 9.0f; do_block = false; do_normal = false; }
+    else if (dtp < 200.0f) { tol = 14.0f; do_block = false; do_normal = false; }
+
+    float tw = 0.0f;
+    float ssh[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float sco = 0.0f, scg = 0.0f, sao = 0.0f, ssky = 0.0f, slum = 0.0f, sspp = 0.0f, smom = 0.0f;
+    int successful = 0;
+    for (int k = 0; k < 5; k++) {
+        float sx = rx + (offx[k] + 0.0f) * tex_w;
+        float sy = ry + (offy[k] + 0.0f) * tex_h;
+        if (!(sx < 1.0f - bth && sx > bth && sy < 1.0f - bth && sy > bth)) continue;
+        Bil f = bil_setup(sx, sy, W, H);
+        float pw = bil(prev_t[f.i00], prev_t[f.i10], prev_t[f.i01], prev_t[f.i11], f.a, f.b);
+        Vec3 pp = position_from(cam, sx, sy, pw);
+        int ni = nearest_idx(sx, sy, W, H);
+        Vec3 pn = vpt::face_normal(prev_face[ni], 1.0f);
+        Vec3 pd = vpt::mk3(fabsf(base_pos.x - pp.x), fabsf(base_pos.y - pp.y), fabsf(base_pos.z - pp.z));
+        float perr = vpt::dot3(pd, pd);
+        int sblock = min(max((int)prev_block[ni], 0), 127);
+        bool valid = false;
+        if (perr < tol && ((pw < 0.0f) == (base_w < 0.0f))) {
+            valid = true;
+            if (do_normal && !(pn.x == base_n.x && pn.y == base_n.y && pn.z == base_n.z)) valid = false;
+            if (do_block && base_block != sblock) valid = false;
+        }
+        if (valid) {
+            float cw = weights[k];
+            Tx h = bil_tx(hist, f);
+            uint2 u00 = hist_util[f.i00], u10 = hist_util[f.i10], u01 = hist_util[f.i01], u11 = hist_util[f.i11];
+            float spp = bil(h2f(u00.x & 0xffffu), h2f(u10.x & 0xffffu), h2f(u01.x & 0xffffu), h2f(u11.x & 0xffffu),
+                            f.a, f.b);
+            float mom = bil(h2f(u00.x >> 16), h2f(u10.x >> 16), h2f(u01.x >> 16), h2f(u11.x >> 16), f.a, f.b);
+            float lum = bil(h2f(u00.y & 0xffffu), h2f(u10.y & 0xffffu), h2f(u01.y & 0xffffu), h2f(u11.y & 0xffffu),
+                            f.a, f.b);
+            for (int c = 0; c < 4; c++) ssh[c] = ssh[c] + h.sh[c] * cw;
+            sco = sco + h.co * cw;
+            scg = scg + h.cg * cw;
+            sspp = sspp + spp * cw;
+            smom = smom + mom * cw;
+            slum = slum + lum * cw;
+            sao = sao + h.ao * cw;
+            ssky = ssky + h.sky * cw;
+            tw = tw + cw;
+            successful++;
+        }
+    }
+    if (tw > 0.001f) {
+        for (int c = 0; c < 4; c++) ssh[c] = ssh[c] / tw;
+        sco = sco / tw;
+        scg = scg / tw;
+        smom = smom / tw;
+        sspp = sspp / tw;
+        slum = slum / tw;
+        sao = sao / tw;
+        ssky = ssky / tw;
+    } else {
+        successful = 0;
+    }
+    float spp_inc = sspp + 1.0f;
+    if (successful <= 0) spp_inc = 0.01f;
+    float blend = fmaxf(1.0f / spp_inc, 0.05f);
+    float moment_factor = fmaxf(1.0f / spp_inc, 0.05f);
+    float util_spp = successful <= 0 ? 0.0f : spp_inc;
+    float util_moment = (1.0f - moment_factor) * smom + moment_factor * (base_lum * base_lum);
+    float store_luma = slum * (1.0f - blend) + base_lum * blend;
+    float osh[4];
+    float oco, ocg, oao, osky;
+    if (successful <= 0) {
+        for (int c = 0; c < 4; c++) osh[c] = base.sh[c];
+        oco = base.co;
+        ocg = base.cg;
+        oao = base.ao;
+        osky = base.sky;
+    } else {
+        for (int c = 0; c < 4; c++) osh[c] = ssh[c] * (1.0f - blend) + base.sh[c] * blend;
+        oco = sco * (1.0f - blend) + base.co * blend;
+        ocg = scg * (1.0f - blend) + base.cg * blend;
+        oao = sao * (1.0f - blend) + base.ao * blend;
+        osky = ssky * (1.0f - blend) + base.sky * blend;
+    }
+    for (int c = 0; c < 4; c++) osh[c] = fclamp(osh[c], -100.0f, 100.0f);
+    store_tx(out, i, osh, fclamp(oco, -10.0f, 100.0f), fclamp(ocg, -10.0f, 100.0f), 0.0f, fclamp(oao, 0.0f, 1.0f),
+             fclamp(osky, 0.0f, 1.0f));
+    uint32_t u0 = (uint32_t)vpt_f32_to_f16(fclamp(util_spp, -150.0f, 150.0f)) |
+                  ((uint32_t)vpt_f32_to_f16(fclamp(util_moment, -150.0f, 150.0f)) << 16);
+    uint32_t u1 = (uint32_t)vpt_f32_to_f16(fclamp(store_luma, -150.0f, 150.0f));
+    out_util[i] = make_uint2(u0, u1);
+}
+
+// ---- VarianceEstimate.main (SVGF/VarianceEstimate.glsl:73-192) -----------
+// The output texel also carries the temporal AO/sky, which the first a-trous
+// pass reads (Pipeline.cpp:2531-2535).
+__global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float* __restrict__ gb_t,
+                                                       const uint8_t* __restrict__ gb_face,
+                                                       const vpt_svgf_texel* __restrict__ in,
+                                                       const uint2* __restrict__ util,
+                                                       vpt_svgf_texel* __restrict__ out)
+{
+    int px, py;
+    if (!pixel_of(W, H, px, py)) return;
+    const size_t i = (size_t)py * W + px;
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    const float thresh = 4.0f + 4.0f + 4.0f;
+    float uvx = texc(px, W), uvy = texc(py, H);
+    float base_w = gb_t[i];
+    Vec3 base_n = vpt::face_normal(gb_face[i], 1.0f);
+    uint2 bu = util[i];
+    float frames = h2f(bu.x & 0xffffu);
+    float base_moment = h2f(bu.x >> 16);
+    Tx base = load_tx(in, i);
+    float base_lum = sh_to_y(base.sh[3]);
+    float osh[4], oco, ocg;
+    float variance;
+    if (frames < thresh) {
+        const float color_phi = 5.0f;
+        float tw = 0.0f, tmom = 0.0f, tlum = 0.0f, tw2 = 0.0f;
+        float tsh[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float tco = 0.0f, tcg = 0.0f;
+        for (int x = -4; x <= 4; x++) {
+            for (int y = -4; y <= 4; y++) {
+                float sx = uvx + (float)x * tex_w;
+                float sy = uvy + (float)y * tex_h;
+                if (!in_screen(sx, sy)) continue;
+                size_t j = (size_t)(py + y) * W + (px + x);
+                float sw = gb_t[j];
+                Vec3 sn = vpt::face_normal(gb_face[j], 1.0f);
+                float smom = h2f(util[j].x >> 16);
+                Tx s = load_tx(in, j);
+                float slum = sh_to_y(s.sh[3]);
+                float nw = vpt_powi(fmaxf(vpt::dot3(base_n, sn), 0.0f), 16);
+                float e = vpt_expf(-fabsf(sw - base_w));
+                float dw = e * e;
+                float lw = fabsf(slum - base_lum) / color_phi;
+                float w = (vpt_expf(-lw) * nw) * dw;
+                float w2 = w;
+                w = fmaxf(w, 0.000000015f);
+                w2 = fmaxf(w2, 0.0000000015f);
+                tw = tw + w;
+                tmom = tmom + smom * w2;
+                for (int k = 0; k < 4; k++) tsh[k] = tsh[k] + s.sh[k] * w;
+                tco = tco + s.co * w;
+                tcg = tcg + s.cg * w;
+                tlum = tlum + slum * w2;
+                tw2 = tw2 + w2;
+            }
+        }
+        if (tw > 0.0f) {
+            tmom = tmom / tw2;
+            tlum = tlum / tw2;
+            tco = tco / tw;
+            tcg = tcg / tw;
+            for (int k = 0; k < 4; k++) tsh[k] = tsh[k] / tw;
+        }
+        for (int k = 0; k < 4; k++) osh[k] = tsh[k];
+        oco = tco;
+        ocg = tcg;
+        float acc = tlum * tlum;
+        variance = tmom - acc;
+        variance = variance * 3.0f;
+    } else {
+        for (int k = 0; k < 4; k++) osh[k] = base.sh[k];
+        oco = base.co;
+        ocg = base.cg;
+        variance = base_moment - base_lum * base_lum;
+    }
+    variance = variance * (thresh / frames);
+    for (int k = 0; k < 4; k++) osh[k] = fclamp(osh[k], -100.0f, 100.0f);
+    store_tx(out, i, osh, fclamp(oco, -10.0f, 100.0f), fclamp(ocg, -10.0f, 100.0f), fclamp(variance, -1.0f, 50.0f),
+             base.ao, base.sky);
+}
+
+// GradientNoise (SpatialFilter.glsl:160-165).
+__device__ inline float gradient_noise(int px, int py, float time)
+{
+    float t = time * 100.493850275f;
+    float m = t - 500.0f * floorf(t / 500.0f);
+    float cx = ((float)px + 0.5f) + m;
+    float cy = ((float)py + 0.5f) + m;
+    float inner = 0.06711056f * cx + 0.00583715f * cy;
+    inner = inner - floorf(inner);
+    float n = 52.9829189f * inner;
+    return n - floorf(n);
+}
+
+// ---- SpatialFilter.main (SVGF/SpatialFilter.glsl:175-355), one pass ------
+__global__ void __launch_bounds__(256) atrous_kernel(int W, int H, const float* __restrict__ gb_t,
+                                                     const uint8_t* __restrict__ gb_face,
+                                                     const vpt_svgf_texel* __restrict__ in,
+                                                     const uint2* __restrict__ tutil, int step, float scale,
+                                                     float color_phi_bias, float time,
+                                                     vpt_svgf_texel* __restrict__ out)
+{
+    int px, py;
+    if (!pixel_of(W, H, px, py)) return;
+    const size_t i = (size_t)py * W + px;
+    const float atrous[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+    const float gauss[2] = {0.60283f, 0.198585f};
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    const bool filter_ao = step <= 4;
+    const bool filter_sky = (step <= 6) || filter_ao;
+    float uvx = texc(px, W), uvy = texc(py, H);
+    int jit = (int)((gradient_noise(px, py, time) - 0.5f) * ((float)step * 0.8f));
+    float base_depth = gb_t[i];
+    Vec3 base_n = vpt::face_normal(gb_face[i], 1.0f);
+    Tx base = load_tx(in, i);
+    float base_lum = sh_to_y(base.sh[3]);
+
+    // GaussianVariance (:94-129) over the previous pass's variance
+    float base_var = 0.0f, vsum = 0.0f, ksum = 0.0f;
+    for (int x = -1; x <= 1; x++) {
+        for (int y = -1; y <= 1; y++) {
+            float sx = uvx + (float)x * tex_w;
+            float sy = uvy + (float)y * tex_h;
+            if (!in_screen(sx, sy)) continue;
+            float kv = gauss[abs(x)] * gauss[abs(y)];
+            float v = (x == 0 && y == 0) ? base.x : load_tx(in, (size_t)(py + y) * W + (px + x)).x;
+            if (x == 0 && y == 0) base_var = v;
+            vsum = vsum + v * kv;
+            ksum = ksum + kv;
+        }
+    }
+    float var_est = vsum / fmaxf(ksum, 0.01f);
+
+    float tsh[4] = {base.sh[0], base.sh[1], base.sh[2], base.sh[3]};
+    float tco = base.co, tcg = base.cg, tw = 1.0f, tvar = base_var;
+    float tao = base.ao, tsky = base.sky, taow = 1.0f;
+    float frames = h2f(tutil[i].x & 0xffffu);
+    bool strong = frames <= 8.0f && step <= 8;
+    float curve = 0.0f;
+    if (var_est < 0.01f) curve = 128.0f;
+    else if (var_est < 0.025f) curve = 112.0f;
+    else if (var_est < 0.05f) curve = 96.0f;
+    else if (var_est < 0.075f) curve = 84.0f;
+    else if (var_est < 0.1f) curve = 70.0f;
+    float tweaked;
+    if (var_est < 0.1f) {
+        float fv = fclamp(var_est, 0.0f, 1.0f);
+        tweaked = fv * vpt_powi(1.0f - fv, (int)(curve + 6.0f));
+    } else {
+        tweaked = var_est;
+    }
+    float phi = sqrtf(fmaxf(0.0f, 0.000001f + tweaked));
+    phi = phi / fmaxf(color_phi_bias, 0.1f);
+
+    if (!(base_depth < 0.0f)) {
+        for (int x = -1; x <= 1; x++) {
+            for (int y = -1; y <= 1; y++) {
+                if (x == 0 && y == 0) continue;
+                float ox = ((float)x * (float)step) * scale + (float)jit * 0.5f;
+                float oy = ((float)y * (float)step) * scale + (float)jit * 0.5f;
+                float sx = uvx + ox * tex_w;
+                float sy = uvy + oy * tex_h;
+                if (!in_screen(sx, sy)) continue;
+                Bil f = bil_setup(sx, sy, W, H);
+                float sdepth = bil(gb_t[f.i00], gb_t[f.i10], gb_t[f.i01], gb_t[f.i11], f.a, f.b);
+                float ddiff = fabsf(sdepth - base_depth);
+                Vec3 sn = vpt::face_normal(gb_face[nearest_idx(sx, sy, W, H)], 1.0f);
+                Tx s = bil_tx(in, f);
+                float slum = sh_to_y(s.sh[3]);
+                float nw = vpt_powi(fmaxf(vpt::dot3(base_n, sn), 0.0f), 32);
+                nw = fclamp(nw, 0.001f, 1.0f);
+                float lw = fabsf(slum - base_lum) / phi;
+                float e = vpt_expf(-fmaxf(ddiff, 0.00001f));
+                float dw = fclamp(e * e, 0.0001f, 1.0f);
+                float w = strong ? (nw * dw) : ((vpt_expf(-lw) * nw) * dw);
+                w = fclamp(w, 0.001f, 1.0f);
+                float xw = atrous[abs(x)], yw = atrous[abs(y)];
+                w = (xw * yw) * w;
+                w = fmaxf(w, 0.00000001f);
+                for (int c = 0; c < 4; c++) tsh[c] = tsh[c] + s.sh[c] * w;
+                tco = tco + s.co * w;
+                tcg = tcg + s.cg * w;
+                tvar = tvar + (w * w) * s.x;
+                tw = tw + w;
+                if (filter_sky || filter_ao) {
+                    float aow = fclamp(((xw * yw) * nw) * dw, 0.000001f, 1.0f);
+                    tao = tao + s.ao * aow;
+                    tsky = tsky + s.sky * aow;
+                    taow = taow + aow;
+                }
+            }
+        }
+    }
+    float osh[4];
+    for (int c = 0; c < 4; c++) osh[c] = fclamp(tsh[c] / tw, -100.0f, 100.0f);
+    float oao = tao / taow, osky = tsky / taow;
+    if (!filter_ao) oao = base.ao;
+    store_tx(out, i, osh, fclamp(tco / tw, -10.0f, 100.0f), fclamp(tcg / tw, -10.0f, 100.0f),
+             fclamp(tvar / (tw * tw), -1.0f, 50.0f), fclamp(oao, 0.0f, 1.0f), fclamp(osky, 0.0f, 1.0f));
+}
+
+// ---- SHToIrridiance(shY, CoCg, v) (ColorPassFrag.glsl:356-368) ------------
+__global__ void __launch_bounds__(256) decode_kernel(int W, int H, const uint8_t* __restrict__ gb_face,
+                                                     const vpt_svgf_texel* __restrict__ in, float* __restrict__ rgb)
+{
+    int px, py;
+    if (!pixel_of(W, H, px, py)) return;
+    const size_t i = (size_t)py * W + px;
+    Tx t = load_tx(in, i);
+    Vec3 v = vpt::face_normal(gb_face[i], 1.0f);
+    float x = (t.sh[0] * v.x + t.sh[1] * v.y) + t.sh[2] * v.z;
+    float Y = 2.0f * (1.023326f * x + 0.886226f * t.sh[3]);
+    Y = fmaxf(Y, 0.0f);
+    float k = (Y * 0.282095f) / (t.sh[3] + 1e-6f);
+    float co = t.co * k, cg = t.cg * k;
+    float T = Y - cg * 0.5f;
+    float G = cg + T;
+    float B = T - co * 0.5f;
+    float R = B + co;
+    rgb[3 * i] = fmaxf(R, 0.0f);
+    rgb[3 * i + 1] = fmaxf(G, 0.0f);
+    rgb[3 * i + 2] = fmaxf(B, 0.0f);
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int fill_zero(void* p, size_t bytes, hipStream_t s)
+{
+    size_t n = bytes / 4;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (uint32_t*)p, n, 0u);
+    HIP_TRY(hipGetLastError());
+    return VPT_OK;
+}
+
+CamView make_cam(const vpt_camera* c)
+{
+    CamView v;
+    for (int i = 0; i < 16; i++) {
+        v.iv[i] = c->inv_view[i];
+        v.ip[i] = c->inv_proj[i];
+    }
+    return v;
+}
+
+}  // namespace
+
+extern "C" VPT_API int vpt_svgf_create(vpt_ctx* ctx, int width, int height, const vpt_svgf_params* prm, vpt_svgf** out)
+{
+    if (!ctx || !out) return set_error(VPT_EINVAL, "vpt_svgf_create: null argument");
+    if (width <= 1 || height <= 1 || width > 16384 || height > 16384)
+        return set_error(VPT_EINVAL, "vpt_svgf_create: bad size %dx%d", width, height);
+    DeviceGuard g(ctx->device);
+    vpt_svgf* s = new vpt_svgf();
+    s->device = ctx->device;
+    s->W = width;
+    s->H = height;
+    if (prm) s->prm = *prm;
+    else vpt_svgf_params_default(&s->prm);
+    size_t n = (size_t)width * height;
+    hipError_t e = hipSuccess;
+    void** planes[] = {(void**)&s->d_pre, (void**)&s->d_temporal[0], (void**)&s->d_temporal[1], (void**)&s->d_var,
+                       (void**)&s->d_atrous[0], (void**)&s->d_atrous[1]};
+    for (void** p : planes)
+        if (e == hipSuccess) e = hipMalloc(p, n * sizeof(vpt_svgf_texel));
+    for (int k = 0; k < 2; k++)
+        if (e == hipSuccess) e = hipMalloc(&s->d_util[k], n * sizeof(uint2));
+    if (e != hipSuccess) {
+        vpt_svgf_destroy(s);
+        return set_error(VPT_ENOMEM, "vpt_svgf_create: hipMalloc failed (%s)", hipGetErrorString(e));
+    }
+    if (int rc = vpt_svgf_reset(s, nullptr)) {
+        vpt_svgf_destroy(s);
+        return rc;
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    *out = s;
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_svgf_destroy(vpt_svgf* s)
+{
+    if (!s) return VPT_OK;
+    DeviceGuard g(s->device);
+    (void)hipFree(s->d_pre);
+    (void)hipFree(s->d_var);
+    for (int k = 0; k < 2; k++) {
+        (void)hipFree(s->d_temporal[k]);
+        (void)hipFree(s->d_util[k]);
+        (void)hipFree(s->d_atrous[k]);
+    }
+    delete s;
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_svgf_reset(vpt_svgf* s, void* stream)
+{
+    if (!s) return set_error(VPT_EINVAL, "vpt_svgf_reset: null object");
+    DeviceGuard g(s->device);
+    hipStream_t st = (hipStream_t)stream;
+    size_t n = (size_t)s->W * s->H;
+    for (int k = 0; k < 2; k++) {
+        if (int rc = fill_zero(s->d_temporal[k], n * sizeof(vpt_svgf_texel), st)) return rc;
+        if (int rc = fill_zero(s->d_util[k], n * sizeof(uint2), st)) return rc;
+    }
+    s->cur = 0;
+    s->have_history = false;
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_svgf_frame(vpt_svgf* s, const vpt_camera* cam, const float prev_viewproj[16],
+                                      const vpt_gbuffer* cur, const vpt_gbuffer* prev, const vpt_svgf_texel* d_in,
+                                      float time, float* d_rgb_out, vpt_svgf_texel* d_texel_out, void* stream)
+{
+    if (!s || !cam || !prev_viewproj || !cur || !prev || !d_in || !d_rgb_out)
+        return set_error(VPT_EINVAL, "vpt_svgf_frame: null argument");
+    const int W = s->W, H = s->H;
+    if (cam->width != W || cam->height != H || cur->width != W || cur->height != H || prev->width != W ||
+        prev->height != H)
+        return set_error(VPT_EINVAL, "vpt_svgf_frame: size mismatch (denoiser %dx%d)", W, H);
+    if (!cur->d_t || !cur->d_face || !cur->d_block || !prev->d_t || !prev->d_face || !prev->d_block)
+        return set_error(VPT_EINVAL, "vpt_svgf_frame: G-buffer planes missing");
+    DeviceGuard g(s->device);
+    hipStream_t st = (hipStream_t)stream;
+    CamView cv = make_cam(cam);
+    Mat16 vp;
+    for (int k = 0; k < 16; k++) vp.m[k] = prev_viewproj[k];
+    dim3 grid = pass_grid(W, H), block(256);
+    const vpt_svgf_texel* temporal_in = d_in;
+    if (s->prm.pre_spatial) {
+        hipLaunchKernelGGL(prespatial_kernel, grid, block, 0, st, cv, W, H, cur->d_t, cur->d_face, d_in, s->d_pre);
+        HIP_TRY(hipGetLastError());
+        temporal_in = s->d_pre;
+    }
+    const int hist = s->cur, next = s->cur ^ 1;
+    hipLaunchKernelGGL(temporal_kernel, grid, block, 0, st, cv, vp, W, H, cur->d_t, cur->d_face, cur->d_block,
+                       prev->d_t, prev->d_face, prev->d_block, temporal_in, s->d_temporal[hist], s->d_util[hist],
+                       s->d_temporal[next], s->d_util[next]);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(variance_kernel, grid, block, 0, st, W, H, cur->d_t, cur->d_face, s->d_temporal[next],
+                       s->d_util[next], s->d_var);
+    HIP_TRY(hipGetLastError());
+    const float scale = 1.0f + (2.4f - 1.0f) * s->prm.resolution_scale;   // mix(1, 2.4, u_ResolutionScale)
+    const vpt_svgf_texel* src = s->d_var;
+    int dst_idx = 0;
+    for (int p = 0; p < 5; p++) {
+        dst_idx = p % 2;   // DiffuseDenoiseFBO / DiffuseDenoisedFBO2 ping-pong (Pipeline.cpp:2501-2503)
+        hipLaunchKernelGGL(atrous_kernel, grid, block, 0, st, W, H, cur->d_t, cur->d_face, src, s->d_util[next],
+                           s->prm.steps[p], scale, s->prm.color_phi_bias, time, s->d_atrous[dst_idx]);
+        HIP_TRY(hipGetLastError());
+        src = s->d_atrous[dst_idx];
+    }
+    hipLaunchKernelGGL(decode_kernel, grid, block, 0, st, W, H, cur->d_face, src, d_rgb_out);
+    HIP_TRY(hipGetLastError());
+    if (d_texel_out)
+        HIP_TRY(hipMemcpyAsync(d_texel_out, src, (size_t)W * H * sizeof(vpt_svgf_texel), hipMemcpyDeviceToDevice, st));
+    s->cur = next;
+    s->last_final = dst_idx;
+    s->have_history = true;
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_svgf_read_utility(vpt_svgf* s, uint16_t* d_dst, void* stream)
+{
+    if (!s || !d_dst) return set_error(VPT_EINVAL, "vpt_svgf_read_utility: null argument");
+    DeviceGuard g(s->device);
+    HIP_TRY(hipMemcpyAsync(d_dst, s->d_util[s->cur], (size_t)s->W * s->H * sizeof(uint2), hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream));
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_svgf_read_plane(vpt_svgf* s, int which, vpt_svgf_texel* d_dst, void* stream)
+{
+    if (!s || !d_dst) return set_error(VPT_EINVAL, "vpt_svgf_read_plane: null argument");
+    const vpt_svgf_texel* src = nullptr;
+    switch (which) {
+    case 0: src = s->d_pre; break;
+    case 1: src = s->d_temporal[s->cur]; break;
+    case 2: src = s->d_var; break;
+    case 3: src = s->d_atrous[s->last_final]; break;
+    default: return set_error(VPT_EINVAL, "vpt_svgf_read_plane: plane %d unknown", which);
+    }
+    DeviceGuard g(s->device);
+    HIP_TRY(hipMemcpyAsync(d_dst, src, (size_t)s->W * s->H * sizeof(vpt_svgf_texel), hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream));
+    return VPT_OK;
+}

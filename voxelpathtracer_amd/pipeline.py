@@ -1,0 +1,126 @@
+"""Frame orchestration of the hot path — the slice of the reference's
+``MainPipeline::StartPipeline`` frame loop that this library replaces
+(Core/Pipeline.cpp:1960-2852): primary trace into a ping-pong G-buffer
+(:1963-2016), diffuse GI trace (:2171-2281), the SVGF chain (:2284-2615)
+and the sun-shadow trace (:2790-2852).  Previous-frame matrices follow
+:1816-1824.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import SvgfParams, call, lib
+from .world import Context, DeviceGBuffer, World, _ptr, _stream, gi_params, make_camera
+
+
+def svgf_params_default() -> SvgfParams:
+    p = SvgfParams()
+    call("vpt_svgf_params_default", C.byref(p))
+    return p
+
+
+def decode_texels(tex: np.ndarray) -> dict:
+    """(H, W, 8) uint16 texels -> float planes (the storage values)."""
+    tex = np.asarray(tex, np.uint16)
+    h = tex[..., :7].view(np.float16).astype(np.float32)
+    ao = (tex[..., 7] & 0xFF).astype(np.float32) / np.float32(255.0)
+    sky = (tex[..., 7] >> 8).astype(np.float32) / np.float32(255.0)
+    return dict(sh=h[..., 0:4], cocg=h[..., 4:6], x=h[..., 6], ao=np.stack([ao, sky], -1))
+
+
+class SVGF:
+    """vpt_svgf: history and ping-pong planes of one view."""
+
+    def __init__(self, ctx: Context, width: int, height: int, params: SvgfParams | None = None):
+        h = C.c_void_p()
+        self.params = params or svgf_params_default()
+        call("vpt_svgf_create", ctx.handle, width, height, C.byref(self.params), C.byref(h))
+        self.handle = h
+        self.width, self.height = width, height
+
+    def reset(self, stream=None):
+        call("vpt_svgf_reset", self.handle, _stream(stream))
+
+    def frame(self, cam, prev_viewproj, cur: DeviceGBuffer, prev: DeviceGBuffer, texels_in, rgb_out, time: float,
+              texels_out=None, stream=None):
+        vp = (C.c_float * 16)(*map(float, np.asarray(prev_viewproj, np.float32).ravel()))
+        call("vpt_svgf_frame", self.handle, C.byref(cam), vp, C.byref(cur.struct()), C.byref(prev.struct()),
+             _ptr(texels_in), time, _ptr(rgb_out), _ptr(texels_out), _stream(stream))
+
+    def read_plane(self, which: int, dst, stream=None):
+        call("vpt_svgf_read_plane", self.handle, which, _ptr(dst), _stream(stream))
+
+    def read_utility(self, dst, stream=None):
+        call("vpt_svgf_read_utility", self.handle, _ptr(dst), _stream(stream))
+
+    def close(self):
+        if self.handle:
+            lib().vpt_svgf_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def viewproj(view: np.ndarray, proj: np.ndarray) -> np.ndarray:
+    """proj * view for column-major float32 matrices, summed in k order in
+    float32 (u_PrevProjection * u_PrevView, TemporalFilter.glsl:64)."""
+    V = np.asarray(view, np.float32).reshape(4, 4)   # V[col][row]
+    P = np.asarray(proj, np.float32).reshape(4, 4)
+    out = np.zeros((4, 4), np.float32)
+    for c in range(4):
+        for r in range(4):
+            acc = np.float32(0.0)
+            for k in range(4):
+                acc = np.float32(acc + np.float32(P[k, r] * V[c, k]))
+            out[c, r] = acc
+    return out.ravel()
+
+
+class Pipeline:
+    """Per-frame pass chain for one view of one world."""
+
+    def __init__(self, ctx: Context, world: World, width: int, height: int, sun_tick: float = 50.0,
+                 max_iter: int = 350, shadow_iter: int = 350):
+        import torch
+        self.ctx, self.world = ctx, world
+        self.width, self.height = width, height
+        self.gbuf = [DeviceGBuffer(width, height), DeviceGBuffer(width, height)]
+        self.rgb = torch.empty((height, width, 3), dtype=torch.float32, device="cuda")
+        self.texels = torch.empty((height, width, 8), dtype=torch.int16, device="cuda")
+        self.denoised = torch.empty((height, width, 3), dtype=torch.float32, device="cuda")
+        self.shadow = torch.empty((height, width), dtype=torch.uint8, device="cuda")
+        self.svgf = SVGF(ctx, width, height)
+        self.sun_tick = sun_tick
+        self.max_iter, self.shadow_iter = max_iter, shadow_iter
+        self.frame_index = 0
+        self.prev_view = None
+        self.prev_proj = None
+
+    def render(self, pos, yaw: float, pitch: float, stream=None):
+        """One frame: primary -> GI -> SVGF -> shadow.  Returns the camera."""
+        cam, view, proj = make_camera(pos, yaw, pitch, self.width, self.height)
+        cur = self.gbuf[self.frame_index % 2]
+        prev = self.gbuf[(self.frame_index + 1) % 2]
+        if self.prev_view is None:
+            self.prev_view, self.prev_proj = view, proj
+        self.world.trace_primary(cam, cur, self.max_iter, stream)
+        prm = gi_params(self.sun_tick, self.frame_index)
+        self.world.trace_gi(cam, cur, prm, self.rgb, self.texels, stream)
+        if self.frame_index == 0:
+            prev = cur
+        self.svgf.frame(cam, viewproj(self.prev_view, self.prev_proj), cur, prev, self.texels, self.denoised,
+                        time=self.frame_index / 60.0, stream=stream)
+        self.world.trace_shadow(cam, cur, list(prm.sun_dir), self.shadow, self.shadow_iter, stream)
+        self.prev_view, self.prev_proj = view, proj
+        self.frame_index += 1
+        return cam
+
+
+__all__ = ["SVGF", "Pipeline", "decode_texels", "svgf_params_default", "viewproj", "_lib"]

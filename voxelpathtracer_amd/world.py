@@ -1,0 +1,208 @@
+"""Host-side mirror of the reference's ``VoxelRT::World`` (Core/World.h:35-222)
+over the C ABI.
+
+The reference keeps the block grid inline (``m_WorldData``), uploads it as a
+3D texture (``Buffer``), builds the distance field with three compute
+dispatches (``InitializeDistanceGenerator`` / ``GenerateDistanceField``) and
+offers CPU block picking (``RaycastDetect``).  Here the grid copy stays on
+the host as a numpy array (z, y, x), the device copy lives in HBM in bricked
+layout, and every GPU step is a libvpt call.  Device buffers handed to the
+ray kernels are torch tensors (memory and streams only).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Camera, GBuffer, Hit, Ray, call, lib
+
+
+def _stream(stream=None) -> C.c_void_p:
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+def _ptr(t) -> C.c_void_p:
+    return C.c_void_p(0 if t is None else t.data_ptr())
+
+
+class Context:
+    """vpt_ctx: binds one HIP device."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        call("vpt_ctx_create", device, C.byref(h))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().vpt_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class World:
+    """VoxelRT::World: block grid + distance field resident in HBM."""
+
+    def __init__(self, ctx: Context, nx: int = 384, ny: int = 128, nz: int = 384):
+        h = C.c_void_p()
+        call("vpt_world_create", ctx.handle, nx, ny, nz, C.byref(h))
+        self.ctx = ctx
+        self.handle = h
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.m_WorldData = np.zeros((nz, ny, nx), np.uint8)
+        self._pending: dict[int, int] = {}
+        self._counter = None
+
+    # -- World.h:46-70 ---------------------------------------------------
+    def GetBlock(self, x: int, y: int, z: int) -> int:
+        return int(self.m_WorldData[z, y, x])
+
+    def SetBlock(self, x: int, y: int, z: int, block: int) -> None:
+        """Host edit; becomes visible on the device at the next
+        ``Buffer`` / ``FlushEdits`` (the reference follows SetBlock with a
+        one-voxel glTexSubImage3D, World.cpp:367-374)."""
+        self.m_WorldData[z, y, x] = block
+        self._pending[x | (y << 10) | (z << 20)] = block
+
+    def FlushEdits(self, stream=None) -> None:
+        if not self._pending:
+            return
+        edits = np.fromiter(self._pending.keys(), np.uint32)
+        ids = np.fromiter(self._pending.values(), np.uint8)
+        call("vpt_world_set_blocks", self.handle, edits.ctypes.data_as(C.c_void_p), ids.ctypes.data_as(C.c_void_p),
+             len(edits), _stream(stream))
+        self._pending.clear()
+
+    # -- World.h:167-174 -------------------------------------------------
+    def Buffer(self, stream=None) -> None:
+        grid = np.ascontiguousarray(self.m_WorldData)
+        call("vpt_world_upload", self.handle, grid.ctypes.data_as(C.c_void_p), _stream(stream))
+        self._pending.clear()
+
+    def InitializeDistanceGenerator(self) -> None:
+        """The DF storage is allocated with the world; nothing to compile."""
+
+    def GenerateDistanceField(self, stream=None) -> None:
+        self.FlushEdits(stream)
+        call("vpt_df_build", self.handle, _stream(stream))
+
+    # -- World.h:186 -----------------------------------------------------
+    def RaycastDetect(self, pos, direction) -> tuple[int, int, int, int]:
+        out = (C.c_int32 * 4)()
+        p = (C.c_float * 3)(*map(float, pos))
+        d = (C.c_float * 3)(*map(float, direction))
+        grid = np.ascontiguousarray(self.m_WorldData)
+        call("vpt_raycast_detect", grid.ctypes.data_as(C.c_void_p), self.nx, self.ny, self.nz, p, d, out)
+        return tuple(out)
+
+    # -- downloads (parity) ------------------------------------------------
+    def download_df(self, stream=None) -> np.ndarray:
+        df = np.empty((self.nz, self.ny, self.nx), np.uint8)
+        call("vpt_df_download", self.handle, df.ctypes.data_as(C.c_void_p), _stream(stream))
+        return df
+
+    def download_grid(self, stream=None) -> np.ndarray:
+        g = np.empty((self.nz, self.ny, self.nx), np.uint8)
+        call("vpt_grid_download", self.handle, g.ctypes.data_as(C.c_void_p), _stream(stream))
+        return g
+
+    def device_bytes(self) -> int:
+        n = C.c_size_t()
+        call("vpt_world_bytes", self.handle, C.byref(n))
+        return n.value
+
+    def set_sample_counter(self, counter_tensor) -> None:
+        """counter_tensor: int64 cuda tensor of two elements (DF samples,
+        traced rays), or None to stop counting."""
+        self._counter = counter_tensor
+        call("vpt_world_set_sample_counter", self.handle, _ptr(counter_tensor))
+
+    # -- tracing ---------------------------------------------------------
+    def trace_primary(self, cam: Camera, gbuf: "DeviceGBuffer", max_iter: int = 350, stream=None) -> None:
+        call("vpt_trace_primary", self.handle, C.byref(cam), max_iter, C.byref(gbuf.struct()), _stream(stream))
+
+    def trace_shadow(self, cam: Camera, gbuf: "DeviceGBuffer", sun_dir, mask, max_iter: int = 350,
+                     stream=None) -> None:
+        sd = (C.c_float * 3)(*map(float, sun_dir))
+        call("vpt_trace_shadow", self.handle, C.byref(cam), C.byref(gbuf.struct()), sd, max_iter, _ptr(mask),
+             _stream(stream))
+
+    def trace_gi(self, cam: Camera, gbuf: "DeviceGBuffer", prm, rgb, texels=None, stream=None) -> None:
+        call("vpt_trace_gi", self.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), _ptr(rgb),
+             _ptr(texels), _stream(stream))
+
+    def trace_rays(self, rays, hits, n: int | None = None, max_iter: int = 350, stream=None) -> None:
+        """rays: uint8/float cuda tensor holding vpt_ray records; hits: tensor for vpt_hit records."""
+        if n is None:
+            n = rays.numel() * rays.element_size() // C.sizeof(Ray)
+        call("vpt_trace_rays", self.handle, _ptr(rays), n, max_iter, _ptr(hits), _stream(stream))
+
+    def close(self):
+        if self.handle:
+            lib().vpt_world_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceGBuffer:
+    """InitialTraceFBO planes (t, face, block, voxel) as cuda tensors."""
+
+    def __init__(self, width: int, height: int, device="cuda"):
+        import torch
+        self.width, self.height = width, height
+        self.t = torch.empty((height, width), dtype=torch.float32, device=device)
+        self.face = torch.empty((height, width), dtype=torch.uint8, device=device)
+        self.block = torch.empty((height, width), dtype=torch.uint8, device=device)
+        self.voxel = torch.empty((height, width), dtype=torch.int32, device=device)
+
+    def struct(self) -> GBuffer:
+        return GBuffer(self.t.data_ptr(), self.face.data_ptr(), self.block.data_ptr(), self.voxel.data_ptr(),
+                       self.width, self.height)
+
+    def numpy(self) -> dict:
+        return dict(t=self.t.cpu().numpy(), face=self.face.cpu().numpy(), block=self.block.cpu().numpy(),
+                    voxel=self.voxel.cpu().numpy().view(np.uint32))
+
+
+def make_camera(pos, yaw_deg: float, pitch_deg: float, width: int, height: int, fov_deg: float = 60.0,
+                znear: float = 0.1, zfar: float = 1000.0):
+    """glm lookAt/perspective camera (FpsCamera.cpp).  Returns (Camera, view, proj)."""
+    cam = Camera()
+    view = (C.c_float * 16)()
+    proj = (C.c_float * 16)()
+    p = (C.c_float * 3)(*map(float, pos))
+    call("vpt_camera_make", p, yaw_deg, pitch_deg, fov_deg, width, height, znear, zfar, C.byref(cam), view, proj)
+    return cam, np.array(view, np.float32), np.array(proj, np.float32)
+
+
+def generate_terrain(nx: int, ny: int, nz: int, seed: int = 0x5EED0001, seed_random: int = 0x5EED0002,
+                     h_min: int = 20, h_max: int = 90, random_permille: int = 10) -> np.ndarray:
+    grid = np.empty((nz, ny, nx), np.uint8)
+    call("vpt_generate_terrain", nx, ny, nz, seed, seed_random, h_min, h_max, random_permille,
+         grid.ctypes.data_as(C.c_void_p))
+    return grid
+
+
+def gi_params(sun_tick: float = 50.0, frame: int = 0, albedo_seed: int = 0x5EED0003):
+    prm = _lib.GiParams()
+    call("vpt_gi_params_default", sun_tick, frame, albedo_seed, C.byref(prm))
+    return prm
+
+
+__all__ = ["Context", "World", "DeviceGBuffer", "make_camera", "generate_terrain", "gi_params", "Ray", "Hit"]
