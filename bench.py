@@ -35,10 +35,22 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--svgf-frames", type=int, default=30)
+    ap.add_argument("--svgf-frames", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_probe(mib: int) -> None:
+    import torch
+    n = mib * (1 << 20)
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
 
 
 def main():
@@ -141,22 +153,41 @@ def main():
     dom = max(pass_ms, key=lambda k: pass_ms[k])
     achieved = alg[dom] / (pass_ms[dom] * 1e-3) / 1e9
 
-    # ---- SVGF --------------------------------------------------------------
+    # ---- SVGF: the 64-frame sequence (camera +0.1 voxel/frame in x and
+    # +0.2 deg yaw/frame); G-buffers and 1-spp GI inputs of every frame are
+    # rendered first and stay resident, then the denoiser alone is timed.
+    nf = args.svgf_frames
+    seq = []
+    prev_vm = None
+    for f in range(nf):
+        pos = (CAM_POS[0] + 0.1 * f, CAM_POS[1], CAM_POS[2])
+        c_f, v_f, p_f = make_camera(pos, CAM_YAW + 0.2 * f, CAM_PITCH, W, H)
+        g_f = DeviceGBuffer(W, H)
+        t_f = torch.empty((H, W, 8), dtype=torch.int16, device="cuda")
+        world.trace_primary(c_f, g_f, 350)
+        world.trace_gi(c_f, g_f, gi_params(50.0, f), rgb, t_f)
+        vp_f = viewproj(*(prev_vm if prev_vm else (v_f, p_f)))
+        seq.append((c_f, vp_f, g_f, t_f))
+        prev_vm = (v_f, p_f)
+    torch.cuda.synchronize()
     sv = SVGF(ctx, W, H)
-    gb_prev = DeviceGBuffer(W, H)
-    world.trace_primary(cam, gb_prev, 350)
     den = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
-    vp = viewproj(view, proj)
-    for f in range(3):
-        sv.frame(cam, vp, gb, gb_prev, tex, den, time=f / 60.0)
+
+    def svgf_sequence():
+        sv.reset()
+        for f, (c_f, vp_f, g_f, t_f) in enumerate(seq):
+            g_prev = seq[f - 1][2] if f else g_f
+            sv.frame(c_f, vp_f, g_f, g_prev, t_f, den, time=f / 60.0)
+
+    svgf_sequence()
     torch.cuda.synchronize()
     s0, s1 = ev(), ev()
     s0.record(stream)
-    for f in range(args.svgf_frames):
-        sv.frame(cam, vp, gb, gb_prev, tex, den, time=f / 60.0)
+    svgf_sequence()
     s1.record(stream)
     torch.cuda.synchronize()
-    svgf_ms = s0.elapsed_time(s1) / args.svgf_frames
+    svgf_ms = s0.elapsed_time(s1) / nf
+    del seq
 
     result = {
         "metric": "Grays/s DDA+GI @1080p (bit-exact hits)",
@@ -184,6 +215,8 @@ def main():
                      "traffic": None, "algorithmic_bytes": alg[dom]},
     }
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(grid, gb, mask, rgb, cam, prm, args.cpu_reps)
     print(json.dumps(result), flush=True)

@@ -189,7 +189,18 @@ VPT_HD uint8_t vpt_unorm8_encode(float x)
     return (uint8_t)floorf(c * 255.0f + 0.5f);
 }
 
-VPT_HD float vpt_unorm8_decode(uint8_t c) { return (float)c / 255.0f; }
+/* c / 255 correctly rounded, as a product and one fused correction step
+ * (q = c * fl(1/255); e = c - q * 255 exactly; q + e * fl(1/255)); equal to
+ * the IEEE quotient for every byte (checked exhaustively by the tests) and
+ * four instructions instead of a full division on the GPU. */
+VPT_HD float vpt_unorm8_decode(uint8_t c)
+{
+    const float x = (float)c;
+    const float r = 0.0039215688593685627f;   /* fl(1/255) */
+    float q = x * r;
+    float e = fmaf(-q, 255.0f, x);
+    return fmaf(e, r, q);
+}
 
 /* DF byte (Manhattan distance 0..254) -> conservative step count
  * E = int(floor(M == 1 ? 1 : M * 0.57735026918)) with M read back through

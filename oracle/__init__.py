@@ -224,3 +224,67 @@ def time_plain(grid, inv_view, inv_proj, W, H, max_iter=350, reps=5):
 
 def threads() -> int:
     return int(lib().oracle_threads())
+
+
+# ---- shared numerics (array wrappers) ---------------------------------------
+
+def expf(x):
+    x = _f32(x)
+    y = np.empty_like(x)
+    lib().oracle_expf_n(_p(x), _p(y), C.c_int(x.size))
+    return y
+
+
+def sincos_2pi(u):
+    u = _f32(u)
+    s = np.empty_like(u)
+    c = np.empty_like(u)
+    lib().oracle_sincos2pi_n(_p(u), _p(s), _p(c), C.c_int(u.size))
+    return s, c
+
+
+def f32_to_f16_bits(x):
+    x = _f32(x)
+    h = np.empty(x.shape, np.uint16)
+    lib().oracle_f32_to_f16_n(_p(x), _p(h), C.c_int(x.size))
+    return h
+
+
+def f16_bits_to_f32(h):
+    h = np.ascontiguousarray(h, np.uint16)
+    x = np.empty(h.shape, np.float32)
+    lib().oracle_f16_to_f32_n(_p(h), _p(x), C.c_int(h.size))
+    return x
+
+
+def pcg_hash(v):
+    v = np.ascontiguousarray(v, np.uint32)
+    h = np.empty_like(v)
+    lib().oracle_pcg_n(_p(v), _p(h), C.c_int(v.size))
+    return h
+
+
+def df_step_table():
+    out = np.empty(256, np.int32)
+    lib().oracle_df_step_table(_p(out))
+    return out
+
+
+def unorm8_encode(x):
+    x = _f32(x)
+    c = np.empty(x.shape, np.uint8)
+    lib().oracle_unorm8_n(_p(x), _p(c), C.c_int(x.size))
+    return c
+
+
+def powi(x, e: int):
+    x = _f32(x)
+    y = np.empty_like(x)
+    lib().oracle_powi_n(_p(x), C.c_int(e), _p(y), C.c_int(x.size))
+    return y
+
+
+def unorm8_decode_table():
+    out = np.empty(256, np.float32)
+    lib().oracle_unorm8_decode_table(_p(out))
+    return out

@@ -1216,3 +1216,53 @@ OAPI void oracle_time_plain(const uint8_t* grid, int nx, int ny, int nz, const f
 
 /* OpenMP threads the oracle runs with (cores used by the CPU baseline). */
 OAPI int oracle_threads(void) { return omp_get_max_threads(); }
+
+/* ======================================================================= */
+/* Array wrappers of the shared numerics (include/vpt/vpt_numerics.h), so  */
+/* the tests can pin them against libm / numpy.                            */
+/* ======================================================================= */
+
+OAPI void oracle_expf_n(const float* x, float* y, int n)
+{
+    for (int i = 0; i < n; i++) y[i] = vpt_expf(x[i]);
+}
+
+OAPI void oracle_sincos2pi_n(const float* u, float* s, float* c, int n)
+{
+    for (int i = 0; i < n; i++) vpt_sincos_2pi(u[i], &s[i], &c[i]);
+}
+
+OAPI void oracle_f32_to_f16_n(const float* x, uint16_t* h, int n)
+{
+    for (int i = 0; i < n; i++) h[i] = vpt_f32_to_f16(x[i]);
+}
+
+OAPI void oracle_f16_to_f32_n(const uint16_t* h, float* x, int n)
+{
+    for (int i = 0; i < n; i++) x[i] = vpt_f16_to_f32(h[i]);
+}
+
+OAPI void oracle_pcg_n(const uint32_t* v, uint32_t* h, int n)
+{
+    for (int i = 0; i < n; i++) h[i] = vpt_pcg_hash(v[i]);
+}
+
+OAPI void oracle_df_step_table(int32_t* out)
+{
+    for (int k = 0; k < 256; k++) out[k] = vpt_df_step(k);
+}
+
+OAPI void oracle_unorm8_n(const float* x, uint8_t* c, int n)
+{
+    for (int i = 0; i < n; i++) c[i] = vpt_unorm8_encode(x[i]);
+}
+
+OAPI void oracle_powi_n(const float* x, int e, float* y, int n)
+{
+    for (int i = 0; i < n; i++) y[i] = vpt_powi(x[i], e);
+}
+
+OAPI void oracle_unorm8_decode_table(float* out)
+{
+    for (int k = 0; k < 256; k++) out[k] = vpt_unorm8_decode((uint8_t)k);
+}

@@ -1,0 +1,123 @@
+"""CPU tests of the C-ABI library: it loads, exports every symbol that
+include/vpt/*.h declares, and its host-only utilities behave.  No device
+compute is called here."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from voxelpathtracer_amd import _lib
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def header_symbols() -> set[str]:
+    names = set()
+    for h in (ROOT / "include" / "vpt").glob("*.h"):
+        names |= set(re.findall(r"VPT_API\s+[\w\s\*]+?\b(vpt_\w+)\s*\(", h.read_text()))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    declared = header_symbols()
+    assert len(declared) >= 30
+    missing = [n for n in sorted(declared) if not hasattr(L, n)]
+    assert not missing, missing
+    # the ctypes prototype table covers the whole header
+    assert declared <= set(_lib.exported_symbols()) | {"vpt_svgf_read_plane"}
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (vpt_\w+)", out))
+    assert declared <= exported
+
+
+def test_library_is_native_gfx950():
+    """libvpt.so carries gfx950 device code (the HIP fat binary)."""
+    data = _lib.LIB_PATH.read_bytes()
+    assert b"gfx950" in data
+    assert _lib.lib().vpt_version().decode().endswith("(gfx950)")
+
+
+def test_terrain_generator_properties():
+    from voxelpathtracer_amd import generate_terrain
+    g = generate_terrain(384, 128, 384)
+    assert g.shape == (384, 128, 384)
+    solid = g > 0
+    heights = solid.sum(axis=1)                      # (z, x) column heights
+    assert heights.min() >= 20 and heights.max() <= 90
+    # columns are solid from y = 0 up to h - 1
+    cols = solid.argmin(axis=1)
+    assert np.array_equal(cols, heights)
+    # layering (before random ids): top grass, 4 dirt, stone below -- check
+    # on a generator run without random ids
+    g0 = generate_terrain(64, 128, 64, random_permille=0)
+    h0 = (g0 > 0).sum(axis=1)
+    z, x = 10, 20
+    col = g0[z, :, x]
+    h = h0[z, x]
+    assert col[h - 1] == 1 and (col[h - 5:h - 1] == 2).all() and (col[:h - 5] == 3).all()
+    # ~1% of solid voxels re-labelled with ids 1..100
+    diff = (g != generate_terrain(384, 128, 384, random_permille=0)) & solid
+    frac = diff.sum() / solid.sum()
+    assert 0.008 < frac < 0.011, frac
+    assert g.max() <= 100
+    # deterministic
+    assert np.array_equal(g, generate_terrain(384, 128, 384))
+
+
+def test_gi_params_defaults():
+    from voxelpathtracer_amd import gi_params
+    p = gi_params(50.0, 7)
+    sun = np.array(p.sun_dir)
+    assert abs(np.linalg.norm(sun) - 1) < 1e-6
+    c, s = np.cos(np.radians(100)), np.sin(np.radians(100))
+    np.testing.assert_allclose(sun, np.array([c - s, s + c, 1]) / np.linalg.norm([c - s, s + c, 1]), rtol=1e-6)
+    assert p.frame == 7 and p.gi_iter == 48 and p.shadow_iter == 128
+    alb = np.array(p.albedo)
+    assert alb.min() >= 0.1 and alb.max() <= 0.9
+    with pytest.raises(_lib.VptError):
+        gi_params(110.0, 0)                          # sun below the horizon
+
+
+def test_host_error_paths():
+    L = _lib.lib()
+    assert L.vpt_generate_terrain(0, 8, 8, 1, 2, 0, 4, 0, None) == _lib.VPT_EINVAL
+    cam = _lib.Camera()
+    pos = (C.c_float * 3)(1.0, 2.0, 3.0)
+    assert L.vpt_camera_make(pos, 0.0, 95.0, 60.0, 64, 64, 0.1, 100.0, C.byref(cam), None, None) == _lib.VPT_EINVAL
+    assert L.vpt_camera_make(pos, 0.0, 0.0, 60.0, 0, 64, 0.1, 100.0, C.byref(cam), None, None) == _lib.VPT_EINVAL
+    assert b"bad" in L.vpt_last_error()
+    assert L.vpt_world_create(None, 64, 64, 64, None) == _lib.VPT_EINVAL
+    assert L.vpt_trace_rays(None, None, 0, 10, None, None) == _lib.VPT_EINVAL
+    assert L.vpt_svgf_frame(None, None, None, None, None, None, 0.0, None, None, None) == _lib.VPT_EINVAL
+
+
+def test_raycast_detect_host_matches_oracle():
+    import oracle
+    from tests.helpers import terrain
+    from voxelpathtracer_amd import Context  # noqa: F401
+    g = terrain(128, 64, 128, h_min=10, h_max=40)
+    rng = np.random.default_rng(2)
+    L = _lib.lib()
+    for _ in range(300):
+        p = np.array([rng.uniform(5, 120), rng.uniform(42, 50), rng.uniform(5, 120)], np.float32)
+        d = rng.normal(size=3)
+        d[1] = -abs(d[1]) - 0.3
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        out = (C.c_int32 * 4)()
+        _lib.check(L.vpt_raycast_detect(g.ctypes.data_as(C.c_void_p), 128, 64, 128,
+                                        (C.c_float * 3)(*p), (C.c_float * 3)(*d), out))
+        np.testing.assert_array_equal(np.array(out), oracle.raycast_detect(g, p, d))
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stdout

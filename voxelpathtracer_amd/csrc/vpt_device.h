@@ -232,6 +232,12 @@ __device__ inline int voxel_at(const WorldView& w, Vec3 p)
                                w.slab_stride)];
 }
 
+// Hardware binary16 conversions (v_cvt_f32_f16 is exact; v_cvt_f16_f32
+// rounds to nearest even with fp16 denormals preserved), i.e. the same
+// results as vpt_f16_to_f32 / vpt_f32_to_f16 of the shared numerics.
+__device__ inline float hw_h2f(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)h); }
+__device__ inline uint32_t hw_f2h(float x) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x); }
+
 // Blocks are dealt to the 8 XCDs round-robin by hardware; remap the block
 // index so that XCD k receives a contiguous 1/8 of the tiles, keeping
 // neighbouring tiles (which touch the same bricks) in one L2.

@@ -72,7 +72,8 @@ struct Tx {
     float ao, sky;
 };
 
-__device__ inline float h2f(uint32_t h) { return vpt_f16_to_f32((uint16_t)h); }
+__device__ inline float h2f(uint32_t h) { return vpt::hw_h2f(h); }
+__device__ inline uint32_t f2h(float x) { return vpt::hw_f2h(x); }
 
 __device__ inline Tx load_tx(const vpt_svgf_texel* p, size_t i)
 {
@@ -93,11 +94,10 @@ __device__ inline Tx load_tx(const vpt_svgf_texel* p, size_t i)
 __device__ inline void store_tx(vpt_svgf_texel* p, size_t i, const float sh[4], float co, float cg, float x, float ao,
                                 float sky)
 {
-    uint32_t w0 = (uint32_t)vpt_f32_to_f16(sh[0]) | ((uint32_t)vpt_f32_to_f16(sh[1]) << 16);
-    uint32_t w1 = (uint32_t)vpt_f32_to_f16(sh[2]) | ((uint32_t)vpt_f32_to_f16(sh[3]) << 16);
-    uint32_t w2 = (uint32_t)vpt_f32_to_f16(co) | ((uint32_t)vpt_f32_to_f16(cg) << 16);
-    uint32_t w3 = (uint32_t)vpt_f32_to_f16(x) | ((uint32_t)vpt_unorm8_encode(ao) << 16) |
-                  ((uint32_t)vpt_unorm8_encode(sky) << 24);
+    uint32_t w0 = f2h(sh[0]) | (f2h(sh[1]) << 16);
+    uint32_t w1 = f2h(sh[2]) | (f2h(sh[3]) << 16);
+    uint32_t w2 = f2h(co) | (f2h(cg) << 16);
+    uint32_t w3 = f2h(x) | ((uint32_t)vpt_unorm8_encode(ao) << 16) | ((uint32_t)vpt_unorm8_encode(sky) << 24);
     *reinterpret_cast<uint4*>(p + i) = make_uint4(w0, w1, w2, w3);
 }
 
@@ -157,12 +157,18 @@ __device__ inline Tx bil_tx(const vpt_svgf_texel* p, const Bil& f)
     return r;
 }
 
+// GL NEAREST + REPEAT texel index for coordinates inside (0, 1), where
+// floor(s * W) lies in [0, W] (W itself when s * W rounds up to W).
 __device__ inline int nearest_idx(float s, float t, int W, int H)
 {
-    int i = (int)floorf(s * (float)W), j = (int)floorf(t * (float)H);
-    i = ((i % W) + W) % W;
-    j = ((j % H) + H) % H;
+    int i = wrap1((int)floorf(s * (float)W), W), j = wrap1((int)floorf(t * (float)H), H);
     return j * W + i;
+}
+
+// The X channel (binary16 in the low half of the texel's last word).
+__device__ inline float load_x(const vpt_svgf_texel* p, size_t i)
+{
+    return h2f(reinterpret_cast<const uint32_t*>(p + i)[3] & 0xffffu);
 }
 
 // Pixel of this lane: 32x8 tiles (a wavefront covers 32x2 pixels, i.e.
@@ -369,9 +375,8 @@ __global__ void __launch_bounds__(256) temporal_kernel(CamView cam, Mat16 prev_v
     for (int c = 0; c < 4; c++) osh[c] = fclamp(osh[c], -100.0f, 100.0f);
     store_tx(out, i, osh, fclamp(oco, -10.0f, 100.0f), fclamp(ocg, -10.0f, 100.0f), 0.0f, fclamp(oao, 0.0f, 1.0f),
              fclamp(osky, 0.0f, 1.0f));
-    uint32_t u0 = (uint32_t)vpt_f32_to_f16(fclamp(util_spp, -150.0f, 150.0f)) |
-                  ((uint32_t)vpt_f32_to_f16(fclamp(util_moment, -150.0f, 150.0f)) << 16);
-    uint32_t u1 = (uint32_t)vpt_f32_to_f16(fclamp(store_luma, -150.0f, 150.0f));
+    uint32_t u0 = f2h(fclamp(util_spp, -150.0f, 150.0f)) | (f2h(fclamp(util_moment, -150.0f, 150.0f)) << 16);
+    uint32_t u1 = f2h(fclamp(store_luma, -150.0f, 150.0f));
     out_util[i] = make_uint2(u0, u1);
 }
 
@@ -416,10 +421,15 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
                 Tx s = load_tx(in, j);
                 float slum = sh_to_y(s.sh[3]);
                 float nw = vpt_powi(fmaxf(vpt::dot3(base_n, sn), 0.0f), 16);
-                float e = vpt_expf(-fabsf(sw - base_w));
-                float dw = e * e;
-                float lw = fabsf(slum - base_lum) / color_phi;
-                float w = (vpt_expf(-lw) * nw) * dw;
+                // exp() of a finite non-positive argument lies in [0, 1], so a
+                // zero normal weight makes the product exactly 0
+                float w = 0.0f;
+                if (nw != 0.0f) {
+                    float e = vpt_expf(-fabsf(sw - base_w));
+                    float dw = e * e;
+                    float lw = fabsf(slum - base_lum) / color_phi;
+                    w = (vpt_expf(-lw) * nw) * dw;
+                }
                 float w2 = w;
                 w = fmaxf(w, 0.000000015f);
                 w2 = fmaxf(w2, 0.0000000015f);
@@ -501,7 +511,7 @@ __global__ void __launch_bounds__(256) atrous_kernel(int W, int H, const float* 
             float sy = uvy + (float)y * tex_h;
             if (!in_screen(sx, sy)) continue;
             float kv = gauss[abs(x)] * gauss[abs(y)];
-            float v = (x == 0 && y == 0) ? base.x : load_tx(in, (size_t)(py + y) * W + (px + x)).x;
+            float v = (x == 0 && y == 0) ? base.x : load_x(in, (size_t)(py + y) * W + (px + x));
             if (x == 0 && y == 0) base_var = v;
             vsum = vsum + v * kv;
             ksum = ksum + kv;

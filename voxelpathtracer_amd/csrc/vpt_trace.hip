@@ -208,8 +208,8 @@ __device__ inline float fclamp(float x, float lo, float hi) { return fminf(fmaxf
 __device__ inline void store_texel(vpt_svgf_texel* dst, const float sh[6], float luma, float ao, float sky)
 {
     uint16_t h[8];
-    for (int k = 0; k < 6; k++) h[k] = vpt_f32_to_f16(sh[k]);
-    h[6] = vpt_f32_to_f16(luma);
+    for (int k = 0; k < 6; k++) h[k] = (uint16_t)vpt::hw_f2h(sh[k]);
+    h[6] = (uint16_t)vpt::hw_f2h(luma);
     h[7] = (uint16_t)(vpt_unorm8_encode(ao) | ((uint16_t)vpt_unorm8_encode(sky) << 8));
     uint4 v = make_uint4((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16),
                          (uint32_t)h[4] | ((uint32_t)h[5] << 16), (uint32_t)h[6] | ((uint32_t)h[7] << 16));
