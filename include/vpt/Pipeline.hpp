@@ -1,0 +1,136 @@
+// C++ façade for the hot-path slice of the reference's frame loop
+// (MainPipeline::StartPipeline, Core/Pipeline.cpp:1960-2852): primary trace
+// into a ping-pong G-buffer (InitialTraceFBO / InitialTraceFBOPrev,
+// :1970-1971), diffuse GI trace (:2171-2281), SVGF (:2284-2615) and the sun
+// shadow trace (:2790-2852).  Device buffers are plain hipMalloc
+// allocations owned by the pipeline.  Header-only; links against libvpt.so
+// and the HIP runtime.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <stdexcept>
+
+#include "World.hpp"
+#include "vpt.h"
+
+namespace VoxelRT {
+
+class Pipeline {
+public:
+    Pipeline(Context& ctx, World& world, int width, int height, float sun_tick = 50.0f, bool persistent = true)
+        : m_World(world), m_W(width), m_H(height), m_SunTick(sun_tick), m_Persistent(persistent)
+    {
+        const size_t px = (size_t)width * height;
+        for (auto& g : m_GBuf) {
+            Alloc(&g.d_t, px * sizeof(float));
+            Alloc(&g.d_face, px);
+            Alloc(&g.d_block, px);
+            Alloc(&g.d_voxel, px * sizeof(uint32_t));
+            g.width = width;
+            g.height = height;
+        }
+        Alloc(&m_Mask, px);
+        Alloc(&m_RGB, px * 3 * sizeof(float));
+        Alloc(&m_Denoised, px * 3 * sizeof(float));
+        Alloc(&m_Texels, px * sizeof(vpt_svgf_texel));
+        vpt_svgf_params sp;
+        vpt_check(vpt_svgf_params_default(&sp));
+        vpt_check(vpt_svgf_create(ctx.Handle(), width, height, &sp, &m_Svgf));
+        if (persistent) vpt_check(vpt_frame_create(ctx.Handle(), width, height, &m_Frame));
+    }
+
+    ~Pipeline()
+    {
+        if (m_Frame) vpt_frame_destroy(m_Frame);
+        vpt_svgf_destroy(m_Svgf);
+        for (auto& g : m_GBuf) {
+            (void)hipFree(g.d_t);
+            (void)hipFree(g.d_face);
+            (void)hipFree(g.d_block);
+            (void)hipFree(g.d_voxel);
+        }
+        (void)hipFree(m_Mask);
+        (void)hipFree(m_RGB);
+        (void)hipFree(m_Denoised);
+        (void)hipFree(m_Texels);
+    }
+
+    Pipeline(const Pipeline&) = delete;
+    Pipeline& operator=(const Pipeline&) = delete;
+
+    // One frame for a camera at `pos` looking along (yaw, pitch) degrees.
+    void RenderFrame(const float pos[3], float yaw, float pitch, void* stream = nullptr)
+    {
+        vpt_camera cam;
+        float view[16], proj[16];
+        vpt_check(vpt_camera_make(pos, yaw, pitch, 60.0f, m_W, m_H, 0.1f, 1000.0f, &cam, view, proj));
+        vpt_gbuffer& cur = m_GBuf[m_Frame_ % 2];
+        vpt_gbuffer& prev = m_Frame_ == 0 ? cur : m_GBuf[(m_Frame_ + 1) % 2];
+        if (m_Frame_ == 0) {
+            for (int i = 0; i < 16; i++) {
+                m_PrevView[i] = view[i];
+                m_PrevProj[i] = proj[i];
+            }
+        }
+        vpt_gi_params prm;
+        vpt_check(vpt_gi_params_default(m_SunTick, (uint32_t)m_Frame_, 0x5EED0003u, &prm));
+        if (m_Persistent) {
+            vpt_check(vpt_frame_trace(m_Frame, m_World.Handle(), &cam, &prm, 350, 350, &cur, m_Mask, m_RGB, m_Texels,
+                                      1, stream));
+        } else {
+            vpt_check(vpt_trace_primary(m_World.Handle(), &cam, 350, &cur, stream));
+            vpt_check(vpt_trace_gi(m_World.Handle(), &cam, &cur, &prm, m_RGB, m_Texels, stream));
+            vpt_check(vpt_trace_shadow(m_World.Handle(), &cam, &cur, prm.sun_dir, 350, m_Mask, stream));
+        }
+        float prev_vp[16];
+        ViewProj(m_PrevView, m_PrevProj, prev_vp);
+        vpt_check(vpt_svgf_frame(m_Svgf, &cam, prev_vp, &cur, &prev, m_Texels, (float)m_Frame_ / 60.0f, m_Denoised,
+                                 nullptr, stream));
+        for (int i = 0; i < 16; i++) {
+            m_PrevView[i] = view[i];
+            m_PrevProj[i] = proj[i];
+        }
+        ++m_Frame_;
+    }
+
+    const vpt_gbuffer& GBuffer() const { return m_GBuf[(m_Frame_ + 1) % 2]; }
+    const float* DenoisedRGB() const { return m_Denoised; }
+    const float* RawRGB() const { return m_RGB; }
+    const uint8_t* ShadowMask() const { return m_Mask; }
+    int FrameIndex() const { return m_Frame_; }
+
+private:
+    template <typename T>
+    static void Alloc(T** p, size_t bytes)
+    {
+        if (hipMalloc(reinterpret_cast<void**>(p), bytes) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+    }
+
+    // proj * view, column-major, summed in k order (u_PrevProjection * u_PrevView).
+    static void ViewProj(const float* v, const float* p, float* out)
+    {
+        for (int c = 0; c < 4; c++)
+            for (int r = 0; r < 4; r++) {
+                float acc = 0.0f;
+                for (int k = 0; k < 4; k++) acc = acc + p[k * 4 + r] * v[c * 4 + k];
+                out[c * 4 + r] = acc;
+            }
+    }
+
+    World& m_World;
+    int m_W, m_H;
+    float m_SunTick;
+    bool m_Persistent;
+    int m_Frame_ = 0;
+    vpt_gbuffer m_GBuf[2] = {};
+    uint8_t* m_Mask = nullptr;
+    float* m_RGB = nullptr;
+    float* m_Denoised = nullptr;
+    vpt_svgf_texel* m_Texels = nullptr;
+    vpt_svgf* m_Svgf = nullptr;
+    vpt_frame* m_Frame = nullptr;
+    float m_PrevView[16] = {}, m_PrevProj[16] = {};
+};
+
+}  // namespace VoxelRT

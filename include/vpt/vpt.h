@@ -182,6 +182,67 @@ VPT_API int vpt_trace_rays(vpt_world* w, const vpt_ray* d_rays, int n, int max_i
  * NULL to disable. */
 VPT_API int vpt_world_set_sample_counter(vpt_world* w, unsigned long long* d_counter);
 
+/* ---- batched / persistent tracing ------------------------------------------
+ * One persistent launch drains a queue of rays from any number of resident
+ * worlds: a lane whose ray has finished refills itself from a wave-private
+ * chunk of the queue, so divergent rays do not idle their wavefront.
+ * Results are bit-identical to vpt_trace_rays. */
+typedef struct vpt_batch vpt_batch;
+
+/* A queued ray: origin, unit direction, meta = max_iter (bits 0..15) |
+ * world slot (bits 16..23), dst = index of its vpt_hit in the output. */
+typedef struct vpt_qray {
+    float origin[3];
+    float dir[3];
+    uint32_t meta;
+    uint32_t dst;
+} vpt_qray;
+
+/* Launch shape of the persistent kernels (chosen by vpt_batch_autotune). */
+typedef struct vpt_launch_cfg {
+    int32_t block;           /* threads per workgroup: 64, 128 or 256 */
+    int32_t waves_per_cu;    /* resident wavefronts per CU launched */
+    int32_t chunk;           /* rays taken from the queue per atomic */
+} vpt_launch_cfg;
+
+VPT_API int vpt_batch_create(vpt_ctx* ctx, int max_worlds, vpt_batch** out);
+VPT_API int vpt_batch_destroy(vpt_batch* b);
+/* Registers (or with NULL clears) a resident world in `slot`. */
+VPT_API int vpt_batch_set_world(vpt_batch* b, int slot, vpt_world* w);
+VPT_API int vpt_batch_get_cfg(const vpt_batch* b, vpt_launch_cfg* cfg);
+VPT_API int vpt_batch_set_cfg(vpt_batch* b, const vpt_launch_cfg* cfg);
+/* Traces n queued rays in one persistent launch: d_hits[ray.dst] = result
+ * (d_hits holds n entries; a ray whose dst is outside [0, n) is traced and
+ * its result dropped; a ray naming an empty world slot misses). */
+VPT_API int vpt_batch_trace(vpt_batch* b, const vpt_qray* d_rays, int n, vpt_hit* d_hits, void* stream);
+/* Times the candidate launch shapes on this ray set and keeps the fastest;
+ * *best_ms receives its time (may be NULL). */
+VPT_API int vpt_batch_autotune(vpt_batch* b, const vpt_qray* d_rays, int n, vpt_hit* d_hits, float* best_ms,
+                               void* stream);
+
+/* The per-frame ray passes of one view through the persistent kernels:
+ * primary rays into the G-buffer, then one fused launch for the sun-shadow
+ * rays and the GI rays (each GI hit continuing as its shadow ray in the
+ * same lane), then shading.  Outputs are bit-identical to vpt_trace_primary
+ * + vpt_trace_shadow + vpt_trace_gi.  The pass chain can be captured into a
+ * HIP graph (vpt_frame_graph) and replayed per frame. */
+typedef struct vpt_frame vpt_frame;
+
+VPT_API int vpt_frame_create(vpt_ctx* ctx, int width, int height, vpt_frame** out);
+VPT_API int vpt_frame_destroy(vpt_frame* f);
+VPT_API int vpt_frame_set_cfg(vpt_frame* f, const vpt_launch_cfg* cfg);
+VPT_API int vpt_frame_get_cfg(const vpt_frame* f, vpt_launch_cfg* cfg);
+/* Runs the passes for camera `cam` (G-buffer `gb`, shadow mask, GI outputs
+ * as in vpt_trace_gi).  use_graph != 0 replays a captured HIP graph
+ * (captured on first use, re-captured when buffers change). */
+VPT_API int vpt_frame_trace(vpt_frame* f, vpt_world* w, const vpt_camera* cam, const vpt_gi_params* prm,
+                            int primary_iter, int shadow_iter, const vpt_gbuffer* gb, uint8_t* d_mask, float* d_rgb,
+                            vpt_svgf_texel* d_svgf_in, int use_graph, void* stream);
+/* Times candidate launch shapes on one frame and keeps the fastest. */
+VPT_API int vpt_frame_autotune(vpt_frame* f, vpt_world* w, const vpt_camera* cam, const vpt_gi_params* prm,
+                               int primary_iter, int shadow_iter, const vpt_gbuffer* gb, uint8_t* d_mask,
+                               float* d_rgb, vpt_svgf_texel* d_svgf_in, float* best_ms, void* stream);
+
 /* ---- SVGF ---------------------------------------------------------------
  * Owns the history and ping-pong planes of one W x H view. */
 typedef struct vpt_svgf_params {

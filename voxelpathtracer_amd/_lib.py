@@ -56,6 +56,14 @@ class Hit(C.Structure):
                 ("samples", C.c_uint16)]
 
 
+class QRay(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("dir", C.c_float * 3), ("meta", C.c_uint32), ("dst", C.c_uint32)]
+
+
+class LaunchCfg(C.Structure):
+    _fields_ = [("block", C.c_int32), ("waves_per_cu", C.c_int32), ("chunk", C.c_int32)]
+
+
 _vp = C.c_void_p
 _i = C.c_int
 _f = C.c_float
@@ -81,6 +89,21 @@ _PROTOS = {
     "vpt_trace_gi": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(GiParams), _vp, _vp, _vp]),
     "vpt_trace_rays": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "vpt_world_set_sample_counter": (_i, [_vp, _vp]),
+    "vpt_batch_create": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "vpt_batch_destroy": (_i, [_vp]),
+    "vpt_batch_set_world": (_i, [_vp, _i, _vp]),
+    "vpt_batch_get_cfg": (_i, [_vp, C.POINTER(LaunchCfg)]),
+    "vpt_batch_set_cfg": (_i, [_vp, C.POINTER(LaunchCfg)]),
+    "vpt_batch_trace": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "vpt_batch_autotune": (_i, [_vp, _vp, _i, _vp, C.POINTER(_f), _vp]),
+    "vpt_frame_create": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
+    "vpt_frame_destroy": (_i, [_vp]),
+    "vpt_frame_set_cfg": (_i, [_vp, C.POINTER(LaunchCfg)]),
+    "vpt_frame_get_cfg": (_i, [_vp, C.POINTER(LaunchCfg)]),
+    "vpt_frame_trace": (_i, [_vp, _vp, C.POINTER(Camera), C.POINTER(GiParams), _i, _i, C.POINTER(GBuffer), _vp, _vp,
+                             _vp, _i, _vp]),
+    "vpt_frame_autotune": (_i, [_vp, _vp, C.POINTER(Camera), C.POINTER(GiParams), _i, _i, C.POINTER(GBuffer), _vp,
+                                _vp, _vp, C.POINTER(_f), _vp]),
     "vpt_svgf_create": (_i, [_vp, _i, _i, C.POINTER(SvgfParams), C.POINTER(_vp)]),
     "vpt_svgf_destroy": (_i, [_vp]),
     "vpt_svgf_reset": (_i, [_vp, _vp]),

@@ -46,7 +46,7 @@ DEVICE = [
     "-packed-fp32-ops",
 ]
 
-SOURCES = ["vpt_host.cpp", "vpt_world.hip", "vpt_trace.hip", "vpt_svgf.hip"]
+SOURCES = ["vpt_host.cpp", "vpt_world.hip", "vpt_trace.hip", "vpt_batch.hip", "vpt_svgf.hip"]
 
 
 def _compile(src: str) -> Path:
@@ -75,7 +75,24 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_demo(verbose)
     return LIB
+
+
+DEMO = OUT / "vpt_frame_demo"
+
+
+def build_demo(verbose: bool = True) -> Path:
+    """The C++ host example over the World / Pipeline façades."""
+    cmd = [HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}",
+           str(CSRC / "examples" / "vpt_frame_demo.cpp"), f"-L{OUT}", "-lvpt", "-Wl,-rpath,$ORIGIN",
+           "-o", str(DEMO)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"demo build failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {DEMO}")
+    return DEMO
 
 
 def build_oracle(verbose: bool = True) -> Path:

@@ -139,88 +139,143 @@ __device__ inline uint32_t pack_cell(int x, int y, int z)
 }
 
 // VoxelTraversalDF (InitialRayTraceFrag.glsl:307-374; identical copies in
-// ShadowRayTraceFrag.glsl:222-289 and DiffuseRayTraceFrag.glsl:1043-1110).
-__device__ inline Hit traverse(const WorldView& w, Vec3 o, Vec3 d, int max_iter)
+// ShadowRayTraceFrag.glsl:222-289 and DiffuseRayTraceFrag.glsl:1043-1110),
+// split into resumable pieces so that a persistent kernel can interleave
+// several rays per lane:
+//   trav_init     the loop's set-up,
+//   trav_prepare  the head of one iteration (iteration cap, IsInVolume, the
+//                 DF address); false = the loop has ended,
+//   trav_advance  the rest of the iteration once the DF byte has arrived;
+//                 false = the loop has ended,
+//   trav_finish   the code after the loop.
+// The per-ray state is the reference's: origin, direction, the reciprocal
+// direction, initial origin, MinIdx, Intersection.
+struct Trav {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float ix, iy, iz;
+    float o0x, o0y, o0z;
+    int left;       // iterations still allowed
+    int min_idx;
+    int inter;
+    int samples;
+};
+
+__device__ inline int sgn(float v) { return v > 0.0f ? 1 : (v < 0.0f ? -1 : 0); }
+
+__device__ __attribute__((always_inline)) inline void trav_init(Trav& r, Vec3 o, Vec3 d, int max_iter)
 {
-    const float ox0 = o.x, oy0 = o.y, oz0 = o.z;
-    const int sx = d.x > 0.0f ? 1 : (d.x < 0.0f ? -1 : 0);
-    const int sy = d.y > 0.0f ? 1 : (d.y < 0.0f ? -1 : 0);
-    const int sz = d.z > 0.0f ? 1 : (d.z < 0.0f ? -1 : 0);
-    const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
-    const int hx = (1 + sx) >> 1, hy = (1 + sy) >> 1, hz = (1 + sz) >> 1;
-    bool inter = false;
-    int min_idx = 0;
-    int samples = 0;
-    for (int itr = 0; itr < max_iter; itr++) {
-        float fx = floorf(o.x), fy = floorf(o.y), fz = floorf(o.z);
-        if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax)) {
-            inter = false;
-            break;
-        }
-        uint32_t off = brick_offset((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, w.row_stride,
-                                    w.slab_stride);
-        int m = w.df[off];
-        samples++;
-        int e = df_step(m);
-        if (e == 0) break;
-        if (e == 1) {
-            int gx = (int)o.x, gy = (int)o.y, gz = (int)o.z;
-            float wx = o.x - (float)gx, wy = o.y - (float)gy, wz = o.z - (float)gz;
-            float fxx = ((float)hx - wx) * ix;
-            float fyy = ((float)hy - wy) * iy;
-            float fzz = ((float)hz - wz) * iz;
-            min_idx = (fxx < fyy && sx != 0) ? ((fxx < fzz || sz == 0) ? 0 : 2)
-                                             : ((fyy < fzz || sz == 0) ? 1 : 2);
-            float fm = min_idx == 0 ? fxx : (min_idx == 1 ? fyy : fzz);
-            wx = wx + d.x * fm;
-            wy = wy + d.y * fm;
-            wz = wz + d.z * fm;
-            if (min_idx == 0) {
-                gx += sx;
-                wx = (float)(1 - hx);
-            } else if (min_idx == 1) {
-                gy += sy;
-                wy = (float)(1 - hy);
-            } else {
-                gz += sz;
-                wz = (float)(1 - hz);
-            }
-            o.x = (float)gx + wx;
-            o.y = (float)gy + wy;
-            o.z = (float)gz + wz;
-            if (min_idx == 0) o.x = o.x + (float)sx * 0.0001f;
-            else if (min_idx == 1) o.y = o.y + (float)sy * 0.0001f;
-            else o.z = o.z + (float)sz * 0.0001f;
-            inter = true;
-        } else {
-            float k = (float)(e - 1);
-            o.x = o.x + k * d.x;
-            o.y = o.y + k * d.y;
-            o.z = o.z + k * d.z;
-        }
+    r.ox = o.x;
+    r.oy = o.y;
+    r.oz = o.z;
+    r.dx = d.x;
+    r.dy = d.y;
+    r.dz = d.z;
+    r.ix = 1.0f / d.x;
+    r.iy = 1.0f / d.y;
+    r.iz = 1.0f / d.z;
+    r.o0x = o.x;
+    r.o0y = o.y;
+    r.o0z = o.z;
+    r.left = max_iter;
+    r.min_idx = 0;
+    r.inter = 0;
+    r.samples = 0;
+}
+
+__device__ __attribute__((always_inline)) inline bool trav_prepare(const WorldView& w, Trav& r, uint32_t& off)
+{
+    if (r.left <= 0) return false;
+    float fx = floorf(r.ox), fy = floorf(r.oy), fz = floorf(r.oz);
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax)) {
+        r.inter = 0;
+        return false;
     }
+    off = brick_offset((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, w.row_stride, w.slab_stride);
+    return true;
+}
+
+__device__ __attribute__((always_inline)) inline bool trav_advance(Trav& r, int m)
+{
+    r.samples++;
+    int e = df_step(m);
+    if (e == 0) return false;
+    if (e == 1) {
+        const int sx = sgn(r.dx), sy = sgn(r.dy), sz = sgn(r.dz);
+        const int hx = (1 + sx) >> 1, hy = (1 + sy) >> 1, hz = (1 + sz) >> 1;
+        int gx = (int)r.ox, gy = (int)r.oy, gz = (int)r.oz;
+        float wx = r.ox - (float)gx, wy = r.oy - (float)gy, wz = r.oz - (float)gz;
+        float fxx = ((float)hx - wx) * r.ix;
+        float fyy = ((float)hy - wy) * r.iy;
+        float fzz = ((float)hz - wz) * r.iz;
+        int mi = (fxx < fyy && sx != 0) ? ((fxx < fzz || sz == 0) ? 0 : 2) : ((fyy < fzz || sz == 0) ? 1 : 2);
+        float fm = mi == 0 ? fxx : (mi == 1 ? fyy : fzz);
+        wx = wx + r.dx * fm;
+        wy = wy + r.dy * fm;
+        wz = wz + r.dz * fm;
+        if (mi == 0) {
+            gx += sx;
+            wx = (float)(1 - hx);
+        } else if (mi == 1) {
+            gy += sy;
+            wy = (float)(1 - hy);
+        } else {
+            gz += sz;
+            wz = (float)(1 - hz);
+        }
+        r.ox = (float)gx + wx;
+        r.oy = (float)gy + wy;
+        r.oz = (float)gz + wz;
+        if (mi == 0) r.ox = r.ox + (float)sx * 0.0001f;
+        else if (mi == 1) r.oy = r.oy + (float)sy * 0.0001f;
+        else r.oz = r.oz + (float)sz * 0.0001f;
+        r.min_idx = mi;
+        r.inter = 1;
+    } else {
+        float k = (float)(e - 1);
+        r.ox = r.ox + k * r.dx;
+        r.oy = r.oy + k * r.dy;
+        r.oz = r.oz + k * r.dz;
+    }
+    r.left--;
+    return r.left > 0;
+}
+
+__device__ __attribute__((always_inline)) inline Hit trav_finish(const WorldView& w, const Trav& r)
+{
     Hit h;
     h.t = -1.0f;
     h.face = 255;
     h.block = 0;
     h.voxel = 0xffffffffu;
-    h.samples = samples;
-    if (inter) {
-        float fx = floorf(o.x), fy = floorf(o.y), fz = floorf(o.z);
+    h.samples = r.samples;
+    if (r.inter) {
+        float fx = floorf(r.ox), fy = floorf(r.oy), fz = floorf(r.oz);
         if (fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax) {
             int cx = (int)fx, cy = (int)fy, cz = (int)fz;
             int b = w.grid[brick_offset(cx, cy, cz, w.row_stride, w.slab_stride)];
             if (b > 0) {
-                float dx = o.x - ox0, dy = o.y - oy0, dz = o.z - oz0;
+                float dx = r.ox - r.o0x, dy = r.oy - r.o0y, dz = r.oz - r.o0z;
                 h.t = sqrtf((dx * dx + dy * dy) + dz * dz);
-                int s = min_idx == 0 ? sx : (min_idx == 1 ? sy : sz);
-                h.face = face_from_axis(min_idx, s);
+                int s = r.min_idx == 0 ? sgn(r.dx) : (r.min_idx == 1 ? sgn(r.dy) : sgn(r.dz));
+                h.face = face_from_axis(r.min_idx, s);
                 h.block = b;
                 h.voxel = pack_cell(cx, cy, cz);
             }
         }
     }
     return h;
+}
+
+__device__ inline Hit traverse(const WorldView& w, Vec3 o, Vec3 d, int max_iter)
+{
+    Trav r;
+    trav_init(r, o, d, max_iter);
+    uint32_t off;
+    while (trav_prepare(w, r, off)) {
+        if (!trav_advance(r, w.df[off])) break;
+    }
+    return trav_finish(w, r);
 }
 
 // GetVoxel(ivec3(floor(p))) (InitialRayTraceFrag.glsl:79-87).
