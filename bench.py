@@ -170,7 +170,8 @@ def main():
     grays_persist = persist_counts["rays"] / (ms_persist * 1e-3) / 1e9
     cfg = fr.cfg
     persistent = {"ms_per_frame": round(ms_persist, 4), "grays": round(grays_persist, 4), "work": persist_counts,
-                  "launch": {"block": cfg.block, "waves_per_cu": cfg.waves_per_cu, "chunk": cfg.chunk},
+                  "launch": {"block": cfg.block, "waves_per_cu": cfg.waves_per_cu, "chunk": cfg.chunk,
+                             "refill": cfg.refill, "tile_primary": bool(cfg.flags & 1)},
                   "autotune_best_ms": round(tuned_ms, 4)}
     if ms_persist < ms_plain:
         ms_per_step, grays, path = ms_persist, grays_persist, "persistent"

@@ -204,5 +204,7 @@ def test_batch_rejects_bad_arguments(ctx):
     with pytest.raises(VptError):
         b.set_world(2, None)
     with pytest.raises(VptError):
-        b.cfg = LaunchCfg(100, 8, 64)
+        b.cfg = LaunchCfg(100, 8, 64, 32, 0)
+    with pytest.raises(VptError):
+        b.cfg = LaunchCfg(256, 8, 64, 0, 0)
     b.close()

@@ -61,7 +61,8 @@ class QRay(C.Structure):
 
 
 class LaunchCfg(C.Structure):
-    _fields_ = [("block", C.c_int32), ("waves_per_cu", C.c_int32), ("chunk", C.c_int32)]
+    _fields_ = [("block", C.c_int32), ("waves_per_cu", C.c_int32), ("chunk", C.c_int32), ("refill", C.c_int32),
+                ("flags", C.c_int32)]
 
 
 _vp = C.c_void_p

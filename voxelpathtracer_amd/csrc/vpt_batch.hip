@@ -97,17 +97,23 @@ struct Cursor {
     bool exhausted;
 };
 
-// Refills the idle lanes of one slot from the wave's chunk, taking a new
-// chunk (one atomicAdd) when the current one is used up.
+// Refills the idle lanes from the wave's chunk, taking a new chunk (one
+// atomicAdd) when the current one is used up.  Runs only once at least
+// `refill_min` lanes are idle (or all of them), so the set-up cost of new
+// rays is paid for a group of lanes at a time.
 template <bool MULTI, int SRC>
-__device__ __attribute__((always_inline)) inline void refill_slot(Slot& s, WorldView& wv, const WorldView& w0, const WorldView* __restrict__ worlds,
-                                   uint32_t nworlds, const vpt_qray* __restrict__ rays, uint32_t n,
-                                   uint32_t* __restrict__ head, uint32_t chunk, const FrameConst* __restrict__ fc,
-                                   int tiles_x, Cursor& c, unsigned long long& n_rays)
+__device__ __attribute__((always_inline)) inline void refill_slot(Slot& s, WorldView& wv, const WorldView& w0,
+                                                                  const WorldView* __restrict__ worlds,
+                                                                  uint32_t nworlds, const vpt_qray* __restrict__ rays,
+                                                                  uint32_t n, uint32_t* __restrict__ head,
+                                                                  uint32_t chunk, uint32_t refill_min,
+                                                                  const FrameConst* __restrict__ fc, int tiles_x,
+                                                                  Cursor& c, unsigned long long& n_rays)
 {
     const int lane = threadIdx.x & 63;
     bool need = !s.active;
     uint64_t idle = __ballot(need);
+    if (c.exhausted || ((uint32_t)__popcll(idle) < refill_min && idle != __ballot(true))) return;
     while (idle != 0ull && !c.exhausted) {
         if (c.next >= c.end) {
             uint32_t base = 0;
@@ -219,7 +225,8 @@ template <bool MULTI, int SRC>
 __global__ void __launch_bounds__(256) persistent_kernel(WorldView w0, const WorldView* __restrict__ worlds,
                                                          uint32_t nworlds, const vpt_qray* __restrict__ rays,
                                                          const uint32_t* __restrict__ d_n, uint32_t n_host,
-                                                         uint32_t* __restrict__ head, uint32_t chunk, Outputs out,
+                                                         uint32_t* __restrict__ head, uint32_t chunk,
+                                                         uint32_t refill_min, Outputs out,
                                                          const FrameConst* __restrict__ fc,
                                                          unsigned long long* __restrict__ counter)
 {
@@ -235,7 +242,8 @@ __global__ void __launch_bounds__(256) persistent_kernel(WorldView w0, const Wor
     const int tiles_x = SRC == 1 ? (fc->cam.W + 7) / 8 : 1;
 
     while (true) {
-        refill_slot<MULTI, SRC>(s, wv, w0, worlds, nworlds, rays, n, head, chunk, fc, tiles_x, c, n_rays);
+        refill_slot<MULTI, SRC>(s, wv, w0, worlds, nworlds, rays, n, head, chunk, refill_min, fc, tiles_x, c,
+                                n_rays);
         if (!__any(s.active)) {
             if (c.exhausted) break;
             continue;
@@ -376,20 +384,53 @@ int cu_count(int device)
 bool cfg_valid(const vpt_launch_cfg& c)
 {
     return (c.block == 64 || c.block == 128 || c.block == 256) && c.waves_per_cu >= 1 && c.waves_per_cu <= 64 &&
-           c.chunk >= 1 && c.chunk <= 65536;
+           c.chunk >= 1 && c.chunk <= 65536 && c.refill >= 1 && c.refill <= 64 && (c.flags & ~1) == 0;
 }
 
-vpt_launch_cfg default_cfg() { return vpt_launch_cfg{256, 16, 256}; }
+vpt_launch_cfg default_cfg() { return vpt_launch_cfg{256, 16, 256, 32, 1}; }
 
 // The autotuner's search space: workgroup size x resident wavefronts per CU
-// x queue chunk per atomic.
-std::vector<vpt_launch_cfg> candidates()
+// x queue chunk per atomic x refill threshold (x primary-ray kernel for the
+// frame path).
+std::vector<vpt_launch_cfg> candidates(bool frame)
 {
     std::vector<vpt_launch_cfg> c;
-    for (int b : {64, 256})
-        for (int wpc : {8, 16, 24, 32})
-            for (int ch : {64, 256, 1024}) c.push_back(vpt_launch_cfg{b, wpc, ch});
+    for (int fl : {0, 1}) {
+        if (fl && !frame) continue;
+        for (int b : {64, 256})
+            for (int wpc : {8, 16, 32})
+                for (int ch : {128, 1024})
+                    for (int rf : {8, 32, 64}) c.push_back(vpt_launch_cfg{b, wpc, ch, rf, fl});
+    }
     return c;
+}
+
+// Primary rays by the one-launch tile kernel (16x16 tiles of four 8x8
+// wavefront squares) with the camera read from the frame constants.
+__global__ void __launch_bounds__(256) frame_primary_kernel(WorldView w, const FrameConst* __restrict__ fc, Outputs out,
+                                                            unsigned long long* __restrict__ counter)
+{
+    const CamView& cam = fc->cam;
+    const uint32_t tiles_x = (uint32_t)(cam.W + 15) / 16u;
+    const uint32_t tiles_y = (uint32_t)(cam.H + 15) / 16u;
+    const uint32_t tile = vpt::xcd_remap(blockIdx.x, tiles_x * tiles_y);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    unsigned long long samples = 0, rays = 0;
+    if (px < cam.W && py < cam.H) {
+        Vec3 dir = vpt::primary_dir(cam, px, py);
+        vpt::Hit h = vpt::traverse(w, vpt::mk3(cam.iv[12], cam.iv[13], cam.iv[14]), dir, fc->primary_iter);
+        const uint32_t i = (uint32_t)py * (uint32_t)cam.W + (uint32_t)px;
+        bool intersect = h.t > 0.0f && h.block > 0;
+        out.gb_t[i] = h.t;
+        out.gb_face[i] = (uint8_t)(intersect ? h.face : 255);
+        out.gb_block[i] = (uint8_t)(intersect ? h.block : 0);
+        if (out.gb_voxel) out.gb_voxel[i] = intersect ? h.voxel : 0xffffffffu;
+        samples = (unsigned long long)h.samples;
+        rays = 1;
+    }
+    wave_count(counter, samples, rays);
 }
 
 // Launches the persistent kernel variant for (MULTI, SRC) with shape cfg.
@@ -401,7 +442,7 @@ int launch_persistent(const vpt_launch_cfg& cfg, int cus, hipStream_t s, WorldVi
     const int waves_per_block = cfg.block / 64;
     const int blocks = std::max(1, cfg.waves_per_cu * cus / waves_per_block);
     hipLaunchKernelGGL((persistent_kernel<MULTI, SRC>), dim3(blocks), dim3(cfg.block), 0, s, w0, worlds, nworlds,
-                       rays, d_n, n_host, head, (uint32_t)cfg.chunk, out, fc, counter);
+                       rays, d_n, n_host, head, (uint32_t)cfg.chunk, (uint32_t)cfg.refill, out, fc, counter);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }
@@ -534,7 +575,7 @@ extern "C" VPT_API int vpt_batch_autotune(vpt_batch* b, const vpt_qray* d_rays, 
     HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
     vpt_launch_cfg best_cfg = b->cfg;
-    for (const vpt_launch_cfg& c : candidates()) {
+    for (const vpt_launch_cfg& c : candidates(false)) {
         if (int rc = batch_launch(b, c, d_rays, n, d_hits, s)) return rc;  // warm-up
         HIP_TRY(hipEventRecord(e0, s));
         for (int r = 0; r < 3; r++)
@@ -661,11 +702,16 @@ static int frame_enqueue(vpt_frame* f, const vpt_launch_cfg& cfg, vpt_world* w, 
     out.gb_face = gb->d_face;
     out.gb_block = gb->d_block;
     out.gb_voxel = gb->d_voxel;
-    const uint32_t n_primary = (uint32_t)(((f->W + 7) / 8) * ((f->H + 7) / 8) * 64);
-    if (int rc = launch_persistent<false, 1>(cfg, f->cus, s, wv, nullptr, 0, nullptr, nullptr, n_primary,
-                                             f->d_counters + 0, out, f->d_fc, w->d_counter))
-        return rc;
     const dim3 tiles((unsigned)(((f->W + 15) / 16) * ((f->H + 15) / 16)));
+    if (cfg.flags & 1) {
+        hipLaunchKernelGGL(frame_primary_kernel, tiles, dim3(256), 0, s, wv, f->d_fc, out, w->d_counter);
+        HIP_TRY(hipGetLastError());
+    } else {
+        const uint32_t n_primary = (uint32_t)(((f->W + 7) / 8) * ((f->H + 7) / 8) * 64);
+        if (int rc = launch_persistent<false, 1>(cfg, f->cus, s, wv, nullptr, 0, nullptr, nullptr, n_primary,
+                                                 f->d_counters + 0, out, f->d_fc, w->d_counter))
+            return rc;
+    }
     hipLaunchKernelGGL(gen_kernel, tiles, dim3(256), 0, s, wv, f->d_fc, gb->d_t, gb->d_face, d_mask, f->d_queue,
                        f->d_counters + 2);
     HIP_TRY(hipGetLastError());
@@ -781,7 +827,7 @@ extern "C" VPT_API int vpt_frame_autotune(vpt_frame* f, vpt_world* w, const vpt_
     HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
     vpt_launch_cfg best_cfg = f->cfg;
-    for (const vpt_launch_cfg& c : candidates()) {
+    for (const vpt_launch_cfg& c : candidates(true)) {
         if (int rc = frame_enqueue(f, c, w, gb, d_mask, d_rgb, d_svgf_in, s)) return rc;
         HIP_TRY(hipEventRecord(e0, s));
         for (int r = 0; r < 3; r++)

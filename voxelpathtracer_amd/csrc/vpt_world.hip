@@ -114,7 +114,36 @@ constexpr int kDfCap = 254;
 
 // X then Y pass over one z-slab held in LDS (slabs of at most kSlabMax
 // bytes; larger slabs use df_x_kernel + df_y_kernel).
-constexpr int kSlabMax = 64 * 1024;
+//
+// X pass: the reference's forward/backward sweep along a row equals
+// min(cap, distance to the nearest solid cell of the row), which one
+// wavefront computes per row with two scans: each lane owns 4 consecutive
+// bytes, a wave prefix-max gives the last solid position at or before every
+// byte and a wave suffix-min the next solid position at or after it.
+// Y pass: one lane per column, forward and backward relaxation with the
+// running value in a register (LDS reads independent of it).
+constexpr int kSlabMax = 128 * 1024;
+constexpr int kNone = 1 << 20;
+
+__device__ inline int wave_incl_max(int v)
+{
+    const int lane = threadIdx.x & 63;
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(v, d);
+        if (lane >= d) v = max(v, t);
+    }
+    return v;
+}
+
+__device__ inline int wave_incl_min_rev(int v)
+{
+    const int lane = threadIdx.x & 63;
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_down(v, d);
+        if (lane + d < 64) v = min(v, t);
+    }
+    return v;
+}
 
 __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restrict__ grid, uint8_t* __restrict__ out,
                                                          int nx, int ny, int nz)
@@ -123,43 +152,83 @@ __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restri
     const int z = blockIdx.x;
     const size_t base = (size_t)z * nx * ny;
     const int bytes = nx * ny;
-    // stage
     for (int i = threadIdx.x * 16; i < bytes; i += blockDim.x * 16)
         *reinterpret_cast<uint4*>(slab + i) = *reinterpret_cast<const uint4*>(grid + base + i);
     __syncthreads();
     const int cap = min(kDfCap, nx + ny + nz);
-    // X pass (ManhattanDistanceX.comp:53-68): one lane per row
-    for (int y = threadIdx.x; y < ny; y += blockDim.x) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    const int chunks = (nx + 255) / 256;   // 256 bytes (64 lanes x 4) per chunk, nx <= 1024
+    // ---- X pass, one wavefront per row ----------------------------------
+    for (int y = wave; y < ny; y += waves) {
         uint8_t* row = slab + y * nx;
-        int prev = row[0] > 0 ? 0 : cap;
-        row[0] = (uint8_t)prev;
-        for (int x = 1; x < nx; x++) {
-            int v = row[x] > 0 ? 0 : min(cap, 1 + prev);
-            row[x] = (uint8_t)v;
-            prev = v;
+        int fwd[4][4];
+        int carry = -kNone;   // last solid position left of the chunk
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (c >= chunks) break;
+            const int x0 = c * 256 + lane * 4;
+            uint32_t v = x0 < nx ? *reinterpret_cast<const uint32_t*>(row + x0) : 0u;
+            int last_local = -kNone;
+            int lastpos[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if ((v >> (8 * k)) & 0xffu) last_local = x0 + k;
+                lastpos[k] = last_local;
+            }
+            int incl = wave_incl_max(last_local);
+            int excl = __shfl_up(incl, 1);
+            if (lane == 0) excl = -kNone;
+            excl = max(excl, carry);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int l = max(lastpos[k], excl);
+                fwd[c][k] = l < 0 ? kNone : x0 + k - l;
+            }
+            carry = max(carry, __shfl(incl, 63));
         }
-        int next = row[nx - 1];
-        for (int x = nx - 2; x >= 0; x--) {
-            int v = row[x];
-            if (next < v) v = 1 + next;
-            row[x] = (uint8_t)v;
-            next = v;
+        int carry_r = kNone;  // next solid position right of the chunk
+#pragma unroll
+        for (int c = 3; c >= 0; c--) {
+            if (c >= chunks) continue;
+            const int x0 = c * 256 + lane * 4;
+            uint32_t v = x0 < nx ? *reinterpret_cast<const uint32_t*>(row + x0) : 0u;
+            int next_local = kNone;
+            int nextpos[4];
+#pragma unroll
+            for (int k = 3; k >= 0; k--) {
+                if ((v >> (8 * k)) & 0xffu) next_local = x0 + k;
+                nextpos[k] = next_local;
+            }
+            int incl = wave_incl_min_rev(next_local);
+            int excl = __shfl_down(incl, 1);
+            if (lane == 63) excl = kNone;
+            excl = min(excl, carry_r);
+            uint32_t outw = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int nx_pos = min(nextpos[k], excl);
+                int bwd = nx_pos >= kNone ? kNone : nx_pos - (x0 + k);
+                int d = min(cap, min(fwd[c][k], bwd));
+                outw |= (uint32_t)d << (8 * k);
+            }
+            if (x0 < nx) *reinterpret_cast<uint32_t*>(row + x0) = outw;
+            carry_r = min(carry_r, __shfl(incl, 0));
         }
     }
     __syncthreads();
-    // Y pass (ManhattanDistanceY.comp:103-118): one lane per column
+    // ---- Y pass (ManhattanDistanceY.comp:103-118), one lane per column ---
     for (int x = threadIdx.x; x < nx; x += blockDim.x) {
         int prev = slab[x];
+#pragma unroll 8
         for (int y = 1; y < ny; y++) {
-            int v = slab[y * nx + x];
-            if (prev < v) v = 1 + prev;
+            int v = min((int)slab[y * nx + x], prev + 1);
             slab[y * nx + x] = (uint8_t)v;
             prev = v;
         }
-        int next = slab[(ny - 1) * nx + x];
+        int next = prev;
+#pragma unroll 8
         for (int y = ny - 2; y >= 0; y--) {
-            int v = slab[y * nx + x];
-            if (next < v) v = 1 + next;
+            int v = min((int)slab[y * nx + x], next + 1);
             slab[y * nx + x] = (uint8_t)v;
             next = v;
         }
@@ -167,6 +236,69 @@ __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restri
     __syncthreads();
     for (int i = threadIdx.x * 16; i < bytes; i += blockDim.x * 16)
         *reinterpret_cast<uint4*>(out + base + i) = *reinterpret_cast<const uint4*>(slab + i);
+}
+
+// Z pass over a 32 x 8 x nz column tile held in LDS (tiles of at most
+// kZTileMax bytes; larger ones use df_z_kernel): one lane per (x, y)
+// column, forward and backward relaxation along z (ManhattanDistanceZ.comp:
+// 151-165) with the running value in a register, then the tile -- four
+// whole 8x8 bricks wide and one brick high -- is written to the bricked DF
+// with 16-byte stores.
+constexpr int kZTileMax = 128 * 1024;
+
+__global__ void __launch_bounds__(256) df_z_tile_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ df_brick,
+                                                        int nx, int ny, int nz, uint32_t row_stride,
+                                                        uint32_t slab_stride)
+{
+    extern __shared__ uint8_t tile[];   // [z][y 0..7][x 0..31]
+    const int tiles_x = nx / 32;
+    const int x0 = (blockIdx.x % tiles_x) * 32, y0 = (blockIdx.x / tiles_x) * 8;
+    const size_t plane = (size_t)nx * ny;
+    // stage: 8 rows of 32 bytes per z, two 16-byte loads per row
+    for (int i = threadIdx.x; i < nz * 16; i += blockDim.x) {
+        const int z = i >> 4, r = (i >> 1) & 7, h = i & 1;
+        *reinterpret_cast<uint4*>(tile + z * 256 + r * 32 + h * 16) =
+            *reinterpret_cast<const uint4*>(in + (size_t)z * plane + (size_t)(y0 + r) * nx + x0 + h * 16);
+    }
+    __syncthreads();
+    {
+        const int c = threadIdx.x;   // column: y = c / 32, x = c % 32
+        int prev = tile[c];
+#pragma unroll 8
+        for (int z = 1; z < nz; z++) {
+            int v = min((int)tile[z * 256 + c], prev + 1);
+            tile[z * 256 + c] = (uint8_t)v;
+            prev = v;
+        }
+        int next = prev;
+#pragma unroll 8
+        for (int z = nz - 2; z >= 0; z--) {
+            int v = min((int)tile[z * 256 + c], next + 1);
+            tile[z * 256 + c] = (uint8_t)v;
+            next = v;
+        }
+    }
+    __syncthreads();
+    // write whole bricks: brick (bx, bz) of this tile, 512 bytes each, one
+    // 16-byte group per lane: bytes m..m+15 of the Morton order
+    const int nbz = nz / 8;
+    for (int i = threadIdx.x; i < 4 * nbz * 32; i += blockDim.x) {
+        const int g = i & 31, brick = i >> 5;       // 32 groups of 16 bytes per brick
+        const int bxl = brick & 3, bz = brick >> 2;
+        uint32_t words[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t m = (uint32_t)(g * 16 + k);   // Morton index in the brick
+            const uint32_t lx = (m & 1u) | ((m >> 2) & 2u) | ((m >> 4) & 4u);
+            const uint32_t ly = ((m >> 1) & 1u) | ((m >> 3) & 2u) | ((m >> 5) & 4u);
+            const uint32_t lz = ((m >> 2) & 1u) | ((m >> 4) & 2u) | ((m >> 6) & 4u);
+            const uint32_t b = tile[(bz * 8 + lz) * 256 + ly * 32 + bxl * 8 + lx];
+            words[k >> 2] |= b << (8 * (k & 3));
+        }
+        const size_t dst = (size_t)((x0 >> 3) + bxl) * vpt::kBrickBytes + (size_t)(y0 >> 3) * row_stride +
+                           (size_t)bz * slab_stride + (size_t)g * 16;
+        *reinterpret_cast<uint4*>(df_brick + dst) = make_uint4(words[0], words[1], words[2], words[3]);
+    }
 }
 
 // X pass in global memory (slabs too large for LDS): one lane per row.
@@ -423,6 +555,8 @@ extern "C" VPT_API int vpt_df_build(vpt_world* w, void* stream)
     const int nx = w->nx, ny = w->ny, nz = w->nz;
     const size_t slab = (size_t)nx * ny;
     if (slab <= (size_t)kSlabMax) {
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_xy_slab_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kSlabMax));
         hipLaunchKernelGGL(df_xy_slab_kernel, dim3(nz), dim3(512), slab, s, w->d_grid_lin, w->d_tmp, nx, ny, nz);
         HIP_TRY(hipGetLastError());
     } else {
@@ -434,10 +568,18 @@ extern "C" VPT_API int vpt_df_build(vpt_world* w, void* stream)
         hipLaunchKernelGGL(df_y_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, w->d_tmp, nx, ny, nz);
         HIP_TRY(hipGetLastError());
     }
-    size_t cols = (size_t)nx * ny;
     uint32_t rs = (uint32_t)w->nbx * vpt::kBrickBytes, ss = rs * (uint32_t)w->nby;
-    hipLaunchKernelGGL(df_z_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, w->d_tmp, w->d_df, nx, ny,
-                       nz, rs, ss);
+    const size_t ztile = (size_t)nz * 256;
+    if (nx % 32 == 0 && ztile <= (size_t)kZTileMax) {
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_z_tile_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kZTileMax));
+        hipLaunchKernelGGL(df_z_tile_kernel, dim3((unsigned)((nx / 32) * (ny / 8))), dim3(256), ztile, s, w->d_tmp,
+                           w->d_df, nx, ny, nz, rs, ss);
+    } else {
+        size_t cols = (size_t)nx * ny;
+        hipLaunchKernelGGL(df_z_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, w->d_tmp, w->d_df, nx,
+                           ny, nz, rs, ss);
+    }
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }
