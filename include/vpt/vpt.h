@@ -49,6 +49,8 @@ enum {
     VPT_ENOMEM = -2,     /* device allocation failed / budget exceeded */
     VPT_EDEVICE = -3,    /* HIP runtime error */
     VPT_ESTATE = -4,     /* object used in the wrong state */
+    VPT_ETIMEOUT = -5,   /* a wait ended before the work completed */
+    VPT_EBUSY = -6,      /* the object is still in use */
 };
 
 typedef struct vpt_ctx vpt_ctx;
@@ -176,6 +178,16 @@ VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer*
 /* Ray queries (serving API): n rays, traversal cap max_iter. */
 VPT_API int vpt_trace_rays(vpt_world* w, const vpt_ray* d_rays, int n, int max_iter, vpt_hit* d_hits, void* stream);
 
+/* GI ray queries: each ray is a diffuse bounce (cap gi_iter); a bounce that
+ * hits a block whose face sees the sun continues as the hit's shadow ray
+ * from hit + N*0.045 toward sun_dir (cap shadow_iter), as GetShadowAt at
+ * DiffuseRayTraceFrag.glsl:1202-1222 is used by CalculateDiffuse (:535-664).
+ * d_hits[i] = the bounce's hit; d_vis[i] = 1 when the sun is blocked at the
+ * hit, 0 when it is visible, 255 when no shadow ray was traced (miss, or
+ * N.L < 0.001).  Two one-launch kernels (bounce, then shadow). */
+VPT_API int vpt_trace_gi_rays(vpt_world* w, const vpt_ray* d_rays, int n, int gi_iter, const float sun_dir[3],
+                              int shadow_iter, vpt_hit* d_hits, uint8_t* d_vis, void* stream);
+
 /* Work counters for bandwidth accounting: when d_counter is non-NULL (two
  * uint64 in device memory), every later trace call on this world adds its
  * DF samples to d_counter[0] and its traced rays to d_counter[1].  Pass
@@ -245,6 +257,98 @@ VPT_API int vpt_frame_autotune(vpt_frame* f, vpt_world* w, const vpt_camera* cam
                                int primary_iter, int shadow_iter, const vpt_gbuffer* gb, uint8_t* d_mask,
                                float* d_rgb, vpt_svgf_texel* d_svgf_in, float* best_ms, void* stream);
 
+/* ---- serving ---------------------------------------------------------------
+ * A request loop over worlds kept resident in HBM.  Clients submit ray-query
+ * requests (primary, shadow or GI rays already in device memory) and frame
+ * requests (the primary + shadow + GI passes of one camera) of any size; a
+ * worker thread admits them while their result memory fits in the server's
+ * arena (first come, first admitted), runs them, and records arrival,
+ * admission, start and completion times.
+ *
+ * mode VPT_SERVE_PLAIN runs one request at a time with the one-launch
+ * kernels (vpt_trace_rays / vpt_trace_gi_rays / vpt_trace_primary +
+ * vpt_trace_shadow + vpt_trace_gi) and waits for each.  VPT_SERVE_BATCHED
+ * does continuous batching: every launch of the persistent kernel drains
+ * the rays of all admitted requests together (a request larger than a launch
+ * is split over several), GI bounces continue as their shadow rays in the
+ * same lane, up to max_inflight launches are queued on the device at once,
+ * and requests that arrive meanwhile join the next launch.  Results are
+ * bit-identical in both modes and do not depend on what else is in flight.
+ *
+ * Results live in server memory until vpt_server_release.  A request's
+ * d_rays must hold its rays when it is submitted (the work producing them
+ * has completed) and stay valid until the request completes.  The server
+ * runs on its own non-blocking stream.  Submit / wait / release may be
+ * called from any threads. */
+typedef struct vpt_server vpt_server;
+
+enum { VPT_SERVE_PLAIN = 0, VPT_SERVE_BATCHED = 1 };
+enum { VPT_REQ_PRIMARY = 0, VPT_REQ_SHADOW = 1, VPT_REQ_GI = 2, VPT_REQ_FRAME = 3 };
+
+typedef struct vpt_server_cfg {
+    int32_t mode;             /* VPT_SERVE_PLAIN or VPT_SERVE_BATCHED */
+    int32_t max_worlds;       /* world slots, 1..256 */
+    uint64_t arena_bytes;     /* request result memory (admission budget) */
+    uint64_t world_budget;    /* bytes of resident worlds (and frame scratch) allowed */
+    uint32_t max_launch_rays; /* rays per persistent launch (batched) */
+    int32_t max_inflight;     /* launches queued on the device at once, 1..4 (batched) */
+} vpt_server_cfg;
+
+typedef struct vpt_request {
+    int32_t world;            /* slot from vpt_server_add_world */
+    int32_t kind;             /* VPT_REQ_* */
+    uint32_t n;               /* rays (ray kinds) */
+    int32_t max_iter;         /* traversal cap (ray kinds; the bounce cap for GI) */
+    const vpt_ray* d_rays;    /* ray kinds: n rays in device memory */
+    float sun_dir[3];         /* GI: continuation shadow ray direction */
+    int32_t shadow_iter;      /* GI: its cap */
+    const vpt_camera* cam;    /* frame: camera (copied at submit) */
+    const vpt_gi_params* gi;  /* frame: GI constants (copied at submit) */
+    int32_t primary_iter;     /* frame */
+    int32_t frame_shadow_iter;/* frame: sun-shadow cap */
+    int64_t arrival_ns;       /* arrival time on vpt_now_ns's clock; 0 = at submit */
+} vpt_request;
+
+typedef struct vpt_request_result {
+    int32_t status;           /* VPT_OK or the error the request ended with */
+    int32_t kind;
+    uint32_t n;               /* rays, or pixels of a frame */
+    vpt_hit* d_hits;          /* ray kinds: n results */
+    uint8_t* d_vis;           /* GI: sun blocked (1) / visible (0) / no shadow ray (255) */
+    vpt_gbuffer gbuffer;      /* frame: G-buffer */
+    uint8_t* d_mask;          /* frame: sun-shadow mask */
+    float* d_rgb;             /* frame: GI radiance, 3 floats per pixel */
+    int64_t t_arrival_ns, t_admit_ns, t_start_ns, t_done_ns;
+} vpt_request_result;
+
+typedef struct vpt_server_stats {
+    uint64_t submitted, completed, rays, launches, frames;
+    uint64_t arena_used, arena_peak, arena_bytes, world_bytes;
+    uint32_t pending, active;
+} vpt_server_stats;
+
+/* Steady clock used for request times (nanoseconds). */
+VPT_API int64_t vpt_now_ns(void);
+VPT_API int vpt_server_create(vpt_ctx* ctx, const vpt_server_cfg* cfg, vpt_server** out);
+/* Waits for all submitted work, then frees the server (not the worlds). */
+VPT_API int vpt_server_destroy(vpt_server* s);
+/* Makes `w` resident in the first free slot (*slot); VPT_ENOMEM when its
+ * bytes exceed what is left of world_budget. */
+VPT_API int vpt_server_add_world(vpt_server* s, vpt_world* w, int* slot);
+/* VPT_EBUSY while a pending or running request names the slot. */
+VPT_API int vpt_server_remove_world(vpt_server* s, int slot);
+/* Persistent-kernel launch shape (batched mode). */
+VPT_API int vpt_server_set_launch_cfg(vpt_server* s, const vpt_launch_cfg* cfg);
+/* Queues a request; *ticket identifies it.  VPT_ENOMEM when its result
+ * memory exceeds the whole arena (it could never be admitted). */
+VPT_API int vpt_server_submit(vpt_server* s, const vpt_request* req, uint64_t* ticket);
+/* Blocks until the request completes (timeout_ms < 0: no limit) and fills
+ * *res; VPT_ETIMEOUT if it has not completed in time. */
+VPT_API int vpt_server_wait(vpt_server* s, uint64_t ticket, int timeout_ms, vpt_request_result* res);
+/* Frees a completed request's result memory (the ticket becomes invalid). */
+VPT_API int vpt_server_release(vpt_server* s, uint64_t ticket);
+VPT_API int vpt_server_stats_get(vpt_server* s, vpt_server_stats* st);
+
 /* ---- SVGF ---------------------------------------------------------------
  * Owns the history and ping-pong planes of one W x H view. */
 typedef struct vpt_svgf_params {
@@ -277,6 +381,58 @@ VPT_API int vpt_svgf_read_plane(vpt_svgf* s, int which, vpt_svgf_texel* d_dst, v
  * TemporalFilter.glsl: accumulated frames, moment, luma as binary16, plus
  * one unused half): 4 halves per pixel. */
 VPT_API int vpt_svgf_read_utility(vpt_svgf* s, uint16_t* d_dst, void* stream);
+
+/* ---- learned denoiser -------------------------------------------------------
+ * A 3-level U-Net (32 / 64 / 128 channels, 3x3 convolutions, 2x2 average
+ * pooling, nearest upsampling with skip concatenation, 1x1 output head)
+ * that maps one frame's 1-spp GI irradiance plus albedo, normal and depth
+ * to the converged irradiance.  bf16 weights and activations, fp32
+ * accumulation on the MFMA units; trained by its own backward kernels and
+ * Adam.  No counterpart in the reference (its denoiser is the SVGF chain
+ * above, which stays the fallback).
+ *
+ * Network input: NHWC bf16 with 32 channels per pixel (vpt_unet_assemble):
+ * log1p(irradiance) 3, albedo 3, normal 3, log1p(t)/8 1, zeros 22.
+ * Output / training target: NHWC fp32, 3 channels, log1p(irradiance)
+ * (mode 0) or irradiance (mode 1).  n x h x w with h, w multiples of 4. */
+typedef struct vpt_unet vpt_unet;
+
+VPT_API int vpt_unet_create(vpt_ctx* ctx, int max_n, int max_h, int max_w, vpt_unet** out);
+VPT_API int vpt_unet_destroy(vpt_unet* u);
+VPT_API int vpt_unet_param_count(const vpt_unet* u, size_t* n);
+/* He-normal initialisation from `seed`; resets the optimiser. */
+VPT_API int vpt_unet_init(vpt_unet* u, uint32_t seed);
+/* Parameters as one fp32 vector: per layer [9][cout][cin] weights then
+ * [cout] biases, then the head [3][32] and [3]; set resets the optimiser. */
+VPT_API int vpt_unet_set_params(vpt_unet* u, const float* host, size_t n);
+VPT_API int vpt_unet_get_params(vpt_unet* u, float* host, size_t n);
+/* Gradients of the last vpt_unet_backward, same layout. */
+VPT_API int vpt_unet_get_grads(vpt_unet* u, float* host, size_t n);
+VPT_API int vpt_unet_forward(vpt_unet* u, const uint16_t* d_x, int n, int h, int w, int mode, float* d_out,
+                             void* stream);
+/* Forward, L1 loss against d_target (log space) and all gradients;
+ * *loss (optional) receives the mean absolute error (synchronises). */
+VPT_API int vpt_unet_backward(vpt_unet* u, const uint16_t* d_x, const float* d_target, int n, int h, int w,
+                              float* loss, void* stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradients. */
+VPT_API int vpt_unet_adam_step(vpt_unet* u, float lr, void* stream);
+/* Network input of one frame from the G-buffer and vpt_trace_gi's texels
+ * (albedo from the table of the context's last GI pass). */
+VPT_API int vpt_unet_assemble(vpt_unet* u, const vpt_gbuffer* gb, const vpt_svgf_texel* d_tex, uint16_t* d_x,
+                              void* stream);
+/* assemble + forward: denoised irradiance, 3 floats per pixel. */
+VPT_API int vpt_unet_denoise(vpt_unet* u, const vpt_gbuffer* gb, const vpt_svgf_texel* d_tex, float* d_rgb,
+                             void* stream);
+/* Debug / tests: activation `index` of the last forward pass (0 input,
+ * 1 e0a, 2 e0b, 3 p0, 4 e1a, 5 e1b, 6 p1, 7 b0, 8 b1, 9 d1a, 10 d1b,
+ * 11 d0a, 12 d0b), NHWC bf16 at its level's resolution. */
+VPT_API int vpt_unet_read_activation(vpt_unet* u, int index, int n, int h, int w, uint16_t* host);
+/* Reference accumulation: d_acc[i][0..5] (+)= SH-Y and CoCg of texel i. */
+VPT_API int vpt_texel_accumulate(const vpt_svgf_texel* d_tex, int npx, float* d_acc, int reset, void* stream);
+/* SHToIrridiance (ColorPassFrag.glsl:356-368) with the G-buffer face
+ * normal, of d_tex or (d_tex NULL) of d_acc * acc_scale. */
+VPT_API int vpt_irradiance_decode(const vpt_gbuffer* gb, const vpt_svgf_texel* d_tex, const float* d_acc,
+                                  float acc_scale, float* d_rgb, void* stream);
 
 /* ---- host utilities ------------------------------------------------------ */
 

@@ -121,3 +121,30 @@ def test_bench_refuses_multi_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stdout
+
+
+def test_ctypes_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror has the C struct's size and field offsets."""
+    import shutil
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not found")
+    structs = {"vpt_camera": _lib.Camera, "vpt_gbuffer": _lib.GBuffer, "vpt_gi_params": _lib.GiParams,
+               "vpt_svgf_params": _lib.SvgfParams, "vpt_ray": _lib.Ray, "vpt_hit": _lib.Hit, "vpt_qray": _lib.QRay,
+               "vpt_launch_cfg": _lib.LaunchCfg, "vpt_server_cfg": _lib.ServerCfg, "vpt_request": _lib.Request,
+               "vpt_request_result": _lib.RequestResult, "vpt_server_stats": _lib.ServerStats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "vpt/vpt.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

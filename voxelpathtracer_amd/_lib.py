@@ -18,6 +18,15 @@ VPT_EINVAL = -1
 VPT_ENOMEM = -2
 VPT_EDEVICE = -3
 VPT_ESTATE = -4
+VPT_ETIMEOUT = -5
+VPT_EBUSY = -6
+
+VPT_SERVE_PLAIN = 0
+VPT_SERVE_BATCHED = 1
+VPT_REQ_PRIMARY = 0
+VPT_REQ_SHADOW = 1
+VPT_REQ_GI = 2
+VPT_REQ_FRAME = 3
 
 
 class VptError(RuntimeError):
@@ -65,6 +74,32 @@ class LaunchCfg(C.Structure):
                 ("flags", C.c_int32)]
 
 
+class ServerCfg(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("max_worlds", C.c_int32), ("arena_bytes", C.c_uint64),
+                ("world_budget", C.c_uint64), ("max_launch_rays", C.c_uint32), ("max_inflight", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("world", C.c_int32), ("kind", C.c_int32), ("n", C.c_uint32), ("max_iter", C.c_int32),
+                ("d_rays", C.c_void_p), ("sun_dir", C.c_float * 3), ("shadow_iter", C.c_int32),
+                ("cam", C.POINTER(Camera)), ("gi", C.POINTER(GiParams)), ("primary_iter", C.c_int32),
+                ("frame_shadow_iter", C.c_int32), ("arrival_ns", C.c_int64)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("kind", C.c_int32), ("n", C.c_uint32), ("d_hits", C.c_void_p),
+                ("d_vis", C.c_void_p), ("gbuffer", GBuffer), ("d_mask", C.c_void_p), ("d_rgb", C.c_void_p),
+                ("t_arrival_ns", C.c_int64), ("t_admit_ns", C.c_int64), ("t_start_ns", C.c_int64),
+                ("t_done_ns", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rays", C.c_uint64),
+                ("launches", C.c_uint64), ("frames", C.c_uint64), ("arena_used", C.c_uint64),
+                ("arena_peak", C.c_uint64), ("arena_bytes", C.c_uint64), ("world_bytes", C.c_uint64),
+                ("pending", C.c_uint32), ("active", C.c_uint32)]
+
+
 _vp = C.c_void_p
 _i = C.c_int
 _f = C.c_float
@@ -89,6 +124,7 @@ _PROTOS = {
     "vpt_trace_shadow": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _vp, _vp]),
     "vpt_trace_gi": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(GiParams), _vp, _vp, _vp]),
     "vpt_trace_rays": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
+    "vpt_trace_gi_rays": (_i, [_vp, _vp, _i, _i, C.POINTER(_f), _i, _vp, _vp, _vp]),
     "vpt_world_set_sample_counter": (_i, [_vp, _vp]),
     "vpt_batch_create": (_i, [_vp, _i, C.POINTER(_vp)]),
     "vpt_batch_destroy": (_i, [_vp]),
@@ -105,6 +141,31 @@ _PROTOS = {
                              _vp, _i, _vp]),
     "vpt_frame_autotune": (_i, [_vp, _vp, C.POINTER(Camera), C.POINTER(GiParams), _i, _i, C.POINTER(GBuffer), _vp,
                                 _vp, _vp, C.POINTER(_f), _vp]),
+    "vpt_now_ns": (C.c_int64, []),
+    "vpt_server_create": (_i, [_vp, C.POINTER(ServerCfg), C.POINTER(_vp)]),
+    "vpt_server_destroy": (_i, [_vp]),
+    "vpt_server_add_world": (_i, [_vp, _vp, C.POINTER(_i)]),
+    "vpt_server_remove_world": (_i, [_vp, _i]),
+    "vpt_server_set_launch_cfg": (_i, [_vp, C.POINTER(LaunchCfg)]),
+    "vpt_server_submit": (_i, [_vp, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "vpt_server_wait": (_i, [_vp, C.c_uint64, _i, C.POINTER(RequestResult)]),
+    "vpt_server_release": (_i, [_vp, C.c_uint64]),
+    "vpt_server_stats_get": (_i, [_vp, C.POINTER(ServerStats)]),
+    "vpt_unet_create": (_i, [_vp, _i, _i, _i, C.POINTER(_vp)]),
+    "vpt_unet_destroy": (_i, [_vp]),
+    "vpt_unet_param_count": (_i, [_vp, C.POINTER(C.c_size_t)]),
+    "vpt_unet_init": (_i, [_vp, _u32]),
+    "vpt_unet_set_params": (_i, [_vp, _vp, C.c_size_t]),
+    "vpt_unet_get_params": (_i, [_vp, _vp, C.c_size_t]),
+    "vpt_unet_get_grads": (_i, [_vp, _vp, C.c_size_t]),
+    "vpt_unet_forward": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp]),
+    "vpt_unet_backward": (_i, [_vp, _vp, _vp, _i, _i, _i, C.POINTER(_f), _vp]),
+    "vpt_unet_adam_step": (_i, [_vp, _f, _vp]),
+    "vpt_unet_assemble": (_i, [_vp, C.POINTER(GBuffer), _vp, _vp, _vp]),
+    "vpt_unet_denoise": (_i, [_vp, C.POINTER(GBuffer), _vp, _vp, _vp]),
+    "vpt_unet_read_activation": (_i, [_vp, _i, _i, _i, _i, _vp]),
+    "vpt_texel_accumulate": (_i, [_vp, _i, _vp, _i, _vp]),
+    "vpt_irradiance_decode": (_i, [C.POINTER(GBuffer), _vp, _vp, _f, _vp, _vp]),
     "vpt_svgf_create": (_i, [_vp, _i, _i, C.POINTER(SvgfParams), C.POINTER(_vp)]),
     "vpt_svgf_destroy": (_i, [_vp]),
     "vpt_svgf_reset": (_i, [_vp, _vp]),

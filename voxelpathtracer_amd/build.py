@@ -46,7 +46,8 @@ DEVICE = [
     "-packed-fp32-ops",
 ]
 
-SOURCES = ["vpt_host.cpp", "vpt_world.hip", "vpt_trace.hip", "vpt_batch.hip", "vpt_svgf.hip"]
+SOURCES = ["vpt_host.cpp", "vpt_world.hip", "vpt_trace.hip", "vpt_batch.hip", "vpt_svgf.hip", "vpt_server.hip",
+           "vpt_unet.hip"]
 
 
 def _compile(src: str) -> Path:
@@ -80,19 +81,25 @@ def build(verbose: bool = True) -> Path:
 
 
 DEMO = OUT / "vpt_frame_demo"
+SERVE_BENCH = OUT / "vpt_serve_bench"
+
+
+def _build_example(src: str, out: Path, verbose: bool) -> Path:
+    cmd = [HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", "-pthread", f"-I{ROOT / 'include'}",
+           str(CSRC / "examples" / src), f"-L{OUT}", "-lvpt", "-Wl,-rpath,$ORIGIN", "-o", str(out)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"example build failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {out}")
+    return out
 
 
 def build_demo(verbose: bool = True) -> Path:
-    """The C++ host example over the World / Pipeline façades."""
-    cmd = [HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}",
-           str(CSRC / "examples" / "vpt_frame_demo.cpp"), f"-L{OUT}", "-lvpt", "-Wl,-rpath,$ORIGIN",
-           "-o", str(DEMO)]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"demo build failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    if verbose:
-        print(f"built {DEMO}")
-    return DEMO
+    """The C++ host examples: the frame loop over the World / Pipeline
+    façades, and the serving benchmark over vpt_server."""
+    _build_example("vpt_serve_bench.cpp", SERVE_BENCH, verbose)
+    return _build_example("vpt_frame_demo.cpp", DEMO, verbose)
 
 
 def build_oracle(verbose: bool = True) -> Path:

@@ -67,6 +67,9 @@ struct Outputs {
     uint8_t* gb_face;
     uint8_t* gb_block;
     uint32_t* gb_voxel;
+    const vpt::ServeSegment* segs;   // served requests: segment table
+    const uint32_t* seg_begin;       // their first ids, then the launch's ray count
+    uint32_t nseg;
 };
 
 struct Slot {
@@ -108,7 +111,8 @@ __device__ __attribute__((always_inline)) inline void refill_slot(Slot& s, World
                                                                   uint32_t n, uint32_t* __restrict__ head,
                                                                   uint32_t chunk, uint32_t refill_min,
                                                                   const FrameConst* __restrict__ fc, int tiles_x,
-                                                                  Cursor& c, unsigned long long& n_rays)
+                                                                  const Outputs& out, Cursor& c,
+                                                                  unsigned long long& n_rays)
 {
     const int lane = threadIdx.x & 63;
     bool need = !s.active;
@@ -152,6 +156,29 @@ __device__ __attribute__((always_inline)) inline void refill_slot(Slot& s, World
                 s.kind = MULTI ? KIND_QUERY : (q.meta >> 24);
                 s.active = true;
                 n_rays++;
+            } else if (SRC == 2) {
+                // the segment holding launch-wide id `id`: seg_begin[lo] <= id < seg_begin[lo + 1]
+                uint32_t lo = 0, hi = out.nseg;
+                while (hi - lo > 1u) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (out.seg_begin[mid] <= id) lo = mid;
+                    else hi = mid;
+                }
+                const vpt::ServeSegment& sg = out.segs[lo];
+                const uint32_t local = sg.first + (id - sg.begin);
+                const vpt_ray q = sg.rays[local];
+                if (sg.world < nworlds) {
+                    wv = worlds[sg.world];
+                } else {
+                    wv = w0;
+                    wv.xmax = wv.ymax = wv.zmax = -1.0f;
+                }
+                vpt::trav_init(s.r, vpt::mk3(q.origin[0], q.origin[1], q.origin[2]),
+                               vpt::mk3(q.dir[0], q.dir[1], q.dir[2]), sg.max_iter);
+                s.dst = local;
+                s.kind = (sg.kind == VPT_REQ_GI ? KIND_GI : KIND_QUERY) | (lo << 8);
+                s.active = true;
+                n_rays++;
             } else {
                 const uint32_t tile = id >> 6, l = id & 63u;
                 const int px = (int)(tile % (uint32_t)tiles_x) * 8 + (int)(l & 7u);
@@ -189,6 +216,35 @@ __device__ __attribute__((always_inline)) inline void finish_slot(Slot& s, const
         out.gb_face[dst] = (uint8_t)(intersect ? h.face : 255);
         out.gb_block[dst] = (uint8_t)(intersect ? h.block : 0);
         if (out.gb_voxel) out.gb_voxel[dst] = intersect ? h.voxel : 0xffffffffu;
+    } else if (SRC == 2) {
+        const uint32_t seg = s.kind >> 8, k = s.kind & 0xffu;
+        const vpt::ServeSegment& sg = out.segs[seg];
+        if (k == KIND_GI_SHADOW) {
+            sg.vis[dst] = h.t > 0.0f ? 1 : 0;
+        } else {
+            vpt_hit o;
+            o.t = h.t;
+            o.voxel = h.voxel;
+            o.face = (uint8_t)h.face;
+            o.block = (uint8_t)h.block;
+            o.samples = (uint16_t)min(h.samples, 65535);
+            sg.hits[dst] = o;
+            if (k == KIND_GI) {
+                const Vec3 L = vpt::mk3(sg.sun[0], sg.sun[1], sg.sun[2]);
+                Vec3 so;
+                float ndotl;
+                if (h.t > 0.0f && h.block > 0 &&
+                    vpt::gi_shadow_origin(vpt::mk3(s.r.o0x, s.r.o0y, s.r.o0z), vpt::mk3(s.r.dx, s.r.dy, s.r.dz),
+                                          h.t, h.face, L, so, ndotl)) {
+                    vpt::trav_init(s.r, so, L, sg.shadow_iter);
+                    s.kind = KIND_GI_SHADOW | (seg << 8);
+                    s.active = true;
+                    n_rays++;
+                } else {
+                    sg.vis[dst] = 255;
+                }
+            }
+        }
     } else if (s.kind == KIND_QUERY) {
         vpt_hit o;
         o.t = h.t;
@@ -220,7 +276,8 @@ __device__ __attribute__((always_inline)) inline void finish_slot(Slot& s, const
 }
 
 // SRC = 0: queue of vpt_qray (rays[id]); SRC = 1: primary ray of the id-th
-// pixel in 8x8 tile order (fc->cam).
+// pixel in 8x8 tile order (fc->cam); SRC = 2: served requests' rays through
+// the segment table in `out`.
 template <bool MULTI, int SRC>
 __global__ void __launch_bounds__(256) persistent_kernel(WorldView w0, const WorldView* __restrict__ worlds,
                                                          uint32_t nworlds, const vpt_qray* __restrict__ rays,
@@ -242,7 +299,7 @@ __global__ void __launch_bounds__(256) persistent_kernel(WorldView w0, const Wor
     const int tiles_x = SRC == 1 ? (fc->cam.W + 7) / 8 : 1;
 
     while (true) {
-        refill_slot<MULTI, SRC>(s, wv, w0, worlds, nworlds, rays, n, head, chunk, refill_min, fc, tiles_x, c,
+        refill_slot<MULTI, SRC>(s, wv, w0, worlds, nworlds, rays, n, head, chunk, refill_min, fc, tiles_x, out, c,
                                 n_rays);
         if (!__any(s.active)) {
             if (c.exhausted) break;
@@ -250,7 +307,7 @@ __global__ void __launch_bounds__(256) persistent_kernel(WorldView w0, const Wor
         }
         // one iteration of the lane's ray (a single-world launch reads the
         // world from the kernel arguments)
-        const WorldView& w = MULTI ? wv : w0;
+        const WorldView& w = (MULTI || SRC == 2) ? wv : w0;
         uint32_t off = 0;
         const bool ld = s.active && vpt::trav_prepare(w, s.r, off);
         const int m = ld ? (int)w.df[off] : 0;
@@ -448,6 +505,29 @@ int launch_persistent(const vpt_launch_cfg& cfg, int cus, hipStream_t s, WorldVi
 }
 
 }  // namespace
+
+int vpt::device_cus(int device) { return cu_count(device); }
+
+bool vpt::launch_cfg_valid(const vpt_launch_cfg& cfg) { return cfg_valid(cfg); }
+
+int vpt::serve_launch(int cus, const vpt_launch_cfg& cfg, const void* d_worlds, uint32_t nworlds,
+                      const ServeSegment* d_segs, const uint32_t* d_begin, uint32_t nseg, uint32_t total,
+                      uint32_t* d_head, void* stream)
+{
+    if (!cfg_valid(cfg) || nseg == 0 || total == 0) return set_error(VPT_EINVAL, "serve_launch: bad launch");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, d_head, 1);
+    HIP_TRY(hipGetLastError());
+    Outputs out;
+    memset(&out, 0, sizeof(out));
+    out.segs = d_segs;
+    out.seg_begin = d_begin;
+    out.nseg = nseg;
+    WorldView w0;
+    memset(&w0, 0, sizeof(w0));
+    return launch_persistent<false, 2>(cfg, cus, s, w0, (const WorldView*)d_worlds, nworlds, nullptr, nullptr, total,
+                                       d_head, out, nullptr, nullptr);
+}
 
 // ===========================================================================
 // Batch (ray queries over resident worlds)

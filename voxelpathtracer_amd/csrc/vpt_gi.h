@@ -96,16 +96,20 @@ __device__ inline void gi_ray(const CamView& cam, int px, int py, float dist, Ve
 // GI hit -> shadow ray toward the sun from ip + hn * 0.045
 // (DiffuseRayTraceFrag.glsl:583-597, GetShadowAt :1202-1222).  Returns
 // false when N.L < 0.001 (no shadow ray; ShadowAt = 0).
-__device__ inline bool gi_shadow_ray(Vec3 o, Vec3 dir, float T, int hit_face, const GiConst& g, Vec3& so,
-                                     float& ndotl)
+__device__ inline bool gi_shadow_origin(Vec3 o, Vec3 dir, float T, int hit_face, Vec3 L, Vec3& so, float& ndotl)
 {
     Vec3 ip = mk3(o.x + dir.x * T, o.y + dir.y * T, o.z + dir.z * T);
     Vec3 hn = face_normal(hit_face, 0.0f);
-    Vec3 L = mk3(g.sun[0], g.sun[1], g.sun[2]);
     ndotl = fmaxf(dot3(hn, L), 0.0f);
     if (ndotl < 0.001f) return false;
     so = mk3(ip.x + hn.x * 0.045f, ip.y + hn.y * 0.045f, ip.z + hn.z * 0.045f);
     return true;
+}
+
+__device__ inline bool gi_shadow_ray(Vec3 o, Vec3 dir, float T, int hit_face, const GiConst& g, Vec3& so,
+                                     float& ndotl)
+{
+    return gi_shadow_origin(o, dir, T, hit_face, mk3(g.sun[0], g.sun[1], g.sun[2]), so, ndotl);
 }
 
 // IrridianceToSH (DiffuseRayTraceFrag.glsl:766-784).

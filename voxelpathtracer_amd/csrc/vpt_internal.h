@@ -28,6 +28,36 @@ struct World {
     size_t voxels() const { return (size_t)nx * ny * nz; }
 };
 
+// One contiguous run of a served request's rays inside a persistent launch:
+// launch-wide ray ids [begin, next segment's begin) are the request's rays
+// first, first + 1, ...
+struct ServeSegment {
+    const vpt_ray* rays;   // the request's rays
+    vpt_hit* hits;         // its results
+    uint8_t* vis;          // GI requests: sun visibility at the bounce hit
+    uint32_t begin;        // launch-wide id of the segment's first ray
+    uint32_t first;        // request-local index of that ray
+    uint32_t world;        // world slot
+    uint32_t kind;         // VPT_REQ_PRIMARY / SHADOW / GI
+    int32_t max_iter;
+    int32_t shadow_iter;   // GI: continuation cap
+    float sun[3];          // GI: continuation direction
+    uint32_t pad;
+};
+
+// Enqueues one persistent launch over `total` rays split in `nseg` segments
+// (d_segs, and d_begin = their begin ids followed by `total`), reading
+// worlds from the table d_worlds of `nworlds` views.  d_head: one zeroed
+// queue counter, reset by the launch itself.
+int serve_launch(int cus, const vpt_launch_cfg& cfg, const void* d_worlds, uint32_t nworlds,
+                 const ServeSegment* d_segs, const uint32_t* d_begin, uint32_t nseg, uint32_t total, uint32_t* d_head,
+                 void* stream);
+
+// Number of compute units of `device` (256 when the query fails).
+int device_cus(int device);
+// Whether `cfg` is a launch shape the persistent kernels accept.
+bool launch_cfg_valid(const vpt_launch_cfg& cfg);
+
 }  // namespace vpt
 
 struct vpt_ctx {

@@ -270,6 +270,35 @@ __global__ void __launch_bounds__(256) rays_kernel(WorldView w, const vpt_ray* _
     count_samples(counter, samples, nrays);
 }
 
+// Shadow rays of GI ray queries (second launch of vpt_trace_gi_rays): the
+// bounce hit in hits[i] continues toward the sun when its face sees it.
+__global__ void __launch_bounds__(256) gi_rays_shadow_kernel(WorldView w, const vpt_ray* __restrict__ rays, int n,
+                                                             const vpt_hit* __restrict__ hits, float lx, float ly,
+                                                             float lz, int shadow_iter, uint8_t* __restrict__ vis,
+                                                             unsigned long long* __restrict__ counter)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int samples = 0, nrays = 0;
+    if (i < n) {
+        const vpt_ray r = rays[i];
+        const vpt_hit h = hits[i];
+        const Vec3 L = vpt::mk3(lx, ly, lz);
+        Vec3 so;
+        float ndotl;
+        uint8_t v = 255;
+        if (h.t > 0.0f && h.block > 0 &&
+            vpt::gi_shadow_origin(vpt::mk3(r.origin[0], r.origin[1], r.origin[2]),
+                                  vpt::mk3(r.dir[0], r.dir[1], r.dir[2]), h.t, h.face, L, so, ndotl)) {
+            vpt::Hit s = vpt::traverse(w, so, L, shadow_iter);
+            v = s.t > 0.0f ? 1 : 0;
+            samples = s.samples;
+            nrays = 1;
+        }
+        vis[i] = v;
+    }
+    count_samples(counter, samples, nrays);
+}
+
 dim3 tile_grid(int W, int H) { return dim3((unsigned)(((W + 15) / 16) * ((H + 15) / 16))); }
 
 }  // namespace
@@ -332,6 +361,27 @@ extern "C" VPT_API int vpt_trace_rays(vpt_world* w, const vpt_ray* d_rays, int n
     DeviceGuard g(w->device);
     hipLaunchKernelGGL(rays_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        vpt::make_view(w), d_rays, n, max_iter, d_hits, w->d_counter);
+    HIP_TRY(hipGetLastError());
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_trace_gi_rays(vpt_world* w, const vpt_ray* d_rays, int n, int gi_iter, const float sun_dir[3],
+                                         int shadow_iter, vpt_hit* d_hits, uint8_t* d_vis, void* stream)
+{
+    if (!w || !sun_dir || (n > 0 && (!d_rays || !d_hits || !d_vis)))
+        return set_error(VPT_EINVAL, "vpt_trace_gi_rays: null argument");
+    if (n < 0) return set_error(VPT_EINVAL, "vpt_trace_gi_rays: negative ray count");
+    if (int rc = vpt::check_iter(gi_iter, "vpt_trace_gi_rays")) return rc;
+    if (int rc = vpt::check_iter(shadow_iter, "vpt_trace_gi_rays")) return rc;
+    if (n == 0) return VPT_OK;
+    DeviceGuard g(w->device);
+    const WorldView wv = vpt::make_view(w);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(rays_kernel, grid, dim3(256), 0, (hipStream_t)stream, wv, d_rays, n, gi_iter, d_hits,
+                       w->d_counter);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(gi_rays_shadow_kernel, grid, dim3(256), 0, (hipStream_t)stream, wv, d_rays, n, d_hits,
+                       sun_dir[0], sun_dir[1], sun_dir[2], shadow_iter, d_vis, w->d_counter);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }

@@ -148,6 +148,14 @@ class World:
             n = rays.numel() * rays.element_size() // C.sizeof(Ray)
         call("vpt_trace_rays", self.handle, _ptr(rays), n, max_iter, _ptr(hits), _stream(stream))
 
+    def trace_gi_rays(self, rays, hits, vis, n: int, gi_iter: int = 48, sun_dir=(0.0, 1.0, 0.0),
+                      shadow_iter: int = 128, stream=None) -> None:
+        """GI ray queries: bounce hits into `hits`, sun visibility at the hit
+        into `vis` (1 blocked, 0 visible, 255 no shadow ray)."""
+        sd = (C.c_float * 3)(*map(float, sun_dir))
+        call("vpt_trace_gi_rays", self.handle, _ptr(rays), n, gi_iter, sd, shadow_iter, _ptr(hits), _ptr(vis),
+             _stream(stream))
+
     def close(self):
         if self.handle:
             lib().vpt_world_destroy(self.handle)
