@@ -38,6 +38,11 @@ def parse():
     ap.add_argument("--svgf-frames", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving benchmark")
+    ap.add_argument("--serve-requests", type=int, default=3000)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-denoiser section")
+    ap.add_argument("--unet-pairs", type=int, default=2048)
+    ap.add_argument("--unet-steps", type=int, default=2000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -207,13 +212,18 @@ def main():
     sv = SVGF(ctx, W, H)
     den = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
 
-    def svgf_sequence():
+    eval_frames = [f for f in (15, 31, 47, 63) if f < nf]
+    svgf_out = {}
+
+    def svgf_sequence(keep=False):
         sv.reset()
         for f, (c_f, vp_f, g_f, t_f) in enumerate(seq):
             g_prev = seq[f - 1][2] if f else g_f
             sv.frame(c_f, vp_f, g_f, g_prev, t_f, den, time=f / 60.0)
+            if keep and f in eval_frames:
+                svgf_out[f] = den.clone()
 
-    svgf_sequence()
+    svgf_sequence(keep=True)
     torch.cuda.synchronize()
     s0, s1 = ev(), ev()
     s0.record(stream)
@@ -221,7 +231,6 @@ def main():
     s1.record(stream)
     torch.cuda.synchronize()
     svgf_ms = s0.elapsed_time(s1) / nf
-    del seq
 
     result = {
         "metric": "Grays/s DDA+GI @1080p (bit-exact hits)",
@@ -251,11 +260,120 @@ def main():
                      "traffic": None, "algorithmic_bytes": alg[dom]},
     }
 
+    if not args.no_learned:
+        result["learned"] = learned(ctx, world, seq, eval_frames, svgf_out, svgf_ms, args)
+    del seq
+    if not args.no_serve:
+        result["serve"] = serve(args)
     if args.copy_probe:
         copy_probe(args.copy_probe)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(grid, gb, mask, rgb, cam, prm, args.cpu_reps)
     print(json.dumps(result), flush=True)
+
+
+def serve(args):
+    """The serving benchmark (C++ host over vpt_server): 16 resident worlds,
+    ray-query requests of log-uniform size in [1K, 2M] rays, kinds
+    primary : shadow : GI = 1 : 1 : 2; plain (one launch per request) vs
+    batched (continuous batching, persistent kernel): saturated throughput
+    and p50 / p99 latency under Poisson arrivals at half the plain path's
+    throughput."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent / "voxelpathtracer_amd" / "_build" / "vpt_serve_bench"
+    cmd = [str(exe), "--requests", str(args.serve_requests), "--loads", "0.5"]
+    try:
+        res = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    except subprocess.TimeoutExpired:
+        return {"error": "serving benchmark timed out"}
+    if res.returncode != 0:
+        return {"error": f"serving benchmark failed ({res.returncode}): {res.stderr[-400:]}"}
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    sat = out["saturated"]
+    ld = out["loads"][0]
+    best = out["best_batched"]
+    out["summary"] = {
+        "batched_kernel": best,
+        "plain_grays_s": sat["plain"]["grays_s"], "batched_grays_s": sat[best]["grays_s"],
+        "throughput_ratio": out["ratio_saturated"],
+        "p50_ms": {"plain": ld["plain"]["p50_ms"], "batched": ld[best]["p50_ms"]},
+        "p99_ms": {"plain": ld["plain"]["p99_ms"], "batched": ld[best]["p99_ms"]},
+        "batched_at_own_half_load": {"grays_s": ld["best_batched_own_load"]["grays_s"],
+                                     "p50_ms": ld["best_batched_own_load"]["p50_ms"],
+                                     "p99_ms": ld["best_batched_own_load"]["p99_ms"]},
+    }
+    return out
+
+
+def learned(ctx, world, seq, eval_frames, svgf_out, svgf_ms, args):
+    """Learned denoiser: trained on the GPU by its own kernels on pairs the
+    tracer renders (1 spp -> 256 spp, 128x128 windows of a 1080p frustum
+    over random seeded worlds and cameras), then run at 1920x1080 on the
+    SVGF sequence.  PSNR (tonemapped x/(1+x)) against 256-spp renders of the
+    same frames for the raw 1-spp irradiance, SVGF and the network; time per
+    frame of the network (assembly + forward) beside SVGF's."""
+    import torch
+    from voxelpathtracer_amd import gi_params
+    from voxelpathtracer_amd.unet import UNet, irradiance_decode, make_dataset, texel_accumulate, tonemap_psnr, train
+    H, W = seq[0][2].height, seq[0][2].width
+    u = UNet(ctx, 1, H, W)
+    u.init(1)
+    t0 = time.perf_counter()
+    x, t = make_dataset(ctx, u, args.unet_pairs, size=128, spp=256, seed=11, worlds=8)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    losses = train(u, x, t, steps=args.unet_steps, batch=16, lr=2e-3)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    del x, t
+    stream = torch.cuda.current_stream()
+    rgb = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    tex = torch.empty((H, W, 8), dtype=torch.int16, device="cuda")
+    acc = torch.empty((H, W, 6), dtype=torch.float32, device="cuda")
+    ref = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    noisy = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    out = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    psnr = {"input_1spp": [], "svgf": [], "unet": []}
+    for f in eval_frames:
+        c_f, _, g_f, t_f = seq[f]
+        for k in range(256):
+            world.trace_gi(c_f, g_f, gi_params(50.0, 100000 + 256 * f + k), rgb, tex)
+            texel_accumulate(tex, W * H, acc, k == 0)
+        irradiance_decode(g_f, ref, acc=acc, scale=1.0 / 256)
+        irradiance_decode(g_f, noisy, tex=t_f)
+        u.denoise(g_f, t_f, out)
+        torch.cuda.synchronize()
+        r = ref.cpu().numpy()
+        psnr["input_1spp"].append(tonemap_psnr(noisy.cpu().numpy(), r))
+        psnr["svgf"].append(tonemap_psnr(svgf_out[f].cpu().numpy(), r))
+        psnr["unet"].append(tonemap_psnr(out.cpu().numpy(), r))
+    # time per frame at 1080p
+    c_f, _, g_f, t_f = seq[-1]
+    for _ in range(3):
+        u.denoise(g_f, t_f, out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(stream)
+    for _ in range(reps):
+        u.denoise(g_f, t_f, out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    unet_ms = e0.elapsed_time(e1) / reps
+    u.close()
+    mean = {k: round(float(np.mean(v)), 3) for k, v in psnr.items()}
+    # MFMA work of one 1080p forward: 2 * 9 * cin * cout per output pixel per layer
+    flops = 0
+    for cin, cout, lv in ((32, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 128, 2), (128, 128, 2),
+                          (192, 64, 1), (64, 64, 1), (96, 32, 0), (32, 32, 0)):
+        flops += 2 * 9 * cin * cout * (W >> lv) * (H >> lv)
+    return {"pairs": args.unet_pairs, "pair_size": 128, "ref_spp": 256, "train_steps": args.unet_steps,
+            "batch": 16, "dataset_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
+            "loss_first": round(float(np.mean(losses[:10])), 5), "loss_last": round(float(np.mean(losses[-50:])), 5),
+            "eval_frames": eval_frames, "psnr_db": {k: [round(x, 3) for x in v] for k, v in psnr.items()},
+            "psnr_mean_db": mean, "unet_ms_per_frame": round(unet_ms, 4), "svgf_ms_per_frame": round(svgf_ms, 4),
+            "unet_tflops": round(flops / (unet_ms * 1e-3) / 1e12, 2), "unet_gflop_per_frame": round(flops / 1e9, 2)}
 
 
 def cpu_baseline(grid, gb, mask, rgb, cam, prm, reps):

@@ -216,7 +216,8 @@ typedef struct vpt_launch_cfg {
     int32_t waves_per_cu;    /* resident wavefronts per CU launched */
     int32_t chunk;           /* rays taken from the queue per atomic */
     int32_t refill;          /* a wavefront refills once this many lanes are idle (1..64) */
-    int32_t flags;           /* frame path, bit 0: primary rays by the one-launch tile kernel */
+    int32_t flags;           /* frame path, bit 0: primary rays by the one-launch tile kernel;
+                                served rays, bit 1: one lane per ray instead of the persistent kernel */
 } vpt_launch_cfg;
 
 VPT_API int vpt_batch_create(vpt_ctx* ctx, int max_worlds, vpt_batch** out);

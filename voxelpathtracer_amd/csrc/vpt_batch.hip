@@ -317,6 +317,51 @@ __global__ void __launch_bounds__(256) persistent_kernel(WorldView w0, const Wor
 }
 
 
+// Served requests, one lane per ray (the launch-wide id selects the
+// segment); a GI bounce that hits a sun-facing block traces its shadow ray
+// in the same lane.  Same traversal code as every other path.
+__global__ void __launch_bounds__(256) serve_direct_kernel(const WorldView* __restrict__ worlds, uint32_t nworlds,
+                                                           Outputs out, uint32_t total)
+{
+    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= total) return;
+    uint32_t lo = 0, hi = out.nseg;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (out.seg_begin[mid] <= id) lo = mid;
+        else hi = mid;
+    }
+    const vpt::ServeSegment& sg = out.segs[lo];
+    const uint32_t local = sg.first + (id - sg.begin);
+    const vpt_ray q = sg.rays[local];
+    WorldView wv;
+    if (sg.world < nworlds) {
+        wv = worlds[sg.world];
+    } else {
+        memset(&wv, 0, sizeof(wv));
+        wv.xmax = wv.ymax = wv.zmax = -1.0f;
+    }
+    const Vec3 o = vpt::mk3(q.origin[0], q.origin[1], q.origin[2]);
+    const Vec3 d = vpt::mk3(q.dir[0], q.dir[1], q.dir[2]);
+    const vpt::Hit h = vpt::traverse(wv, o, d, sg.max_iter);
+    vpt_hit r;
+    r.t = h.t;
+    r.voxel = h.voxel;
+    r.face = (uint8_t)h.face;
+    r.block = (uint8_t)h.block;
+    r.samples = (uint16_t)min(h.samples, 65535);
+    sg.hits[local] = r;
+    if (sg.kind == VPT_REQ_GI) {
+        const Vec3 L = vpt::mk3(sg.sun[0], sg.sun[1], sg.sun[2]);
+        Vec3 so;
+        float ndotl;
+        uint8_t v = 255;
+        if (h.t > 0.0f && h.block > 0 && vpt::gi_shadow_origin(o, d, h.t, h.face, L, so, ndotl))
+            v = vpt::traverse(wv, so, L, sg.shadow_iter).t > 0.0f ? 1 : 0;
+        sg.vis[local] = v;
+    }
+}
+
 __global__ void reset_kernel(uint32_t* counters, int n)
 {
     if ((int)threadIdx.x < n) counters[threadIdx.x] = 0u;
@@ -441,7 +486,7 @@ int cu_count(int device)
 bool cfg_valid(const vpt_launch_cfg& c)
 {
     return (c.block == 64 || c.block == 128 || c.block == 256) && c.waves_per_cu >= 1 && c.waves_per_cu <= 64 &&
-           c.chunk >= 1 && c.chunk <= 65536 && c.refill >= 1 && c.refill <= 64 && (c.flags & ~1) == 0;
+           c.chunk >= 1 && c.chunk <= 65536 && c.refill >= 1 && c.refill <= 64 && (c.flags & ~3) == 0;
 }
 
 vpt_launch_cfg default_cfg() { return vpt_launch_cfg{256, 16, 256, 32, 1}; }
@@ -516,16 +561,28 @@ int vpt::serve_launch(int cus, const vpt_launch_cfg& cfg, const void* d_worlds, 
 {
     if (!cfg_valid(cfg) || nseg == 0 || total == 0) return set_error(VPT_EINVAL, "serve_launch: bad launch");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, d_head, 1);
-    HIP_TRY(hipGetLastError());
     Outputs out;
     memset(&out, 0, sizeof(out));
     out.segs = d_segs;
     out.seg_begin = d_begin;
     out.nseg = nseg;
+    if (cfg.flags & 2) {
+        hipLaunchKernelGGL(serve_direct_kernel, dim3((total + 255) / 256), dim3(256), 0, s, (const WorldView*)d_worlds,
+                           nworlds, out, total);
+        HIP_TRY(hipGetLastError());
+        return VPT_OK;
+    }
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, d_head, 1);
+    HIP_TRY(hipGetLastError());
+    // a launch smaller than the resident lanes gives each wavefront at most
+    // one ray per lane, so no ray waits behind another in its lane
+    vpt_launch_cfg c = cfg;
+    const uint64_t waves = (uint64_t)std::max(1, c.waves_per_cu * cus / (c.block / 64)) * (c.block / 64);
+    const uint64_t per_wave = (total + waves - 1) / waves;
+    c.chunk = (int)std::min<uint64_t>((uint64_t)c.chunk, std::max<uint64_t>(64, (per_wave + 63) / 64 * 64));
     WorldView w0;
     memset(&w0, 0, sizeof(w0));
-    return launch_persistent<false, 2>(cfg, cus, s, w0, (const WorldView*)d_worlds, nworlds, nullptr, nullptr, total,
+    return launch_persistent<false, 2>(c, cus, s, w0, (const WorldView*)d_worlds, nworlds, nullptr, nullptr, total,
                                        d_head, out, nullptr, nullptr);
 }
 

@@ -178,7 +178,7 @@ struct vpt_server {
     int device = 0;
     int cus = 256;
     vpt_server_cfg cfg{};
-    vpt_launch_cfg launch{256, 16, 256, 32, 0};
+    vpt_launch_cfg launch{256, 16, 256, 32, 2};   // one lane per ray (bit 1); the persistent kernel is optional
 
     std::vector<vpt_world*> worlds;
     std::vector<int> world_refs;
