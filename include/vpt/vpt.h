@@ -154,6 +154,14 @@ VPT_API int vpt_world_set_blocks(vpt_world* w, const uint32_t* edits, const uint
  * DF = min(254, L1 distance to the nearest block with id > 0). */
 VPT_API int vpt_df_build(vpt_world* w, void* stream);
 
+/* Raw world files (SaveWorld / LoadWorld, WorldFileHandler.cpp:10-83): the
+ * nx*ny*nz block-id bytes in the linear layout, no header.  Load checks the
+ * file size against the world's dims and uploads; it does not rebuild the
+ * DF: call vpt_df_build after it, as the reference does after LoadWorld
+ * (Pipeline.cpp:1206, 1293-1294). */
+VPT_API int vpt_world_save(vpt_world* w, const char* path);
+VPT_API int vpt_world_load(vpt_world* w, const char* path, void* stream);
+
 /* Read back (parity tests / World::GetBlock): linear x-fastest layout. */
 VPT_API int vpt_df_download(vpt_world* w, uint8_t* host_df, void* stream);
 VPT_API int vpt_grid_download(vpt_world* w, uint8_t* host_grid, void* stream);

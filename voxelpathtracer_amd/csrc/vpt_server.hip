@@ -338,6 +338,13 @@ struct vpt_server {
             q->scheduled += take[i];
             nseg++;
         }
+        // rays of one world next to each other in the launch: the wavefronts
+        // in flight at any moment then walk few worlds' distance fields
+        std::stable_sort(L.parts.begin(), L.parts.end(), [](const Part& a, const Part& b) {
+            const bool fa = a.req->r.kind == VPT_REQ_FRAME, fb = b.req->r.kind == VPT_REQ_FRAME;
+            if (fa != fb) return fb;   // frames last (they run after the ray kernel anyway)
+            return a.req->r.world < b.req->r.world;
+        });
         return !L.parts.empty();
     }
 

@@ -17,7 +17,7 @@
 // [tap][cout][cin] copies of fp32 master weights; every product accumulates
 // in fp32 on the MFMA units (v_mfma_f32_16x16x32_bf16).  A 3x3 convolution
 // is an implicit GEMM: M = 16 pixels of an output row, N = 16 output
-// channels, K = 32 input channels of one tap.  A 16x16 pixel tile and its
+// channels, K = 32 input channels of one tap.  A 16x32 pixel tile and its
 // one-pixel halo are staged in LDS per 32-channel slice together with the
 // slice's weights for all nine taps; upsampling and concatenation are folded
 // into that staging, bias / ReLU / 2x2 average pooling into the epilogue.
@@ -73,8 +73,14 @@ __host__ __device__ inline float bf2f(uint16_t h)
 // a bf16 value > 0 (the ReLU mask of a stored activation)
 __device__ inline bool bf_pos(uint16_t h) { return h != 0 && !(h & 0x8000u); }
 
-constexpr int kTile = 16;          // output tile edge (pixels)
-constexpr int kHalo = kTile + 2;
+// Two bf16 values of v kept where the matching halves of m are > 0.
+__device__ inline uint32_t mask_pair(uint32_t v, uint32_t m)
+{
+    const uint32_t lo = bf_pos((uint16_t)(m & 0xffffu)) ? (v & 0xffffu) : 0u;
+    const uint32_t hi = bf_pos((uint16_t)(m >> 16)) ? (v & 0xffff0000u) : 0u;
+    return lo | hi;
+}
+
 constexpr int kCi = 32;            // input channels per K slice
 constexpr int kRow = 40;           // LDS row pitch in halves (32 + 8: conflict-free 16-byte reads)
 
@@ -108,50 +114,96 @@ __device__ inline uint4 load_in8(const ConvArgs& p, int img, int gy, int gx, int
     return *reinterpret_cast<const uint4*>(p.b + off);
 }
 
-// 3x3 convolution, stride 1, zero padding.  Grid: x = 16x16 tiles of all
-// images, y = output channel blocks of COUT_T.  256 threads: wave v owns
-// output rows 4v..4v+3 of the tile.
+// 3x3 convolution, stride 1, zero padding, 512 threads on 16 x 32 pixel
+// tiles (wave v owns rows 4(v%4)..+3, columns 16(v/4)..+15), persistent: a workgroup
+// walks (tile, 32-channel slice) steps, tile = blockIdx.x + k * gridDim.x,
+// with the next step's global loads in flight (in registers) while the
+// current step is multiplied and the finished tile is written out.  The
+// epilogue goes through LDS so each lane stores 16 contiguous bytes.
+// Per wave and tap: NT weight fragments and 4 pixel-row fragments from LDS
+// (conflict-free 16-byte reads at an 80-byte pixel pitch) feed 4 x NT MFMAs.
 template <int COUT_T>
-__global__ void __launch_bounds__(256) conv3x3_kernel(ConvArgs p)
+__global__ void __launch_bounds__(512) conv3x3_wide_kernel(ConvArgs p)
 {
     constexpr int NT = COUT_T / 16;
-    __shared__ __attribute__((aligned(16))) uint16_t sA[kHalo * kHalo * kRow];
-    __shared__ __attribute__((aligned(16))) uint16_t sB[9 * COUT_T * kRow];
+    constexpr int TW = 32, TH = 16, HW = TW + 2, HH = TH + 2;
+    constexpr int A_ELEMS = HH * HW * 4;          // 16-byte pieces of the halo slice
+    constexpr int B_ELEMS = 9 * COUT_T * 4;       // 16-byte pieces of the weight slice
+    constexpr int A_PER = (A_ELEMS + 511) / 512;
+    constexpr int B_PER = (B_ELEMS + 511) / 512;
+    constexpr int OROW = COUT_T + 8;              // epilogue pitch (halves) per pixel
+    constexpr int A_BYTES = HH * HW * kRow * 2, B_BYTES = 9 * COUT_T * kRow * 2;
+    static_assert(8 * 64 * OROW * 2 <= A_BYTES + B_BYTES, "epilogue staging does not fit");
+    __shared__ __attribute__((aligned(16))) uint8_t smem[A_BYTES + B_BYTES];
+    uint16_t* sA = reinterpret_cast<uint16_t*>(smem);
+    uint16_t* sB = reinterpret_cast<uint16_t*>(smem + A_BYTES);
 
-    const int tiles_x = (p.w_ + kTile - 1) / kTile, tiles_y = (p.h + kTile - 1) / kTile;
+    const int tiles_x = (p.w_ + TW - 1) / TW, tiles_y = (p.h + TH - 1) / TH;
     const int per_img = tiles_x * tiles_y;
-    const int t = blockIdx.x;
-    const int img = t / per_img;
-    const int ty0 = ((t % per_img) / tiles_x) * kTile, tx0 = ((t % per_img) % tiles_x) * kTile;
+    const int ntiles = p.n * per_img;
     const int co0 = blockIdx.y * COUT_T;
     const int cin = p.ca + p.cb;
+    const int nch = cin / kCi;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lm = lane & 15, lk = (lane >> 4) * 8;
+    const int wr = (wave & 3) * 4, wc = (wave >> 2) * 16;
+    if ((int)blockIdx.x >= ntiles) return;
+    const int my_tiles = (ntiles - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int nsteps = my_tiles * nch;
+
+    uint4 ra[A_PER], rb[B_PER];
+    // loads of step `st` (tile blockIdx.x + (st / nch) * gridDim.x, slice st % nch)
+#define VPT_FETCH(ST)                                                                                               \
+    do {                                                                                                            \
+        const int t_ = (int)blockIdx.x + ((ST) / nch) * (int)gridDim.x;                                             \
+        const int c_ = ((ST) % nch) * kCi;                                                                          \
+        const int im_ = t_ / per_img;                                                                               \
+        const int y_ = ((t_ % per_img) / tiles_x) * TH, x_ = ((t_ % per_img) % tiles_x) * TW;                       \
+        _Pragma("unroll") for (int k = 0; k < A_PER; k++)                                                           \
+        {                                                                                                           \
+            const int e = tid + k * 512;                                                                            \
+            const int pix = e >> 2, q = e & 3;                                                                      \
+            ra[k] = e < A_ELEMS ? load_in8(p, im_, y_ + pix / HW - 1, x_ + pix % HW - 1, c_ + q * 8)                \
+                                : make_uint4(0u, 0u, 0u, 0u);                                                       \
+        }                                                                                                           \
+        _Pragma("unroll") for (int k = 0; k < B_PER; k++)                                                           \
+        {                                                                                                           \
+            const int e = tid + k * 512;                                                                            \
+            const int row = e >> 2, q = e & 3;                                                                      \
+            const int tap = row / COUT_T, co = row % COUT_T;                                                        \
+            rb[k] = e < B_ELEMS ? *reinterpret_cast<const uint4*>(p.w + ((size_t)tap * p.cout + co0 + co) * cin +   \
+                                                                  c_ + q * 8)                                       \
+                                : make_uint4(0u, 0u, 0u, 0u);                                                       \
+        }                                                                                                           \
+    } while (0)
+#define VPT_STASH()                                                                                                 \
+    do {                                                                                                            \
+        _Pragma("unroll") for (int k = 0; k < A_PER; k++)                                                           \
+        {                                                                                                           \
+            const int e = tid + k * 512;                                                                            \
+            if (e < A_ELEMS) *reinterpret_cast<uint4*>(&sA[(e >> 2) * kRow + (e & 3) * 8]) = ra[k];                \
+        }                                                                                                           \
+        _Pragma("unroll") for (int k = 0; k < B_PER; k++)                                                           \
+        {                                                                                                           \
+            const int e = tid + k * 512;                                                                            \
+            if (e < B_ELEMS) *reinterpret_cast<uint4*>(&sB[(e >> 2) * kRow + (e & 3) * 8]) = rb[k];                \
+        }                                                                                                           \
+    } while (0)
 
     f32x4 acc[4][NT];
+    VPT_FETCH(0);
+    VPT_STASH();
+    __syncthreads();
+    for (int st = 0; st < nsteps; st++) {
+        const int chunk = st % nch;
+        if (chunk == 0) {
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+            for (int r = 0; r < 4; r++)
 #pragma unroll
-        for (int j = 0; j < NT; j++) acc[r][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-
-    for (int c0 = 0; c0 < cin; c0 += kCi) {
-        __syncthreads();
-        // input halo: 18 x 18 pixels x 32 channels, 4 x 16 bytes per pixel
-        for (int e = tid; e < kHalo * kHalo * 4; e += 256) {
-            const int pix = e >> 2, q = e & 3;
-            const int hy = pix / kHalo, hx = pix % kHalo;
-            const uint4 v = load_in8(p, img, ty0 + hy - 1, tx0 + hx - 1, c0 + q * 8);
-            *reinterpret_cast<uint4*>(&sA[pix * kRow + q * 8]) = v;
+                for (int j = 0; j < NT; j++) acc[r][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         }
-        // weights of this slice for all taps: 9 x COUT_T rows of 32 channels
-        for (int e = tid; e < 9 * COUT_T * 4; e += 256) {
-            const int row = e >> 2, q = e & 3;
-            const int tap = row / COUT_T, co = row % COUT_T;
-            const uint4 v =
-                *reinterpret_cast<const uint4*>(p.w + ((size_t)tap * p.cout + co0 + co) * cin + c0 + q * 8);
-            *reinterpret_cast<uint4*>(&sB[row * kRow + q * 8]) = v;
-        }
-        __syncthreads();
+        const bool more = st + 1 < nsteps;
+        if (more) VPT_FETCH(st + 1);
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;
@@ -161,58 +213,89 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvArgs p)
                 bfr[j] = *reinterpret_cast<const bf16x8*>(&sB[(tap * COUT_T + j * 16 + lm) * kRow + lk]);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int hy = wave * 4 + r + dy, hx = lm + dx;
-                const bf16x8 afr = *reinterpret_cast<const bf16x8*>(&sA[(hy * kHalo + hx) * kRow + lk]);
+                const int hy = wr + r + dy, hx = wc + lm + dx;
+                const bf16x8 afr = *reinterpret_cast<const bf16x8*>(&sA[(hy * HW + hx) * kRow + lk]);
 #pragma unroll
                 for (int j = 0; j < NT; j++)
                     acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[j], acc[r][j], 0, 0, 0);
             }
         }
-    }
-
-    // epilogue: lane holds D[pixel x = 4*(lane/16) + i][channel lane%16] of each (row, n-tile)
-    const int xb = tx0 + 4 * (lane >> 4);
+        __syncthreads();
+        if (chunk == nch - 1) {
+            // the tile is complete: epilogue, part 1: bias / ReLU in
+            // registers, bf16 values into this wave's LDS block [64 px][COUT_T],
+            // pooled values stored directly
+            const int t = (int)blockIdx.x + (st / nch) * (int)gridDim.x;
+            const int img = t / per_img;
+            const int ty0 = ((t % per_img) / tiles_x) * TH, tx0 = ((t % per_img) % tiles_x) * TW;
+            uint16_t* so = reinterpret_cast<uint16_t*>(smem) + wave * 64 * OROW;
+            const int xl = 4 * (lane >> 4);
 #pragma unroll
-    for (int j = 0; j < NT; j++) {
-        const int co = co0 + j * 16 + lm;
-        const float bias = p.bias ? p.bias[co] : 0.0f;
-        float q[4][4];
+            for (int j = 0; j < NT; j++) {
+                const int col = j * 16 + lm;
+                const float bias = p.bias ? p.bias[co0 + col] : 0.0f;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int gy = ty0 + wave * 4 + r;
+                for (int r = 0; r < 4; r++) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int gx = xb + i;
-                float v = acc[r][j][i];
-                const bool in = gy < p.h && gx < p.w_;
-                const size_t off = (((size_t)img * p.h + gy) * p.w_ + gx) * p.cout + co;
-                if (p.mask) {
-                    v = (in && bf_pos(p.mask[off])) ? v : 0.0f;
-                } else {
-                    v += bias;
-                    if (p.relu) v = fmaxf(v, 0.0f);
+                    for (int i = 0; i < 4; i++) {
+                        float v = acc[r][j][i];
+                        if (!p.mask) {
+                            v += bias;
+                            if (p.relu) v = fmaxf(v, 0.0f);
+                        }
+                        const uint16_t hv = f2bf(v);
+                        acc[r][j][i] = bf2f(hv);   // the stored value, for the pooled output
+                        so[(r * 16 + xl + i) * OROW + col] = hv;
+                    }
                 }
-                const uint16_t hv = f2bf(v);
-                q[r][i] = bf2f(hv);
-                if (in) p.y[off] = hv;
-            }
-        }
-        if (p.pool) {
-            const int ph = p.h >> 1, pw = p.w_ >> 1;
+                if (p.pool) {
+                    const int ph = p.h >> 1, pw = p.w_ >> 1;
 #pragma unroll
-            for (int r = 0; r < 4; r += 2) {
-                const int gy = ty0 + wave * 4 + r;
+                    for (int r2 = 0; r2 < 2; r2++) {
+                        const int gy = ty0 + wr + 2 * r2;
 #pragma unroll
-                for (int i = 0; i < 4; i += 2) {
-                    const int gx = xb + i;
-                    if (gy < p.h && gx < p.w_) {
-                        const float s = ((q[r][i] + q[r][i + 1]) + (q[r + 1][i] + q[r + 1][i + 1])) * 0.25f;
-                        p.pool[(((size_t)img * ph + (gy >> 1)) * pw + (gx >> 1)) * p.cout + co] = f2bf(s);
+                        for (int i2 = 0; i2 < 2; i2++) {
+                            const int gx = tx0 + wc + xl + 2 * i2;
+                            if (gy < p.h && gx < p.w_) {
+                                const f32x4 a = acc[2 * r2][j], b = acc[2 * r2 + 1][j];
+                                const float sm = ((a[2 * i2] + a[2 * i2 + 1]) + (b[2 * i2] + b[2 * i2 + 1])) * 0.25f;
+                                p.pool[(((size_t)img * ph + (gy >> 1)) * pw + (gx >> 1)) * p.cout + co0 + col] =
+                                    f2bf(sm);
+                            }
+                        }
                     }
                 }
             }
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            // part 2: 16-byte stores of 8 channels of one pixel (mask applied here)
+            constexpr int CH8 = COUT_T / 8;
+#pragma unroll
+            for (int k = 0; k < CH8; k++) {
+                const int e = lane + k * 64;
+                const int px = e / CH8, c8 = e % CH8;
+                const int gy = ty0 + wr + (px >> 4), gx = tx0 + wc + (px & 15);
+                if (gy >= p.h || gx >= p.w_) continue;
+                uint4 v = *reinterpret_cast<const uint4*>(&so[px * OROW + c8 * 8]);
+                const size_t off = (((size_t)img * p.h + gy) * p.w_ + gx) * p.cout + co0 + c8 * 8;
+                if (p.mask) {
+                    const uint4 mk = *reinterpret_cast<const uint4*>(p.mask + off);
+                    v.x = mask_pair(v.x, mk.x);
+                    v.y = mask_pair(v.y, mk.y);
+                    v.z = mask_pair(v.z, mk.z);
+                    v.w = mask_pair(v.w, mk.w);
+                }
+                *reinterpret_cast<uint4*>(p.y + off) = v;
+            }
+            __syncthreads();
+        }
+        if (more) {
+            VPT_STASH();
+            __syncthreads();
         }
     }
+#undef VPT_FETCH
+#undef VPT_STASH
 }
 
 // Weight gradient of a 3x3 convolution:
@@ -590,13 +673,21 @@ const Layer kNet[kLayers] = {
     {D1B, E0B, 1, D0A, -1, 96, 32, 0, 0, 0},    {D0A, -1, 0, D0B, -1, 32, 32, 0, 0, 0},
 };
 
+// Persistent workgroups: one per CU and output-channel block (the wide
+// kernel holds one workgroup per CU), never more than the tiles.
+int g_cus = 0;
+
 int launch_conv(const ConvArgs& a, hipStream_t s)
 {
-    const int tiles = a.n * ((a.h + kTile - 1) / kTile) * ((a.w_ + kTile - 1) / kTile);
-    if (a.cout % 64 == 0) {
-        hipLaunchKernelGGL(conv3x3_kernel<64>, dim3((unsigned)tiles, (unsigned)(a.cout / 64)), dim3(256), 0, s, a);
+    const int tiles = a.n * ((a.h + 15) / 16) * ((a.w_ + 31) / 32);
+    const int cout_t = a.cout % 64 == 0 ? 64 : 32;
+    const int blocks_y = a.cout / cout_t;
+    const int per_block = std::max(1, g_cus / blocks_y);
+    const unsigned gx = (unsigned)std::min(tiles, per_block);
+    if (cout_t == 64) {
+        hipLaunchKernelGGL(conv3x3_wide_kernel<64>, dim3(gx, (unsigned)blocks_y), dim3(512), 0, s, a);
     } else {
-        hipLaunchKernelGGL(conv3x3_kernel<32>, dim3((unsigned)tiles, (unsigned)(a.cout / 32)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(conv3x3_wide_kernel<32>, dim3(gx, (unsigned)blocks_y), dim3(512), 0, s, a);
     }
     HIP_TRY(hipGetLastError());
     return VPT_OK;
@@ -868,6 +959,7 @@ extern "C" VPT_API int vpt_unet_create(vpt_ctx* ctx, int max_n, int max_h, int m
                          max_n, max_h, max_w);
     DeviceGuard g(ctx->device);
     vpt_unet* u = new vpt_unet();
+    if (!g_cus) g_cus = vpt::device_cus(ctx->device);
     u->ctx = ctx;
     u->device = ctx->device;
     u->max_n = max_n;

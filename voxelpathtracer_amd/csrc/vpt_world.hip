@@ -22,7 +22,10 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <stdio.h>
+
 #include <algorithm>
+#include <vector>
 
 #include "vpt_device.h"
 #include "vpt_internal.h"
@@ -617,5 +620,40 @@ extern "C" VPT_API int vpt_world_set_sample_counter(vpt_world* w, unsigned long 
 {
     if (int rc = check_world(w, "vpt_world_set_sample_counter")) return rc;
     w->d_counter = d_counter;
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_world_save(vpt_world* w, const char* path)
+{
+    if (int rc = check_world(w, "vpt_world_save")) return rc;
+    if (!path) return set_error(VPT_EINVAL, "vpt_world_save: null path");
+    std::vector<uint8_t> host(w->voxels());
+    DeviceGuard g(w->device);
+    HIP_TRY(hipMemcpy(host.data(), w->d_grid_lin, host.size(), hipMemcpyDeviceToHost));
+    FILE* f = fopen(path, "wb");
+    if (!f) return set_error(VPT_EINVAL, "vpt_world_save: cannot open %s for writing", path);
+    const size_t n = fwrite(host.data(), 1, host.size(), f);
+    const int closed = fclose(f);
+    if (n != host.size() || closed != 0) return set_error(VPT_EINVAL, "vpt_world_save: short write to %s", path);
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_world_load(vpt_world* w, const char* path, void* stream)
+{
+    if (int rc = check_world(w, "vpt_world_load")) return rc;
+    if (!path) return set_error(VPT_EINVAL, "vpt_world_load: null path");
+    FILE* f = fopen(path, "rb");
+    if (!f) return set_error(VPT_EINVAL, "vpt_world_load: cannot open %s", path);
+    std::vector<uint8_t> host(w->voxels());
+    const size_t n = fread(host.data(), 1, host.size(), f);
+    const bool extra = fgetc(f) != EOF;
+    fclose(f);
+    if (n != host.size() || extra)
+        return set_error(VPT_EINVAL, "vpt_world_load: %s does not hold exactly %zu bytes (%dx%dx%d)", path,
+                         host.size(), w->nx, w->ny, w->nz);
+    if (int rc = vpt_world_upload(w, host.data(), stream)) return rc;
+    // the host buffer is pageable and goes out of scope: finish the copy first
+    DeviceGuard g(w->device);
+    HIP_TRY(hipStreamSynchronize(as_stream(stream)));
     return VPT_OK;
 }

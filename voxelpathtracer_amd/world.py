@@ -156,6 +156,16 @@ class World:
         call("vpt_trace_gi_rays", self.handle, _ptr(rays), n, gi_iter, sd, shadow_iter, _ptr(hits), _ptr(vis),
              _stream(stream))
 
+    def SaveWorld(self, path: str) -> None:
+        """Raw world file: nx*ny*nz block ids, x fastest (WorldFileHandler.cpp:10-37)."""
+        call("vpt_world_save", self.handle, str(path).encode())
+
+    def LoadWorld(self, path: str, stream=None) -> None:
+        """Loads a raw world file into the device grid and m_WorldData
+        (WorldFileHandler.cpp:39-83); rebuild the DF afterwards."""
+        call("vpt_world_load", self.handle, str(path).encode(), _stream(stream))
+        call("vpt_grid_download", self.handle, self.m_WorldData.ctypes.data_as(C.c_void_p), _stream(stream))
+
     def close(self):
         if self.handle:
             lib().vpt_world_destroy(self.handle)
