@@ -257,7 +257,7 @@ def main():
         "svgf_ms_per_frame": round(svgf_ms, 4),
         "roofline": {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "algorithmic_bytes": alg[dom]},
+                     "traffic": profiled_traffic(f"{dom}_kernel"), "algorithmic_bytes": alg[dom]},
     }
 
     if not args.no_learned:
@@ -268,8 +268,28 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
     if not args.no_cpu_baseline:
+        # the frame's GPU outputs again (the buffers served as scratch since)
+        world.trace_primary(cam, gb, 350)
+        world.trace_shadow(cam, gb, sun, mask, 350)
+        world.trace_gi(cam, gb, prm, rgb, tex)
+        torch.cuda.synchronize()
         result["cpu_baseline"] = cpu_baseline(grid, gb, mask, rgb, cam, prm, args.cpu_reps)
     print(json.dumps(result), flush=True)
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE runs scaled by a plain copy's readings), as recorded in
+    profiles/frame_1080p_kernels.md; None when not recorded."""
+    from pathlib import Path
+    p = Path(__file__).resolve().parent / "profiles" / "frame_1080p_kernels.md"
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def serve(args):

@@ -120,7 +120,13 @@ struct Part {
     vpt_world* world;
 };
 
+// A launch slot: its own stream (launches in different slots overlap on
+// the device, so one launch's tail is filled by the next), world table,
+// segment table and staging.
 struct Slot {
+    hipStream_t stream = nullptr;
+    WorldView* d_worlds = nullptr;
+    bool table_dirty = true;
     ServeSegment* d_segs = nullptr;
     uint32_t* d_begin = nullptr;
     uint32_t* d_head = nullptr;
@@ -183,14 +189,12 @@ struct vpt_server {
     std::vector<vpt_world*> worlds;
     std::vector<int> world_refs;
     std::vector<WorldView> world_table;
-    bool table_dirty = true;
-    WorldView* d_worlds = nullptr;
     uint64_t world_bytes = 0;
 
-    std::map<uint64_t, vpt_frame*> frames;   // (W << 32 | H) -> persistent frame chain
+    // persistent frame chains, one per (slot, W, H): slot << 56 | W << 28 | H
+    std::map<uint64_t, vpt_frame*> frames;
 
     Arena arena;
-    hipStream_t stream = nullptr;
     Slot slots[kMaxInflight];
 
     std::mutex mu;
@@ -304,8 +308,7 @@ struct vpt_server {
             if (q->r.kind == VPT_REQ_FRAME) {
                 if (frame_taken) continue;   // one frame per launch
                 frame_taken = true;
-                const uint64_t key = ((uint64_t)q->cam.width << 32) | (uint32_t)q->cam.height;
-                L.parts.push_back(Part{q, 0, q->units, frames.at(key), worlds[q->r.world]});
+                L.parts.push_back(Part{q, 0, q->units, frames.at(frame_key(slot, q->cam)), worlds[q->r.world]});
                 q->scheduled = q->units;
                 q->res.t_start_ns = clock_ns();
             } else {
@@ -353,8 +356,9 @@ struct vpt_server {
     int issue(const Launch& L, const std::vector<WorldView>* table)
     {
         Slot& sl = slots[L.slot];
+        hipStream_t stream = sl.stream;
         if (table) {
-            if (hipMemcpyAsync(d_worlds, table->data(), sizeof(WorldView) * table->size(), hipMemcpyHostToDevice,
+            if (hipMemcpyAsync(sl.d_worlds, table->data(), sizeof(WorldView) * table->size(), hipMemcpyHostToDevice,
                                stream) != hipSuccess)
                 return set_error(VPT_EDEVICE, "vpt_server: world table upload failed");
         }
@@ -387,7 +391,7 @@ struct vpt_server {
                 hipMemcpyAsync(sl.d_begin, sl.h_begin, sizeof(uint32_t) * (nseg + 1), hipMemcpyHostToDevice,
                                stream) != hipSuccess)
                 return set_error(VPT_EDEVICE, "vpt_server: segment upload failed");
-            if (int rc = vpt::serve_launch(cus, L.cfg, d_worlds, L.nworlds, sl.d_segs, sl.d_begin, nseg, total,
+            if (int rc = vpt::serve_launch(cus, L.cfg, sl.d_worlds, L.nworlds, sl.d_segs, sl.d_begin, nseg, total,
                                            sl.d_head, stream))
                 return rc;
         }
@@ -405,6 +409,7 @@ struct vpt_server {
 
     int run_plain(Req* q, vpt_world* w)
     {
+        hipStream_t stream = slots[0].stream;
         const vpt_request& r = q->r;
         int rc = VPT_OK;
         switch (r.kind) {
@@ -429,12 +434,23 @@ struct vpt_server {
         return rc;
     }
 
-    const std::vector<WorldView>* take_table(std::vector<WorldView>& copy)
+    // The world table for a launch in `slot` when that slot's copy is stale.
+    const std::vector<WorldView>* take_table(int slot, std::vector<WorldView>& copy)
     {
-        if (!table_dirty) return nullptr;
+        if (!slots[slot].table_dirty) return nullptr;
         copy = world_table;
-        table_dirty = false;
+        slots[slot].table_dirty = false;
         return &copy;
+    }
+
+    void tables_dirty()
+    {
+        for (Slot& sl : slots) sl.table_dirty = true;
+    }
+
+    static uint64_t frame_key(int slot, const vpt_camera& c)
+    {
+        return ((uint64_t)slot << 56) | ((uint64_t)c.width << 28) | (uint64_t)c.height;
     }
 
     void run()
@@ -473,7 +489,7 @@ struct vpt_server {
                 Launch L;
                 if (build(L)) {
                     slots[L.slot].busy = true;
-                    const std::vector<WorldView>* tbl = take_table(table_copy);
+                    const std::vector<WorldView>* tbl = take_table(L.slot, table_copy);
                     lk.unlock();
                     const int rc = issue(L, tbl);
                     lk.lock();
@@ -482,7 +498,7 @@ struct vpt_server {
                     if (rc) {
                         // the device stream is in an unknown state: wait it out, fail the parts
                         lk.unlock();
-                        (void)hipStreamSynchronize(stream);
+                        for (int i = 0; i < cfg.max_inflight; i++) (void)hipStreamSynchronize(slots[i].stream);
                         lk.lock();
                         while (!inflight.empty()) retire_front(rc);
                     }
@@ -546,11 +562,11 @@ extern "C" VPT_API int vpt_server_destroy(vpt_server* s)
         (void)hipFree(sl.d_head);
         (void)hipHostFree(sl.h_segs);
         (void)hipHostFree(sl.h_begin);
+        (void)hipFree(sl.d_worlds);
         if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
-    (void)hipFree(s->d_worlds);
     (void)hipFree(s->arena.base);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return VPT_OK;
 }
@@ -581,12 +597,12 @@ extern "C" VPT_API int vpt_server_create(vpt_ctx* ctx, const vpt_server_cfg* cfg
         v.xmax = v.ymax = v.zmax = -1.0f;   // empty slot: every ray leaves at once
     }
     const size_t arena = round_up(cfg->arena_bytes);
-    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&s->d_worlds, sizeof(WorldView) * cfg->max_worlds);
-    if (e == hipSuccess) e = hipMalloc(&s->arena.base, arena);
+    hipError_t e = hipMalloc(&s->arena.base, arena);
     for (int i = 0; i < cfg->max_inflight && e == hipSuccess; i++) {
         Slot& sl = s->slots[i];
-        e = hipMalloc(&sl.d_segs, sizeof(ServeSegment) * kMaxSegs);
+        e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_worlds, sizeof(WorldView) * cfg->max_worlds);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_segs, sizeof(ServeSegment) * kMaxSegs);
         if (e == hipSuccess) e = hipMalloc(&sl.d_begin, sizeof(uint32_t) * (kMaxSegs + 1));
         if (e == hipSuccess) e = hipMalloc(&sl.d_head, sizeof(uint32_t) * 4);
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_segs, sizeof(ServeSegment) * kMaxSegs, hipHostMallocDefault);
@@ -620,7 +636,7 @@ extern "C" VPT_API int vpt_server_add_world(vpt_server* s, vpt_world* w, int* sl
                          w->bytes, (unsigned long long)s->world_bytes, (unsigned long long)s->cfg.world_budget);
     s->worlds[free_slot] = w;
     s->world_table[free_slot] = vpt::make_view(w);
-    s->table_dirty = true;
+    s->tables_dirty();
     s->world_bytes += w->bytes;
     s->st.world_bytes = s->world_bytes;
     *slot = free_slot;
@@ -641,7 +657,7 @@ extern "C" VPT_API int vpt_server_remove_world(vpt_server* s, int slot)
     s->worlds[slot] = nullptr;
     memset(&s->world_table[slot], 0, sizeof(WorldView));
     s->world_table[slot].xmax = s->world_table[slot].ymax = s->world_table[slot].zmax = -1.0f;
-    s->table_dirty = true;
+    s->tables_dirty();
     return VPT_OK;
 }
 
@@ -697,25 +713,25 @@ extern "C" VPT_API int vpt_server_submit(vpt_server* s, const vpt_request* req, 
         delete q;
         return set_error(VPT_ENOMEM, "vpt_server_submit: %zu result bytes exceed the arena (%zu)", b, s->arena.size);
     }
-    if (r.kind == VPT_REQ_FRAME) {
-        const uint64_t key = ((uint64_t)q->cam.width << 32) | (uint32_t)q->cam.height;
-        if (!s->frames.count(key)) {
-            // scratch of the persistent frame chain for this resolution, kept resident
-            const uint64_t px = units;
-            const uint64_t scratch = px * (2 * sizeof(vpt_qray) + 7) + 4096;
-            if (s->world_bytes + scratch > s->cfg.world_budget) {
-                delete q;
-                return set_error(VPT_ENOMEM, "vpt_server_submit: frame scratch exceeds the world budget");
-            }
+    if (r.kind == VPT_REQ_FRAME && !s->frames.count(vpt_server::frame_key(0, q->cam))) {
+        // scratch of the persistent frame chain for this resolution, one per
+        // launch slot (slots run concurrently), kept resident
+        const uint64_t px = units;
+        const uint64_t scratch = (px * (2 * sizeof(vpt_qray) + 7) + 4096) * (uint64_t)s->cfg.max_inflight;
+        if (s->world_bytes + scratch > s->cfg.world_budget) {
+            delete q;
+            return set_error(VPT_ENOMEM, "vpt_server_submit: frame scratch exceeds the world budget");
+        }
+        for (int slot = 0; slot < s->cfg.max_inflight; slot++) {
             vpt_frame* f = nullptr;
             if (int rc = vpt_frame_create(s->ctx, q->cam.width, q->cam.height, &f)) {
                 delete q;
                 return rc;
             }
-            s->frames[key] = f;
-            s->world_bytes += scratch;
-            s->st.world_bytes = s->world_bytes;
+            s->frames[vpt_server::frame_key(slot, q->cam)] = f;
         }
+        s->world_bytes += scratch;
+        s->st.world_bytes = s->world_bytes;
     }
     q->ticket = s->next_ticket++;
     q->res.t_arrival_ns = r.arrival_ns ? r.arrival_ns : clock_ns();
