@@ -418,11 +418,20 @@ __global__ void __launch_bounds__(256) gen_kernel(WorldView w, const FrameConst*
             emit_g = true;
         }
     }
+    // compaction: wave counts by ballot + popcount, one atomicAdd per
+    // workgroup (a single queue counter shared by every wave serialises)
+    __shared__ uint32_t wave_total[4], group_base;
     const uint64_t bs = __ballot(emit_s), bg = __ballot(emit_g);
     const uint32_t ns = (uint32_t)__popcll(bs), total = ns + (uint32_t)__popcll(bg);
-    uint32_t base = 0;
-    if (lane == 0 && total) base = atomicAdd(qlen, total);
-    base = __shfl(base, 0);
+    if (lane == 0) wave_total[wave] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t all = (wave_total[0] + wave_total[1]) + (wave_total[2] + wave_total[3]);
+        group_base = all ? atomicAdd(qlen, all) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = group_base;
+    for (int v = 0; v < wave; v++) base += wave_total[v];
     if (emit_s) queue[base + (uint32_t)__popcll(bs & lanemask_lt())] = rs;
     if (emit_g) queue[base + ns + (uint32_t)__popcll(bg & lanemask_lt())] = rg;
 }
