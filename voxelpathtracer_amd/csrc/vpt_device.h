@@ -257,7 +257,12 @@ __device__ __attribute__((always_inline)) inline Hit trav_finish(const WorldView
             if (b > 0) {
                 float dx = r.ox - r.o0x, dy = r.oy - r.o0y, dz = r.oz - r.o0z;
                 h.t = sqrtf((dx * dx + dy * dy) + dz * dz);
-                int s = r.min_idx == 0 ? sgn(r.dx) : (r.min_idx == 1 ? sgn(r.dy) : sgn(r.dz));
+                // the three signs first, then a select: an indexed pick of the
+                // direction components can end up as a scratch-memory load
+                const int sx = sgn(r.dx), sy = sgn(r.dy), sz = sgn(r.dz);
+                int s = sx;
+                s = r.min_idx == 1 ? sy : s;
+                s = r.min_idx == 2 ? sz : s;
                 h.face = face_from_axis(r.min_idx, s);
                 h.block = b;
                 h.voxel = pack_cell(cx, cy, cz);
