@@ -131,8 +131,8 @@ VPT_API int vpt_ctx_device(const vpt_ctx* ctx, int* device);
 
 /* ---- world (World.h:35-222) ----------------------------------------------
  * Device-resident block grid plus distance field, both stored in 8x8x8
- * bricks with Morton order inside a brick.  Dims must be multiples of 8,
- * nx, ny, nz in [8, 1024]. */
+ * bricks with Morton order inside a brick.  nx must be a multiple of 16,
+ * ny and nz multiples of 8, all in [8, 1024] (nx >= 16). */
 VPT_API int vpt_world_create(vpt_ctx* ctx, int nx, int ny, int nz, vpt_world** out);
 VPT_API int vpt_world_destroy(vpt_world* w);
 VPT_API int vpt_world_dims(const vpt_world* w, int* nx, int* ny, int* nz);

@@ -148,3 +148,35 @@ def test_ctypes_struct_layouts_match_header(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_argument_errors_without_a_device():
+    """Entry points reject null handles and bad arguments with status codes
+    before touching a device (the reference printed or threw instead)."""
+    L = _lib.lib()
+    out = C.c_void_p()
+    assert L.vpt_world_create(None, 64, 64, 64, C.byref(out)) == _lib.VPT_EINVAL
+    assert b"null" in L.vpt_last_error()
+    assert L.vpt_trace_rays(None, None, 1, 350, None, None) == _lib.VPT_EINVAL
+    assert L.vpt_trace_gi_rays(None, None, 1, 48, None, 128, None, None, None) == _lib.VPT_EINVAL
+    assert L.vpt_batch_create(None, 4, C.byref(out)) == _lib.VPT_EINVAL
+    assert L.vpt_svgf_create(None, 64, 64, None, C.byref(out)) == _lib.VPT_EINVAL
+    assert L.vpt_unet_create(None, 1, 64, 64, C.byref(out)) == _lib.VPT_EINVAL
+    cfg = _lib.ServerCfg(_lib.VPT_SERVE_BATCHED, 4, 1 << 20, 1 << 30, 1 << 16, 2)
+    assert L.vpt_server_create(None, C.byref(cfg), C.byref(out)) == _lib.VPT_EINVAL
+    assert L.vpt_server_submit(None, None, None) == _lib.VPT_EINVAL
+    assert L.vpt_server_wait(None, C.c_uint64(1), 0, None) == _lib.VPT_EINVAL
+    assert L.vpt_world_load(None, b"/nonexistent", None) == _lib.VPT_EINVAL
+    assert L.vpt_world_save(None, b"/nonexistent") == _lib.VPT_EINVAL
+    # host utilities
+    grid = np.zeros(16 * 8 * 8, np.uint8)
+    assert L.vpt_generate_terrain(0, 8, 8, 1, 2, 1, 2, 0, grid.ctypes.data_as(C.c_void_p)) == _lib.VPT_EINVAL
+    prm = _lib.GiParams()
+    assert L.vpt_gi_params_default(C.c_float(150.0), 0, 3, C.byref(prm)) == _lib.VPT_EINVAL   # sun below the horizon
+    assert L.vpt_gi_params_default(C.c_float(50.0), 0, 3, C.byref(prm)) == _lib.VPT_OK
+    pos = (C.c_float * 3)(1.5, 2.5, 3.5)
+    d = (C.c_float * 3)(0.0, -1.0, 0.0)
+    res = (C.c_int32 * 4)()
+    assert L.vpt_raycast_detect(None, 16, 8, 8, pos, d, res) == _lib.VPT_EINVAL
+    # no device work was needed for any of the above: the error text names the call
+    assert b"vpt_raycast_detect" in L.vpt_last_error()

@@ -303,3 +303,57 @@ def test_svgf_parity_1080p(ctx, terrain_world):
     res = _run_svgf(ctx, w, g, df, 1920, 1080, 2)
     for r in res:
         assert r["rgb_rel_l1"] <= 1e-3
+
+
+@pytest.mark.parametrize("max_iter", [0, 1, 2, 65535])
+def test_ray_queries_iteration_caps(terrain_world, max_iter):
+    """The loop cap at its edges: 0 (no sample), 1, 2 and the largest cap."""
+    g, df, w = terrain_world
+    torch = _torch()
+    rng = np.random.default_rng(31 + max_iter)
+    n = 20_000
+    org = np.stack([rng.uniform(0, 384, n), rng.uniform(0, 128, n), rng.uniform(0, 384, n)], 1)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([org, d], 1).astype(np.float32)
+    dev_rays = torch.from_numpy(rays.view(np.uint8).copy()).cuda()
+    hits = torch.empty(n * C.sizeof(Hit), dtype=torch.uint8, device="cuda")
+    w.trace_rays(dev_rays, hits, n, max_iter)
+    torch.cuda.synchronize()
+    dt = np.dtype([("t", "<f4"), ("voxel", "<u4"), ("face", "u1"), ("block", "u1"), ("samples", "<u2")])
+    got = hits.cpu().numpy().view(dt)
+    ref = oracle.trace_rays(g, df, rays, max_iter)
+    for k in ("t", "face", "block", "samples"):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    assert got["samples"].max() <= max_iter
+
+
+@pytest.mark.parametrize("dims", [(16, 8, 8), (1024, 16, 8), (16, 1024, 16), (16, 8, 1024)])
+def test_extreme_world_shapes(ctx, dims):
+    """Smallest world and worlds at the 1024 limit along each axis: DF and
+    ray queries against the oracle."""
+    torch = _torch()
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx + 3 * ny + 7 * nz)
+    g = random_world(rng, nx, ny, nz, 0.05)
+    w = World(ctx, nx, ny, nz)
+    w.m_WorldData[...] = g
+    w.Buffer()
+    w.GenerateDistanceField()
+    df = oracle.df_build(g)
+    np.testing.assert_array_equal(w.download_df(), df)
+    n = 5000
+    org = np.stack([rng.uniform(-2, nx + 2, n), rng.uniform(-2, ny + 2, n), rng.uniform(-2, nz + 2, n)], 1)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([org, d], 1).astype(np.float32)
+    dev_rays = torch.from_numpy(rays.view(np.uint8).copy()).cuda()
+    hits = torch.empty(n * C.sizeof(Hit), dtype=torch.uint8, device="cuda")
+    w.trace_rays(dev_rays, hits, n, 350)
+    torch.cuda.synchronize()
+    dt = np.dtype([("t", "<f4"), ("voxel", "<u4"), ("face", "u1"), ("block", "u1"), ("samples", "<u2")])
+    got = hits.cpu().numpy().view(dt)
+    ref = oracle.trace_rays(g, df, rays, 350)
+    for k in ("t", "face", "block", "samples"):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    w.close()

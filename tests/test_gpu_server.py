@@ -226,6 +226,11 @@ def test_world_slots_and_errors(ctx, scene):
     with pytest.raises(VptError):
         srv.remove_world(s1)              # the request is not released yet
     srv.release(t)
+    # an empty request completes at once with no results
+    t0 = srv.submit_rays(s1, "gi", reqs[0]["dev"], 0, 48, SUN, 128)
+    out0 = srv.fetch(t0)
+    assert out0["hits"].numel() == 0 and out0["vis"].numel() == 0
+    srv.release(t0)
     srv.remove_world(s1)
     s2 = srv.add_world(worlds[2])
     assert s2 == s1
