@@ -233,7 +233,7 @@ def main():
     svgf_ms = s0.elapsed_time(s1) / nf
 
     result = {
-        "metric": "Grays/s DDA+GI @1080p (bit-exact hits)",
+        "metric": "Grays/s DDA+GI @1080p (bit-exact hits); SVGF ms/frame; served p50/p99 latency",
         "value": round(grays, 4),
         "unit": "Grays/s",
         "n_gpus": 1,
