@@ -8,7 +8,8 @@ Workload (one step = one frame of rays, inputs resident in HBM):
   ray per hit pixel plus a sun-shadow ray at the GI hit.  Grays/s counts
   every ray whose traversal loop is entered.  SVGF ms/frame (pre-spatial,
   temporal, variance, 5 a-trous, decode at 1920x1080) and DF-build ms are
-  reported beside it.
+  reported beside it, with the wall time of an incremental DF update after
+  a one-block edit.
 
 Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
 """
@@ -91,6 +92,7 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     df_ms = e0.elapsed_time(e1) / df_reps
+    df_update = df_update_timing(world, grid, torch)
 
     # ---- rays --------------------------------------------------------------
     cam, view, proj = make_camera(CAM_POS, CAM_YAW, CAM_PITCH, W, H)
@@ -254,6 +256,7 @@ def main():
                   "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()}, "work": counts},
         "persistent": persistent,
         "df_build_ms": round(df_ms, 4),
+        "df_update": df_update,
         "svgf_ms_per_frame": round(svgf_ms, 4),
         "roofline": {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
@@ -275,6 +278,45 @@ def main():
         torch.cuda.synchronize()
         result["cpu_baseline"] = cpu_baseline(grid, gb, mask, rgb, cam, prm, args.cpu_reps)
     print(json.dumps(result), flush=True)
+
+
+def df_update_timing(world, grid, torch, edits: int = 24):
+    """One block placed on the terrain surface, then removed again: GPU time
+    (events around the flush + vpt_world_set_blocks + vpt_df_update) and
+    wall time to completion, beside the recomputed box's size (read in a
+    second pass over the same edits).  The world ends as it started."""
+    rng = np.random.default_rng(7)
+    sites = []
+    while len(sites) < edits:
+        x, z = int(rng.integers(0, NX)), int(rng.integers(0, NZ))
+        y = int(np.argmin(grid[z, :, x] > 0))
+        if not grid[z, y, x]:                 # skip columns solid to the top
+            sites.append((x, y, z))
+    stream = torch.cuda.current_stream()
+    gpu_ms, wall_ms, cells = [], [], []
+    for x, y, z in sites:
+        for block in (5, 0):
+            world.SetBlock(x, y, z, block)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            e0.record(stream)
+            world.UpdateDistanceField()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            wall_ms.append((time.perf_counter() - t0) * 1e3)
+            gpu_ms.append(e0.elapsed_time(e1))
+    for x, y, z in sites:
+        for block in (5, 0):
+            world.SetBlock(x, y, z, block)
+            x0, y0, z0, x1, y1, z1 = world.UpdateDistanceField(region=True)
+            cells.append((x1 - x0) * (y1 - y0) * (z1 - z0))
+    gpu_ms, wall_ms = gpu_ms[4:], wall_ms[4:]     # first calls warm up
+    return {"edit": "one block placed on / removed from the surface", "samples": len(gpu_ms),
+            "gpu_ms_median": round(float(np.median(gpu_ms)), 4), "gpu_ms_p90": round(float(np.percentile(gpu_ms, 90)), 4),
+            "wall_ms_median": round(float(np.median(wall_ms)), 4),
+            "box_cells_median": int(np.median(cells)), "box_cells_max": int(max(cells)),
+            "local_fraction": float(np.mean([c < NX * NY * NZ for c in cells]))}
 
 
 def profiled_traffic(kernel: str):

@@ -71,6 +71,19 @@ public:
         vpt_check(vpt_df_build(m_World, stream));
     }
 
+    // Pending edits flushed, then the DF updated around them only (the
+    // reference follows SetBlock with a full GenerateDistanceField,
+    // World.cpp:367-375); asynchronous on `stream`.
+    void UpdateDistanceField(void* stream = nullptr)
+    {
+        if (m_Pending.empty()) return;
+        std::vector<uint32_t> e;
+        e.reserve(m_Pending.size());
+        for (auto& kv : m_Pending) e.push_back(kv.first);
+        FlushEdits(stream);
+        vpt_check(vpt_df_update(m_World, e.data(), (int)e.size(), nullptr, stream));
+    }
+
     // World.h:186: CPU block picking
     std::array<int, 4> RaycastDetect(const float pos[3], const float dir[3]) const
     {

@@ -162,6 +162,22 @@ VPT_API int vpt_df_build(vpt_world* w, void* stream);
 VPT_API int vpt_world_save(vpt_world* w, const char* path);
 VPT_API int vpt_world_load(vpt_world* w, const char* path, void* stream);
 
+/* Incremental GenerateDistanceField after vpt_world_set_blocks with the same
+ * edit list (the reference rebuilds the whole field after an edit,
+ * World.cpp:367-375 -> World.cpp:69-113).  One workgroup finds, on the
+ * device, a box around the edits outside which no distance can change
+ * (every cell on each of its faces has an old distance below its L1
+ * distance to the edits' bounding box) and rebuilds the field inside it
+ * from its blocks and the unchanged one-cell shell around it, in LDS.  A box
+ * that does not fit (128 KiB with its shell) runs the full rebuild instead,
+ * enqueued behind a flag the search sets, so the call never waits for the
+ * device unless `region` is given: then it synchronises `stream` and
+ * region[6] receives the recomputed box [x0, y0, z0, x1, y1, z1) (the whole
+ * world after a full rebuild, all zero for an empty list).  The result
+ * equals vpt_df_build's; edits outside the world are VPT_EINVAL.  Calls on
+ * one world must not overlap (the world holds one search record). */
+VPT_API int vpt_df_update(vpt_world* w, const uint32_t* edits, int n, int32_t* region, void* stream);
+
 /* Read back (parity tests / World::GetBlock): linear x-fastest layout. */
 VPT_API int vpt_df_download(vpt_world* w, uint8_t* host_df, void* stream);
 VPT_API int vpt_grid_download(vpt_world* w, uint8_t* host_grid, void* stream);

@@ -118,6 +118,7 @@ _PROTOS = {
     "vpt_world_upload_device": (_i, [_vp, _vp, _vp]),
     "vpt_world_set_blocks": (_i, [_vp, _vp, _vp, _i, _vp]),
     "vpt_df_build": (_i, [_vp, _vp]),
+    "vpt_df_update": (_i, [_vp, _vp, _i, _vp, _vp]),
     "vpt_world_save": (_i, [_vp, C.c_char_p]),
     "vpt_world_load": (_i, [_vp, C.c_char_p, _vp]),
     "vpt_df_download": (_i, [_vp, _vp, _vp]),

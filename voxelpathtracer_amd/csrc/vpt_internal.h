@@ -69,4 +69,5 @@ struct vpt_ctx {
 
 struct vpt_world : vpt::World {
     vpt_ctx* ctx = nullptr;
+    int32_t* d_dfbox = nullptr;   // incremental DF update: box found on the device (8 ints)
 };

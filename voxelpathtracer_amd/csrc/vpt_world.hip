@@ -25,6 +25,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "vpt_device.h"
@@ -111,6 +112,25 @@ __global__ void set_blocks_kernel(uint8_t* __restrict__ lin, uint8_t* __restrict
     brk[vpt::brick_offset(x, y, z, row_stride, slab_stride)] = ids[i];
 }
 
+// Short edit lists travel in the kernel arguments: no staging buffer and
+// no host-to-device copy (an interactive edit is a handful of blocks).
+constexpr int kArgEdits = 128;
+struct ArgEdits {
+    uint32_t edit[kArgEdits];
+    uint8_t id[kArgEdits];
+};
+
+__global__ void set_blocks_args_kernel(uint8_t* __restrict__ lin, uint8_t* __restrict__ brk, ArgEdits ed, int n,
+                                       int nx, int ny, uint32_t row_stride, uint32_t slab_stride)
+{
+    const int i = threadIdx.x;
+    if (i >= n) return;
+    const uint32_t e = ed.edit[i];
+    const int x = (int)(e & 1023u), y = (int)((e >> 10) & 1023u), z = (int)((e >> 20) & 1023u);
+    lin[(size_t)z * nx * ny + (size_t)y * nx + x] = ed.id[i];
+    brk[vpt::brick_offset(x, y, z, row_stride, slab_stride)] = ed.id[i];
+}
+
 // ---- DF passes ---------------------------------------------------------------
 
 constexpr int kDfCap = 254;
@@ -149,8 +169,9 @@ __device__ inline int wave_incl_min_rev(int v)
 }
 
 __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restrict__ grid, uint8_t* __restrict__ out,
-                                                         int nx, int ny, int nz)
+                                                         int nx, int ny, int nz, const int32_t* __restrict__ gate)
 {
+    if (gate && !*gate) return;
     extern __shared__ uint8_t slab[];
     const int z = blockIdx.x;
     const size_t base = (size_t)z * nx * ny;
@@ -251,8 +272,9 @@ constexpr int kZTileMax = 128 * 1024;
 
 __global__ void __launch_bounds__(256) df_z_tile_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ df_brick,
                                                         int nx, int ny, int nz, uint32_t row_stride,
-                                                        uint32_t slab_stride)
+                                                        uint32_t slab_stride, const int32_t* __restrict__ gate)
 {
+    if (gate && !*gate) return;
     extern __shared__ uint8_t tile[];   // [z][y 0..7][x 0..31]
     const int tiles_x = nx / 32;
     const int x0 = (blockIdx.x % tiles_x) * 32, y0 = (blockIdx.x / tiles_x) * 8;
@@ -305,8 +327,10 @@ __global__ void __launch_bounds__(256) df_z_tile_kernel(const uint8_t* __restric
 }
 
 // X pass in global memory (slabs too large for LDS): one lane per row.
-__global__ void df_x_kernel(const uint8_t* __restrict__ grid, uint8_t* __restrict__ out, int nx, int ny, int nz)
+__global__ void df_x_kernel(const uint8_t* __restrict__ grid, uint8_t* __restrict__ out, int nx, int ny, int nz,
+                            const int32_t* __restrict__ gate)
 {
+    if (gate && !*gate) return;
     size_t row_id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row_id >= (size_t)ny * nz) return;
     const int cap = min(kDfCap, nx + ny + nz);
@@ -329,8 +353,9 @@ __global__ void df_x_kernel(const uint8_t* __restrict__ grid, uint8_t* __restric
 }
 
 // Y pass in global memory: one lane per (x, z) column, lanes along x.
-__global__ void df_y_kernel(uint8_t* __restrict__ buf, int nx, int ny, int nz)
+__global__ void df_y_kernel(uint8_t* __restrict__ buf, int nx, int ny, int nz, const int32_t* __restrict__ gate)
 {
+    if (gate && !*gate) return;
     size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= (size_t)nx * nz) return;
     int x = (int)(col % nx), z = (int)(col / nx);
@@ -355,8 +380,9 @@ __global__ void df_y_kernel(uint8_t* __restrict__ buf, int nx, int ny, int nz)
 // The final value of every cell is known during the backward sweep and is
 // written straight into the bricked DF.
 __global__ void df_z_kernel(uint8_t* __restrict__ buf, uint8_t* __restrict__ df_brick, int nx, int ny, int nz,
-                            uint32_t row_stride, uint32_t slab_stride)
+                            uint32_t row_stride, uint32_t slab_stride, const int32_t* __restrict__ gate)
 {
+    if (gate && !*gate) return;
     size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= (size_t)nx * ny) return;
     int x = (int)(col % nx), y = (int)(col / nx);
@@ -444,11 +470,13 @@ extern "C" VPT_API int vpt_world_create(vpt_ctx* ctx, int nx, int ny, int nz, vp
     if (e == hipSuccess) e = hipMalloc(&w->d_grid, v);
     if (e == hipSuccess) e = hipMalloc(&w->d_df, v);
     if (e == hipSuccess) e = hipMalloc(&w->d_tmp, v);
+    if (e == hipSuccess) e = hipMalloc(&w->d_dfbox, 8 * sizeof(int32_t));
     if (e != hipSuccess) {
         (void)hipFree(w->d_grid_lin);
         (void)hipFree(w->d_grid);
         (void)hipFree(w->d_df);
         (void)hipFree(w->d_tmp);
+        (void)hipFree(w->d_dfbox);
         delete w;
         return set_error(VPT_ENOMEM, "vpt_world_create: hipMalloc of %zu bytes failed", 4 * v);
     }
@@ -468,6 +496,7 @@ extern "C" VPT_API int vpt_world_destroy(vpt_world* w)
     (void)hipFree(w->d_grid);
     (void)hipFree(w->d_df);
     (void)hipFree(w->d_tmp);
+    (void)hipFree(w->d_dfbox);
     delete w;
     return VPT_OK;
 }
@@ -535,6 +564,16 @@ extern "C" VPT_API int vpt_world_set_blocks(vpt_world* w, const uint32_t* edits,
     }
     DeviceGuard g(w->device);
     hipStream_t s = as_stream(stream);
+    if (n <= kArgEdits) {
+        ArgEdits ed;
+        std::copy(edits, edits + n, ed.edit);
+        std::copy(ids, ids + n, ed.id);
+        const uint32_t rs = (uint32_t)w->nbx * vpt::kBrickBytes, ss = rs * (uint32_t)w->nby;
+        hipLaunchKernelGGL(set_blocks_args_kernel, dim3(1), dim3(kArgEdits), 0, s, w->d_grid_lin, w->d_grid, ed, n,
+                           w->nx, w->ny, rs, ss);
+        HIP_TRY(hipGetLastError());
+        return VPT_OK;
+    }
     uint32_t* d_e = nullptr;
     uint8_t* d_i = nullptr;
     HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d_e), sizeof(uint32_t) * n, s));
@@ -550,25 +589,28 @@ extern "C" VPT_API int vpt_world_set_blocks(vpt_world* w, const uint32_t* edits,
     return VPT_OK;
 }
 
-extern "C" VPT_API int vpt_df_build(vpt_world* w, void* stream)
+namespace {
+
+// GenerateDistanceField's three passes (World.cpp:69-113).  With a gate
+// pointer every kernel returns at once unless *gate is nonzero, so a caller
+// can enqueue the rebuild behind a decision taken on the device.
+int df_build_launch(vpt_world* w, hipStream_t s, const int32_t* gate)
 {
-    if (int rc = check_world(w, "vpt_df_build")) return rc;
-    DeviceGuard g(w->device);
-    hipStream_t s = as_stream(stream);
     const int nx = w->nx, ny = w->ny, nz = w->nz;
     const size_t slab = (size_t)nx * ny;
     if (slab <= (size_t)kSlabMax) {
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_xy_slab_kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kSlabMax));
-        hipLaunchKernelGGL(df_xy_slab_kernel, dim3(nz), dim3(512), slab, s, w->d_grid_lin, w->d_tmp, nx, ny, nz);
+        hipLaunchKernelGGL(df_xy_slab_kernel, dim3(nz), dim3(512), slab, s, w->d_grid_lin, w->d_tmp, nx, ny, nz, gate);
         HIP_TRY(hipGetLastError());
     } else {
         size_t rows = (size_t)ny * nz;
         hipLaunchKernelGGL(df_x_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, w->d_grid_lin,
-                           w->d_tmp, nx, ny, nz);
+                           w->d_tmp, nx, ny, nz, gate);
         HIP_TRY(hipGetLastError());
         size_t cols = (size_t)nx * nz;
-        hipLaunchKernelGGL(df_y_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, w->d_tmp, nx, ny, nz);
+        hipLaunchKernelGGL(df_y_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, w->d_tmp, nx, ny, nz,
+                           gate);
         HIP_TRY(hipGetLastError());
     }
     uint32_t rs = (uint32_t)w->nbx * vpt::kBrickBytes, ss = rs * (uint32_t)w->nby;
@@ -577,14 +619,23 @@ extern "C" VPT_API int vpt_df_build(vpt_world* w, void* stream)
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_z_tile_kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kZTileMax));
         hipLaunchKernelGGL(df_z_tile_kernel, dim3((unsigned)((nx / 32) * (ny / 8))), dim3(256), ztile, s, w->d_tmp,
-                           w->d_df, nx, ny, nz, rs, ss);
+                           w->d_df, nx, ny, nz, rs, ss, gate);
     } else {
         size_t cols = (size_t)nx * ny;
         hipLaunchKernelGGL(df_z_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, w->d_tmp, w->d_df, nx,
-                           ny, nz, rs, ss);
+                           ny, nz, rs, ss, gate);
     }
     HIP_TRY(hipGetLastError());
     return VPT_OK;
+}
+
+}  // namespace
+
+extern "C" VPT_API int vpt_df_build(vpt_world* w, void* stream)
+{
+    if (int rc = check_world(w, "vpt_df_build")) return rc;
+    DeviceGuard g(w->device);
+    return df_build_launch(w, as_stream(stream), nullptr);
 }
 
 static int download_bricked(vpt_world* w, const uint8_t* d_src, uint8_t* host, hipStream_t s)
@@ -655,5 +706,409 @@ extern "C" VPT_API int vpt_world_load(vpt_world* w, const char* path, void* stre
     // the host buffer is pageable and goes out of scope: finish the copy first
     DeviceGuard g(w->device);
     HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return VPT_OK;
+}
+
+// ---- incremental DF update ---------------------------------------------------
+//
+// The DF holds min(cap, L1 distance to the nearest block).  After block edits
+// a cell p can change only if |p - e|_1 <= D_old(p) for some edit e (an added
+// block shortens p's distance only when it is nearer than D_old(p); a removed
+// one lengthens it only for cells whose nearest block it was, with
+// D_old(p) = |p - e| < cap).  Let B be a box around the edits' bounding box
+// K such that every cell q on a face of B that is not on the world's border
+// has D_old(q) < dist_1(q, K).  A changed cell p outside B would, on a
+// monotone lattice path back to its edit e, enter B at such a face cell q
+// with D_old(q) >= D_old(p) - |p - q| >= |p - e| - |p - q| = |q - e| >=
+// dist_1(q, K) (D_old is 1-Lipschitz): impossible.  So every cell outside B
+// keeps its value, and inside B
+//   D_new(p) = min(cap, min over blocks s in B of |p - s|,
+//                       min over shell cells q of D_old(q) + |p - q|)
+// (the shell is the one-cell layer around B: a shortest path to a block
+// outside B leaves B through it).  Three separable min-plus sweeps over the
+// lines through B compute this: B's cells start at 0 (block) or cap, the
+// shell cells keep D_old (already their swept value, D_old being
+// 1-Lipschitz), and every intermediate value stays an upper bound of D_new.
+//
+// One workgroup does it all.  It grows B from K in steps: each step tests,
+// for every face, its current position and kCands - 1 positions further
+// out at geometrically growing margins ((m + 1) 2^c - 1), all at once (one
+// wavefront-aligned slice of face cells per face and candidate), and moves
+// the face to the nearest position that passes; a step in which every face
+// passes where it stands ends the search.  B plus its shell is then staged
+// in LDS, swept, and B written back.  A box whose shell-extended volume
+// passes the LDS budget, or a search past kSearchSteps steps, sets a gate
+// instead, and the full rebuild enqueued behind the kernel runs.
+
+namespace {
+
+constexpr int kSlackBias = 4096;          // > the largest L1 distance in a world
+constexpr int kSearchSteps = 64;
+constexpr int kCands = 8;                 // positions tested per face and step
+constexpr int kSlices = 6 * kCands;
+constexpr int kStepCells = 256 * 1024;    // face cells per step before only the current positions are tested
+constexpr int kLocalLds = 128 * 1024;     // cells of B plus shell held in LDS, one byte each
+constexpr int kShortLine = 16;            // lines up to this length: one lane each; longer: one wavefront
+constexpr int kBatch = 4;                 // global loads in flight per thread
+constexpr int kRow = 16;                  // face-cell slices are padded to DPP rows of 16 lanes
+constexpr int kZRun = 4;                  // cells along z per thread when staging / writing back
+
+// The world dims and the edits' bounding box, inclusive.
+struct EditBox {
+    int k0[3], k1[3];
+    int dims[3];
+};
+
+// Inclusive wavefront scans with DPP: row shifts by 1, 2, 4, 8 within each
+// row of 16 lanes (lanes shifted in from outside the row read the identity),
+// then row_bcast:15 and row_bcast:31 carry rows 0 and 0-1 into the rows
+// above.  No LDS traffic, unlike shuffles.
+template <bool kMax>
+__device__ inline int wave_scan_dpp(int v)
+{
+    constexpr int kId = kMax ? INT_MIN : INT_MAX;
+    auto op = [](int a, int b) { return kMax ? max(a, b) : min(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(kId, v, 0x111, 0xf, 0xf, false));   // row_shr:1
+    v = op(v, __builtin_amdgcn_update_dpp(kId, v, 0x112, 0xf, 0xf, false));   // row_shr:2
+    v = op(v, __builtin_amdgcn_update_dpp(kId, v, 0x114, 0xf, 0xf, false));   // row_shr:4
+    v = op(v, __builtin_amdgcn_update_dpp(kId, v, 0x118, 0xf, 0xf, false));   // row_shr:8
+    v = op(v, __builtin_amdgcn_update_dpp(kId, v, 0x142, 0xa, 0xf, false));   // row_bcast:15 into rows 1, 3
+    v = op(v, __builtin_amdgcn_update_dpp(kId, v, 0x143, 0xc, 0xf, false));   // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+// Exact j / n and j % n for 0 <= j < 2^20, 0 < n <= 2^11, from a float
+// reciprocal: the estimate is within one of the quotient, then corrected.
+struct FastDiv {
+    int n;
+    float inv;
+    __device__ explicit FastDiv(int n_) : n(n_), inv(1.0f / (float)n_) {}
+    __device__ inline int divmod(int j, int& r) const
+    {
+        int q = (int)((float)j * inv);
+        r = j - q * n;
+        if (r < 0) {
+            q--;
+            r += n;
+        } else if (r >= n) {
+            q++;
+            r -= n;
+        }
+        return q;
+    }
+};
+
+__device__ inline int cand_margin(int m, int c) { return c == 0 ? m : (m + 1) * (1 << c) - 1; }
+
+// Min-plus sweep of one line of L cells at stride st, forward then
+// backward: cell k ends at min over j of v[j] + |k - j|.
+__device__ inline void sweep_line_serial(uint8_t* p, int L, int st)
+{
+    int run = 1 << 20;
+    for (int k = 0; k < L; k++) {
+        run = min((int)p[k * st], run + 1);
+        p[k * st] = (uint8_t)run;
+    }
+    run = 1 << 20;
+    for (int k = L - 1; k >= 0; k--) {
+        run = min((int)p[k * st], run + 1);
+        p[k * st] = (uint8_t)run;
+    }
+}
+
+// The same with one wavefront per line: forward f[k] = k + min_{j<=k}(v[j] - j)
+// and backward b[k] = min_{j>=k}(f[j] + j) - k, both as prefix scans over
+// lanes (the backward pass maps lane l of chunk c to k = L - 1 - 64 c - l).
+__device__ inline void sweep_line_wave(uint8_t* p, int L, int st)
+{
+    const int lane = threadIdx.x & 63;
+    constexpr int kBig = 1 << 20;
+    int carry = kBig;
+    for (int t = 0; t < L; t += 64) {
+        const int k = t + lane;
+        int a = k < L ? (int)p[k * st] - k : kBig;
+        a = min(wave_scan_dpp<false>(a), carry);
+        if (k < L) p[k * st] = (uint8_t)(a + k);
+        carry = __builtin_amdgcn_readlane(a, 63);
+    }
+    carry = kBig;
+    for (int t = 0; t < L; t += 64) {
+        const int k = L - 1 - t - lane;
+        int a = k >= 0 ? (int)p[k * st] + k : kBig;
+        a = min(wave_scan_dpp<false>(a), carry);
+        if (k >= 0) p[k * st] = (uint8_t)(a - k);
+        carry = __builtin_amdgcn_readlane(a, 63);
+    }
+}
+
+// out: [0..2] box begin, [3..5] box end (half-open), [6] 1 = updated here,
+// [7] 1 = full rebuild needed (the box then reads as the whole world).
+__global__ void __launch_bounds__(1024) df_update_kernel(const uint8_t* __restrict__ grid_lin,
+                                                        uint8_t* __restrict__ df, EditBox eb, int cap,
+                                                        uint32_t row_stride, uint32_t slab_stride,
+                                                        int32_t* __restrict__ out)
+{
+    extern __shared__ uint8_t cells[];   // during the search: slice of each row of face cells
+    __shared__ int s_margin[6];
+    __shared__ int s_grow[6];            // face still looking for its position: test candidates
+    __shared__ unsigned int s_slack[kSlices];
+    __shared__ int s_begin[kSlices + 1];
+    __shared__ int s_fixed[kSlices], s_area[kSlices], s_nu[kSlices];
+    __shared__ int s_state;   // 0 search on, 1 box found, 2 full rebuild
+    const int tid = threadIdx.x;
+    if (tid < 6) {
+        s_margin[tid] = 0;
+        s_grow[tid] = 1;
+    }
+    int b0[3], b1[3];
+    for (int step = 0;; step++) {
+        __syncthreads();
+        unsigned long long vol = 1;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            b0[a] = max(0, eb.k0[a] - s_margin[2 * a]);
+            b1[a] = min(eb.dims[a], eb.k1[a] + s_margin[2 * a + 1] + 1);
+            vol *= (unsigned long long)(b1[a] - b0[a]);
+        }
+        const int wx = b1[0] - b0[0], wy = b1[1] - b0[1], wz = b1[2] - b0[2];
+        const int face_cells = 2 * (wy * wz + wx * wz + wx * wy);
+        const bool full = vol > (unsigned long long)kLocalLds || step >= kSearchSteps;
+        const bool wide = face_cells * kCands <= kStepCells;   // candidates affordable this step
+        if (!full) {
+            // slice f * kCands + c: face f at candidate c, padded to whole DPP rows
+            if (tid < 64) {
+                int count = 0;
+                if (tid < kSlices) {
+                    const int f = tid / kCands, c = tid % kCands, a = f >> 1, side = f & 1;
+                    const int k0a = a == 0 ? eb.k0[0] : (a == 1 ? eb.k0[1] : eb.k0[2]);
+                    const int k1a = a == 0 ? eb.k1[0] : (a == 1 ? eb.k1[1] : eb.k1[2]);
+                    const int dima = a == 0 ? eb.dims[0] : (a == 1 ? eb.dims[1] : eb.dims[2]);
+                    const int nu = a == 0 ? wy : wx, nv = a == 2 ? wy : wz;
+                    const int mc = cand_margin(s_margin[f], c);
+                    const int pos = side ? k1a + mc : k0a - mc;
+                    const bool border = side ? pos >= dima - 1 : pos <= 0;
+                    const bool tested = c == 0 || (wide && s_grow[f]);
+                    s_fixed[tid] = pos;
+                    s_area[tid] = nu * nv;
+                    s_nu[tid] = nu;
+                    s_slack[tid] = 0;
+                    count = border || !tested ? 0 : (nu * nv + kRow - 1) & ~(kRow - 1);
+                }
+                int incl = count;   // exclusive prefix of the counts
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int t = __shfl_up(incl, d);
+                    if (tid >= d) incl += t;
+                }
+                if (tid <= kSlices) s_begin[tid] = incl - count;
+            }
+            __syncthreads();
+            const int total = s_begin[kSlices], rows = total / kRow;
+            for (int r = tid; r < rows; r += blockDim.x) {
+                const int i = r * kRow;
+                int lo = 0;   // last slice with s_begin <= i
+#pragma unroll
+                for (int w = 32; w > 0; w >>= 1)
+                    if (lo + w <= kSlices - 1 && s_begin[lo + w] <= i) lo += w;
+                cells[r] = (uint8_t)lo;
+            }
+            __syncthreads();
+            // kBatch face cells per thread in flight; rows of 16 lanes share a
+            // slice; every lane of a wavefront reaches the row reductions
+            const int total64 = (total + 63) & ~63;
+            for (int i0 = tid; i0 < total64; i0 += kBatch * 1024) {
+                int sl[kBatch], d[kBatch], dist[kBatch];
+#pragma unroll
+                for (int b = 0; b < kBatch; b++) {
+                    const int i = i0 + b * 1024;
+                    sl[b] = i < total ? (int)cells[i / kRow] : -1;
+                }
+#pragma unroll
+                for (int b = 0; b < kBatch; b++) {
+                    d[b] = 0;
+                    dist[b] = 1 << 20;   // padding lanes: slack far below zero
+                    const int lo = sl[b];
+                    if (lo >= 0) {
+                        const int j = i0 + b * 1024 - s_begin[lo];
+                        if (j < s_area[lo]) {
+                            const int f = lo / kCands, fix = s_fixed[lo];
+                            int ju;
+                            const int jv = FastDiv(s_nu[lo]).divmod(j, ju);
+                            const int x = f < 2 ? fix : b0[0] + ju;
+                            const int y = f < 2 ? b0[1] + ju : (f < 4 ? fix : b0[1] + jv);
+                            const int z = f < 4 ? b0[2] + jv : fix;
+                            dist[b] = max(0, max(eb.k0[0] - x, x - eb.k1[0])) +
+                                      max(0, max(eb.k0[1] - y, y - eb.k1[1])) +
+                                      max(0, max(eb.k0[2] - z, z - eb.k1[2]));
+                            d[b] = df[vpt::brick_offset(x, y, z, row_stride, slab_stride)];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < kBatch; b++) {
+                    if (i0 - (tid & 63) + b * 1024 >= total64) continue;   // uniform per wavefront
+                    int m = max(0, d[b] - dist[b] + kSlackBias);
+                    m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x111, 0xf, 0xf, false));   // row_shr:1
+                    m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x112, 0xf, 0xf, false));   // row_shr:2
+                    m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x114, 0xf, 0xf, false));   // row_shr:4
+                    m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x118, 0xf, 0xf, false));   // row_shr:8
+                    if ((tid & 15) == 15 && sl[b] >= 0 && m) atomicMax(&s_slack[sl[b]], (unsigned int)m);
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) s_state = full ? 2 : 1;
+        __syncthreads();
+        if (!full && tid < 6) {
+            // a slice passes when it has no cells (world border) or every slack is negative
+            const int f = tid, m0 = s_margin[f];
+            const bool tested = wide && s_grow[f];
+            auto passes = [&](int c) { return s_slack[f * kCands + c] < (unsigned int)kSlackBias; };
+            if (passes(0)) {
+                s_grow[f] = 0;
+            } else {
+                int next = -1;
+                for (int c = 1; tested && c < kCands && next < 0; c++)
+                    if (passes(c)) next = cand_margin(m0, c);
+                s_grow[f] = next < 0;
+                if (next < 0) {
+                    const int v = (int)s_slack[f * kCands] - kSlackBias;
+                    next = max(max(m0 + v + 1, 2 * m0), tested ? cand_margin(m0, kCands - 1) + 1 : 0);
+                }
+                s_margin[f] = next;
+                s_state = 0;   // every writer stores the same value
+            }
+        }
+        __syncthreads();
+        if (s_state) break;
+    }
+    // B plus its shell, clipped to the world
+    int e0[3], e1[3];
+    for (int a = 0; a < 3; a++) {
+        e0[a] = max(0, b0[a] - 1);
+        e1[a] = min(eb.dims[a], b1[a] + 1);
+    }
+    const int ex = e1[0] - e0[0], ey = e1[1] - e0[1], ez = e1[2] - e0[2];
+    const bool local = s_state == 1 && (long long)ex * ey * ez <= kLocalLds;
+    if (tid == 0) {
+        for (int a = 0; a < 3; a++) {
+            out[a] = local ? b0[a] : 0;
+            out[3 + a] = local ? b1[a] : eb.dims[a];
+        }
+        out[6] = local;
+        out[7] = !local;
+    }
+    if (!local) return;   // uniform: every thread computed the same box
+    const int nx = eb.dims[0], ny = eb.dims[1];
+    // seeds: B's cells 0 (block) or cap, shell cells their old distance.
+    // One item = kZRun cells along z at one (x, y): the x-y part of both
+    // addresses is computed once, and all loads of the item are in flight.
+    const int zruns = (ez + kZRun - 1) / kZRun;
+    const int items = ex * ey * zruns;
+    const FastDiv div_ex(ex), div_ey(ey);
+    const size_t plane = (size_t)nx * ny;
+    for (int it = tid; it < items; it += blockDim.x) {
+        int xl, yl;
+        const int zr = div_ey.divmod(div_ex.divmod(it, xl), yl);
+        const int x = e0[0] + xl, y = e0[1] + yl;
+        const bool in_xy = x >= b0[0] && x < b1[0] && y >= b0[1] && y < b1[1];
+        const uint8_t* g = grid_lin + (size_t)y * nx + x;
+        const uint8_t* o = df + vpt::brick_offset(x, y, 0, row_stride, slab_stride);
+        uint8_t blk[kZRun], old[kZRun];
+#pragma unroll
+        for (int k = 0; k < kZRun; k++) {
+            const int z = e0[2] + min(zr * kZRun + k, ez - 1);
+            blk[k] = g[(size_t)z * plane];
+            old[k] = o[vpt::brick_offset(0, 0, z, row_stride, slab_stride)];
+        }
+#pragma unroll
+        for (int k = 0; k < kZRun; k++) {
+            const int zl = zr * kZRun + k, z = e0[2] + zl;
+            if (zl < ez)
+                cells[(zl * ey + yl) * ex + xl] =
+                    in_xy && z >= b0[2] && z < b1[2] ? (blk[k] > 0 ? 0 : (uint8_t)cap) : old[k];
+        }
+    }
+    __syncthreads();
+    // sweeps along x, y, z over the lines through B (E-local coordinates)
+    const int lo[3] = {b0[0] - e0[0], b0[1] - e0[1], b0[2] - e0[2]};
+    const int len[3] = {ex, ey, ez};
+    const int stride[3] = {1, ex, ex * ey};
+    const int wave = tid >> 6, waves = blockDim.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+        const int nu = b1[u] - b0[u], lines = nu * (b1[v] - b0[v]);
+        if (len[a] <= kShortLine) {
+            for (int i = tid; i < lines; i += blockDim.x)
+                sweep_line_serial(cells + (lo[u] + i % nu) * stride[u] + (lo[v] + i / nu) * stride[v], len[a],
+                                  stride[a]);
+        } else {
+            for (int i = wave; i < lines; i += waves)
+                sweep_line_wave(cells + (lo[u] + i % nu) * stride[u] + (lo[v] + i / nu) * stride[v], len[a],
+                                stride[a]);
+        }
+        __syncthreads();
+    }
+    const int bx = b1[0] - b0[0], by = b1[1] - b0[1], bz = b1[2] - b0[2];
+    const int bruns = (bz + kZRun - 1) / kZRun;
+    const FastDiv div_bx(bx), div_by(by);
+    for (int it = tid; it < bx * by * bruns; it += blockDim.x) {
+        int xl, yl;
+        const int zr = div_by.divmod(div_bx.divmod(it, xl), yl);
+        const int x = b0[0] + xl, y = b0[1] + yl;
+        uint8_t* o = df + vpt::brick_offset(x, y, 0, row_stride, slab_stride);
+        const uint8_t* c = cells + ((b0[2] - e0[2]) * ey + (y - e0[1])) * ex + (x - e0[0]);
+#pragma unroll
+        for (int k = 0; k < kZRun; k++) {
+            const int zl = zr * kZRun + k;
+            if (zl < bz) o[vpt::brick_offset(0, 0, b0[2] + zl, row_stride, slab_stride)] = c[zl * ex * ey];
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" VPT_API int vpt_df_update(vpt_world* w, const uint32_t* edits, int n, int* region, void* stream)
+{
+    if (int rc = check_world(w, "vpt_df_update")) return rc;
+    if (n < 0 || (n > 0 && !edits)) return set_error(VPT_EINVAL, "vpt_df_update: bad edit list");
+    if (n == 0) {
+        if (region) std::fill(region, region + 6, 0);
+        return VPT_OK;
+    }
+    const int nx = w->nx, ny = w->ny, nz = w->nz;
+    EditBox eb;
+    eb.dims[0] = nx;
+    eb.dims[1] = ny;
+    eb.dims[2] = nz;
+    for (int a = 0; a < 3; a++) {
+        eb.k0[a] = eb.dims[a];
+        eb.k1[a] = -1;
+    }
+    for (int i = 0; i < n; i++) {
+        const int c[3] = {(int)(edits[i] & 1023u), (int)((edits[i] >> 10) & 1023u), (int)((edits[i] >> 20) & 1023u)};
+        if (c[0] >= nx || c[1] >= ny || c[2] >= nz || (edits[i] >> 30))
+            return set_error(VPT_EINVAL, "vpt_df_update: edit %d at (%d,%d,%d) outside the world", i, c[0], c[1], c[2]);
+        for (int a = 0; a < 3; a++) {
+            eb.k0[a] = std::min(eb.k0[a], c[a]);
+            eb.k1[a] = std::max(eb.k1[a], c[a]);
+        }
+    }
+    DeviceGuard g(w->device);
+    hipStream_t s = as_stream(stream);
+    const uint32_t rs = (uint32_t)w->nbx * vpt::kBrickBytes, ss = rs * (uint32_t)w->nby;
+    const int cap = std::min(kDfCap, nx + ny + nz);
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_update_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, kLocalLds));
+    hipLaunchKernelGGL(df_update_kernel, dim3(1), dim3(1024), kLocalLds, s, w->d_grid_lin, w->d_df, eb, cap, rs, ss,
+                       w->d_dfbox);
+    HIP_TRY(hipGetLastError());
+    if (int rc = df_build_launch(w, s, w->d_dfbox + 7)) return rc;
+    if (region) {
+        int32_t box[8];
+        HIP_TRY(hipMemcpyAsync(box, w->d_dfbox, sizeof(box), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        std::copy(box, box + 6, region);
+    }
     return VPT_OK;
 }

@@ -97,6 +97,18 @@ class World:
         self.FlushEdits(stream)
         call("vpt_df_build", self.handle, _stream(stream))
 
+    def UpdateDistanceField(self, stream=None, region: bool = False):
+        """Flushes pending SetBlock edits and updates the DF around them
+        (vpt_df_update, asynchronous).  With region=True waits for the
+        stream and returns the recomputed box (x0, y0, z0, x1, y1, z1)."""
+        if not self._pending:
+            return None
+        edits = np.fromiter(self._pending.keys(), np.uint32)
+        self.FlushEdits(stream)
+        box = (C.c_int32 * 6)() if region else None
+        call("vpt_df_update", self.handle, edits.ctypes.data_as(C.c_void_p), len(edits), box, _stream(stream))
+        return tuple(box) if region else None
+
     # -- World.h:186 -----------------------------------------------------
     def RaycastDetect(self, pos, direction) -> tuple[int, int, int, int]:
         out = (C.c_int32 * 4)()
