@@ -37,6 +37,20 @@ __host__ __device__ inline uint32_t brick_offset(uint32_t x, uint32_t y, uint32_
            ((z >> 3) * slab_stride + (spread3(z & 7u) << 2));
 }
 
+// brick_offset for the traversal's hot loop, same value: the Morton spread
+// of each axis's low three bits is one v_perm_b32 byte lookup in an 8-entry
+// table held in its two source operands (byte k of {hi, lo} = spread of k),
+// and the brick products are 24-bit multiplies (brick index < 2^7, strides
+// < 2^24).  Full-rate instructions only.
+__device__ inline uint32_t brick_offset_fast(uint32_t x, uint32_t y, uint32_t z, uint32_t row_stride,
+                                             uint32_t slab_stride)
+{
+    const uint32_t mx = __builtin_amdgcn_perm(0x49484140u, 0x09080100u, (x & 7u) | 0x0c0c0c00u);
+    const uint32_t my = __builtin_amdgcn_perm(0x92908280u, 0x12100200u, (y & 7u) | 0x0c0c0c00u);
+    const uint32_t mz = __builtin_amdgcn_perm(0x49484140u, 0x09080100u, (z & 7u) | 0x0c0c0c00u);
+    return ((x >> 3) << 9) + mx + __umul24(y >> 3, row_stride) + my + __umul24(z >> 3, slab_stride) + (mz << 2);
+}
+
 struct Vec3 {
     float x, y, z;
 };
@@ -191,7 +205,7 @@ __device__ __attribute__((always_inline)) inline bool trav_prepare(const WorldVi
         r.inter = 0;
         return false;
     }
-    off = brick_offset((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, w.row_stride, w.slab_stride);
+    off = brick_offset_fast((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, w.row_stride, w.slab_stride);
     return true;
 }
 
@@ -201,34 +215,30 @@ __device__ __attribute__((always_inline)) inline bool trav_advance(Trav& r, int 
     int e = df_step(m);
     if (e == 0) return false;
     if (e == 1) {
+        // the one-cell step with selects on the step axis (no divergent
+        // branches: fewer exec-mask moves in the loop)
         const int sx = sgn(r.dx), sy = sgn(r.dy), sz = sgn(r.dz);
         const int hx = (1 + sx) >> 1, hy = (1 + sy) >> 1, hz = (1 + sz) >> 1;
-        int gx = (int)r.ox, gy = (int)r.oy, gz = (int)r.oz;
+        const int gx = (int)r.ox, gy = (int)r.oy, gz = (int)r.oz;
         float wx = r.ox - (float)gx, wy = r.oy - (float)gy, wz = r.oz - (float)gz;
-        float fxx = ((float)hx - wx) * r.ix;
-        float fyy = ((float)hy - wy) * r.iy;
-        float fzz = ((float)hz - wz) * r.iz;
-        int mi = (fxx < fyy && sx != 0) ? ((fxx < fzz || sz == 0) ? 0 : 2) : ((fyy < fzz || sz == 0) ? 1 : 2);
-        float fm = mi == 0 ? fxx : (mi == 1 ? fyy : fzz);
-        wx = wx + r.dx * fm;
-        wy = wy + r.dy * fm;
-        wz = wz + r.dz * fm;
-        if (mi == 0) {
-            gx += sx;
-            wx = (float)(1 - hx);
-        } else if (mi == 1) {
-            gy += sy;
-            wy = (float)(1 - hy);
-        } else {
-            gz += sz;
-            wz = (float)(1 - hz);
-        }
-        r.ox = (float)gx + wx;
-        r.oy = (float)gy + wy;
-        r.oz = (float)gz + wz;
-        if (mi == 0) r.ox = r.ox + (float)sx * 0.0001f;
-        else if (mi == 1) r.oy = r.oy + (float)sy * 0.0001f;
-        else r.oz = r.oz + (float)sz * 0.0001f;
+        const float fxx = ((float)hx - wx) * r.ix;
+        const float fyy = ((float)hy - wy) * r.iy;
+        const float fzz = ((float)hz - wz) * r.iz;
+        const int mi = (fxx < fyy && sx != 0) ? ((fxx < fzz || sz == 0) ? 0 : 2) : ((fyy < fzz || sz == 0) ? 1 : 2);
+        const bool m0 = mi == 0, m1 = mi == 1, m2 = mi == 2;
+        const float fm = m0 ? fxx : (m1 ? fyy : fzz);
+        wx = m0 ? (float)(1 - hx) : wx + r.dx * fm;
+        wy = m1 ? (float)(1 - hy) : wy + r.dy * fm;
+        wz = m2 ? (float)(1 - hz) : wz + r.dz * fm;
+        const float nx = (float)(gx + (m0 ? sx : 0)) + wx;
+        const float ny = (float)(gy + (m1 ? sy : 0)) + wy;
+        const float nz = (float)(gz + (m2 ? sz : 0)) + wz;
+        const float ax = m0 ? nx + (float)sx * 0.0001f : nx;
+        const float ay = m1 ? ny + (float)sy * 0.0001f : ny;
+        const float az = m2 ? nz + (float)sz * 0.0001f : nz;
+        r.ox = ax;
+        r.oy = ay;
+        r.oz = az;
         r.min_idx = mi;
         r.inter = 1;
     } else {
@@ -253,7 +263,7 @@ __device__ __attribute__((always_inline)) inline Hit trav_finish(const WorldView
         float fx = floorf(r.ox), fy = floorf(r.oy), fz = floorf(r.oz);
         if (fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax) {
             int cx = (int)fx, cy = (int)fy, cz = (int)fz;
-            int b = w.grid[brick_offset(cx, cy, cz, w.row_stride, w.slab_stride)];
+            int b = w.grid[brick_offset_fast(cx, cy, cz, w.row_stride, w.slab_stride)];
             if (b > 0) {
                 float dx = r.ox - r.o0x, dy = r.oy - r.o0y, dz = r.oz - r.o0z;
                 h.t = sqrtf((dx * dx + dy * dy) + dz * dz);
@@ -288,8 +298,8 @@ __device__ inline int voxel_at(const WorldView& w, Vec3 p)
 {
     float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
     if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax)) return 0;
-    return w.grid[brick_offset((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, w.row_stride,
-                               w.slab_stride)];
+    return w.grid[brick_offset_fast((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, w.row_stride,
+                                    w.slab_stride)];
 }
 
 // Hardware binary16 conversions (v_cvt_f32_f16 is exact; v_cvt_f16_f32
