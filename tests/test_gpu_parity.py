@@ -303,6 +303,9 @@ def test_svgf_parity_1080p(ctx, terrain_world):
     res = _run_svgf(ctx, w, g, df, 1920, 1080, 2)
     for r in res:
         assert r["rgb_rel_l1"] <= 1e-3
+        # the workload's own frames: every plane bit-exact as well
+        assert r["pre"] == 1.0 and r["temporal"] == 1.0 and r["util"] == 1.0 and r["var"] == 1.0, r
+        assert r["final"] == 1.0, r
 
 
 @pytest.mark.parametrize("max_iter", [0, 1, 2, 65535])

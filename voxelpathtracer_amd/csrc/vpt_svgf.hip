@@ -421,14 +421,22 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
                 Tx s = load_tx(in, j);
                 float slum = sh_to_y(s.sh[3]);
                 float nw = vpt_powi(fmaxf(vpt::dot3(base_n, sn), 0.0f), 16);
-                // exp() of a finite non-positive argument lies in [0, 1], so a
-                // zero normal weight makes the product exactly 0
+                // vpt_expf of a non-positive argument lies in [0, 1] (checked
+                // for every float), so a zero normal weight makes the product
+                // exactly 0.  At a depth gap of 25 or more the product is below
+                // exp(-50) * 3^16 < 1.5e-9 (3^16 is the largest normal weight,
+                // sky against sky), under both clamps below, so it is taken as
+                // 0 without evaluating the exps.  vpt_expf(-0) is exactly 1.
                 float w = 0.0f;
-                if (nw != 0.0f) {
-                    float e = vpt_expf(-fabsf(sw - base_w));
+                const float dd = fabsf(sw - base_w);
+                if (nw != 0.0f && dd < 25.0f) {
+                    float e = 1.0f;
+                    if (dd != 0.0f) e = vpt_expf(-dd);
                     float dw = e * e;
                     float lw = fabsf(slum - base_lum) / color_phi;
-                    w = (vpt_expf(-lw) * nw) * dw;
+                    float el = 1.0f;
+                    if (lw != 0.0f) el = vpt_expf(-lw);
+                    w = (el * nw) * dw;
                 }
                 float w2 = w;
                 w = fmaxf(w, 0.000000015f);
@@ -586,9 +594,17 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
                 float nw = vpt_powi(fmaxf(vpt::dot3(base_n, sn), 0.0f), 32);
                 nw = fclamp(nw, 0.001f, 1.0f);
                 float lw = fabsf(slum - base_lum) / phi;
-                float e = vpt_expf(-fmaxf(ddiff, 0.00001f));
-                float dw = fclamp(e * e, 0.0001f, 1.0f);
-                float w = strong ? (nw * dw) : ((vpt_expf(-lw) * nw) * dw);
+                // exp(-d)^2 < 1e-4 for every float d >= 4.7, so the depth
+                // weight sits on its lower clamp there without the exp
+                float dw = 0.0001f;
+                if (ddiff < 4.7f) {
+                    float e = vpt_expf(-fmaxf(ddiff, 0.00001f));
+                    dw = fclamp(e * e, 0.0001f, 1.0f);
+                }
+                // vpt_expf(-lw) <= 1, so (exp * nw) * dw <= nw * dw: once that
+                // is at most 0.001 the weight sits on its lower clamp
+                float w = nw * dw;
+                if (!strong && w > 0.001f) w = (vpt_expf(-lw) * nw) * dw;
                 w = fclamp(w, 0.001f, 1.0f);
                 float xw = atrous[abs(x)], yw = atrous[abs(y)];
                 w = (xw * yw) * w;
