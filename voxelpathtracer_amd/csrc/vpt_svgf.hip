@@ -383,29 +383,71 @@ __global__ void __launch_bounds__(256) temporal_kernel(CamView cam, Mat16 prev_v
 // ---- VarianceEstimate.main (SVGF/VarianceEstimate.glsl:73-192) -----------
 // The output texel also carries the temporal AO/sky, which the first a-trous
 // pass reads (Pipeline.cpp:2531-2535).
+// The variance pass stages its neighbourhood: a 32x8 block plus the 4-pixel
+// apron of the 9x9 spatial estimate, decoded once into LDS (depth, normal
+// index, SH, CoCg, luminance, moment) instead of once per tap.  A block in
+// which no pixel has a short history skips the staging.  Taps are visited
+// in the reference's order and combined with the same operations.
+constexpr int kVarR = 4;
+constexpr int kVarSW = 32 + 2 * kVarR, kVarSH = 8 + 2 * kVarR, kVarCells = kVarSW * kVarSH;
+
+__device__ inline int normal_index(uint32_t face) { return face <= 5u ? (int)face : 6; }
+
 __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float* __restrict__ gb_t,
                                                        const uint8_t* __restrict__ gb_face,
                                                        const vpt_svgf_texel* __restrict__ in,
                                                        const uint2* __restrict__ util,
                                                        vpt_svgf_texel* __restrict__ out)
 {
-    int px, py;
-    if (!pixel_of(W, H, px, py)) return;
-    const size_t i = (size_t)py * W + px;
-    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    __shared__ float4 s_sh[kVarCells];
+    __shared__ float4 s_aux[kVarCells];   // co, cg, luminance, moment
+    __shared__ float s_t[kVarCells];
+    __shared__ uint8_t s_n[kVarCells];
+    __shared__ float s_nw[7 * 8];         // normal weight by (base, sample) normal index
+    const uint32_t tiles_x = (uint32_t)(W + 31) / 32u;
+    const uint32_t tile = vpt::xcd_remap(blockIdx.x, tiles_x * ((uint32_t)(H + 7) / 8u));
+    const int x0 = (int)(tile % tiles_x) * 32, y0 = (int)(tile / tiles_x) * 8;
+    const int lx = (int)(threadIdx.x & 31), ly = (int)(threadIdx.x >> 5);
+    const int px = x0 + lx, py = y0 + ly;
+    const bool valid = px < W && py < H;
     const float thresh = 4.0f + 4.0f + 4.0f;
+    const size_t i = valid ? (size_t)py * W + px : 0;
+    float frames = 0.0f, base_moment = 0.0f;
+    if (valid) {
+        const uint2 bu = util[i];
+        frames = h2f(bu.x & 0xffffu);
+        base_moment = h2f(bu.x >> 16);
+    }
+    const bool spatial = valid && frames < thresh;
+    if (__syncthreads_or(spatial)) {
+        for (int c = (int)threadIdx.x; c < kVarCells; c += (int)blockDim.x) {
+            const int gx = x0 - kVarR + c % kVarSW, gy = y0 - kVarR + c / kVarSW;
+            if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;   // off screen: never read
+            const size_t j = (size_t)gy * W + gx;
+            const Tx t = load_tx(in, j);
+            s_sh[c] = make_float4(t.sh[0], t.sh[1], t.sh[2], t.sh[3]);
+            s_aux[c] = make_float4(t.co, t.cg, sh_to_y(t.sh[3]), h2f(util[j].x >> 16));
+            s_t[c] = gb_t[j];
+            s_n[c] = (uint8_t)normal_index(gb_face[j]);
+        }
+        if (threadIdx.x < 49) {
+            const int a = (int)threadIdx.x / 7, b = (int)threadIdx.x % 7;
+            const Vec3 na = vpt::face_normal(a == 6 ? 255 : a, 1.0f), nb = vpt::face_normal(b == 6 ? 255 : b, 1.0f);
+            s_nw[a * 8 + b] = vpt_powi(fmaxf(vpt::dot3(na, nb), 0.0f), 16);
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
     float uvx = texc(px, W), uvy = texc(py, H);
-    float base_w = gb_t[i];
-    Vec3 base_n = vpt::face_normal(gb_face[i], 1.0f);
-    uint2 bu = util[i];
-    float frames = h2f(bu.x & 0xffffu);
-    float base_moment = h2f(bu.x >> 16);
     Tx base = load_tx(in, i);
     float base_lum = sh_to_y(base.sh[3]);
     float osh[4], oco, ocg;
     float variance;
-    if (frames < thresh) {
+    if (spatial) {
         const float color_phi = 5.0f;
+        const float base_w = gb_t[i];
+        const float* nw_row = s_nw + normal_index(gb_face[i]) * 8;
         float tw = 0.0f, tmom = 0.0f, tlum = 0.0f, tw2 = 0.0f;
         float tsh[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         float tco = 0.0f, tcg = 0.0f;
@@ -413,14 +455,13 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
             for (int y = -4; y <= 4; y++) {
                 float sx = uvx + (float)x * tex_w;
                 float sy = uvy + (float)y * tex_h;
-                if (!in_screen(sx, sy)) continue;
-                size_t j = (size_t)(py + y) * W + (px + x);
-                float sw = gb_t[j];
-                Vec3 sn = vpt::face_normal(gb_face[j], 1.0f);
-                float smom = h2f(util[j].x >> 16);
-                Tx s = load_tx(in, j);
-                float slum = sh_to_y(s.sh[3]);
-                float nw = vpt_powi(fmaxf(vpt::dot3(base_n, sn), 0.0f), 16);
+                if (!in_screen(sx, sy)) continue;   // exactly the taps inside the image
+                const int c = (ly + kVarR + y) * kVarSW + (lx + kVarR + x);
+                const float sw = s_t[c];
+                const float nw = nw_row[s_n[c]];
+                const float4 a = s_aux[c];
+                const float4 sh = s_sh[c];
+                const float slum = a.z, smom = a.w;
                 // vpt_expf of a non-positive argument lies in [0, 1] (checked
                 // for every float), so a zero normal weight makes the product
                 // exactly 0.  At a depth gap of 25 or more the product is below
@@ -443,9 +484,12 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
                 w2 = fmaxf(w2, 0.0000000015f);
                 tw = tw + w;
                 tmom = tmom + smom * w2;
-                for (int k = 0; k < 4; k++) tsh[k] = tsh[k] + s.sh[k] * w;
-                tco = tco + s.co * w;
-                tcg = tcg + s.cg * w;
+                tsh[0] = tsh[0] + sh.x * w;
+                tsh[1] = tsh[1] + sh.y * w;
+                tsh[2] = tsh[2] + sh.z * w;
+                tsh[3] = tsh[3] + sh.w * w;
+                tco = tco + a.x * w;
+                tcg = tcg + a.y * w;
                 tlum = tlum + slum * w2;
                 tw2 = tw2 + w2;
             }
