@@ -9,7 +9,8 @@ Workload (one step = one frame of rays, inputs resident in HBM):
   every ray whose traversal loop is entered.  SVGF ms/frame (pre-spatial,
   temporal, variance, 5 a-trous, decode at 1920x1080) and DF-build ms are
   reported beside it, with the wall time of an incremental DF update after
-  a one-block edit.
+  a one-block edit and the sun-shadow denoiser's ms/frame over the same
+  sequence.
 
 Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
 """
@@ -66,7 +67,8 @@ def main():
                           "(the path shards by pixel tiles / requests with no collective, see DESIGN.md)"}))
         sys.exit(2)
     import torch
-    from voxelpathtracer_amd import SVGF, Context, World, generate_terrain, gi_params, make_camera, viewproj
+    from voxelpathtracer_amd import (SVGF, Context, ShadowDenoiser, World, generate_terrain, gi_params, make_camera,
+                                     viewproj)
     from voxelpathtracer_amd.world import DeviceGBuffer
 
     W, H = args.width, args.height
@@ -199,6 +201,7 @@ def main():
     # rendered first and stay resident, then the denoiser alone is timed.
     nf = args.svgf_frames
     seq = []
+    shadow_in = []   # per frame: shadow mask and transversal (vpt_trace_shadow2)
     prev_vm = None
     for f in range(nf):
         pos = (CAM_POS[0] + 0.1 * f, CAM_POS[1], CAM_POS[2])
@@ -207,8 +210,12 @@ def main():
         t_f = torch.empty((H, W, 8), dtype=torch.int16, device="cuda")
         world.trace_primary(c_f, g_f, 350)
         world.trace_gi(c_f, g_f, gi_params(50.0, f), rgb, t_f)
+        m_f = torch.empty((H, W), dtype=torch.uint8, device="cuda")
+        tv_f = torch.empty((H, W), dtype=torch.int16, device="cuda")
+        world.trace_shadow2(c_f, g_f, sun, m_f, tv_f, 350)
         vp_f = viewproj(*(prev_vm if prev_vm else (v_f, p_f)))
         seq.append((c_f, vp_f, g_f, t_f))
+        shadow_in.append((m_f, tv_f))
         prev_vm = (v_f, p_f)
     torch.cuda.synchronize()
     sv = SVGF(ctx, W, H)
@@ -234,6 +241,38 @@ def main():
     torch.cuda.synchronize()
     svgf_ms = s0.elapsed_time(s1) / nf
 
+    # ---- sun-shadow denoise (ShadowTemporalFilter + ShadowFilter) over the
+    # same sequence, inputs resident; and the shadow pass with its
+    # transversal output on the frame itself
+    sd = ShadowDenoiser(ctx, W, H)
+    sh_out = torch.empty((H, W), dtype=torch.uint8, device="cuda")
+
+    def shadow_sequence():
+        sd.reset()
+        for f, (c_f, vp_f, g_f, _) in enumerate(seq):
+            g_prev = seq[f - 1][2] if f else g_f
+            m_f, tv_f = shadow_in[f]
+            sd.frame(c_f, vp_f, g_f, g_prev, m_f, tv_f, sh_out)
+
+    shadow_sequence()
+    torch.cuda.synchronize()
+    s0, s1 = ev(), ev()
+    s0.record(stream)
+    shadow_sequence()
+    s1.record(stream)
+    torch.cuda.synchronize()
+    shadow_denoise_ms = s0.elapsed_time(s1) / nf
+    m_tmp, tv_tmp = shadow_in[0]
+    s0, s1 = ev(), ev()
+    s0.record(stream)
+    for _ in range(10):
+        world.trace_shadow2(cam, gb, sun, m_tmp, tv_tmp, 350)
+    s1.record(stream)
+    torch.cuda.synchronize()
+    shadow2_ms = s0.elapsed_time(s1) / 10
+    sd.close()
+    del shadow_in
+
     result = {
         "metric": "Grays/s DDA+GI @1080p (bit-exact hits); SVGF ms/frame; served p50/p99 latency",
         "value": round(grays, 4),
@@ -258,6 +297,8 @@ def main():
         "df_build_ms": round(df_ms, 4),
         "df_update": df_update,
         "svgf_ms_per_frame": round(svgf_ms, 4),
+        "shadow_denoise": {"ms_per_frame": round(shadow_denoise_ms, 4), "shadow_pass_with_transversal_ms":
+                           round(shadow2_ms, 4), "frames": nf},
         "roofline": {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": profiled_traffic(f"{dom}_kernel"), "algorithmic_bytes": alg[dom]},

@@ -56,6 +56,7 @@ enum {
 typedef struct vpt_ctx vpt_ctx;
 typedef struct vpt_world vpt_world;
 typedef struct vpt_svgf vpt_svgf;
+typedef struct vpt_shadow_denoise vpt_shadow_denoise;
 
 /* Camera as the passes see it: inverse view / inverse projection
  * (u_InverseView, u_InverseProjection; ray origin = inv_view[12..14]). */
@@ -192,6 +193,15 @@ VPT_API int vpt_trace_primary(vpt_world* w, const vpt_camera* cam, int max_iter,
  * resolution, unjittered: d_mask[i] = 1 in shadow, 0 lit or sky. */
 VPT_API int vpt_trace_shadow(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const float sun_dir[3],
                              int max_iter, uint8_t* d_mask, void* stream);
+
+/* SHADOW_TRACE with its second output (ShadowRayTraceFrag.glsl:7-8,
+ * 505-511), the input of the shadow denoiser: the same d_mask, and
+ * d_transversal (optional, binary16 bits per pixel) = clamp(T / 100, 1e-5,
+ * 196) of the shadow ray, 4.25 / 100 when it misses, 1 / 100 when
+ * N.L <= 0.01, 64 for sky.  Unlike vpt_trace_shadow, the ray is also traced
+ * when its origin cell is solid, as in the reference. */
+VPT_API int vpt_trace_shadow2(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const float sun_dir[3],
+                              int max_iter, uint8_t* d_mask, uint16_t* d_transversal, void* stream);
 
 /* DIFFUSE_TRACE (DiffuseRayTraceFrag.glsl:822-935), 1 spp, 1 bounce with a
  * shadow ray at the bounce hit.  d_rgb: 3 floats per pixel (clamped to
@@ -406,6 +416,27 @@ VPT_API int vpt_svgf_read_plane(vpt_svgf* s, int which, vpt_svgf_texel* d_dst, v
  * TemporalFilter.glsl: accumulated frames, moment, luma as binary16, plus
  * one unused half): 4 halves per pixel. */
 VPT_API int vpt_svgf_read_utility(vpt_svgf* s, uint16_t* d_dst, void* stream);
+
+/* ---- sun-shadow denoiser -----------------------------------------------------
+ * ShadowTemporalFilter + ShadowFilter (ShadowTemporalFilter.glsl:98-267,
+ * ShadowFilter.glsl:69-165; wiring Pipeline.cpp:2854-2948, SoftShadows and
+ * DenoiseSunShadows on).  Holds the temporal history (colour unorm8, frame
+ * count binary16, ping-pong) for one width x height. */
+VPT_API int vpt_shadow_denoise_create(vpt_ctx* ctx, int width, int height, vpt_shadow_denoise** out);
+VPT_API int vpt_shadow_denoise_destroy(vpt_shadow_denoise* d);
+/* Clears the history (colour 0, frame count 0). */
+VPT_API int vpt_shadow_denoise_reset(vpt_shadow_denoise* d, void* stream);
+/* One frame.  cur/prev: this and the previous frame's G-buffers (t, face;
+ * prev: t); d_mask / d_transversal: vpt_trace_shadow2's outputs for cur;
+ * prev_viewproj: the previous frame's projection * view; filter_scale:
+ * u_ShadowFilterScale (1 by default, Pipeline.cpp:132).  d_out: the
+ * filtered shadow, one unorm8 byte per pixel (ShadowFiltered). */
+VPT_API int vpt_shadow_denoise_frame(vpt_shadow_denoise* d, const vpt_camera* cam, const float prev_viewproj[16],
+                                     const vpt_gbuffer* cur, const vpt_gbuffer* prev, const uint8_t* d_mask,
+                                     const uint16_t* d_transversal, float filter_scale, uint8_t* d_out, void* stream);
+/* The last frame's temporal planes: which = 0 colour (1 byte per pixel),
+ * 1 frame count (binary16 bits). */
+VPT_API int vpt_shadow_denoise_read(vpt_shadow_denoise* d, int which, void* d_dst, void* stream);
 
 /* ---- learned denoiser -------------------------------------------------------
  * A 3-level U-Net (32 / 64 / 128 channels, 3x3 convolutions, 2x2 average

@@ -20,6 +20,12 @@
  *     exp(r) on [-ln2/2, ln2/2], least-squares fitted on Chebyshev nodes
  *     (max rel error ~1e-7), and exponent reconstruction by bit assembly.
  *   - vpt_powi: x^n for integer n by binary exponentiation (fixed order).
+ *   - vpt_logf: natural log.  x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+ *     log m = 2 atanh(s), s = (m-1)/(m+1), summed as the odd series
+ *     2(s + s^3/3 + ... + s^9/9) (|s| <= 0.1716, truncation < 1e-9), plus
+ *     e ln2 with ln2 split as in vpt_expf.
+ *   - vpt_powf: x^y = vpt_expf(y vpt_logf(x)) for x > 0 (GLSL pow is
+ *     undefined for x < 0; 0^y = 0 for y > 0, x^0 = 1).
  *   - vpt_f32_to_f16 / vpt_f16_to_f32: IEEE binary16 conversion with
  *     round-to-nearest-even, the rounding of a GL RGBA16F render target.
  *   - vpt_unorm8_encode / vpt_unorm8_decode: GL unorm8 conversions
@@ -119,6 +125,46 @@ VPT_HD float vpt_expf(float x)
     float s1 = vpt_u2f((uint32_t)(k1 + 127) << 23);
     float s2 = vpt_u2f((uint32_t)(k2 + 127) << 23);
     return (p * s1) * s2;
+}
+
+/* Natural log by the atanh series (see the header comment). */
+VPT_HD float vpt_logf(float x)
+{
+    if (!(x == x)) return x;                                  /* NaN */
+    if (x < 0.0f) return vpt_u2f(0x7fc00000u);                /* NaN */
+    if (x == 0.0f) return vpt_u2f(0xff800000u);               /* -inf */
+    if (x == vpt_u2f(0x7f800000u)) return x;                  /* +inf */
+    uint32_t u = vpt_f2u(x);
+    int e = (int)((u >> 23) & 0xffu) - 127;
+    if (e == -127) {                                          /* subnormal: scale by 2^23 */
+        u = vpt_f2u(x * 8388608.0f);
+        e = (int)((u >> 23) & 0xffu) - 127 - 23;
+    }
+    float m = vpt_u2f((u & 0x007fffffu) | 0x3f800000u);       /* [1, 2) */
+    if (m > 1.41421353816986083984375f) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    const float LN2_HI = 0.693145751953125f;
+    const float LN2_LO = 1.428606765330187045e-06f;
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p = 0.111111111111f;                                /* 1/9 */
+    p = p * s2 + 0.142857142857f;                             /* 1/7 */
+    p = p * s2 + 0.2f;                                        /* 1/5 */
+    p = p * s2 + 0.333333333333f;                             /* 1/3 */
+    float lm = (s + s) + (s + s) * (s2 * p);
+    float fe = (float)e;
+    return fe * LN2_HI + (fe * LN2_LO + lm);
+}
+
+/* x^y for x >= 0 (see the header comment). */
+VPT_HD float vpt_powf(float x, float y)
+{
+    if (y == 0.0f) return 1.0f;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : vpt_u2f(0x7f800000u);
+    if (x == 1.0f) return 1.0f;
+    return vpt_expf(y * vpt_logf(x));
 }
 
 /* x^n, n >= 0, by binary exponentiation (squares taken in a fixed order). */

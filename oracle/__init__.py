@@ -99,6 +99,60 @@ def trace_shadow(grid, df, inv_view, inv_proj, gb_t, gb_face, sun_dir, max_iter:
     return dict(mask=mask, samples=samples, rays=rays)
 
 
+def trace_shadow2(grid, df, inv_view, inv_proj, gb_t, gb_face, sun_dir, max_iter: int = 350):
+    """Shadow pass with its transversal output (binary16 bits), ShadowRayTraceFrag.glsl:414-513."""
+    nz, ny, nx = grid.shape
+    H, W = gb_t.shape
+    mask = np.empty((H, W), np.uint8)
+    tv = np.empty((H, W), np.uint16)
+    samples = np.empty((H, W), np.int32)
+    rays = np.empty((H, W), np.int32)
+    lib().oracle_trace_shadow2(_p(grid), _p(df), nx, ny, nz, _p(_f32(inv_view)), _p(_f32(inv_proj)), W, H,
+                               _p(_f32(gb_t)), _p(np.ascontiguousarray(gb_face, np.uint8)), _p(_f32(sun_dir)), max_iter,
+                               _p(mask), _p(tv), _p(samples), _p(rays))
+    return dict(mask=mask, transversal=tv, samples=samples, rays=rays)
+
+
+def shadow_temporal(inv_view, inv_proj, prev_viewproj, cur_t, cur_face, prev_t, mask, transversal, prev_color,
+                    prev_frames):
+    """ShadowTemporalFilter.main: (colour u8, frame count binary16 bits)."""
+    H, W = cur_t.shape
+    col = np.empty((H, W), np.uint8)
+    frm = np.empty((H, W), np.uint16)
+    u8 = lambda a: np.ascontiguousarray(a, np.uint8)
+    u16 = lambda a: np.ascontiguousarray(a, np.uint16)
+    lib().oracle_shadow_temporal(_p(_f32(inv_view)), _p(_f32(inv_proj)), _p(_f32(prev_viewproj)), W, H,
+                                 _p(_f32(cur_t)), _p(u8(cur_face)), _p(_f32(prev_t)), _p(u8(mask)), _p(u16(transversal)),
+                                 _p(u8(prev_color)), _p(u16(prev_frames)), _p(col), _p(frm))
+    return col, frm
+
+
+def shadow_filter(inv_view, inv_proj, gb_t, gb_face, color, transversal, frames, scale: float = 1.0):
+    """ShadowFilter.main: filtered shadow, u8 unorm."""
+    H, W = gb_t.shape
+    out = np.empty((H, W), np.uint8)
+    lib().oracle_shadow_filter(_p(_f32(inv_view)), _p(_f32(inv_proj)), W, H, _p(_f32(gb_t)),
+                               _p(np.ascontiguousarray(gb_face, np.uint8)), _p(np.ascontiguousarray(color, np.uint8)),
+                               _p(np.ascontiguousarray(transversal, np.uint16)),
+                               _p(np.ascontiguousarray(frames, np.uint16)), C.c_float(scale), _p(out))
+    return out
+
+
+def logf(x):
+    x = _f32(x)
+    y = np.empty_like(x)
+    lib().oracle_logf_n(_p(x), _p(y), C.c_int(x.size))
+    return y
+
+
+def powf(x, e):
+    x, e = np.broadcast_arrays(_f32(x), _f32(e))
+    x, e = np.ascontiguousarray(x), np.ascontiguousarray(e)
+    y = np.empty_like(x)
+    lib().oracle_powf_n(_p(x), _p(e), _p(y), C.c_int(x.size))
+    return y
+
+
 def gi_params_from(prm) -> OGiParams:
     """Copies a product-side GiParams (same layout) into the oracle struct."""
     o = OGiParams()

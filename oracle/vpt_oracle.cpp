@@ -21,6 +21,8 @@
  *                   SVGF/VarianceEstimate.glsl, SVGF/SpatialFilter.glsl,
  *                   wiring Core/Pipeline.cpp:2284-2611
  *   SH decode       Core/Shaders/ColorPassFrag.glsl:356-368
+ *   shadow denoise  Core/Shaders/ShadowTemporalFilter.glsl:98-267,
+ *                   ShadowFilter.glsl:69-165, wiring Core/Pipeline.cpp:2854-2948
  *   CPU picking     Core/World.cpp:496-546 (RaycastDetect)
  *
  * Parity status per function (what pins it besides this transcription):
@@ -35,6 +37,17 @@
  *                          normalisation, running-mean closed form,
  *                          disocclusion passthrough                -> pinned
  *   oracle_camera          closed forms of glm lookAt/perspective  -> pinned
+ *   oracle_trace_shadow2   transversal: a known slab distance, the
+ *                          miss / N.L / sky constants; mask equal to
+ *                          oracle_trace_shadow                     -> pinned
+ *   oracle_shadow_temporal static-camera frame-count ladder, closed-
+ *                          form blend of uniform images, sky
+ *                          passthrough                             -> pinned
+ *   oracle_shadow_filter   uniform-image identity, sharp and sky
+ *                          passthrough                             -> pinned
+ *                          (per-tap weights of both follow the
+ *                          transcription only)
+ *   vpt_logf / vpt_powf    error bounds against double libm        -> pinned
  */
 #include <stdint.h>
 #include <string.h>
@@ -348,10 +361,15 @@ static inline V3 gbuffer_position(const float* inv_view, const float* inv_proj, 
 
 /* Sun shadow (ShadowRayTraceFrag.glsl:414-513, soft-shadow cone off,
  * alpha test off).  mask: 1 = in shadow, 0 = lit or sky. */
-OAPI void oracle_trace_shadow(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
-                              const float* inv_view, const float* inv_proj, int W, int H,
-                              const float* gb_t, const uint8_t* gb_face, const float* sun_dir, int max_iter,
-                              uint8_t* mask, int32_t* samples, int32_t* rays)
+/* SHADOW_TRACE main (ShadowRayTraceFrag.glsl:414-513), both outputs.  The
+ * second, o_IntersectionTransversal (R16F, :8), is clamp(T / 100, 1e-5,
+ * 196), 4.25 / 100 when the ray misses, 1 / 100 when N.L <= 0.01 and 64 for
+ * sky (:433-434, 486-487, 505-511); the ray is traced also when its origin
+ * cell is solid. */
+static void shadow_impl(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+                        const float* inv_view, const float* inv_proj, int W, int H,
+                        const float* gb_t, const uint8_t* gb_face, const float* sun_dir, int max_iter,
+                        uint8_t* mask, uint16_t* transversal, int32_t* samples, int32_t* rays)
 {
     Dims d = {nx, ny, nz};
     V3 L = v3(sun_dir[0], sun_dir[1], sun_dir[2]);
@@ -360,27 +378,57 @@ OAPI void oracle_trace_shadow(const uint8_t* grid, const uint8_t* df, int nx, in
         for (int px = 0; px < W; px++) {
             size_t i = (size_t)py * W + px;
             int smp = 0, nr = 0;
+            float tv;
             float dist = gb_t[i];
-            if (dist < 0.0f) { mask[i] = 0; if (samples) samples[i] = 0; if (rays) rays[i] = 0; continue; }
-            V3 P = gbuffer_position(inv_view, inv_proj, px, py, W, H, dist);
-            V3 N = face_normal(gb_face[i], 1.0f);
-            float ndotl = dot3(N, L);
-            if (ndotl <= 0.01f) { mask[i] = 1; if (samples) samples[i] = 0; if (rays) rays[i] = 0; continue; }
-            V3 bias = v3(N.x * 0.06f, N.y * 0.06f, N.z * 0.06f);
-            V3 o = v3(P.x + bias.x, P.y + bias.y, P.z + bias.z);
-            float T = -1.0f;
-            int block_at = get_voxel_at(grid, d, o);
-            if (dist > 0.0f) {
-                TraceResult tr = voxel_traversal_df(grid, df, d, o, L, max_iter);
-                T = tr.t;
-                smp = tr.samples;
-                nr = 1;
+            if (dist < 0.0f) {
+                mask[i] = 0;
+                tv = 64.0f;
+            } else {
+                V3 P = gbuffer_position(inv_view, inv_proj, px, py, W, H, dist);
+                V3 N = face_normal(gb_face[i], 1.0f);
+                float ndotl = dot3(N, L);
+                if (ndotl <= 0.01f) {
+                    mask[i] = 1;
+                    tv = 1.0f / 100.0f;
+                } else {
+                    V3 bias = v3(N.x * 0.06f, N.y * 0.06f, N.z * 0.06f);
+                    V3 o = v3(P.x + bias.x, P.y + bias.y, P.z + bias.z);
+                    float T = -1.0f;
+                    int block_at = get_voxel_at(grid, d, o);
+                    if (dist > 0.0f) {
+                        TraceResult tr = voxel_traversal_df(grid, df, d, o, L, max_iter);
+                        T = tr.t;
+                        smp = tr.samples;
+                        nr = 1;
+                    }
+                    mask[i] = (uint8_t)((T > 0.0f || block_at > 0) ? 1 : 0);
+                    tv = fclamp(T / 100.0f, 0.00001f, 196.0f);
+                    if (T < 0.0f) tv = 4.25f / 100.0f;
+                }
             }
-            mask[i] = (uint8_t)((T > 0.0f || block_at > 0) ? 1 : 0);
+            if (transversal) transversal[i] = vpt_f32_to_f16(tv);
             if (samples) samples[i] = smp;
             if (rays) rays[i] = nr;
         }
     }
+}
+
+OAPI void oracle_trace_shadow(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+                              const float* inv_view, const float* inv_proj, int W, int H,
+                              const float* gb_t, const uint8_t* gb_face, const float* sun_dir, int max_iter,
+                              uint8_t* mask, int32_t* samples, int32_t* rays)
+{
+    shadow_impl(grid, df, nx, ny, nz, inv_view, inv_proj, W, H, gb_t, gb_face, sun_dir, max_iter, mask, nullptr,
+                samples, rays);
+}
+
+OAPI void oracle_trace_shadow2(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+                               const float* inv_view, const float* inv_proj, int W, int H,
+                               const float* gb_t, const uint8_t* gb_face, const float* sun_dir, int max_iter,
+                               uint8_t* mask, uint16_t* transversal, int32_t* samples, int32_t* rays)
+{
+    shadow_impl(grid, df, nx, ny, nz, inv_view, inv_proj, W, H, gb_t, gb_face, sun_dir, max_iter, mask, transversal,
+                samples, rays);
 }
 
 /* Parameters of the diffuse GI pass.  Host-side constants of the frame
@@ -1215,6 +1263,222 @@ OAPI void oracle_time_plain(const uint8_t* grid, int nx, int ny, int nz, const f
 }
 
 /* OpenMP threads the oracle runs with (cores used by the CPU baseline). */
+/* ======================================================================= */
+/* Sun-shadow denoise: ShadowTemporalFilter + ShadowFilter                  */
+/* ======================================================================= */
+/* Planes: mask u8 0/1 (the shadow pass's o_Shadow), transversal binary16
+ * (o_IntersectionTransversal), temporal colour u8 unorm (GL_RED
+ * UNSIGNED_BYTE) and frame count binary16 (R16F) per pixel
+ * (Pipeline.cpp:1152-1154).  texture() at the pixel's own centre or at
+ * whole-texel offsets from it reads the texel; other coordinates are GL
+ * LINEAR + REPEAT (bilinear above); normal ids are NEAREST.  GLSL pow with
+ * an integer-valued literal exponent is vpt_powi, with a run-time exponent
+ * vpt_powf. */
+namespace {
+
+/* GetShadowSpatial (ShadowTemporalFilter.glsl:98-150) over the raw mask. */
+static float shadow_spatial_pre(int px, int py, int W, int H, float tva, float base_depth, const float* cur_t,
+                                const uint8_t* cur_face, const uint8_t* mask)
+{
+    const float unit_diagonal = sqrtf(2.0f);
+    if (tva <= unit_diagonal * 2.0f) return 1.0f;
+    const size_t i = (size_t)py * W + px;
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    const float uvx = texcoord(px, W), uvy = texcoord(py, H);
+    float total = (float)mask[i];
+    const float base = total;
+    float weight = 1.0f;
+    const V3 bn = face_normal(cur_face[i], 1.0f);
+    const float b = 0.03f;
+    for (int x = -1; x <= 1; x++) {
+        for (int y = -1; y <= 1; y++) {
+            if (x == 0 && y == 0) continue;
+            float sx = uvx + (float)x * tex_w, sy = uvy + (float)y * tex_h;
+            if (!(sx > b && sx < 1.0f - b && sy > b && sy < 1.0f - b)) continue;
+            const size_t j = (size_t)(py + y) * W + (px + x);
+            const float sd = cur_t[j];
+            const V3 sn = face_normal(cur_face[j], 1.0f);
+            if (sn.x == bn.x && sn.y == bn.y && sn.z == bn.z && fabsf(sd - base_depth) < 1.0f) {
+                const float smp = (float)mask[j];
+                float wa = fclamp(1.0f - fclamp(fabsf(smp - base) / 3.0f, 0.0f, 1.0f), 0.0f, 1.0f);
+                wa = fclamp(vpt_powi(wa, 7), 0.000001f, 1.0f);
+                total = total + smp * wa;
+                weight = weight + wa;
+            }
+        }
+    }
+    return total / weight;
+}
+
+} /* namespace */
+
+/* ShadowTemporalFilter.main (ShadowTemporalFilter.glsl:187-267) with
+ * u_ShadowTemporal true (Pipeline.cpp:2856-2908). */
+OAPI void oracle_shadow_temporal(const float* inv_view, const float* inv_proj, const float* prev_viewproj, int W, int H,
+                                 const float* cur_t, const uint8_t* cur_face, const float* prev_t, const uint8_t* mask,
+                                 const uint16_t* transversal, const uint8_t* prev_color, const uint16_t* prev_frames,
+                                 uint8_t* o_color, uint16_t* o_frames)
+{
+    Frame fr = {inv_view, inv_proj, W, H};
+    std::vector<float> pc((size_t)W * H), pf((size_t)W * H);
+    for (size_t i = 0; i < pc.size(); i++) {
+        pc[i] = vpt_unorm8_decode(prev_color[i]);
+        pf[i] = vpt_f16_to_f32(prev_frames[i]);
+    }
+    Tex Tpc = {pc.data(), W, H, 1}, Tpf = {pf.data(), W, H, 1}, Tpt = {prev_t, W, H, 1};
+#pragma omp parallel for schedule(static)
+    for (int py = 0; py < H; py++) {
+        for (int px = 0; px < W; px++) {
+            const size_t i = (size_t)py * W + px;
+            const float uvx = texcoord(px, W), uvy = texcoord(py, H);
+            const float dist = cur_t[i];
+            float oc, of;
+            if (dist > 0.0f) {
+                const V3 pos = position_from(fr, uvx, uvy, dist);
+                float p4[4] = {pos.x, pos.y, pos.z, 1.0f}, q[4];
+                mat_vec4(prev_viewproj, p4, q);
+                const float rx = (q[0] / q[3]) * 0.5f + 0.5f, ry = (q[1] / q[3]) * 0.5f + 0.5f;
+                const float tva = vpt_f16_to_f32(transversal[i]) * 100.0f;
+                float cur = shadow_spatial_pre(px, py, W, H, tva, dist, cur_t, cur_face, mask);
+                const float bias = 0.005f;
+                const bool rejected = !(rx > 0.0f + bias && rx < 1.0f - bias && ry > 0.0f + bias && ry < 1.0f - bias);
+                if (!rejected) {
+                    const float prev_orig = bilinear(Tpc, rx, ry, 0);
+                    float prev = prev_orig;
+                    if (tva < 1.414f * 3.0f) {   /* ClipShadow (:166-185) */
+                        const int offs[5][2] = {{-1, 0}, {1, 0}, {0, 0}, {0, -1}, {0, 1}};
+                        float mn = 100.0f, mx = -100.0f;
+                        for (int k = 0; k < 5; k++) {
+                            const float smp = (float)mask[(size_t)wrap(py + offs[k][1], H) * W + wrap(px + offs[k][0], W)];
+                            mn = fminf(smp, mn);
+                            mx = fmaxf(smp, mx);
+                        }
+                        const float lo = mn - 0.025f, hi = mx + 0.025f;   /* clipAABB (:156-164) */
+                        const float pcl = 0.5f * (hi + lo), ecl = 0.5f * (hi - lo);
+                        const float vcl = prev_orig - pcl;
+                        const float denom = fabsf(vcl / ecl);
+                        prev = denom > 1.0f ? pcl + vcl / denom : prev_orig;
+                    }
+                    const float pw = bilinear(Tpt, rx, ry, 0);
+                    const V3 pp = position_from(fr, rx, ry, pw);
+                    const V3 dd = v3(pp.x - pos.x, pp.y - pos.y, pp.z - pos.z);
+                    const float d = sqrtf(dot3(dd, dd));
+                    cur = fclamp(cur, 0.0f, 1.0f);
+                    prev = fclamp(prev, 0.0f, 1.0f);
+                    const float vx = (uvx - rx) * (float)W, vy = (uvy - ry) * (float)H;
+                    const float clip_err = fabsf(prev_orig - prev);
+                    const float inc = clip_err < 0.2f ? 1.0f : 0.6f;
+                    const float fc = bilinear(Tpf, rx, ry, 0);
+                    const float fi = fc + inc;
+                    float bf = fclamp((1.0f - (1.0f / fi)) * 1.2f, 0.01f, 0.97f);
+                    const float vr = fclamp(vpt_expf(-sqrtf(vx * vx + vy * vy)) * 0.8f + 0.6f, 0.00000001f, 1.0f);
+                    bf = bf * vr;
+                    float dr = 1.0f;
+                    if (d > 0.4f) {
+                        dr = vpt_powi(vpt_expf(-d), 48);
+                        bf = bf * fclamp(dr, 0.0f, 1.0f);
+                    }
+                    const float a = fclamp(bf, 0.0f, 0.97f);
+                    oc = cur * (1.0f - a) + prev * a;
+                    const float mults = dr * vr;
+                    of = fc + fclamp(mults * 1.1f, 0.0f, 1.0f);
+                    if (mults < 0.1f) of = 0.0f;
+                    else if (mults <= 0.2f + 0.001f) of = 2.0f;
+                    else if (mults <= 0.3f + 0.001f) of = 3.25f;
+                } else {
+                    oc = cur;
+                    of = 0.0f;
+                }
+            } else {
+                oc = (float)mask[i];
+                of = 0.0f;
+            }
+            of = fclamp(of, 0.0f, 256.0f);
+            o_color[i] = vpt_unorm8_encode(oc);
+            o_frames[i] = vpt_f32_to_f16(of);
+        }
+    }
+}
+
+/* ShadowFilter.main / ShadowSpatial (ShadowFilter.glsl:69-165) over the
+ * temporal output, u_ShadowFilterScale = scale (Pipeline.cpp:2912-2948). */
+OAPI void oracle_shadow_filter(const float* inv_view, const float* inv_proj, int W, int H, const float* gb_t,
+                               const uint8_t* gb_face, const uint8_t* color, const uint16_t* transversal,
+                               const uint16_t* frames, float scale, uint8_t* out)
+{
+    (void)inv_view;
+    (void)inv_proj;
+    std::vector<float> cf((size_t)W * H);
+    for (size_t i = 0; i < cf.size(); i++) cf[i] = vpt_unorm8_decode(color[i]);
+    Tex Tt = {gb_t, W, H, 1}, Tc = {cf.data(), W, H, 1};
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+#pragma omp parallel for schedule(static)
+    for (int py = 0; py < H; py++) {
+        for (int px = 0; px < W; px++) {
+            const size_t i = (size_t)py * W + px;
+            const float uvx = texcoord(px, W), uvy = texcoord(py, H);
+            const float fr = vpt_f16_to_f32(frames[i]);
+            const bool apply_luma = fr > 7.5f;
+            const float dist = gb_t[i];
+            const bool sky = dist < 0.0f;
+            const V3 cn = face_normal(gb_face[i], 1.0f);
+            const float cs = cf[i];
+            const float T = vpt_f16_to_f32(transversal[i]) * 100.0f;
+            const float cutoff = sqrtf(2.0f);
+            if ((T > 0.0f && T < cutoff) || sky) {
+                out[i] = vpt_unorm8_encode(cs);
+                continue;
+            }
+            const int size = T < cutoff * 1.414f ? 1 : 3;
+            float sc = 1.0f;
+            if (T > 6.0f) sc = 2.0f;
+            if (T > 16.0f) sc = 2.4f;
+            if (T > 32.0f) sc = 2.6f;
+            const float ct = fclamp(T, 0.0f, 10.0f);
+            const float a = ct / 10.0f;
+            float ve = (20.0f * (1.0f - a) + 6.0f * a) + (T < 6.0f ? 5.0f : 2.0f);
+            ve = fclamp(ve - 1.75f, 0.0000001f, 64.0f);
+            float lm = 1.0f;
+            if (!apply_luma) {
+                const float fn = fr / 7.5f;
+                lm = 0.1f * (1.0f - fn) + 0.5f * fn;
+            }
+            const float expo = (ve * lm) * 0.9f;
+            float tw = 0.0f, ts = 0.0f;
+            for (int x = -size; x <= size; x++) {
+                for (int y = -size; y <= size; y++) {
+                    const float sx = uvx + (((float)x * tex_w) * 1.2f * sc) * scale;
+                    const float sy = uvy + (((float)y * tex_h) * 1.2f * sc) * scale;
+                    const float sd = bilinear(Tt, sx, sy, 0);
+                    const V3 sn = face_normal(nearest_u8(gb_face, W, H, sx, sy), 1.0f);
+                    const float dw = vpt_powi(vpt_expf(-fabsf(dist - sd)), 3);
+                    const float nw = vpt_powi(fmaxf(dot3(cn, sn), 0.000000001f), 32);
+                    const float sa = bilinear(Tc, sx, sy, 0);
+                    const float le = fclamp(1.0f - fclamp(fabsf(sa - cs) / 3.0f, 0.0f, 1.0f), 0.0f, 1.0f);
+                    float w = 1.0f;
+                    w = w * fclamp(vpt_powf(le, expo), 0.0f, 1.0f);
+                    w = w * dw;
+                    w = w * nw;
+                    w = fclamp(w, 0.000000001f, 1.0f);
+                    ts = ts + sa * w;
+                    tw = tw + w;
+                }
+            }
+            out[i] = vpt_unorm8_encode(ts / fmaxf(tw, 0.01f));
+        }
+    }
+}
+
+OAPI void oracle_logf_n(const float* x, float* y, int n)
+{
+    for (int i = 0; i < n; i++) y[i] = vpt_logf(x[i]);
+}
+
+OAPI void oracle_powf_n(const float* x, const float* e, float* y, int n)
+{
+    for (int i = 0; i < n; i++) y[i] = vpt_powf(x[i], e[i]);
+}
+
 OAPI int oracle_threads(void) { return omp_get_max_threads(); }
 
 /* ======================================================================= */

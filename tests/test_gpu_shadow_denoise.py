@@ -1,0 +1,108 @@
+"""GPU parity of the sun-shadow denoise chain with the oracle: the shadow
+pass's transversal output (ShadowRayTraceFrag.glsl:505-511), then
+ShadowTemporalFilter and ShadowFilter (ShadowTemporalFilter.glsl:98-267,
+ShadowFilter.glsl:69-165) over a moving-camera sequence.  Every plane of
+every frame is compared bit for bit."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import camera, svgf_camera_path, terrain
+from voxelpathtracer_amd import ShadowDenoiser, World, gi_params, viewproj
+from voxelpathtracer_amd.world import DeviceGBuffer
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def terrain_world(ctx):
+    g = terrain(384, 128, 384)
+    w = World(ctx, 384, 128, 384)
+    w.m_WorldData[...] = g
+    w.Buffer()
+    w.GenerateDistanceField()
+    yield g, oracle.df_build(g), w
+    w.close()
+
+
+def _run(ctx, w, g, df, W, H, frames, scale=1.0):
+    import torch
+    sd = ShadowDenoiser(ctx, W, H)
+    gbs = [DeviceGBuffer(W, H), DeviceGBuffer(W, H)]
+    mask = torch.empty((H, W), dtype=torch.uint8, device="cuda")
+    mask0 = torch.empty((H, W), dtype=torch.uint8, device="cuda")
+    tv = torch.empty((H, W), dtype=torch.int16, device="cuda")
+    out = torch.empty((H, W), dtype=torch.uint8, device="cuda")
+    col = torch.empty((H, W), dtype=torch.uint8, device="cuda")
+    frm = torch.empty((H, W), dtype=torch.int16, device="cuda")
+    sun = np.float32(list(gi_params(50.0, 0).sun_dir))
+    hist_c = np.zeros((H, W), np.uint8)
+    hist_f = np.zeros((H, W), np.uint16)
+    prev_vm, prev_host = None, None
+    for f in range(frames):
+        pos, yaw, pitch = svgf_camera_path(f)
+        cam, view, proj, iv, ip = camera(W, H, pos, yaw, pitch)
+        cur = gbs[f % 2]
+        prev = gbs[(f + 1) % 2] if f else cur
+        vp = viewproj(*(prev_vm if prev_vm else (view, proj)))
+        print(f"shadow denoise frame {f} {W}x{H}", flush=True)
+        w.trace_primary(cam, cur, 350)
+        w.trace_shadow2(cam, cur, sun, mask, tv, 350)
+        w.trace_shadow(cam, cur, sun, mask0, 350)
+        sd.frame(cam, vp, cur, prev, mask, tv, out, filter_scale=scale)
+        sd.read(0, col)
+        sd.read(1, frm)
+        torch.cuda.synchronize()
+        host = cur.numpy()
+        prev_t = host["t"] if prev_host is None else prev_host["t"]
+        ref = oracle.trace_shadow2(g, df, iv, ip, host["t"], host["face"], sun)
+        np.testing.assert_array_equal(mask.cpu().numpy(), ref["mask"])
+        np.testing.assert_array_equal(mask0.cpu().numpy(), ref["mask"])
+        np.testing.assert_array_equal(tv.cpu().numpy().view(np.uint16), ref["transversal"])
+        oc, of = oracle.shadow_temporal(iv, ip, vp, host["t"], host["face"], prev_t, ref["mask"], ref["transversal"],
+                                        hist_c, hist_f)
+        np.testing.assert_array_equal(col.cpu().numpy(), oc, err_msg=f"temporal colour, frame {f}")
+        np.testing.assert_array_equal(frm.cpu().numpy().view(np.uint16), of, err_msg=f"frame count, frame {f}")
+        oo = oracle.shadow_filter(iv, ip, host["t"], host["face"], oc, ref["transversal"], of, scale)
+        np.testing.assert_array_equal(out.cpu().numpy(), oo, err_msg=f"filtered, frame {f}")
+        hist_c, hist_f = oc, of
+        prev_vm, prev_host = (view, proj), host
+    sd.close()
+    return hist_f
+
+
+def test_shadow_denoise_sequence_small(ctx, terrain_world):
+    g, df, w = terrain_world
+    frames = _run(ctx, w, g, df, 320, 180, 6)
+    assert (frames.view(np.float16) > 0).mean() > 0.3   # history accumulates on most lit pixels
+
+
+def test_shadow_denoise_1080p(ctx, terrain_world):
+    g, df, w = terrain_world
+    _run(ctx, w, g, df, 1920, 1080, 2)
+
+
+def test_shadow_denoise_filter_scale(ctx, terrain_world):
+    """u_ShadowFilterScale 2.5 (taps reach ~23 texels, past the image edge)."""
+    g, df, w = terrain_world
+    _run(ctx, w, g, df, 256, 144, 3, scale=2.5)
+
+
+def test_shadow_denoise_argument_errors(ctx):
+    from voxelpathtracer_amd._lib import VptError
+    with pytest.raises(VptError):
+        ShadowDenoiser(ctx, 1, 100)
+    sd = ShadowDenoiser(ctx, 64, 32)
+    import torch
+    gb = DeviceGBuffer(64, 32)
+    cam, view, proj, iv, ip = camera(64, 32)
+    m = torch.zeros((32, 64), dtype=torch.uint8, device="cuda")
+    t = torch.zeros((32, 64), dtype=torch.int16, device="cuda")
+    with pytest.raises(VptError):
+        sd.frame(cam, viewproj(view, proj), gb, gb, m, t, m, filter_scale=0.0)
+    gb2 = DeviceGBuffer(32, 32)
+    with pytest.raises(VptError):
+        sd.frame(cam, viewproj(view, proj), gb2, gb2, m, t, m)
+    sd.close()

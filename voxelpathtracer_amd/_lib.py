@@ -125,6 +125,7 @@ _PROTOS = {
     "vpt_grid_download": (_i, [_vp, _vp, _vp]),
     "vpt_trace_primary": (_i, [_vp, C.POINTER(Camera), _i, C.POINTER(GBuffer), _vp]),
     "vpt_trace_shadow": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _vp, _vp]),
+    "vpt_trace_shadow2": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _vp, _vp, _vp]),
     "vpt_trace_gi": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(GiParams), _vp, _vp, _vp]),
     "vpt_trace_rays": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "vpt_trace_gi_rays": (_i, [_vp, _vp, _i, _i, C.POINTER(_f), _i, _vp, _vp, _vp]),
@@ -176,6 +177,12 @@ _PROTOS = {
                             _vp, _vp, _vp]),
     "vpt_svgf_read_plane": (_i, [_vp, _i, _vp, _vp]),
     "vpt_svgf_read_utility": (_i, [_vp, _vp, _vp]),
+    "vpt_shadow_denoise_create": (_i, [_vp, _i, _i, _vp]),
+    "vpt_shadow_denoise_destroy": (_i, [_vp]),
+    "vpt_shadow_denoise_reset": (_i, [_vp, _vp]),
+    "vpt_shadow_denoise_frame": (_i, [_vp, C.POINTER(Camera), C.POINTER(_f), C.POINTER(GBuffer), C.POINTER(GBuffer),
+                                      _vp, _vp, _f, _vp, _vp]),
+    "vpt_shadow_denoise_read": (_i, [_vp, _i, _vp, _vp]),
     "vpt_generate_terrain": (_i, [_i, _i, _i, _u32, _u32, _i, _i, _i, _vp]),
     "vpt_camera_make": (_i, [C.POINTER(_f), _f, _f, _f, _i, _i, _f, _f, C.POINTER(Camera), C.POINTER(_f),
                              C.POINTER(_f)]),

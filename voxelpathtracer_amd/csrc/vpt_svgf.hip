@@ -40,6 +40,14 @@ struct vpt_svgf {
     bool have_history = false;
 };
 
+struct vpt_shadow_denoise {
+    int device = 0;
+    int W = 0, H = 0;
+    uint8_t* d_color[2] = {};     // ShadowTemporalFBO colour (GL_RED unorm8), ping-pong history
+    uint16_t* d_frames[2] = {};   // ShadowTemporalFBO frame count (R16F bits), ping-pong
+    int cur = 0;                  // index of the newest temporal planes
+};
+
 namespace {
 
 struct DeviceGuard {
@@ -717,6 +725,231 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
+// ---- Sun-shadow denoise: ShadowTemporalFilter + ShadowFilter ---------------
+// Planes: raw mask u8 0/1 (o_Shadow), transversal binary16
+// (o_IntersectionTransversal), temporal colour u8 unorm and frame count
+// binary16 (Pipeline.cpp:1152-1154).  texture() at the pixel's own centre or
+// at whole-texel offsets from it reads the texel; other coordinates are GL
+// LINEAR + REPEAT; normal ids are NEAREST.  GLSL pow with an integer-valued
+// literal exponent is vpt_powi, with a run-time exponent vpt_powf.
+
+__device__ inline int wrapn(int i, int n)
+{
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// GL LINEAR + REPEAT footprint for any coordinate (the shadow filter's taps
+// reach several texels past the image).
+__device__ inline Bil bil_setup_any(float s, float t, int W, int H)
+{
+    float u = s * (float)W - 0.5f;
+    float v = t * (float)H - 0.5f;
+    float fu = floorf(u), fv = floorf(v);
+    Bil b;
+    b.a = u - fu;
+    b.b = v - fv;
+    const int x0 = wrapn((int)fu, W), x1 = wrapn((int)fu + 1, W);
+    const int y0 = wrapn((int)fv, H), y1 = wrapn((int)fv + 1, H);
+    b.i00 = (size_t)y0 * W + x0;
+    b.i10 = (size_t)y0 * W + x1;
+    b.i01 = (size_t)y1 * W + x0;
+    b.i11 = (size_t)y1 * W + x1;
+    return b;
+}
+
+__device__ inline float dec8(uint8_t c) { return vpt_unorm8_decode(c); }
+
+// GetShadowSpatial (ShadowTemporalFilter.glsl:98-150) over the raw mask.
+__device__ inline float shadow_spatial_pre(int px, int py, int W, int H, float tva, float base_depth,
+                                           const float* __restrict__ cur_t, const uint8_t* __restrict__ cur_face,
+                                           const uint8_t* __restrict__ mask)
+{
+    const float unit_diagonal = sqrtf(2.0f);
+    if (tva <= unit_diagonal * 2.0f) return 1.0f;
+    const size_t i = (size_t)py * W + px;
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    const float uvx = texc(px, W), uvy = texc(py, H);
+    float total = (float)mask[i];
+    const float base = total;
+    float weight = 1.0f;
+    const Vec3 bn = vpt::face_normal(cur_face[i], 1.0f);
+    const float b = 0.03f;
+    for (int x = -1; x <= 1; x++) {
+        for (int y = -1; y <= 1; y++) {
+            if (x == 0 && y == 0) continue;
+            const float sx = uvx + (float)x * tex_w, sy = uvy + (float)y * tex_h;
+            if (!(sx > b && sx < 1.0f - b && sy > b && sy < 1.0f - b)) continue;
+            const size_t j = (size_t)(py + y) * W + (px + x);
+            const float sd = cur_t[j];
+            const Vec3 sn = vpt::face_normal(cur_face[j], 1.0f);
+            if (sn.x == bn.x && sn.y == bn.y && sn.z == bn.z && fabsf(sd - base_depth) < 1.0f) {
+                const float smp = (float)mask[j];
+                float wa = fclamp(1.0f - fclamp(fabsf(smp - base) / 3.0f, 0.0f, 1.0f), 0.0f, 1.0f);
+                wa = fclamp(vpt_powi(wa, 7), 0.000001f, 1.0f);
+                total = total + smp * wa;
+                weight = weight + wa;
+            }
+        }
+    }
+    return total / weight;
+}
+
+// ShadowTemporalFilter.main (ShadowTemporalFilter.glsl:187-267), u_ShadowTemporal true.
+__global__ void __launch_bounds__(256) shadow_temporal_kernel(CamView cam, Mat16 prev_vp, int W, int H,
+                                                              const float* __restrict__ cur_t,
+                                                              const uint8_t* __restrict__ cur_face,
+                                                              const float* __restrict__ prev_t,
+                                                              const uint8_t* __restrict__ mask,
+                                                              const uint16_t* __restrict__ tv,
+                                                              const uint8_t* __restrict__ hist_color,
+                                                              const uint16_t* __restrict__ hist_frames,
+                                                              uint8_t* __restrict__ out_color,
+                                                              uint16_t* __restrict__ out_frames)
+{
+    int px, py;
+    if (!pixel_of(W, H, px, py)) return;
+    const size_t i = (size_t)py * W + px;
+    const float uvx = texc(px, W), uvy = texc(py, H);
+    const float dist = cur_t[i];
+    float oc, of;
+    if (dist > 0.0f) {
+        const Vec3 pos = position_from(cam, uvx, uvy, dist);
+        const float q0 = vpt::mat_row(prev_vp.m, 0, pos.x, pos.y, pos.z, 1.0f);
+        const float q1 = vpt::mat_row(prev_vp.m, 1, pos.x, pos.y, pos.z, 1.0f);
+        const float q3 = vpt::mat_row(prev_vp.m, 3, pos.x, pos.y, pos.z, 1.0f);
+        const float rx = (q0 / q3) * 0.5f + 0.5f, ry = (q1 / q3) * 0.5f + 0.5f;
+        const float tva = h2f(tv[i]) * 100.0f;
+        float cur = shadow_spatial_pre(px, py, W, H, tva, dist, cur_t, cur_face, mask);
+        const float bias = 0.005f;
+        const bool rejected = !(rx > 0.0f + bias && rx < 1.0f - bias && ry > 0.0f + bias && ry < 1.0f - bias);
+        if (!rejected) {
+            const Bil f = bil_setup(rx, ry, W, H);
+            const float prev_orig =
+                bil(dec8(hist_color[f.i00]), dec8(hist_color[f.i10]), dec8(hist_color[f.i01]), dec8(hist_color[f.i11]),
+                    f.a, f.b);
+            float prev = prev_orig;
+            if (tva < 1.414f * 3.0f) {   // ClipShadow (:166-185) and clipAABB (:156-164)
+                const int ox[5] = {-1, 1, 0, 0, 0}, oy[5] = {0, 0, 0, -1, 1};
+                float mn = 100.0f, mx = -100.0f;
+#pragma unroll
+                for (int k = 0; k < 5; k++) {
+                    const float smp = (float)mask[(size_t)wrap1(py + oy[k], H) * W + wrap1(px + ox[k], W)];
+                    mn = fminf(smp, mn);
+                    mx = fmaxf(smp, mx);
+                }
+                const float lo = mn - 0.025f, hi = mx + 0.025f;
+                const float pcl = 0.5f * (hi + lo), ecl = 0.5f * (hi - lo);
+                const float vcl = prev_orig - pcl;
+                const float denom = fabsf(vcl / ecl);
+                prev = denom > 1.0f ? pcl + vcl / denom : prev_orig;
+            }
+            const float pw = bil(prev_t[f.i00], prev_t[f.i10], prev_t[f.i01], prev_t[f.i11], f.a, f.b);
+            const Vec3 pp = position_from(cam, rx, ry, pw);
+            const Vec3 dd = vpt::mk3(pp.x - pos.x, pp.y - pos.y, pp.z - pos.z);
+            const float d = sqrtf(vpt::dot3(dd, dd));
+            cur = fclamp(cur, 0.0f, 1.0f);
+            prev = fclamp(prev, 0.0f, 1.0f);
+            const float vx = (uvx - rx) * (float)W, vy = (uvy - ry) * (float)H;
+            const float clip_err = fabsf(prev_orig - prev);
+            const float inc = clip_err < 0.2f ? 1.0f : 0.6f;
+            const float fc = bil(h2f(hist_frames[f.i00]), h2f(hist_frames[f.i10]), h2f(hist_frames[f.i01]),
+                                 h2f(hist_frames[f.i11]), f.a, f.b);
+            const float fi = fc + inc;
+            float bf = fclamp((1.0f - (1.0f / fi)) * 1.2f, 0.01f, 0.97f);
+            const float vr = fclamp(vpt_expf(-sqrtf(vx * vx + vy * vy)) * 0.8f + 0.6f, 0.00000001f, 1.0f);
+            bf = bf * vr;
+            float dr = 1.0f;
+            if (d > 0.4f) {
+                dr = vpt_powi(vpt_expf(-d), 48);
+                bf = bf * fclamp(dr, 0.0f, 1.0f);
+            }
+            const float a = fclamp(bf, 0.0f, 0.97f);
+            oc = cur * (1.0f - a) + prev * a;
+            const float mults = dr * vr;
+            of = fc + fclamp(mults * 1.1f, 0.0f, 1.0f);
+            if (mults < 0.1f) of = 0.0f;
+            else if (mults <= 0.2f + 0.001f) of = 2.0f;
+            else if (mults <= 0.3f + 0.001f) of = 3.25f;
+        } else {
+            oc = cur;
+            of = 0.0f;
+        }
+    } else {
+        oc = (float)mask[i];
+        of = 0.0f;
+    }
+    of = fclamp(of, 0.0f, 256.0f);
+    out_color[i] = vpt_unorm8_encode(oc);
+    out_frames[i] = (uint16_t)f2h(of);
+}
+
+// ShadowFilter.main / ShadowSpatial (ShadowFilter.glsl:69-165).
+__global__ void __launch_bounds__(256) shadow_filter_kernel(int W, int H, const float* __restrict__ gb_t,
+                                                            const uint8_t* __restrict__ gb_face,
+                                                            const uint8_t* __restrict__ color,
+                                                            const uint16_t* __restrict__ tv,
+                                                            const uint16_t* __restrict__ frames, float scale,
+                                                            uint8_t* __restrict__ out)
+{
+    int px, py;
+    if (!pixel_of(W, H, px, py)) return;
+    const size_t i = (size_t)py * W + px;
+    const float uvx = texc(px, W), uvy = texc(py, H);
+    const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
+    const float fr = h2f(frames[i]);
+    const bool apply_luma = fr > 7.5f;
+    const float dist = gb_t[i];
+    const bool sky = dist < 0.0f;
+    const Vec3 cn = vpt::face_normal(gb_face[i], 1.0f);
+    const float cs = dec8(color[i]);
+    const float T = h2f(tv[i]) * 100.0f;
+    const float cutoff = sqrtf(2.0f);
+    if ((T > 0.0f && T < cutoff) || sky) {
+        out[i] = vpt_unorm8_encode(cs);
+        return;
+    }
+    const int size = T < cutoff * 1.414f ? 1 : 3;
+    float sc = 1.0f;
+    if (T > 6.0f) sc = 2.0f;
+    if (T > 16.0f) sc = 2.4f;
+    if (T > 32.0f) sc = 2.6f;
+    const float ct = fclamp(T, 0.0f, 10.0f);
+    const float am = ct / 10.0f;
+    float ve = (20.0f * (1.0f - am) + 6.0f * am) + (T < 6.0f ? 5.0f : 2.0f);
+    ve = fclamp(ve - 1.75f, 0.0000001f, 64.0f);
+    float lm = 1.0f;
+    if (!apply_luma) {
+        const float fn = fr / 7.5f;
+        lm = 0.1f * (1.0f - fn) + 0.5f * fn;
+    }
+    const float expo = (ve * lm) * 0.9f;
+    float tw = 0.0f, ts = 0.0f;
+    for (int x = -size; x <= size; x++) {
+        for (int y = -size; y <= size; y++) {
+            const float sx = uvx + (((float)x * tex_w) * 1.2f * sc) * scale;
+            const float sy = uvy + (((float)y * tex_h) * 1.2f * sc) * scale;
+            const Bil f = bil_setup_any(sx, sy, W, H);
+            const float sd = bil(gb_t[f.i00], gb_t[f.i10], gb_t[f.i01], gb_t[f.i11], f.a, f.b);
+            const int nxp = wrapn((int)floorf(sx * (float)W), W), nyp = wrapn((int)floorf(sy * (float)H), H);
+            const Vec3 sn = vpt::face_normal(gb_face[(size_t)nyp * W + nxp], 1.0f);
+            const float dw = vpt_powi(vpt_expf(-fabsf(dist - sd)), 3);
+            const float nw = vpt_powi(fmaxf(vpt::dot3(cn, sn), 0.000000001f), 32);
+            const float sa = bil(dec8(color[f.i00]), dec8(color[f.i10]), dec8(color[f.i01]), dec8(color[f.i11]), f.a,
+                                 f.b);
+            const float le = fclamp(1.0f - fclamp(fabsf(sa - cs) / 3.0f, 0.0f, 1.0f), 0.0f, 1.0f);
+            float w = 1.0f;
+            w = w * fclamp(vpt_powf(le, expo), 0.0f, 1.0f);
+            w = w * dw;
+            w = w * nw;
+            w = fclamp(w, 0.000000001f, 1.0f);
+            ts = ts + sa * w;
+            tw = tw + w;
+        }
+    }
+    out[i] = vpt_unorm8_encode(ts / fmaxf(tw, 0.01f));
+}
+
 int fill_zero(void* p, size_t bytes, hipStream_t s)
 {
     size_t n = bytes / 4;
@@ -875,5 +1108,111 @@ extern "C" VPT_API int vpt_svgf_read_plane(vpt_svgf* s, int which, vpt_svgf_texe
     DeviceGuard g(s->device);
     HIP_TRY(hipMemcpyAsync(d_dst, src, (size_t)s->W * s->H * sizeof(vpt_svgf_texel), hipMemcpyDeviceToDevice,
                            (hipStream_t)stream));
+    return VPT_OK;
+}
+
+// ---- sun-shadow denoise object ----------------------------------------------
+
+extern "C" VPT_API int vpt_shadow_denoise_create(vpt_ctx* ctx, int width, int height, vpt_shadow_denoise** out)
+{
+    if (!ctx || !out) return set_error(VPT_EINVAL, "vpt_shadow_denoise_create: null argument");
+    if (width <= 1 || height <= 1 || width > 16384 || height > 16384)
+        return set_error(VPT_EINVAL, "vpt_shadow_denoise_create: bad size %dx%d", width, height);
+    DeviceGuard g(ctx->device);
+    vpt_shadow_denoise* d = new vpt_shadow_denoise();
+    d->device = ctx->device;
+    d->W = width;
+    d->H = height;
+    const size_t n = (size_t)width * height;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2; k++) {
+        if (e == hipSuccess) e = hipMalloc(&d->d_color[k], n);
+        if (e == hipSuccess) e = hipMalloc(&d->d_frames[k], n * sizeof(uint16_t));
+    }
+    if (e != hipSuccess) {
+        vpt_shadow_denoise_destroy(d);
+        return set_error(VPT_ENOMEM, "vpt_shadow_denoise_create: hipMalloc failed (%s)", hipGetErrorString(e));
+    }
+    if (int rc = vpt_shadow_denoise_reset(d, nullptr)) {
+        vpt_shadow_denoise_destroy(d);
+        return rc;
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    *out = d;
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_shadow_denoise_destroy(vpt_shadow_denoise* d)
+{
+    if (!d) return VPT_OK;
+    DeviceGuard g(d->device);
+    for (int k = 0; k < 2; k++) {
+        (void)hipFree(d->d_color[k]);
+        (void)hipFree(d->d_frames[k]);
+    }
+    delete d;
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_shadow_denoise_reset(vpt_shadow_denoise* d, void* stream)
+{
+    if (!d) return set_error(VPT_EINVAL, "vpt_shadow_denoise_reset: null object");
+    DeviceGuard g(d->device);
+    const size_t n = (size_t)d->W * d->H;
+    for (int k = 0; k < 2; k++) {
+        if (int rc = fill_zero(d->d_color[k], n, (hipStream_t)stream)) return rc;
+        if (int rc = fill_zero(d->d_frames[k], n * sizeof(uint16_t), (hipStream_t)stream)) return rc;
+    }
+    d->cur = 0;
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_shadow_denoise_frame(vpt_shadow_denoise* d, const vpt_camera* cam,
+                                                const float prev_viewproj[16], const vpt_gbuffer* cur,
+                                                const vpt_gbuffer* prev, const uint8_t* d_mask,
+                                                const uint16_t* d_transversal, float filter_scale, uint8_t* d_out,
+                                                void* stream)
+{
+    if (!d || !cam || !prev_viewproj || !cur || !prev || !d_mask || !d_transversal || !d_out)
+        return set_error(VPT_EINVAL, "vpt_shadow_denoise_frame: null argument");
+    const int W = d->W, H = d->H;
+    if (cam->width != W || cam->height != H || cur->width != W || cur->height != H || prev->width != W ||
+        prev->height != H)
+        return set_error(VPT_EINVAL, "vpt_shadow_denoise_frame: size mismatch (denoiser %dx%d)", W, H);
+    if (!cur->d_t || !cur->d_face || !prev->d_t)
+        return set_error(VPT_EINVAL, "vpt_shadow_denoise_frame: G-buffer planes missing");
+    if (!(filter_scale > 0.0f && filter_scale <= 16.0f))
+        return set_error(VPT_EINVAL, "vpt_shadow_denoise_frame: filter scale %g outside (0, 16]", filter_scale);
+    DeviceGuard g(d->device);
+    hipStream_t st = (hipStream_t)stream;
+    const CamView cv = make_cam(cam);
+    Mat16 vp;
+    for (int k = 0; k < 16; k++) vp.m[k] = prev_viewproj[k];
+    const dim3 grid = pass_grid(W, H), block(256);
+    const int hist = d->cur, next = d->cur ^ 1;
+    hipLaunchKernelGGL(shadow_temporal_kernel, grid, block, 0, st, cv, vp, W, H, cur->d_t, cur->d_face, prev->d_t,
+                       d_mask, d_transversal, d->d_color[hist], d->d_frames[hist], d->d_color[next],
+                       d->d_frames[next]);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(shadow_filter_kernel, grid, block, 0, st, W, H, cur->d_t, cur->d_face, d->d_color[next],
+                       d_transversal, d->d_frames[next], filter_scale, d_out);
+    HIP_TRY(hipGetLastError());
+    d->cur = next;
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_shadow_denoise_read(vpt_shadow_denoise* d, int which, void* d_dst, void* stream)
+{
+    if (!d || !d_dst) return set_error(VPT_EINVAL, "vpt_shadow_denoise_read: null argument");
+    const size_t n = (size_t)d->W * d->H;
+    const void* src = nullptr;
+    size_t bytes = 0;
+    switch (which) {
+    case 0: src = d->d_color[d->cur]; bytes = n; break;
+    case 1: src = d->d_frames[d->cur]; bytes = n * sizeof(uint16_t); break;
+    default: return set_error(VPT_EINVAL, "vpt_shadow_denoise_read: plane %d unknown", which);
+    }
+    DeviceGuard g(d->device);
+    HIP_TRY(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
     return VPT_OK;
 }

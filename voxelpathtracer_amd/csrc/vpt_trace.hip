@@ -205,6 +205,57 @@ __global__ void __launch_bounds__(256) shadow_kernel(WorldView w, CamView cam, c
     count_samples(counter, samples, rays);
 }
 
+// SHADOW_TRACE main with both outputs (ShadowRayTraceFrag.glsl:7-8, 414-513):
+// the mask as shadow_kernel, and o_IntersectionTransversal as binary16:
+// clamp(T / 100, 1e-5, 196), 4.25 / 100 when the ray misses, 1 / 100 when
+// N.L <= 0.01, 64 for sky.  The ray is traced also when its origin cell is
+// solid (the mask is 1 there either way).
+__global__ void __launch_bounds__(256) shadow2_kernel(WorldView w, CamView cam, const float* __restrict__ gb_t,
+                                                      const uint8_t* __restrict__ gb_face, float lx, float ly,
+                                                      float lz, int max_iter, uint8_t* __restrict__ mask,
+                                                      uint16_t* __restrict__ transversal,
+                                                      unsigned long long* __restrict__ counter)
+{
+    int px, py;
+    tile_pixel(cam.W, cam.H, px, py);
+    int samples = 0, rays = 0;
+    if (px < cam.W && py < cam.H) {
+        const size_t i = (size_t)py * cam.W + px;
+        const Vec3 L = vpt::mk3(lx, ly, lz);
+        const float dist = gb_t[i];
+        uint8_t m;
+        float tv;
+        if (dist < 0.0f) {
+            m = 0;
+            tv = 64.0f;
+        } else {
+            const Vec3 P = vpt::gbuffer_position(cam, px, py, dist);
+            const Vec3 N = vpt::face_normal(gb_face[i], 1.0f);
+            const float ndotl = vpt::dot3(N, L);
+            if (ndotl <= 0.01f) {
+                m = 1;
+                tv = 1.0f / 100.0f;
+            } else {
+                const Vec3 o = vpt::mk3(P.x + N.x * 0.06f, P.y + N.y * 0.06f, P.z + N.z * 0.06f);
+                float T = -1.0f;
+                const int block_at = vpt::voxel_at(w, o);
+                if (dist > 0.0f) {
+                    const vpt::Hit h = vpt::traverse(w, o, L, max_iter);
+                    T = h.t;
+                    samples = h.samples;
+                    rays = 1;
+                }
+                m = (T > 0.0f || block_at > 0) ? 1 : 0;
+                tv = vpt::fclamp(T / 100.0f, 0.00001f, 196.0f);
+                if (T < 0.0f) tv = 4.25f / 100.0f;
+            }
+        }
+        mask[i] = m;
+        if (transversal) transversal[i] = (uint16_t)vpt::hw_f2h(tv);
+    }
+    count_samples(counter, samples, rays);
+}
+
 // DIFFUSE_TRACE main + CalculateDiffuse (DiffuseRayTraceFrag.glsl:535-664,
 // 822-935), 1 spp, 1 bounce; Lambert sun term, PCG randoms, per-id albedo,
 // constant sky.
@@ -329,6 +380,22 @@ extern "C" VPT_API int vpt_trace_shadow(vpt_world* w, const vpt_camera* cam, con
     hipLaunchKernelGGL(shadow_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, (hipStream_t)stream,
                        vpt::make_view(w), vpt::make_cam(cam), gb->d_t, gb->d_face, sun_dir[0], sun_dir[1], sun_dir[2],
                        max_iter, d_mask, w->d_counter);
+    HIP_TRY(hipGetLastError());
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_trace_shadow2(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                         const float sun_dir[3], int max_iter, uint8_t* d_mask,
+                                         uint16_t* d_transversal, void* stream)
+{
+    if (!w || !sun_dir || !d_mask) return set_error(VPT_EINVAL, "vpt_trace_shadow2: null argument");
+    if (int rc = vpt::check_cam(cam, "vpt_trace_shadow2")) return rc;
+    if (int rc = vpt::check_gbuffer(gb, cam, "vpt_trace_shadow2", false)) return rc;
+    if (int rc = vpt::check_iter(max_iter, "vpt_trace_shadow2")) return rc;
+    DeviceGuard g(w->device);
+    hipLaunchKernelGGL(shadow2_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, (hipStream_t)stream,
+                       vpt::make_view(w), vpt::make_cam(cam), gb->d_t, gb->d_face, sun_dir[0], sun_dir[1], sun_dir[2],
+                       max_iter, d_mask, d_transversal, w->d_counter);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }

@@ -68,6 +68,43 @@ class SVGF:
             pass
 
 
+class ShadowDenoiser:
+    """vpt_shadow_denoise: ShadowTemporalFilter + ShadowFilter with their
+    history (Pipeline.cpp:2854-2948)."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        h = C.c_void_p()
+        call("vpt_shadow_denoise_create", ctx.handle, width, height, C.byref(h))
+        self.handle = h
+        self.width, self.height = width, height
+
+    def reset(self, stream=None):
+        call("vpt_shadow_denoise_reset", self.handle, _stream(stream))
+
+    def frame(self, cam, prev_viewproj, cur: DeviceGBuffer, prev: DeviceGBuffer, mask, transversal, out,
+              filter_scale: float = 1.0, stream=None):
+        """mask / transversal: vpt_trace_shadow2's outputs (uint8, binary16
+        bits); out: uint8 tensor for the filtered shadow."""
+        vp = (C.c_float * 16)(*map(float, np.asarray(prev_viewproj, np.float32).ravel()))
+        call("vpt_shadow_denoise_frame", self.handle, C.byref(cam), vp, C.byref(cur.struct()), C.byref(prev.struct()),
+             _ptr(mask), _ptr(transversal), float(filter_scale), _ptr(out), _stream(stream))
+
+    def read(self, which: int, dst, stream=None):
+        """which 0: temporal colour (uint8), 1: frame count (binary16 bits)."""
+        call("vpt_shadow_denoise_read", self.handle, which, _ptr(dst), _stream(stream))
+
+    def close(self):
+        if self.handle:
+            lib().vpt_shadow_denoise_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def viewproj(view: np.ndarray, proj: np.ndarray) -> np.ndarray:
     """proj * view for column-major float32 matrices, summed in k order in
     float32 (u_PrevProjection * u_PrevView, TemporalFilter.glsl:64)."""

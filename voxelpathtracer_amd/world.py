@@ -150,6 +150,13 @@ class World:
         call("vpt_trace_shadow", self.handle, C.byref(cam), C.byref(gbuf.struct()), sd, max_iter, _ptr(mask),
              _stream(stream))
 
+    def trace_shadow2(self, cam: Camera, gbuf: "DeviceGBuffer", sun_dir, mask, transversal, max_iter: int = 350,
+                      stream=None) -> None:
+        """The shadow pass with its transversal output (binary16 bits in an int16/uint16 tensor)."""
+        sd = (C.c_float * 3)(*map(float, sun_dir))
+        call("vpt_trace_shadow2", self.handle, C.byref(cam), C.byref(gbuf.struct()), sd, max_iter, _ptr(mask),
+             _ptr(transversal), _stream(stream))
+
     def trace_gi(self, cam: Camera, gbuf: "DeviceGBuffer", prm, rgb, texels=None, stream=None) -> None:
         call("vpt_trace_gi", self.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), _ptr(rgb),
              _ptr(texels), _stream(stream))
