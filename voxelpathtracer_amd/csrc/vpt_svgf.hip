@@ -735,7 +735,10 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v)
 
 __device__ inline int wrapn(int i, int n)
 {
-    const int r = i % n;
+    if (i >= 0 && i < n) return i;
+    if (i < 0 && i >= -n) return i + n;
+    if (i >= n && i < 2 * n) return i - n;
+    const int r = i % n;   // taps further out than one image (tiny images, large scales)
     return r < 0 ? r + n : r;
 }
 
@@ -751,10 +754,11 @@ __device__ inline Bil bil_setup_any(float s, float t, int W, int H)
     b.b = v - fv;
     const int x0 = wrapn((int)fu, W), x1 = wrapn((int)fu + 1, W);
     const int y0 = wrapn((int)fv, H), y1 = wrapn((int)fv + 1, H);
-    b.i00 = (size_t)y0 * W + x0;
-    b.i10 = (size_t)y0 * W + x1;
-    b.i01 = (size_t)y1 * W + x0;
-    b.i11 = (size_t)y1 * W + x1;
+    const uint32_t r0 = (uint32_t)y0 * (uint32_t)W, r1 = (uint32_t)y1 * (uint32_t)W;   // W * H < 2^28
+    b.i00 = r0 + (uint32_t)x0;
+    b.i10 = r0 + (uint32_t)x1;
+    b.i01 = r1 + (uint32_t)x0;
+    b.i11 = r1 + (uint32_t)x1;
     return b;
 }
 
@@ -930,19 +934,33 @@ __global__ void __launch_bounds__(256) shadow_filter_kernel(int W, int H, const 
             const float sx = uvx + (((float)x * tex_w) * 1.2f * sc) * scale;
             const float sy = uvy + (((float)y * tex_h) * 1.2f * sc) * scale;
             const Bil f = bil_setup_any(sx, sy, W, H);
-            const float sd = bil(gb_t[f.i00], gb_t[f.i10], gb_t[f.i01], gb_t[f.i11], f.a, f.b);
             const int nxp = wrapn((int)floorf(sx * (float)W), W), nyp = wrapn((int)floorf(sy * (float)H), H);
-            const Vec3 sn = vpt::face_normal(gb_face[(size_t)nyp * W + nxp], 1.0f);
-            const float dw = vpt_powi(vpt_expf(-fabsf(dist - sd)), 3);
-            const float nw = vpt_powi(fmaxf(vpt::dot3(cn, sn), 0.000000001f), 32);
+            const Vec3 sn = vpt::face_normal(gb_face[(uint32_t)nyp * (uint32_t)W + (uint32_t)nxp], 1.0f);
             const float sa = bil(dec8(color[f.i00]), dec8(color[f.i10]), dec8(color[f.i01]), dec8(color[f.i11]), f.a,
                                  f.b);
-            const float le = fclamp(1.0f - fclamp(fabsf(sa - cs) / 3.0f, 0.0f, 1.0f), 0.0f, 1.0f);
-            float w = 1.0f;
-            w = w * fclamp(vpt_powf(le, expo), 0.0f, 1.0f);
-            w = w * dw;
-            w = w * nw;
-            w = fclamp(w, 0.000000001f, 1.0f);
+            // The weight is ((clamp(pow) * dw) * nw) with pow <= 1 (vpt_expf of a
+            // non-positive argument, checked for every float), so it sits on
+            // its 1e-9 clamp once dw * nw < 1e-9: nw is exactly 0 or 1 here
+            // (axis normals, or (1,1,1) against an axis normal), and
+            // exp(-d)^3 < 1e-9 for every float d >= 21.  Those taps skip the
+            // exp and the pow.
+            const float nw = vpt_powi(fmaxf(vpt::dot3(cn, sn), 0.000000001f), 32);
+            float w = 0.000000001f;
+            if (nw != 0.0f) {
+                const float sd = bil(gb_t[f.i00], gb_t[f.i10], gb_t[f.i01], gb_t[f.i11], f.a, f.b);
+                const float dd = fabsf(dist - sd);
+                if (dd < 21.0f) {
+                    const float dw = vpt_powi(vpt_expf(-dd), 3);
+                    if (dw * nw >= 0.000000001f) {
+                        const float le = fclamp(1.0f - fclamp(fabsf(sa - cs) / 3.0f, 0.0f, 1.0f), 0.0f, 1.0f);
+                        w = 1.0f;
+                        w = w * fclamp(vpt_powf(le, expo), 0.0f, 1.0f);
+                        w = w * dw;
+                        w = w * nw;
+                        w = fclamp(w, 0.000000001f, 1.0f);
+                    }
+                }
+            }
             ts = ts + sa * w;
             tw = tw + w;
         }
