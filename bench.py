@@ -25,6 +25,9 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# the kernels each frame pass launches (the roofline covers the whole pass)
+PASS_KERNELS = {"primary": "primary_kernel", "shadow": "shadow_kernel",
+                "gi": "gi pass: gi_bounce_kernel + gi_shadow_kernel + gi_shade_kernel"}
 NX, NY, NZ = 384, 128, 384
 CAM_POS = (192.5, 100.5, 192.5)
 CAM_YAW, CAM_PITCH = 45.0, -20.0
@@ -299,9 +302,9 @@ def main():
         "svgf_ms_per_frame": round(svgf_ms, 4),
         "shadow_denoise": {"ms_per_frame": round(shadow_denoise_ms, 4), "shadow_pass_with_transversal_ms":
                            round(shadow2_ms, 4), "frames": nf},
-        "roofline": {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": round(achieved, 2),
+        "roofline": {"bound": "hbm", "kernel": PASS_KERNELS[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": profiled_traffic(f"{dom}_kernel"), "algorithmic_bytes": alg[dom]},
+                     "traffic": profiled_traffic(f"{dom} pass"), "algorithmic_bytes": alg[dom]},
     }
 
     if not args.no_learned:
@@ -361,8 +364,9 @@ def df_update_timing(world, grid, torch, edits: int = 24):
 
 
 def profiled_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` measured by rocprofv3 (FETCH_SIZE and
-    WRITE_SIZE runs scaled by a plain copy's readings), as recorded in
+    """HBM bytes per launch of `kernel` (or per call of a pass, summed over
+    its kernels) measured by rocprofv3 (FETCH_SIZE and WRITE_SIZE runs scaled
+    by a plain copy's readings), as recorded in
     profiles/frame_1080p_kernels.md; None when not recorded."""
     from pathlib import Path
     p = Path(__file__).resolve().parent / "profiles" / "frame_1080p_kernels.md"

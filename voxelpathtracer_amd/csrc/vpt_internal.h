@@ -65,6 +65,12 @@ struct vpt_ctx {
     float* d_albedo = nullptr;       // GI albedo table (128 ids x RGB)
     float albedo_host[128 * 3] = {};
     bool albedo_valid = false;
+    // GI pass scratch, grown on demand: per-pixel hit records (2 words),
+    // shadow-ray jobs (4 words each) and the job counter
+    uint32_t* d_gi_hits = nullptr;
+    uint32_t* d_gi_jobs = nullptr;
+    uint32_t* d_gi_njobs = nullptr;
+    size_t gi_cap = 0;   // pixels
 };
 
 struct vpt_world : vpt::World {

@@ -258,43 +258,116 @@ __global__ void __launch_bounds__(256) shadow2_kernel(WorldView w, CamView cam, 
 
 // DIFFUSE_TRACE main + CalculateDiffuse (DiffuseRayTraceFrag.glsl:535-664,
 // 822-935), 1 spp, 1 bounce; Lambert sun term, PCG randoms, per-id albedo,
-// constant sky.
-__global__ void __launch_bounds__(256) gi_kernel(WorldView w, CamView cam, const float* __restrict__ gb_t,
-                                                 const uint8_t* __restrict__ gb_face, GiConst g,
-                                                 const float* __restrict__ albedo, float* __restrict__ rgb,
-                                                 vpt_svgf_texel* __restrict__ texels,
-                                                 unsigned long long* __restrict__ counter)
+// constant sky.  Three kernels, so that the sun-shadow rays at the bounce
+// hits (up to 128 steps) do not hold idle the lanes whose bounce missed or
+// faced away:
+//   gi_bounce_kernel   one lane per pixel: the bounce ray; stores the hit
+//                      record and, when the hit faces the sun, appends the
+//                      shadow ray to a compacted job list (one atomic per
+//                      wavefront);
+//   gi_shadow_kernel   one lane per job: the shadow ray, which sets the
+//                      record's shadowed bit;
+//   gi_shade_kernel    one lane per pixel: shading from the record.
+// Every pixel goes through the same functions as a single-kernel pass, so
+// the outputs are identical.
+struct GiJob {
+    float ox, oy, oz;
+    uint32_t pix;
+};
+
+// Hit record: x = T (float bits), y = face | block << 8 | hit << 16 | shadowed << 17.
+__device__ inline uint2 gi_record(float T, int face, int block, bool hit)
+{
+    return make_uint2(__float_as_uint(T),
+                      (uint32_t)(face & 0xff) | ((uint32_t)(block & 0xff) << 8) | ((uint32_t)hit << 16));
+}
+
+__global__ void gi_jobs_clear_kernel(uint32_t* __restrict__ n) { *n = 0u; }
+
+__global__ void __launch_bounds__(256) gi_bounce_kernel(WorldView w, CamView cam, const float* __restrict__ gb_t,
+                                                        const uint8_t* __restrict__ gb_face, GiConst g,
+                                                        uint2* __restrict__ hits, GiJob* __restrict__ jobs,
+                                                        uint32_t* __restrict__ njobs,
+                                                        unsigned long long* __restrict__ counter)
 {
     int px, py;
     tile_pixel(cam.W, cam.H, px, py);
     int samples = 0, rays = 0;
+    bool job = false;
+    GiJob jb = {0.0f, 0.0f, 0.0f, 0u};
     if (px < cam.W && py < cam.H) {
         const size_t i = (size_t)py * cam.W + px;
         const float dist = gb_t[i];
-        const Vec3 N = vpt::face_normal(gb_face[i], 0.5f);
-        vpt_svgf_texel* tx = texels ? texels + i : nullptr;
-        if (dist < 0.0f) {
-            vpt::gi_write_sky_pixel(g, N, rgb + 3 * i, tx);
-        } else {
+        if (!(dist < 0.0f)) {   // sky pixels are shaded from the G-buffer alone
             Vec3 o, dir;
-            vpt::gi_ray(cam, px, py, dist, N, g.frame, o, dir);
-            vpt::Hit h = vpt::traverse(w, o, dir, g.gi_iter);
+            vpt::gi_ray(cam, px, py, dist, vpt::face_normal(gb_face[i], 0.5f), g.frame, o, dir);
+            const vpt::Hit h = vpt::traverse(w, o, dir, g.gi_iter);
             samples = h.samples;
             rays = 1;
-            bool hit = h.t > 0.0f && h.block > 0;
-            float shadow_at = 0.0f;
+            const bool hit = h.t > 0.0f && h.block > 0;
+            hits[i] = gi_record(h.t, h.face, h.block, hit);
             Vec3 so;
             float ndotl;
             if (hit && vpt::gi_shadow_ray(o, dir, h.t, h.face, g, so, ndotl)) {
-                vpt::Hit sh = vpt::traverse(w, so, vpt::mk3(g.sun[0], g.sun[1], g.sun[2]), g.shadow_iter);
-                samples += sh.samples;
-                rays += 1;
-                shadow_at = sh.t > 0.0f ? 1.0f : 0.0f;
+                job = true;
+                jb.ox = so.x;
+                jb.oy = so.y;
+                jb.oz = so.z;
+                jb.pix = (uint32_t)i;
             }
-            vpt::gi_write_hit_pixel(g, albedo, dir, h.t, hit, h.face, h.block, shadow_at, rgb + 3 * i, tx);
         }
     }
+    // compacted append, one atomic per wavefront (every lane reaches it)
+    const uint64_t m = __ballot(job);
+    if (m) {
+        const int lane = (int)(threadIdx.x & 63), first = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == first) base = atomicAdd(njobs, (uint32_t)__popcll(m));
+        base = __shfl(base, first);
+        if (job) jobs[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = jb;
+    }
     count_samples(counter, samples, rays);
+}
+
+__global__ void __launch_bounds__(256) gi_shadow_kernel(WorldView w, GiConst g, const GiJob* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ njobs, uint2* __restrict__ hits,
+                                                        unsigned long long* __restrict__ counter)
+{
+    const uint32_t n = *njobs;   // at most one job per pixel: n <= the grid's lanes
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    int samples = 0, rays = 0;
+    if (k < n) {
+        const GiJob jb = jobs[k];
+        const vpt::Hit sh =
+            vpt::traverse(w, vpt::mk3(jb.ox, jb.oy, jb.oz), vpt::mk3(g.sun[0], g.sun[1], g.sun[2]), g.shadow_iter);
+        samples = sh.samples;
+        rays = 1;
+        if (sh.t > 0.0f) hits[jb.pix].y |= 1u << 17;   // one job per pixel: no other writer
+    }
+    count_samples(counter, samples, rays);
+}
+
+__global__ void __launch_bounds__(256) gi_shade_kernel(CamView cam, const float* __restrict__ gb_t,
+                                                       const uint8_t* __restrict__ gb_face, GiConst g,
+                                                       const uint2* __restrict__ hits, const float* __restrict__ albedo,
+                                                       float* __restrict__ rgb, vpt_svgf_texel* __restrict__ texels)
+{
+    int px, py;
+    tile_pixel(cam.W, cam.H, px, py);
+    if (px >= cam.W || py >= cam.H) return;
+    const size_t i = (size_t)py * cam.W + px;
+    const float dist = gb_t[i];
+    const Vec3 N = vpt::face_normal(gb_face[i], 0.5f);
+    vpt_svgf_texel* tx = texels ? texels + i : nullptr;
+    if (dist < 0.0f) {
+        vpt::gi_write_sky_pixel(g, N, rgb + 3 * i, tx);
+        return;
+    }
+    Vec3 o, dir;
+    vpt::gi_ray(cam, px, py, dist, N, g.frame, o, dir);
+    const uint2 rec = hits[i];
+    vpt::gi_write_hit_pixel(g, albedo, dir, __uint_as_float(rec.x), (rec.y >> 16) & 1u, (int)(rec.y & 0xffu),
+                            (int)((rec.y >> 8) & 0xffu), ((rec.y >> 17) & 1u) ? 1.0f : 0.0f, rgb + 3 * i, tx);
 }
 
 // Ray queries: one lane per ray.
@@ -411,9 +484,31 @@ extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const v
     DeviceGuard g(w->device);
     hipStream_t s = (hipStream_t)stream;
     if (int rc = vpt::upload_albedo(w->ctx, prm, s)) return rc;
-    hipLaunchKernelGGL(gi_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w),
-                       vpt::make_cam(cam), gb->d_t, gb->d_face, vpt::make_gi(prm), w->ctx->d_albedo, d_rgb, d_svgf_in,
-                       w->d_counter);
+    vpt_ctx* ctx = w->ctx;
+    const size_t npx = (size_t)cam->width * cam->height;
+    if (ctx->gi_cap < npx) {
+        (void)hipFree(ctx->d_gi_hits);   // hipFree waits for work that may still use them
+        (void)hipFree(ctx->d_gi_jobs);
+        ctx->d_gi_hits = ctx->d_gi_jobs = nullptr;
+        ctx->gi_cap = 0;
+        HIP_TRY(hipMalloc(&ctx->d_gi_hits, npx * 2 * sizeof(uint32_t)));
+        HIP_TRY(hipMalloc(&ctx->d_gi_jobs, npx * sizeof(GiJob)));
+        if (!ctx->d_gi_njobs) HIP_TRY(hipMalloc(&ctx->d_gi_njobs, sizeof(uint32_t)));
+        ctx->gi_cap = npx;
+    }
+    uint2* hits = reinterpret_cast<uint2*>(ctx->d_gi_hits);
+    GiJob* jobs = reinterpret_cast<GiJob*>(ctx->d_gi_jobs);
+    const GiConst gc = vpt::make_gi(prm);
+    hipLaunchKernelGGL(gi_jobs_clear_kernel, dim3(1), dim3(1), 0, s, ctx->d_gi_njobs);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(gi_bounce_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w),
+                       vpt::make_cam(cam), gb->d_t, gb->d_face, gc, hits, jobs, ctx->d_gi_njobs, w->d_counter);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(gi_shadow_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, vpt::make_view(w), gc,
+                       jobs, ctx->d_gi_njobs, hits, w->d_counter);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(gi_shade_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_cam(cam),
+                       gb->d_t, gb->d_face, gc, hits, ctx->d_albedo, d_rgb, d_svgf_in);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }

@@ -437,6 +437,9 @@ extern "C" VPT_API int vpt_ctx_destroy(vpt_ctx* ctx)
     if (!ctx) return VPT_OK;
     DeviceGuard g(ctx->device);
     if (ctx->d_albedo) (void)hipFree(ctx->d_albedo);
+    (void)hipFree(ctx->d_gi_hits);
+    (void)hipFree(ctx->d_gi_jobs);
+    (void)hipFree(ctx->d_gi_njobs);
     delete ctx;
     return VPT_OK;
 }
