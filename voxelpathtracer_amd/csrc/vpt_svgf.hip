@@ -558,8 +558,11 @@ struct GlobalSrc {
     __device__ float xch(int vx, int vy) const { return load_x(in, idx(vx, vy)); }
 };
 
-// One pixel of the pass, reading through `src`.
-template <class Src>
+// One pixel of the pass, reading through `src`.  kMode is the pass's AO /
+// sky filtering as a compile-time value (0: both, step <= 4; 1: sky only,
+// step <= 6; 2: neither), so that the passes that filter neither skip those
+// two channels' taps entirely.
+template <class Src, int kMode>
 __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, int W, int H, int px, int py,
                                                                 const uint2* __restrict__ tutil, int step,
                                                                 float scale, float color_phi_bias, float time,
@@ -569,8 +572,8 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
     const float atrous[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
     const float gauss[2] = {0.60283f, 0.198585f};
     const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
-    const bool filter_ao = step <= 4;
-    const bool filter_sky = (step <= 6) || filter_ao;
+    constexpr bool filter_ao = kMode == 0;
+    constexpr bool filter_sky = kMode <= 1;
     float uvx = texc(px, W), uvy = texc(py, H);
     int jit = (int)((gradient_noise(px, py, time) - 0.5f) * ((float)step * 0.8f));
     float base_depth = src.depth(px, py);
@@ -683,6 +686,7 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
              fclamp(tvar / (tw * tw), -1.0f, 50.0f), fclamp(oao, 0.0f, 1.0f), fclamp(osky, 0.0f, 1.0f));
 }
 
+template <int kMode>
 __global__ void __launch_bounds__(256) atrous_kernel(int W, int H, const float* __restrict__ gb_t,
                                                      const uint8_t* __restrict__ gb_face,
                                                      const vpt_svgf_texel* __restrict__ in,
@@ -693,7 +697,7 @@ __global__ void __launch_bounds__(256) atrous_kernel(int W, int H, const float* 
     int px, py;
     if (!pixel_of(W, H, px, py)) return;
     const GlobalSrc src{in, gb_t, gb_face, W, H};
-    atrous_px(src, W, H, px, py, tutil, step, scale, color_phi_bias, time, out);
+    atrous_px<GlobalSrc, kMode>(src, W, H, px, py, tutil, step, scale, color_phi_bias, time, out);
 }
 
 // ---- SHToIrridiance(shY, CoCg, v) (ColorPassFrag.glsl:356-368) ------------
@@ -1088,8 +1092,11 @@ extern "C" VPT_API int vpt_svgf_frame(vpt_svgf* s, const vpt_camera* cam, const 
     int dst_idx = 0;
     for (int p = 0; p < 5; p++) {
         dst_idx = p % 2;   // DiffuseDenoiseFBO / DiffuseDenoisedFBO2 ping-pong (Pipeline.cpp:2501-2503)
-        hipLaunchKernelGGL(atrous_kernel, grid, block, 0, st, W, H, cur->d_t, cur->d_face, src, s->d_util[next],
-                           s->prm.steps[p], scale, s->prm.color_phi_bias, time, s->d_atrous[dst_idx]);
+        // FilterAO = step <= 4, FilterSky = step <= 6 || FilterAO (SpatialFilter.glsl:183-185)
+        const int step = s->prm.steps[p];
+        auto kern = step <= 4 ? atrous_kernel<0> : (step <= 6 ? atrous_kernel<1> : atrous_kernel<2>);
+        hipLaunchKernelGGL(kern, grid, block, 0, st, W, H, cur->d_t, cur->d_face, src, s->d_util[next],
+                           step, scale, s->prm.color_phi_bias, time, s->d_atrous[dst_idx]);
         HIP_TRY(hipGetLastError());
         src = s->d_atrous[dst_idx];
     }
