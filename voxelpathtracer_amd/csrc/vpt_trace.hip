@@ -261,15 +261,21 @@ __global__ void __launch_bounds__(256) shadow2_kernel(WorldView w, CamView cam, 
 // constant sky.  Three kernels, so that the sun-shadow rays at the bounce
 // hits (up to 128 steps) do not hold idle the lanes whose bounce missed or
 // faced away:
-//   gi_bounce_kernel   one lane per pixel: the bounce ray; stores the hit
-//                      record and, when the hit faces the sun, appends the
-//                      shadow ray to a compacted job list (one atomic per
-//                      wavefront);
+//   gi_bounce_kernel   one lane per bounce ray: the block's 16x16 pixels
+//                      set up their rays, which are regrouped onto lanes by
+//                      the ray's y direction (16 bins, through LDS), so that
+//                      a wavefront traces rays of similar length (rays
+//                      leaving upward escape in a few large steps, grazing
+//                      ones walk the terrain); stores the hit record and,
+//                      when the hit faces the sun, appends the shadow ray to
+//                      a compacted job list (one atomic per wavefront);
 //   gi_shadow_kernel   one lane per job: the shadow ray, which sets the
 //                      record's shadowed bit;
 //   gi_shade_kernel    one lane per pixel: shading from the record.
 // Every pixel goes through the same functions as a single-kernel pass, so
-// the outputs are identical.
+// the outputs are identical; only the lane that traces a pixel changes.
+constexpr int kGiBins = 16;
+
 struct GiJob {
     float ox, oy, oz;
     uint32_t pix;
@@ -290,31 +296,67 @@ __global__ void __launch_bounds__(256) gi_bounce_kernel(WorldView w, CamView cam
                                                         uint32_t* __restrict__ njobs,
                                                         unsigned long long* __restrict__ counter)
 {
+    __shared__ float s_ray[6][256];
+    __shared__ uint32_t s_pix[256];
+    __shared__ int s_bin[kGiBins + 1];   // counts, then first slot of each bin; the last bin: no ray
     int px, py;
     tile_pixel(cam.W, cam.H, px, py);
     int samples = 0, rays = 0;
     bool job = false;
     GiJob jb = {0.0f, 0.0f, 0.0f, 0u};
+    if (threadIdx.x <= kGiBins) s_bin[threadIdx.x] = 0;
+    __syncthreads();
+    // set up this pixel's ray and rank it within its y-direction bin
+    Vec3 ro = vpt::mk3(0.0f, 0.0f, 0.0f), rd = ro;
+    int bin = kGiBins;
+    uint32_t rpix = 0xffffffffu;
     if (px < cam.W && py < cam.H) {
-        const size_t i = (size_t)py * cam.W + px;
-        const float dist = gb_t[i];
+        const size_t i0 = (size_t)py * cam.W + px;
+        const float dist = gb_t[i0];
         if (!(dist < 0.0f)) {   // sky pixels are shaded from the G-buffer alone
-            Vec3 o, dir;
-            vpt::gi_ray(cam, px, py, dist, vpt::face_normal(gb_face[i], 0.5f), g.frame, o, dir);
-            const vpt::Hit h = vpt::traverse(w, o, dir, g.gi_iter);
-            samples = h.samples;
-            rays = 1;
-            const bool hit = h.t > 0.0f && h.block > 0;
-            hits[i] = gi_record(h.t, h.face, h.block, hit);
-            Vec3 so;
-            float ndotl;
-            if (hit && vpt::gi_shadow_ray(o, dir, h.t, h.face, g, so, ndotl)) {
-                job = true;
-                jb.ox = so.x;
-                jb.oy = so.y;
-                jb.oz = so.z;
-                jb.pix = (uint32_t)i;
-            }
+            vpt::gi_ray(cam, px, py, dist, vpt::face_normal(gb_face[i0], 0.5f), g.frame, ro, rd);
+            bin = min(kGiBins - 1, max(0, (int)((rd.y + 1.0f) * (0.5f * kGiBins))));
+            rpix = (uint32_t)i0;
+        }
+    }
+    const int rank = atomicAdd(&s_bin[bin], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int b = 0; b <= kGiBins; b++) {
+            const int c = s_bin[b];
+            s_bin[b] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    const int slot = s_bin[bin] + rank;   // < 256: the slots of the block's 256 pixels
+    s_ray[0][slot] = ro.x;
+    s_ray[1][slot] = ro.y;
+    s_ray[2][slot] = ro.z;
+    s_ray[3][slot] = rd.x;
+    s_ray[4][slot] = rd.y;
+    s_ray[5][slot] = rd.z;
+    s_pix[slot] = rpix;
+    __syncthreads();
+    // trace the ray in this lane's slot
+    const uint32_t pix = s_pix[threadIdx.x];
+    if (pix != 0xffffffffu) {
+        const Vec3 o = vpt::mk3(s_ray[0][threadIdx.x], s_ray[1][threadIdx.x], s_ray[2][threadIdx.x]);
+        const Vec3 dir = vpt::mk3(s_ray[3][threadIdx.x], s_ray[4][threadIdx.x], s_ray[5][threadIdx.x]);
+        const vpt::Hit h = vpt::traverse(w, o, dir, g.gi_iter);
+        samples = h.samples;
+        rays = 1;
+        const bool hit = h.t > 0.0f && h.block > 0;
+        hits[pix] = gi_record(h.t, h.face, h.block, hit);
+        Vec3 so;
+        float ndotl;
+        if (hit && vpt::gi_shadow_ray(o, dir, h.t, h.face, g, so, ndotl)) {
+            job = true;
+            jb.ox = so.x;
+            jb.oy = so.y;
+            jb.oz = so.z;
+            jb.pix = pix;
         }
     }
     // compacted append, one atomic per wavefront (every lane reaches it)
