@@ -9,8 +9,10 @@ Workload (one step = one frame of rays, inputs resident in HBM):
   every ray whose traversal loop is entered.  SVGF ms/frame (pre-spatial,
   temporal, variance, 5 a-trous, decode at 1920x1080) and DF-build ms are
   reported beside it, with the wall time of an incremental DF update after
-  a one-block edit and the sun-shadow denoiser's ms/frame over the same
-  sequence.
+  a one-block edit, the sun-shadow denoiser's ms/frame over the same
+  sequence, and the mirror-reflection pass (reflected ray + sun shadow at
+  its hit, 64 / 150 steps) off the frame's G-buffer at 1/4 and full
+  resolution.
 
 Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
 """
@@ -276,6 +278,29 @@ def main():
     sd.close()
     del shadow_in
 
+    # ---- mirror reflections (the reflection pass's traversal core) off the
+    # frame's G-buffer: at the reference's quarter resolution and at full
+    reflect = {}
+    for div in (4, 1):
+        Wr, Hr = W // div, H // div
+        r_dist = torch.empty((Hr, Wr), dtype=torch.int16, device="cuda")
+        r_hit = torch.empty((Hr, Wr), dtype=torch.int32, device="cuda")
+        r_ctr = torch.zeros(2, dtype=torch.int64, device="cuda")
+        world.set_sample_counter(r_ctr)
+        world.trace_reflect(cam, gb, sun, r_dist, r_hit, div=div)
+        torch.cuda.synchronize()
+        world.set_sample_counter(None)
+        r_rays = int(r_ctr[1].item())
+        s0, s1 = ev(), ev()
+        s0.record(stream)
+        for _ in range(20):
+            world.trace_reflect(cam, gb, sun, r_dist, r_hit, div=div)
+        s1.record(stream)
+        torch.cuda.synchronize()
+        r_ms = s0.elapsed_time(s1) / 20
+        reflect[f"div{div}"] = {"target": [Wr, Hr], "ms": round(r_ms, 4), "rays": r_rays,
+                                "samples": int(r_ctr[0].item()), "grays": round(r_rays / r_ms / 1e6, 4)}
+
     result = {
         "metric": "Grays/s DDA+GI @1080p (bit-exact hits); SVGF ms/frame; served p50/p99 latency",
         "value": round(grays, 4),
@@ -302,6 +327,7 @@ def main():
         "svgf_ms_per_frame": round(svgf_ms, 4),
         "shadow_denoise": {"ms_per_frame": round(shadow_denoise_ms, 4), "shadow_pass_with_transversal_ms":
                            round(shadow2_ms, 4), "frames": nf},
+        "reflect": reflect,
         "roofline": {"bound": "hbm", "kernel": PASS_KERNELS[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": profiled_traffic(f"{dom} pass"), "algorithmic_bytes": alg[dom]},

@@ -203,6 +203,28 @@ VPT_API int vpt_trace_shadow(vpt_world* w, const vpt_camera* cam, const vpt_gbuf
 VPT_API int vpt_trace_shadow2(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const float sun_dir[3],
                               int max_iter, uint8_t* d_mask, uint16_t* d_transversal, void* stream);
 
+/* REFLECTION_TRACE (ReflectionTraceFrag.glsl:764-1035) reduced to its
+ * traversal core: smooth mirror reflections (u_RoughReflections off), one
+ * sample, no jitter, screen-space reprojection, LPV and player off, flat
+ * materials (the reflection normal is the G-buffer face normal).  Textured
+ * shading of the hit (albedo, PBR, emissive, normal maps) is not computed.
+ * The target is (width / div) x (height / div), div 1..16; the reference's
+ * ReflectionTraceResolution 0.25 (Pipeline.cpp:101, 1744) is div = 4.  Its
+ * pixels sample the G-buffer at their centres (t LINEAR, face NEAREST).
+ * The reflected ray starts at P + N * 0.035 in direction reflect(I, N),
+ * I = normalize(P - camera position), and takes at most max_iter steps
+ * (u_ReflectionTraceLength, 64); a hit casts the sun shadow ray from
+ * hit + normal * 0.055 toward light_dir with shadow_iter steps (150 in
+ * the shader).
+ *   d_dist: binary16 bits per target pixel of o_HitDistance (R16F,
+ *           Pipeline.cpp:1141): -1 for sky and misses, else 0.001 + T,
+ *           clamped to [-10, 200];
+ *   d_hit (optional): face | block << 8 | shadowed << 16 of the reflected
+ *           hit, 255 without one. */
+VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const float light_dir[3],
+                              int div, int max_iter, int shadow_iter, uint16_t* d_dist, uint32_t* d_hit,
+                              void* stream);
+
 /* DIFFUSE_TRACE (DiffuseRayTraceFrag.glsl:822-935), 1 spp, 1 bounce with a
  * shadow ray at the bounce hit.  d_rgb: 3 floats per pixel (clamped to
  * [0, 8]).  d_svgf_in (optional): SVGF input texels. */

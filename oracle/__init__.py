@@ -113,6 +113,24 @@ def trace_shadow2(grid, df, inv_view, inv_proj, gb_t, gb_face, sun_dir, max_iter
     return dict(mask=mask, transversal=tv, samples=samples, rays=rays)
 
 
+def trace_reflect(grid, df, inv_view, inv_proj, gb_t, gb_face, light_dir, div: int = 4, max_iter: int = 64,
+                  shadow_iter: int = 150):
+    """Mirror reflections (ReflectionTraceFrag.glsl:764-1035, traversal core)
+    at (W // div) x (H // div): hit distance (binary16 bits), hit record
+    face | block << 8 | shadowed << 16 (255 without a hit), samples, rays."""
+    nz, ny, nx = grid.shape
+    H, W = gb_t.shape
+    Hr, Wr = H // div, W // div
+    dist = np.empty((Hr, Wr), np.uint16)
+    hit = np.empty((Hr, Wr), np.uint32)
+    samples = np.empty((Hr, Wr), np.int32)
+    rays = np.empty((Hr, Wr), np.int32)
+    lib().oracle_trace_reflect(_p(grid), _p(df), nx, ny, nz, _p(_f32(inv_view)), _p(_f32(inv_proj)), W, H,
+                               _p(_f32(gb_t)), _p(np.ascontiguousarray(gb_face, np.uint8)), _p(_f32(light_dir)), div,
+                               max_iter, shadow_iter, _p(dist), _p(hit), _p(samples), _p(rays))
+    return dict(dist=dist, hit=hit, samples=samples, rays=rays)
+
+
 def shadow_temporal(inv_view, inv_proj, prev_viewproj, cur_t, cur_face, prev_t, mask, transversal, prev_color,
                     prev_frames):
     """ShadowTemporalFilter.main: (colour u8, frame count binary16 bits)."""

@@ -23,6 +23,8 @@
  *   SH decode       Core/Shaders/ColorPassFrag.glsl:356-368
  *   shadow denoise  Core/Shaders/ShadowTemporalFilter.glsl:98-267,
  *                   ShadowFilter.glsl:69-165, wiring Core/Pipeline.cpp:2854-2948
+ *   reflections     Core/Shaders/ReflectionTraceFrag.glsl:764-1035 (traversal
+ *                   core), wiring Core/Pipeline.cpp:3000-3060
  *   CPU picking     Core/World.cpp:496-546 (RaycastDetect)
  *
  * Parity status per function (what pins it besides this transcription):
@@ -48,6 +50,11 @@
  *                          (per-tap weights of both follow the
  *                          transcription only)
  *   vpt_logf / vpt_powf    error bounds against double libm        -> pinned
+ *   oracle_trace_reflect   corridor closed forms (first hit, face,
+ *                          block, distance incl. the 0.001 offset,
+ *                          shadow at the hit), an independent GL
+ *                          LINEAR / NEAREST resampler, the 200
+ *                          clamp                                   -> pinned
  */
 #include <stdint.h>
 #include <string.h>
@@ -1529,4 +1536,90 @@ OAPI void oracle_powi_n(const float* x, int e, float* y, int n)
 OAPI void oracle_unorm8_decode_table(float* out)
 {
     for (int k = 0; k < 256; k++) out[k] = vpt_unorm8_decode((uint8_t)k);
+}
+
+/* REFLECTION_TRACE (ReflectionTraceFrag.glsl:764-1035) reduced to its
+ * traversal core: smooth mirror reflections (u_RoughReflections off), one
+ * sample, no jitter, screen-space reprojection, LPV and player off, flat
+ * materials (the normal-mapped G-buffer normal is the face normal).
+ *
+ * The target is (W / div) x (H / div) (ReflectionTraceFBO, Pipeline.cpp:
+ * 1141, 1744: the reference's resolution 0.25 is div = 4).  Each target
+ * pixel samples the full-resolution G-buffer at its centre: t LINEAR (the
+ * R16F position texture, Pipeline.cpp:1094; at div = 1 the texel itself),
+ * face NEAREST, decoded by GetNormalFromID (:227-238, ids above 5 -> 1,1,1).
+ *   I = normalize(P - eye)                                   (:783)
+ *   o = P + N * 0.035                                        (:785)
+ *   R = reflect(I, N) = I - 2 dot(N, I) N                    (:828-829)
+ *   T = VoxelTraversalDF(o, R), max_iter steps (TRACE_LENGTH = 64)
+ *   on a hit: the sun shadow GetShadowAt(o + R T + n * 0.055, L) with
+ *   shadow_iter steps (150 in the shader, :1098) -> 1 when it hits (:1327-1345)
+ *   hit distance: sky -1 (:767-773); else 0.001 + T on a hit, -1 on a miss
+ *   (AveragedHitDistance :808, 1024-1031), clamped to [-10, 200] (:1036),
+ *   stored as binary16 (R16F, Pipeline.cpp:1141).
+ * eye is the inverse view's translation (the reference's u_ViewerPosition,
+ * the position its view matrix was built from).  hit_out (optional):
+ * face | block << 8 | shadowed << 16 of the reflected hit, 255 | 0 without
+ * one.  samples / rays (optional): DF samples and traced rays per pixel. */
+OAPI void oracle_trace_reflect(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+                               const float* inv_view, const float* inv_proj, int W, int H,
+                               const float* gb_t, const uint8_t* gb_face, const float* light_dir, int div,
+                               int max_iter, int shadow_iter, uint16_t* dist_out, uint32_t* hit_out,
+                               int32_t* samples, int32_t* rays)
+{
+    const Dims d = {nx, ny, nz};
+    const int Wr = W / div, Hr = H / div;
+    const Frame f = {inv_view, inv_proj, Wr, Hr};
+    const Tex tt = {gb_t, W, H, 1};
+    const V3 L = v3(light_dir[0], light_dir[1], light_dir[2]);
+    const V3 eye = f.origin();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int ry = 0; ry < Hr; ry++) {
+        for (int rx = 0; rx < Wr; rx++) {
+            const size_t i = (size_t)ry * Wr + rx;
+            const float sx = texcoord(rx, Wr), sy = texcoord(ry, Hr);
+            float dist;
+            int face;
+            if (div == 1) {
+                dist = gb_t[i];
+                face = gb_face[i];
+            } else {
+                dist = bilinear(tt, sx, sy, 0);
+                face = nearest_u8(gb_face, W, H, sx, sy);
+            }
+            float hd = -1.0f;
+            uint32_t rec = 255u;
+            int smp = 0, nr = 0;
+            if (!(dist < 0.0f)) {
+                const V3 P = position_from(f, sx, sy, dist);
+                const V3 N = face_normal(face, 1.0f);
+                const V3 I = normalize3(v3(P.x - eye.x, P.y - eye.y, P.z - eye.z));
+                const V3 o = v3(P.x + N.x * 0.035f, P.y + N.y * 0.035f, P.z + N.z * 0.035f);
+                const float k = 2.0f * dot3(N, I);
+                const V3 R = v3(I.x - k * N.x, I.y - k * N.y, I.z - k * N.z);
+                const TraceResult tr = voxel_traversal_df(grid, df, d, o, R, max_iter);
+                smp = tr.samples;
+                nr = 1;
+                if (tr.t > 0.0f) {
+                    const V3 hp = v3(o.x + R.x * tr.t, o.y + R.y * tr.t, o.z + R.z * tr.t);
+                    /* the traversal's normal vector (normal[MinIdx] = -RaySign[MinIdx]).
+                     * It is never zero on a hit: a step along an axis whose
+                     * direction component is 0 moves the other coordinates
+                     * by d * inf, and the ray leaves the volume. */
+                    const V3 hn = face_normal(tr.face, 0.0f);
+                    const V3 so = v3(hp.x + hn.x * 0.055f, hp.y + hn.y * 0.055f, hp.z + hn.z * 0.055f);
+                    const TraceResult sr = voxel_traversal_df(grid, df, d, so, L, shadow_iter);
+                    smp += sr.samples;
+                    nr = 2;
+                    hd = 0.001f + tr.t;
+                    rec = (uint32_t)tr.face | ((uint32_t)tr.block << 8) | ((uint32_t)(sr.t > 0.0f) << 16);
+                }
+                hd = fclamp(hd, -10.0f, 200.0f);
+            }
+            dist_out[i] = vpt_f32_to_f16(hd);
+            if (hit_out) hit_out[i] = rec;
+            if (samples) samples[i] = smp;
+            if (rays) rays[i] = nr;
+        }
+    }
 }

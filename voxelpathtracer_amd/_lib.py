@@ -126,6 +126,7 @@ _PROTOS = {
     "vpt_trace_primary": (_i, [_vp, C.POINTER(Camera), _i, C.POINTER(GBuffer), _vp]),
     "vpt_trace_shadow": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _vp, _vp]),
     "vpt_trace_shadow2": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _vp, _vp, _vp]),
+    "vpt_trace_reflect": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _i, _i, _vp, _vp, _vp]),
     "vpt_trace_gi": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(GiParams), _vp, _vp, _vp]),
     "vpt_trace_rays": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "vpt_trace_gi_rays": (_i, [_vp, _vp, _i, _i, C.POINTER(_f), _i, _vp, _vp, _vp]),

@@ -465,6 +465,103 @@ __global__ void __launch_bounds__(256) gi_rays_shadow_kernel(WorldView w, const 
     count_samples(counter, samples, nrays);
 }
 
+// ---- mirror reflections (ReflectionTraceFrag.glsl:764-1035) ---------------
+// The traversal core of the reflection pass: smooth mirror reflections, one
+// sample, no jitter, reprojection / LPV / player off, flat materials.  The
+// target is (W / div) x (H / div); its pixels sample the full-resolution
+// G-buffer at their centres: t GL LINEAR (the texel itself at div = 1),
+// face GL NEAREST, both with REPEAT wrap.
+
+__device__ inline int repeat_idx(int i, int n)
+{
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// texture(sampler2D LINEAR REPEAT, (s, t)) of a float plane, fp32 weights.
+__device__ inline float gl_linear(const float* __restrict__ p, int W, int H, float s, float t)
+{
+    const float u = s * (float)W - 0.5f, v = t * (float)H - 0.5f;
+    const float fu = floorf(u), fv = floorf(v);
+    const float a = u - fu, b = v - fv;
+    const int x0 = repeat_idx((int)fu, W), x1 = repeat_idx((int)fu + 1, W);
+    const size_t r0 = (size_t)repeat_idx((int)fv, H) * W, r1 = (size_t)repeat_idx((int)fv + 1, H) * W;
+    const float top = p[r0 + x0] * (1.0f - a) + p[r0 + x1] * a;
+    const float bot = p[r1 + x0] * (1.0f - a) + p[r1 + x1] * a;
+    return top * (1.0f - b) + bot * b;
+}
+
+// texture(sampler2D NEAREST REPEAT, (s, t)) of a byte plane.
+__device__ inline int gl_nearest(const uint8_t* __restrict__ p, int W, int H, float s, float t)
+{
+    const int x = repeat_idx((int)floorf(s * (float)W), W), y = repeat_idx((int)floorf(t * (float)H), H);
+    return p[(size_t)y * W + x];
+}
+
+// One lane per target pixel: the reflected ray (max_iter steps) and, on a
+// hit, its sun shadow ray (shadow_iter steps) in the same lane.  Mirror
+// reflections of neighbouring pixels stay coherent, so the lanes of a
+// wavefront walk similar paths.
+//   dist_out: binary16 of the hit distance (sky / miss -1, else 0.001 + T,
+//             clamped to [-10, 200]);
+//   hit_out:  face | block << 8 | shadowed << 16, 255 without a hit.
+__global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, int gW, int gH, int div,
+                                                      const float* __restrict__ gb_t,
+                                                      const uint8_t* __restrict__ gb_face, float lx, float ly,
+                                                      float lz, int max_iter, int shadow_iter,
+                                                      uint16_t* __restrict__ dist_out, uint32_t* __restrict__ hit_out,
+                                                      unsigned long long* __restrict__ counter)
+{
+    int px, py;
+    tile_pixel(cam.W, cam.H, px, py);
+    int samples = 0, rays = 0;
+    if (px < cam.W && py < cam.H) {
+        const size_t i = (size_t)py * cam.W + px;
+        float dist;
+        int face;
+        if (div == 1) {
+            dist = gb_t[i];
+            face = gb_face[i];
+        } else {
+            const float sx = vpt::texcoord(px, cam.W), sy = vpt::texcoord(py, cam.H);
+            dist = gl_linear(gb_t, gW, gH, sx, sy);
+            face = gl_nearest(gb_face, gW, gH, sx, sy);
+        }
+        float hd = -1.0f;
+        uint32_t rec = 255u;
+        if (!(dist < 0.0f)) {   // sky: o_HitDistance -1
+            const Vec3 P = vpt::gbuffer_position(cam, px, py, dist);
+            const Vec3 N = vpt::face_normal(face, 1.0f);   // GetNormalFromID: ids above 5 -> (1, 1, 1)
+            const Vec3 I = vpt::normalize3(vpt::mk3(P.x - cam.iv[12], P.y - cam.iv[13], P.z - cam.iv[14]));
+            const Vec3 o = vpt::mk3(P.x + N.x * 0.035f, P.y + N.y * 0.035f, P.z + N.z * 0.035f);
+            const float k = 2.0f * vpt::dot3(N, I);
+            const Vec3 R = vpt::mk3(I.x - k * N.x, I.y - k * N.y, I.z - k * N.z);   // reflect(I, N)
+            const vpt::Hit h = vpt::traverse(w, o, R, max_iter);
+            samples = h.samples;
+            rays = 1;
+            if (h.t > 0.0f) {
+                // GetShadowAt(HitPosition + Normal * 0.055); a hit's normal is
+                // never zero (a step along an axis with a zero direction
+                // component moves the other coordinates by d * inf, and the
+                // ray leaves the volume)
+                const Vec3 hn = vpt::face_normal(h.face, 0.0f);
+                const Vec3 hp = vpt::mk3(o.x + R.x * h.t, o.y + R.y * h.t, o.z + R.z * h.t);
+                const Vec3 so = vpt::mk3(hp.x + hn.x * 0.055f, hp.y + hn.y * 0.055f, hp.z + hn.z * 0.055f);
+                const vpt::Hit sh = vpt::traverse(w, so, vpt::mk3(lx, ly, lz), shadow_iter);
+                samples += sh.samples;
+                rays = 2;
+                hd = 0.001f + h.t;
+                rec = (uint32_t)(h.face & 0xff) | ((uint32_t)(h.block & 0xff) << 8) |
+                      ((uint32_t)(sh.t > 0.0f) << 16);
+            }
+            hd = vpt::fclamp(hd, -10.0f, 200.0f);
+        }
+        dist_out[i] = (uint16_t)vpt::hw_f2h(hd);
+        if (hit_out) hit_out[i] = rec;
+    }
+    count_samples(counter, samples, rays);
+}
+
 dim3 tile_grid(int W, int H) { return dim3((unsigned)(((W + 15) / 16) * ((H + 15) / 16))); }
 
 }  // namespace
@@ -511,6 +608,28 @@ extern "C" VPT_API int vpt_trace_shadow2(vpt_world* w, const vpt_camera* cam, co
     hipLaunchKernelGGL(shadow2_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, (hipStream_t)stream,
                        vpt::make_view(w), vpt::make_cam(cam), gb->d_t, gb->d_face, sun_dir[0], sun_dir[1], sun_dir[2],
                        max_iter, d_mask, d_transversal, w->d_counter);
+    HIP_TRY(hipGetLastError());
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                         const float light_dir[3], int div, int max_iter, int shadow_iter,
+                                         uint16_t* d_dist, uint32_t* d_hit, void* stream)
+{
+    if (!w || !light_dir || !d_dist) return set_error(VPT_EINVAL, "vpt_trace_reflect: null argument");
+    if (int rc = vpt::check_cam(cam, "vpt_trace_reflect")) return rc;
+    if (int rc = vpt::check_gbuffer(gb, cam, "vpt_trace_reflect", false)) return rc;
+    if (int rc = vpt::check_iter(max_iter, "vpt_trace_reflect")) return rc;
+    if (int rc = vpt::check_iter(shadow_iter, "vpt_trace_reflect")) return rc;
+    if (div < 1 || div > 16 || cam->width / div < 1 || cam->height / div < 1)
+        return set_error(VPT_EINVAL, "vpt_trace_reflect: div must be 1..16 and leave at least one pixel");
+    DeviceGuard g(w->device);
+    CamView rc = vpt::make_cam(cam);
+    rc.W = cam->width / div;
+    rc.H = cam->height / div;
+    hipLaunchKernelGGL(reflect_kernel, tile_grid(rc.W, rc.H), dim3(256), 0, (hipStream_t)stream, vpt::make_view(w), rc,
+                       cam->width, cam->height, div, gb->d_t, gb->d_face, light_dir[0], light_dir[1], light_dir[2],
+                       max_iter, shadow_iter, d_dist, d_hit, w->d_counter);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }

@@ -157,6 +157,16 @@ class World:
         call("vpt_trace_shadow2", self.handle, C.byref(cam), C.byref(gbuf.struct()), sd, max_iter, _ptr(mask),
              _ptr(transversal), _stream(stream))
 
+    def trace_reflect(self, cam: Camera, gbuf: "DeviceGBuffer", light_dir, dist, hit=None, div: int = 4,
+                      max_iter: int = 64, shadow_iter: int = 150, stream=None) -> None:
+        """Mirror reflections of the G-buffer (ReflectionTraceFrag.glsl:764-1035,
+        traversal core) at (W // div) x (H // div): `dist` receives the
+        binary16 bits of o_HitDistance (int16/uint16 tensor), `hit` (optional,
+        int32/uint32) face | block << 8 | shadowed << 16, 255 without a hit."""
+        ld = (C.c_float * 3)(*map(float, light_dir))
+        call("vpt_trace_reflect", self.handle, C.byref(cam), C.byref(gbuf.struct()), ld, div, max_iter, shadow_iter,
+             _ptr(dist), _ptr(hit), _stream(stream))
+
     def trace_gi(self, cam: Camera, gbuf: "DeviceGBuffer", prm, rgb, texels=None, stream=None) -> None:
         call("vpt_trace_gi", self.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), _ptr(rgb),
              _ptr(texels), _stream(stream))
