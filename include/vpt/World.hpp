@@ -63,6 +63,17 @@ public:
         m_Pending.clear();
     }
 
+    // MCWorldImporter::ImportWorld (NBT/Importer.cpp:150-168): the region
+    // files of `dir` into m_WorldData, then Buffer(); rebuild the DF after it
+    // (Pipeline.cpp:1236, 1293-1294).  names: palette name -> legacy id.
+    void ImportMinecraftWorld(const char* dir, const float origin[3], const uint8_t mc_to_block[256],
+                              const vpt_mc_block* names = nullptr, int n_names = 0, void* stream = nullptr)
+    {
+        vpt_check(vpt_mc_import(dir, m_X, m_Y, m_Z, origin, mc_to_block, names, n_names,
+                                reinterpret_cast<uint8_t*>(m_WorldData.data()), nullptr));
+        Buffer(stream);
+    }
+
     // World.h:173-174
     void InitializeDistanceGenerator() {}
     void GenerateDistanceField(void* stream = nullptr)

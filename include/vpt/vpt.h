@@ -163,6 +163,32 @@ VPT_API int vpt_df_build(vpt_world* w, void* stream);
 VPT_API int vpt_world_save(vpt_world* w, const char* path);
 VPT_API int vpt_world_load(vpt_world* w, const char* path, void* stream);
 
+/* Minecraft world import: MCWorldImporter::ImportWorld (NBT/Importer.cpp:
+ * 150-168, ImportRegionFile :86-148) with the enkiMI chunk reader it links
+ * (Dependencies/enkiMI/enkimi.c).  Host only, no device work: every *.mca
+ * region file in `dir` (in name order) is decoded into host_grid (nx*ny*nz
+ * bytes, x fastest, cleared first), for the caller to upload with
+ * vpt_world_upload and follow with vpt_df_build, as the reference does
+ * (Pipeline.cpp:1236, 1293-1294).
+ *   Chunk formats: before 1.13 (Level.Sections Blocks / Data), 1.13-1.17
+ *   (Level.Sections Palette / BlockStates) and 1.18+ (sections[].block_states).
+ *   Palette names resolve through `names` to legacy (id, data value); a name
+ *   not listed is stone (1:0), as in the reference's table lookup.
+ *   Only voxels with data value 0 are written, as mc_to_block[id]
+ *   (BlockDatabase::GetIDFromMCID; mc_to_block[0] must be 0), at
+ *   (x, y, z) - (int)origin + (nx/2, 0, nz/2) when inside the world (the
+ *   reference's fixed 384 x 128 x 384 world is nx = 384, nz = 384).
+ *   stats (optional, 3 ints): region files read, chunks decoded, sections
+ *   imported.  VPT_EINVAL for a missing directory or unreadable file. */
+typedef struct vpt_mc_block {
+    const char* name;   /* namespaced id, e.g. "minecraft:stone" */
+    uint8_t id;         /* legacy block id */
+    uint8_t data;       /* legacy data value */
+} vpt_mc_block;
+VPT_API int vpt_mc_import(const char* dir, int nx, int ny, int nz, const float origin[3],
+                          const uint8_t mc_to_block[256], const vpt_mc_block* names, int n_names, uint8_t* host_grid,
+                          int32_t* stats);
+
 /* Incremental GenerateDistanceField after vpt_world_set_blocks with the same
  * edit list (the reference rebuilds the whole field after an edit,
  * World.cpp:367-375 -> World.cpp:69-113).  One workgroup finds, on the

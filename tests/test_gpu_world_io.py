@@ -46,3 +46,26 @@ def test_load_rejects_wrong_size(ctx, tmp_path):
     with pytest.raises(VptError):
         w.LoadWorld(str(tmp_path / "missing.bin"))
     w.close()
+
+
+def test_minecraft_import_then_df(ctx, tmp_path):
+    """MCWorldImporter::ImportWorld then the DF build (Pipeline.cpp:1236,
+    1293-1294): the imported grid equals the reference reader's (committed
+    fixture) and the DF built on the GPU equals the oracle's of that grid."""
+    from tests.golden import mca_synth as M
+    from pathlib import Path
+    z = np.load(Path(__file__).resolve().parent / "golden" / "mca_legacy.npz")
+    d = tmp_path / "region"
+    d.mkdir()
+    for k in z.files:
+        if k.startswith("file:"):
+            (d / k[5:]).write_bytes(z[k].tobytes())
+    nx, ny, nz = M.DIMS
+    w = World(ctx, nx, ny, nz)
+    print("minecraft import 48^3", flush=True)
+    stats = w.ImportMinecraftWorld(str(d), M.ORIGIN, M.block_lut(), M.NAMES)
+    assert stats[2] == int(z["sections"])
+    np.testing.assert_array_equal(w.m_WorldData, z["expected"])
+    w.GenerateDistanceField()
+    np.testing.assert_array_equal(w.download_df(), oracle.df_build(z["expected"]))
+    w.close()

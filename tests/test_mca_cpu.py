@@ -1,0 +1,127 @@
+"""Minecraft world import (vpt_mc_import, host only) against the reference's
+own reader: the committed fixtures hold the grids the reference's enkiMI
+reader and import loop (Core/NBT/Importer.cpp:68-168) produce from
+synthetic region files in every chunk layout; where the reference tree is
+present, fresh cases are compared with that reader live.  A round trip of a
+known grid through legacy region files pins the import independently."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests.golden import mca_synth as M
+from voxelpathtracer_amd._lib import VptError
+from voxelpathtracer_amd.world import mc_import
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+REF_LIB = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "libmca_ref.so"
+
+
+def _write(tmp_path: Path, files: dict) -> Path:
+    d = tmp_path / "region"
+    d.mkdir(exist_ok=True)
+    for name, data in files.items():
+        (d / name).write_bytes(bytes(data))
+    return d
+
+
+@pytest.mark.parametrize("case", M.CASES)
+def test_fixture_matches_reference(tmp_path, case):
+    z = np.load(GOLDEN / f"mca_{case}.npz")
+    files = {k[5:]: z[k].tobytes() for k in z.files if k.startswith("file:")}
+    grid, stats = mc_import(_write(tmp_path, files), M.DIMS, M.ORIGIN, M.block_lut(), M.NAMES)
+    np.testing.assert_array_equal(grid, z["expected"], err_msg=case)
+    assert stats[0] == len(files) and stats[2] == int(z["sections"])
+    assert (grid > 0).sum() > 1000
+
+
+@pytest.mark.skipif(not REF_LIB.exists(), reason="the reference's reader is built only where the reference is present")
+@pytest.mark.parametrize("seed", [2, 3, 4, 5])
+def test_live_against_reference(tmp_path, seed):
+    from tests.golden.make_mca_golden import reference_import
+    ref = C.CDLL(str(REF_LIB))
+    rng = np.random.default_rng(seed)
+    origin = tuple(rng.uniform(-20, 20, 3))
+    lut = M.block_lut(seed)
+    for case in M.CASES:
+        files = M.build_case(case, 100 + seed)
+        want, sections = reference_import(ref, files, M.DIMS, origin, lut)
+        (tmp_path / case).mkdir()
+        d = _write(tmp_path / case, files)
+        got, stats = mc_import(d, M.DIMS, origin, lut, M.NAMES)
+        np.testing.assert_array_equal(got, want, err_msg=f"{case} seed {seed}")
+        assert stats[2] == sections
+
+
+def test_legacy_round_trip(tmp_path):
+    """A known grid written as pre-1.13 chunks (data values 0) imports back
+    unchanged with the identity id map: chunk (cx, cz) section Y holds world
+    cells x = 16 cx + i, y = 16 Y + j, z = 16 cz + k, and the world centres
+    x and z, so origin (nx/2 - x0, ..) cancels the centring."""
+    nx, ny, nz = 32, 32, 48
+    rng = np.random.default_rng(7)
+    want = np.where(rng.random((nz, ny, nx)) < 0.3, rng.integers(1, 256, (nz, ny, nx)), 0).astype(np.uint8)
+    chunks = {}
+    for cz in range(nz // 16):
+        for cx in range(nx // 16):
+            secs = []
+            for sy in range(ny // 16):
+                block = want[16 * cz:16 * cz + 16, 16 * sy:16 * sy + 16, 16 * cx:16 * cx + 16]   # (z, y, x)
+                blocks = np.ascontiguousarray(block.transpose(1, 0, 2)).reshape(-1)          # y*256 + z*16 + x
+                secs.append(M.compound(("Y", M.Tag(M.BYTE, sy)), ("Blocks", M.Tag(M.BYTE_ARRAY, blocks)),
+                                       ("Data", M.Tag(M.BYTE_ARRAY, np.zeros(2048, np.uint8)))))
+            level = M.compound(("xPos", M.Tag(M.INT, cx)), ("zPos", M.Tag(M.INT, cz)),
+                               ("Sections", M.lst(M.COMPOUND, secs)))
+            chunks[cz * 32 + cx] = M.encode_root(M.compound(("Level", level)))
+    d = _write(tmp_path, {"r.0.0.mca": M.region(chunks)})
+    got, stats = mc_import(d, (nx, ny, nz), (nx // 2, 0, nz // 2), None, None)
+    np.testing.assert_array_equal(got, want)
+    assert stats == [1, len(chunks), len(chunks) * (ny // 16)]
+    # data values other than 0 are not imported; the id map applies
+    lut = np.arange(256, dtype=np.uint8)[::-1].copy()
+    lut[0] = 0
+    got2, _ = mc_import(d, (nx, ny, nz), (nx // 2, 0, nz // 2), lut, None)
+    np.testing.assert_array_equal(got2, np.where(want > 0, 255 - want, 0).astype(np.uint8))
+
+
+def test_palette_semantics(tmp_path):
+    """1.18 chunk: a name missing from the table imports as stone (1:0), a
+    non-zero data value is skipped, indices past the palette are air, and a
+    section whose palette has one entry (no data array) is not imported."""
+    pal = [M.compound(("Name", M.Tag(M.STRING, n))) for n in ("minecraft:air", "test:unknown", "minecraft:granite",
+                                                               "minecraft:dirt")]
+    idx = np.tile(np.arange(6), 4096 // 6 + 1)[:4096]            # 4, 5: past the 4-entry palette
+    # data before palette, and a sibling after block_states: the palette loop
+    # ends on block_states' End and the block_states loop on `biomes`, so the
+    # section's own End records it at its Y (the reader's order dependence)
+    biomes = M.compound(("palette", M.lst(M.STRING, [M.Tag(M.STRING, "minecraft:plains")])))
+    sec = M.compound(("Y", M.Tag(M.BYTE, 0)), ("block_states", M.compound(
+        ("data", M.Tag(M.LONG_ARRAY, M.pack_indices(idx, 4, False))), ("palette", M.lst(M.COMPOUND, pal)))),
+        ("biomes", biomes))
+    single = M.compound(("Y", M.Tag(M.BYTE, 1)), ("block_states", M.compound(
+        ("palette", M.lst(M.COMPOUND, [M.compound(("Name", M.Tag(M.STRING, "minecraft:stone")))])))),
+        ("biomes", biomes))
+    root = M.compound(("DataVersion", M.Tag(M.INT, 2975)), ("xPos", M.Tag(M.INT, 0)), ("zPos", M.Tag(M.INT, 0)),
+                      ("sections", M.lst(M.COMPOUND, [sec, single])))
+    d = _write(tmp_path, {"r.0.0.mca": M.region({0: M.encode_root(root)})})
+    names = {"minecraft:granite": (1, 1), "minecraft:dirt": (3, 0), "minecraft:air": (0, 0)}
+    grid, stats = mc_import(d, (16, 32, 16), (8, 0, 8), None, names)
+    assert stats == [1, 1, 1]
+    cells = grid[:, :16, :].transpose(1, 0, 2).reshape(-1)       # y*256 + z*16 + x
+    want = np.array([0, 1, 0, 3, 0, 0], np.uint8)[idx]            # air, stone, granite has data 1, dirt, air, air
+    np.testing.assert_array_equal(cells, want)
+    assert (grid[:, 16:, :] == 0).all()
+
+
+def test_import_errors(tmp_path):
+    with pytest.raises(VptError):
+        mc_import(tmp_path / "missing", (16, 16, 16))
+    lut = np.arange(256, dtype=np.uint8)
+    lut[0] = 3
+    with pytest.raises(VptError):
+        mc_import(tmp_path, (16, 16, 16), (0, 0, 0), lut)
+    grid, stats = mc_import(tmp_path, (16, 16, 16))            # no region files: an empty world
+    assert stats == [0, 0, 0] and not grid.any()

@@ -56,6 +56,10 @@ class SvgfParams(C.Structure):
                 ("steps", C.c_int32 * 5)]
 
 
+class McBlock(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("id", C.c_uint8), ("data", C.c_uint8)]
+
+
 class Ray(C.Structure):
     _fields_ = [("origin", C.c_float * 3), ("dir", C.c_float * 3)]
 
@@ -121,6 +125,7 @@ _PROTOS = {
     "vpt_df_update": (_i, [_vp, _vp, _i, _vp, _vp]),
     "vpt_world_save": (_i, [_vp, C.c_char_p]),
     "vpt_world_load": (_i, [_vp, C.c_char_p, _vp]),
+    "vpt_mc_import": (_i, [C.c_char_p, _i, _i, _i, C.POINTER(_f), _vp, _vp, _i, _vp, _vp]),
     "vpt_df_download": (_i, [_vp, _vp, _vp]),
     "vpt_grid_download": (_i, [_vp, _vp, _vp]),
     "vpt_trace_primary": (_i, [_vp, C.POINTER(Camera), _i, C.POINTER(GBuffer), _vp]),

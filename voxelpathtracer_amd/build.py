@@ -46,7 +46,7 @@ DEVICE = [
     "-packed-fp32-ops",
 ]
 
-SOURCES = ["vpt_host.cpp", "vpt_world.hip", "vpt_trace.hip", "vpt_batch.hip", "vpt_svgf.hip", "vpt_server.hip",
+SOURCES = ["vpt_host.cpp", "vpt_mca.cpp", "vpt_world.hip", "vpt_trace.hip", "vpt_batch.hip", "vpt_svgf.hip", "vpt_server.hip",
            "vpt_unet.hip"]
 
 
@@ -69,7 +69,7 @@ def build(verbose: bool = True) -> Path:
     with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT / "libvpt.so.tmp"
-    cmd = [HIPCC, *DEVICE, "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, *DEVICE, "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lz"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
@@ -111,6 +111,14 @@ def build_oracle(verbose: bool = True) -> Path:
     lib = ROOT / "oracle" / "_build" / "libvpt_oracle.so"
     if verbose:
         print(f"built {lib}")
+    # the reference's own Minecraft region reader, where the reference tree
+    # is present (the importer's checker; the GPU box only runs GPU tests)
+    if Path("/root/reference/Dependencies/enkiMI/enkimi.c").exists():
+        res = subprocess.run([make, "-s", "-C", str(ROOT / "oracle"), "ref"], capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"oracle/_ref build failed:\n{res.stdout}\n{res.stderr}")
+        if verbose:
+            print(f"built {ROOT / 'oracle' / '_ref' / 'libmca_ref.so'}")
     return lib
 
 

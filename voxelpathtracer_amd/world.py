@@ -185,6 +185,17 @@ class World:
         call("vpt_trace_gi_rays", self.handle, _ptr(rays), n, gi_iter, sd, shadow_iter, _ptr(hits), _ptr(vis),
              _stream(stream))
 
+    def ImportMinecraftWorld(self, directory: str, origin=(0.0, 0.0, 0.0), mc_to_block=None, names=None,
+                             stream=None) -> list:
+        """MCWorldImporter::ImportWorld (NBT/Importer.cpp:150-168): the
+        region files of `directory` into m_WorldData, then Buffer().  See
+        mc_import for the arguments; returns its stats.  Rebuild the DF
+        afterwards (Pipeline.cpp:1293-1294)."""
+        grid, stats = mc_import(directory, (self.nx, self.ny, self.nz), origin, mc_to_block, names)
+        self.m_WorldData[...] = grid
+        self.Buffer(stream)
+        return stats
+
     def SaveWorld(self, path: str) -> None:
         """Raw world file: nx*ny*nz block ids, x fastest (WorldFileHandler.cpp:10-37)."""
         call("vpt_world_save", self.handle, str(path).encode())
@@ -253,3 +264,30 @@ def gi_params(sun_tick: float = 50.0, frame: int = 0, albedo_seed: int = 0x5EED0
 
 
 __all__ = ["Context", "World", "DeviceGBuffer", "make_camera", "generate_terrain", "gi_params", "Ray", "Hit"]
+
+
+def mc_import(directory: str, dims, origin=(0.0, 0.0, 0.0), mc_to_block=None, names=None):
+    """Minecraft region files -> a (nz, ny, nx) uint8 block grid and stats
+    [files, chunks, sections] (vpt_mc_import, host only).
+
+    mc_to_block: 256 legacy-id -> block-id bytes (BlockDatabase::
+    GetIDFromMCID); the identity when None.  names: {namespaced id: (legacy
+    id, data value)} for palette-format chunks; a name not listed imports as
+    stone (1:0)."""
+    nx, ny, nz = map(int, dims)
+    lut = np.arange(256, dtype=np.uint8) if mc_to_block is None else np.ascontiguousarray(mc_to_block, np.uint8)
+    if lut.shape != (256,):
+        raise ValueError("mc_to_block must hold 256 entries")
+    items = list((names or {}).items())
+    table = (_lib.McBlock * max(len(items), 1))()
+    keep = []
+    for k, (name, (bid, dv)) in enumerate(items):
+        b = name.encode()
+        keep.append(b)
+        table[k] = _lib.McBlock(b, int(bid), int(dv))
+    grid = np.empty((nz, ny, nx), np.uint8)
+    stats = (C.c_int32 * 3)()
+    org = (C.c_float * 3)(*map(float, origin))
+    call("vpt_mc_import", str(directory).encode(), nx, ny, nz, org, lut.ctypes.data_as(C.c_void_p),
+         C.cast(table, C.c_void_p), len(items), grid.ctypes.data_as(C.c_void_p), stats)
+    return grid, list(stats)
