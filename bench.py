@@ -5,8 +5,10 @@ Workload (one step = one frame of rays, inputs resident in HBM):
   grass/dirt/stone layering, 1% of solid voxels given random ids 1-100),
   distance field built on the GPU; at 1920x1080, 1 spp: one primary ray per
   pixel, one sun-shadow ray per lit pixel, one cosine-weighted diffuse GI
-  ray per hit pixel plus a sun-shadow ray at the GI hit.  Grays/s counts
-  every ray whose traversal loop is entered.  SVGF ms/frame (pre-spatial,
+  ray per hit pixel plus a sun-shadow ray at the GI hit, run as the library
+  runs a frame (vpt_trace_frame: the shadow pass beside GI on a second
+  stream; the passes one after another are reported too, with each pass's
+  time).  Grays/s counts every ray whose traversal loop is entered.  SVGF ms/frame (pre-spatial,
   temporal, variance, 5 a-trous, decode at 1920x1080) and DF-build ms are
   reported beside it, with the wall time of an incremental DF update after
   a one-block edit, the sun-shadow denoiser's ms/frame over the same
@@ -110,7 +112,11 @@ def main():
     rgb = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
     tex = torch.empty((H, W, 8), dtype=torch.int16, device="cuda")
 
-    def step(events=None):
+    def step():
+        # the frame as the library runs it: the shadow pass beside GI
+        world.trace_frame(cam, gb, prm, mask, rgb, tex, 350, 350)
+
+    def step_seq(events=None):
         if events is not None:
             events[0].record(stream)
         world.trace_primary(cam, gb, 350)
@@ -140,16 +146,26 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    evs = [[ev() for _ in range(4)] for _ in range(args.steps)]
     t_start, t_end = ev(), ev()
-    torch.cuda.synchronize()
     t_start.record(stream)
     for k in range(args.steps):
-        step(evs[k])
+        step()
     t_end.record(stream)
     torch.cuda.synchronize()
     total_ms = t_start.elapsed_time(t_end)
     ms_per_step = total_ms / args.steps
+    # each pass on its own (one stream, in order) for the per-pass times
+    for _ in range(args.warmup):
+        step_seq()
+    evs = [[ev() for _ in range(4)] for _ in range(args.steps)]
+    q0, q1 = ev(), ev()
+    torch.cuda.synchronize()
+    q0.record(stream)
+    for k in range(args.steps):
+        step_seq(evs[k])
+    q1.record(stream)
+    torch.cuda.synchronize()
+    ms_sequential = q0.elapsed_time(q1) / args.steps
     pass_ms = {
         "primary": float(np.mean([e[0].elapsed_time(e[1]) for e in evs])),
         "shadow": float(np.mean([e[1].elapsed_time(e[2]) for e in evs])),
@@ -320,6 +336,7 @@ def main():
                    "parallelism": "single GPU"},
         "path": path,
         "plain": {"ms_per_frame": round(ms_plain, 4), "grays": round(grays_plain, 4),
+                  "ms_per_frame_passes_in_order": round(ms_sequential, 4),
                   "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()}, "work": counts},
         "persistent": persistent,
         "df_build_ms": round(df_ms, 4),

@@ -79,9 +79,7 @@ public:
             vpt_check(vpt_frame_trace(m_Frame, m_World.Handle(), &cam, &prm, 350, 350, &cur, m_Mask, m_RGB, m_Texels,
                                       1, stream));
         } else {
-            vpt_check(vpt_trace_primary(m_World.Handle(), &cam, 350, &cur, stream));
-            vpt_check(vpt_trace_gi(m_World.Handle(), &cam, &cur, &prm, m_RGB, m_Texels, stream));
-            vpt_check(vpt_trace_shadow(m_World.Handle(), &cam, &cur, prm.sun_dir, 350, m_Mask, stream));
+            vpt_check(vpt_trace_frame(m_World.Handle(), &cam, &prm, 350, 350, &cur, m_Mask, m_RGB, m_Texels, stream));
         }
         float prev_vp[16];
         ViewProj(m_PrevView, m_PrevProj, prev_vp);

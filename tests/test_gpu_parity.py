@@ -178,6 +178,53 @@ def test_gi_parity(terrain_world, gbuffer_1080):
     np.testing.assert_array_equal(d["ao"], planes["ao"])
 
 
+def test_trace_frame_parity(terrain_world):
+    """vpt_trace_frame (shadow pass beside GI on the context's side stream)
+    equals the oracle's three passes, frame after frame, with the DF samples
+    and rays of all three counted."""
+    g, df, w = terrain_world
+    torch = _torch()
+    W, H = 640, 360
+    cam, view, proj, iv, ip = camera(W, H)
+    gb = DeviceGBuffer(W, H)
+    mask = torch.full((H, W), 7, dtype=torch.uint8, device="cuda")
+    rgb = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    tex = torch.empty((H, W, 8), dtype=torch.int16, device="cuda")
+    counter = torch.zeros(2, dtype=torch.int64, device="cuda")
+    for frame in range(3):
+        prm = gi_params(frame=frame)
+        print(f"trace_frame {W}x{H} frame {frame}", flush=True)
+        counter.zero_()
+        w.set_sample_counter(counter)
+        w.trace_frame(cam, gb, prm, mask, rgb, tex)
+        w.set_sample_counter(None)
+        torch.cuda.synchronize()
+        host = gb.numpy()
+        p = oracle.trace_primary(g, df, iv, ip, W, H, 350)
+        _check_gbuffer(host, p)
+        sd = np.array(prm.sun_dir, np.float32)
+        sh = oracle.trace_shadow(g, df, iv, ip, host["t"], host["face"], sd, 350)
+        np.testing.assert_array_equal(mask.cpu().numpy(), sh["mask"])
+        gi = oracle.trace_gi(g, df, iv, ip, host["t"], host["face"], oracle.gi_params_from(prm))
+        np.testing.assert_array_equal(rgb.cpu().numpy(), gi["rgb"])
+        # the same work as the three passes called one after another
+        c = counter.cpu().numpy().copy()
+        counter.zero_()
+        w.set_sample_counter(counter)
+        w.trace_primary(cam, gb, 350)
+        w.trace_shadow(cam, gb, list(prm.sun_dir), mask, 350)
+        w.trace_gi(cam, gb, prm, rgb, tex)
+        w.set_sample_counter(None)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(c, counter.cpu().numpy())
+        assert c[1] > W * H
+    from voxelpathtracer_amd._lib import VptError
+    with pytest.raises(VptError):
+        w.trace_frame(cam, gb, prm, None, rgb, tex)
+    with pytest.raises(VptError):
+        w.trace_frame(cam, gb, prm, mask, rgb, tex, primary_iter=-1)
+
+
 def test_ray_queries_parity(terrain_world):
     g, df, w = terrain_world
     torch = _torch()

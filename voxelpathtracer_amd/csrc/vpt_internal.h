@@ -71,6 +71,11 @@ struct vpt_ctx {
     uint32_t* d_gi_jobs = nullptr;
     uint32_t* d_gi_njobs = nullptr;
     size_t gi_cap = 0;   // pixels
+    // vpt_trace_frame: the stream the shadow pass runs on beside GI, and the
+    // fork / join events (created on first use, default priority)
+    void* side_stream = nullptr;
+    void* ev_fork = nullptr;
+    void* ev_join = nullptr;
 };
 
 struct vpt_world : vpt::World {

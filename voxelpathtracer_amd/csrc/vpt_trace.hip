@@ -634,6 +634,42 @@ extern "C" VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, co
     return VPT_OK;
 }
 
+extern "C" VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, const vpt_gi_params* prm,
+                                       int primary_iter, int shadow_iter, const vpt_gbuffer* gb, uint8_t* d_mask,
+                                       float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream)
+{
+    if (!w || !prm || !d_mask || !d_rgb) return set_error(VPT_EINVAL, "vpt_trace_frame: null argument");
+    if (int rc = vpt::check_cam(cam, "vpt_trace_frame")) return rc;
+    if (int rc = vpt::check_gbuffer(gb, cam, "vpt_trace_frame", true)) return rc;
+    if (int rc = vpt::check_iter(primary_iter, "vpt_trace_frame")) return rc;
+    if (int rc = vpt::check_iter(shadow_iter, "vpt_trace_frame")) return rc;
+    if (int rc = vpt::check_iter(prm->gi_iter, "vpt_trace_frame")) return rc;
+    if (int rc = vpt::check_iter(prm->shadow_iter, "vpt_trace_frame")) return rc;
+    DeviceGuard g(w->device);
+    vpt_ctx* ctx = w->ctx;
+    if (!ctx->side_stream) {
+        hipStream_t side;
+        hipEvent_t e0, e1;
+        HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&e0, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+        ctx->side_stream = side;
+        ctx->ev_fork = e0;
+        ctx->ev_join = e1;
+    }
+    hipStream_t s = (hipStream_t)stream, side = (hipStream_t)ctx->side_stream;
+    if (int rc = vpt_trace_primary(w, cam, primary_iter, gb, stream)) return rc;
+    // fork: the shadow pass reads the G-buffer beside GI (both only read
+    // it); join before anything later on `stream`
+    HIP_TRY(hipEventRecord((hipEvent_t)ctx->ev_fork, s));
+    HIP_TRY(hipStreamWaitEvent(side, (hipEvent_t)ctx->ev_fork, 0));
+    const int rc_shadow = vpt_trace_shadow(w, cam, gb, prm->sun_dir, shadow_iter, d_mask, side);
+    HIP_TRY(hipEventRecord((hipEvent_t)ctx->ev_join, side));
+    const int rc_gi = vpt_trace_gi(w, cam, gb, prm, d_rgb, d_svgf_in, stream);
+    HIP_TRY(hipStreamWaitEvent(s, (hipEvent_t)ctx->ev_join, 0));
+    return rc_shadow ? rc_shadow : rc_gi;
+}
+
 extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
                                     const vpt_gi_params* prm, float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream)
 {

@@ -440,6 +440,9 @@ extern "C" VPT_API int vpt_ctx_destroy(vpt_ctx* ctx)
     (void)hipFree(ctx->d_gi_hits);
     (void)hipFree(ctx->d_gi_jobs);
     (void)hipFree(ctx->d_gi_njobs);
+    if (ctx->side_stream) (void)hipStreamDestroy((hipStream_t)ctx->side_stream);
+    if (ctx->ev_fork) (void)hipEventDestroy((hipEvent_t)ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy((hipEvent_t)ctx->ev_join);
     delete ctx;
     return VPT_OK;
 }

@@ -141,20 +141,19 @@ class Pipeline:
         self.prev_proj = None
 
     def render(self, pos, yaw: float, pitch: float, stream=None):
-        """One frame: primary -> GI -> SVGF -> shadow.  Returns the camera."""
+        """One frame: primary -> shadow beside GI -> SVGF.  Returns the camera."""
         cam, view, proj = make_camera(pos, yaw, pitch, self.width, self.height)
         cur = self.gbuf[self.frame_index % 2]
         prev = self.gbuf[(self.frame_index + 1) % 2]
         if self.prev_view is None:
             self.prev_view, self.prev_proj = view, proj
-        self.world.trace_primary(cam, cur, self.max_iter, stream)
         prm = gi_params(self.sun_tick, self.frame_index)
-        self.world.trace_gi(cam, cur, prm, self.rgb, self.texels, stream)
+        self.world.trace_frame(cam, cur, prm, self.shadow, self.rgb, self.texels, self.max_iter, self.shadow_iter,
+                               stream)
         if self.frame_index == 0:
             prev = cur
         self.svgf.frame(cam, viewproj(self.prev_view, self.prev_proj), cur, prev, self.texels, self.denoised,
                         time=self.frame_index / 60.0, stream=stream)
-        self.world.trace_shadow(cam, cur, list(prm.sun_dir), self.shadow, self.shadow_iter, stream)
         self.prev_view, self.prev_proj = view, proj
         self.frame_index += 1
         return cam

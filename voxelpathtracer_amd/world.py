@@ -167,6 +167,13 @@ class World:
         call("vpt_trace_reflect", self.handle, C.byref(cam), C.byref(gbuf.struct()), ld, div, max_iter, shadow_iter,
              _ptr(dist), _ptr(hit), _stream(stream))
 
+    def trace_frame(self, cam: Camera, gbuf: "DeviceGBuffer", prm, mask, rgb, texels=None, primary_iter: int = 350,
+                    shadow_iter: int = 350, stream=None) -> None:
+        """Primary, shadow and GI passes of one frame; the shadow pass runs
+        beside GI on a stream of the context (vpt_trace_frame)."""
+        call("vpt_trace_frame", self.handle, C.byref(cam), C.byref(prm), primary_iter, shadow_iter,
+             C.byref(gbuf.struct()), _ptr(mask), _ptr(rgb), _ptr(texels), _stream(stream))
+
     def trace_gi(self, cam: Camera, gbuf: "DeviceGBuffer", prm, rgb, texels=None, stream=None) -> None:
         call("vpt_trace_gi", self.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), _ptr(rgb),
              _ptr(texels), _stream(stream))
