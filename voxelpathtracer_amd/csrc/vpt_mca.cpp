@@ -18,6 +18,7 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <memory>
 #include <string>
 #include <string_view>
 #include <unordered_map>
@@ -479,7 +480,7 @@ int import_region(const std::string& path, const Target& t, const NameTable& nam
     fclose(f);
     int chunks = 0;
     std::vector<uint8_t> nbt;
-    Chunk* c = new Chunk;
+    std::unique_ptr<Chunk> c(new Chunk);
     for (int i = 0; i < 1024 && file.size() >= kRegionHeader; i++) {
         const size_t loc = (((size_t)file[4 * i] << 16) | ((size_t)file[4 * i + 1] << 8) | file[4 * i + 2]) * kSector;
         if (loc < kRegionHeader || loc + 6 > file.size()) continue;
@@ -503,7 +504,6 @@ int import_region(const std::string& path, const Target& t, const NameTable& nam
                     }
         }
     }
-    delete c;
     return chunks;
 }
 

@@ -648,11 +648,16 @@ extern "C" VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, cons
     DeviceGuard g(w->device);
     vpt_ctx* ctx = w->ctx;
     if (!ctx->side_stream) {
-        hipStream_t side;
-        hipEvent_t e0, e1;
-        HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&e0, hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+        hipStream_t side = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&e0, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&e1, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            if (e0) (void)hipEventDestroy(e0);
+            if (side) (void)hipStreamDestroy(side);
+            return set_error(VPT_EDEVICE, "vpt_trace_frame: side stream: %s", hipGetErrorString(e));
+        }
         ctx->side_stream = side;
         ctx->ev_fork = e0;
         ctx->ev_join = e1;
