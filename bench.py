@@ -12,9 +12,10 @@ Workload (one step = one frame of rays, inputs resident in HBM):
   temporal, variance, 5 a-trous, decode at 1920x1080) and DF-build ms are
   reported beside it, with the wall time of an incremental DF update after
   a one-block edit, the sun-shadow denoiser's ms/frame over the same
-  sequence, and the mirror-reflection pass (reflected ray + sun shadow at
-  its hit, 64 / 150 steps) off the frame's G-buffer at 1/4 and full
-  resolution.
+  sequence, and the reflection pass (reflected rays + sun shadow at their
+  hits, 64 / 150 steps) off the frame's G-buffer with the reference's
+  settings (1/4 resolution, 2 rough samples) and as full-resolution
+  mirrors.
 
 Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
 """
@@ -294,28 +295,32 @@ def main():
     sd.close()
     del shadow_in
 
-    # ---- mirror reflections (the reflection pass's traversal core) off the
-    # frame's G-buffer: at the reference's quarter resolution and at full
+    # ---- reflections (the reflection pass's traversal core) off the frame's
+    # G-buffer: the reference's settings (1/4 resolution, 2 rough samples)
+    # and full-resolution mirror reflections
+    from voxelpathtracer_amd.world import reflect_params
     reflect = {}
-    for div in (4, 1):
+    for name, kw in (("reference_settings", reflect_params(sun, 0)),
+                     ("mirror_full_res", dict(div=1, max_iter=64, shadow_iter=150, spp=1, rough=0))):
+        div = kw["div"]
         Wr, Hr = W // div, H // div
         r_dist = torch.empty((Hr, Wr), dtype=torch.int16, device="cuda")
         r_hit = torch.empty((Hr, Wr), dtype=torch.int32, device="cuda")
         r_ctr = torch.zeros(2, dtype=torch.int64, device="cuda")
         world.set_sample_counter(r_ctr)
-        world.trace_reflect(cam, gb, sun, r_dist, r_hit, div=div)
+        world.trace_reflect(cam, gb, sun, r_dist, r_hit, **kw)
         torch.cuda.synchronize()
         world.set_sample_counter(None)
         r_rays = int(r_ctr[1].item())
         s0, s1 = ev(), ev()
         s0.record(stream)
         for _ in range(20):
-            world.trace_reflect(cam, gb, sun, r_dist, r_hit, div=div)
+            world.trace_reflect(cam, gb, sun, r_dist, r_hit, **kw)
         s1.record(stream)
         torch.cuda.synchronize()
         r_ms = s0.elapsed_time(s1) / 20
-        reflect[f"div{div}"] = {"target": [Wr, Hr], "ms": round(r_ms, 4), "rays": r_rays,
-                                "samples": int(r_ctr[0].item()), "grays": round(r_rays / r_ms / 1e6, 4)}
+        reflect[name] = {"target": [Wr, Hr], "spp": kw["spp"], "rough": kw["rough"], "ms": round(r_ms, 4),
+                         "rays": r_rays, "samples": int(r_ctr[0].item()), "grays": round(r_rays / r_ms / 1e6, 4)}
 
     result = {
         "metric": "Grays/s DDA+GI @1080p (bit-exact hits); SVGF ms/frame; served p50/p99 latency",

@@ -230,26 +230,49 @@ VPT_API int vpt_trace_shadow2(vpt_world* w, const vpt_camera* cam, const vpt_gbu
                               int max_iter, uint8_t* d_mask, uint16_t* d_transversal, void* stream);
 
 /* REFLECTION_TRACE (ReflectionTraceFrag.glsl:764-1035) reduced to its
- * traversal core: smooth mirror reflections (u_RoughReflections off), one
- * sample, no jitter, screen-space reprojection, LPV and player off, flat
- * materials (the reflection normal is the G-buffer face normal).  Textured
- * shading of the hit (albedo, PBR, emissive, normal maps) is not computed.
- * The target is (width / div) x (height / div), div 1..16; the reference's
- * ReflectionTraceResolution 0.25 (Pipeline.cpp:101, 1744) is div = 4.  Its
- * pixels sample the G-buffer at their centres (t LINEAR, face NEAREST).
- * The reflected ray starts at P + N * 0.035 in direction reflect(I, N),
- * I = normalize(P - camera position), and takes at most max_iter steps
- * (u_ReflectionTraceLength, 64); a hit casts the sun shadow ray from
- * hit + normal * 0.055 toward light_dir with shadow_iter steps (150 in
- * the shader).
- *   d_dist: binary16 bits per target pixel of o_HitDistance (R16F,
- *           Pipeline.cpp:1141): -1 for sky and misses, else 0.001 + T,
- *           clamped to [-10, 200];
- *   d_hit (optional): face | block << 8 | shadowed << 16 of the reflected
- *           hit, 255 without one. */
-VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const float light_dir[3],
-                              int div, int max_iter, int shadow_iter, uint16_t* d_dist, uint32_t* d_hit,
-                              void* stream);
+ * traversal core: flat materials (the reflection normal is the G-buffer
+ * face normal, the roughness one value per call), no jitter, screen-space
+ * reprojection, LPV and player off.  Textured shading of the hits (albedo,
+ * PBR, emissive, normal maps) is not computed.
+ *   target   (width / div) x (height / div), div 1..16; the reference's
+ *            ReflectionTraceResolution 0.25 (Pipeline.cpp:101, 1744) is
+ *            div = 4.  Its pixels sample the G-buffer at their centres (t
+ *            LINEAR, face NEAREST).
+ *   rays     per sample (u_SPP, 1..16): from P + N * 0.035 in direction
+ *            reflect(I, H), I = normalize(P - camera position), at most
+ *            max_iter steps (u_ReflectionTraceLength, 64).  H = N for
+ *            mirror reflections (rough = 0); with rough = 1
+ *            (u_RoughReflections) H is the GGX half vector nearest N of
+ *            three (GetReflectionDirection, :617-640, ImportanceSampleGGX
+ *            :345-366) at roughness max(clamp(roughness, 0.01, 1), 0.05),
+ *            from randoms u = pcg_hash chain of (px + W*py + W*H*frame) in
+ *            place of the reference's blue-noise table, scaled by (0.9, 0.65).
+ *            The first max(spp / 4, 1) samples that hit cast the sun shadow
+ *            ray from hit + normal * 0.055 toward light_dir (shadow_iter
+ *            steps, 150 in the shader).
+ *   d_dist   binary16 bits per target pixel of o_HitDistance (R16F,
+ *            Pipeline.cpp:1141): -1 for sky and when no sample hits, else
+ *            (0.001 + sum of T) / hits, clamped to [-10, 200].
+ *   d_hit    (optional) face | block << 8 of the first sample that hit |
+ *            shadow << 16 (the last shadow ray's result) | hits << 24; 255
+ *            without a hit. */
+typedef struct vpt_reflect_params {
+    float light_dir[3];     /* u_StrongerLightDirection */
+    int32_t div;            /* target = G-buffer / div */
+    int32_t max_iter;       /* reflected ray: traversal steps */
+    int32_t shadow_iter;    /* sun shadow ray: traversal steps */
+    int32_t spp;            /* u_SPP */
+    int32_t rough;          /* u_RoughReflections */
+    float roughness;        /* RoughnessAt * RoughnessBias of the flat material */
+    uint32_t frame;         /* seed of the GGX samples */
+} vpt_reflect_params;
+
+/* The reference's defaults (Pipeline.cpp:100-114): div 4, 64 / 150 steps,
+ * 2 samples, rough reflections at roughness 0.5 * 0.85 (RoughnessBias). */
+VPT_API int vpt_reflect_params_default(const float light_dir[3], uint32_t frame, vpt_reflect_params* out);
+
+VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                              const vpt_reflect_params* prm, uint16_t* d_dist, uint32_t* d_hit, void* stream);
 
 /* DIFFUSE_TRACE (DiffuseRayTraceFrag.glsl:822-935), 1 spp, 1 bounce with a
  * shadow ray at the bounce hit.  d_rgb: 3 floats per pixel (clamped to

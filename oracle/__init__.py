@@ -114,10 +114,11 @@ def trace_shadow2(grid, df, inv_view, inv_proj, gb_t, gb_face, sun_dir, max_iter
 
 
 def trace_reflect(grid, df, inv_view, inv_proj, gb_t, gb_face, light_dir, div: int = 4, max_iter: int = 64,
-                  shadow_iter: int = 150):
-    """Mirror reflections (ReflectionTraceFrag.glsl:764-1035, traversal core)
-    at (W // div) x (H // div): hit distance (binary16 bits), hit record
-    face | block << 8 | shadowed << 16 (255 without a hit), samples, rays."""
+                  shadow_iter: int = 150, spp: int = 1, rough: int = 0, roughness: float = 0.425, frame: int = 0):
+    """Reflections (ReflectionTraceFrag.glsl:764-1035, traversal core) at
+    (W // div) x (H // div), mirror (rough 0) or GGX-sampled (rough 1), spp
+    samples: hit distance (binary16 bits), hit record face | block << 8 |
+    shadow << 16 | hits << 24 (255 without a hit), samples, rays."""
     nz, ny, nx = grid.shape
     H, W = gb_t.shape
     Hr, Wr = H // div, W // div
@@ -127,7 +128,8 @@ def trace_reflect(grid, df, inv_view, inv_proj, gb_t, gb_face, light_dir, div: i
     rays = np.empty((Hr, Wr), np.int32)
     lib().oracle_trace_reflect(_p(grid), _p(df), nx, ny, nz, _p(_f32(inv_view)), _p(_f32(inv_proj)), W, H,
                                _p(_f32(gb_t)), _p(np.ascontiguousarray(gb_face, np.uint8)), _p(_f32(light_dir)), div,
-                               max_iter, shadow_iter, _p(dist), _p(hit), _p(samples), _p(rays))
+                               max_iter, shadow_iter, spp, rough, C.c_float(roughness), C.c_uint32(frame), _p(dist),
+                               _p(hit), _p(samples), _p(rays))
     return dict(dist=dist, hit=hit, samples=samples, rays=rays)
 
 

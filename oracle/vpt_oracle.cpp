@@ -1538,10 +1538,51 @@ OAPI void oracle_unorm8_decode_table(float* out)
     for (int k = 0; k < 256; k++) out[k] = vpt_unorm8_decode((uint8_t)k);
 }
 
+/* ImportanceSampleGGX (ReflectionTraceFrag.glsl:345-366): the GGX half
+ * vector around N for roughness r and randoms (x0, x1), phi = 2 pi x0 through
+ * vpt_sincos_2pi. */
+static inline V3 ggx_half(V3 N, float r, float x0, float x1)
+{
+    const float alpha = r * r;
+    const float alpha2 = alpha * alpha;
+    float sp, cp;
+    vpt_sincos_2pi(x0, &sp, &cp);
+    const float cos_t = sqrtf((1.0f - x1) / (1.0f + (alpha2 - 1.0f) * x1));
+    const float sin_t = sqrtf(1.0f - cos_t * cos_t);
+    const V3 h = v3(cp * sin_t, sp * sin_t, cos_t);
+    const V3 up = fabsf(N.z) < 0.999f ? v3(0.0f, 0.0f, 1.0f) : v3(1.0f, 0.0f, 0.0f);
+    const V3 t = normalize3(cross3(up, N));
+    const V3 b = cross3(N, t);
+    return normalize3(v3((t.x * h.x + b.x * h.y) + N.x * h.z, (t.y * h.x + b.y * h.y) + N.y * h.z,
+                         (t.z * h.x + b.z * h.y) + N.z * h.z));
+}
+
+/* GetReflectionDirection (:617-640): of three GGX half vectors, the first
+ * with the largest dot with N; randoms drawn from the pcg_hash chain. */
+static inline V3 reflection_normal(V3 N, float r, uint32_t* state)
+{
+    r = fmaxf(r, 0.05f);
+    float nearest = -100.0f;
+    V3 best = N;
+    for (int i = 0; i < 3; i++) {
+        *state = vpt_pcg_hash(*state);
+        const float x0 = vpt_u32_to_unit(*state) * 0.9f;
+        *state = vpt_pcg_hash(*state);
+        const float x1 = vpt_u32_to_unit(*state) * 0.65f;
+        const V3 h = ggx_half(N, r, x0, x1);
+        const float d = dot3(h, N);
+        if (d > nearest) {
+            best = h;
+            nearest = d;
+        }
+    }
+    return best;
+}
+
 /* REFLECTION_TRACE (ReflectionTraceFrag.glsl:764-1035) reduced to its
- * traversal core: smooth mirror reflections (u_RoughReflections off), one
- * sample, no jitter, screen-space reprojection, LPV and player off, flat
- * materials (the normal-mapped G-buffer normal is the face normal).
+ * traversal core: flat materials (the normal-mapped G-buffer normal is the
+ * face normal, the roughness one value), no jitter, screen-space
+ * reprojection, LPV and player off.
  *
  * The target is (W / div) x (H / div) (ReflectionTraceFBO, Pipeline.cpp:
  * 1141, 1744: the reference's resolution 0.25 is div = 4).  Each target
@@ -1550,22 +1591,28 @@ OAPI void oracle_unorm8_decode_table(float* out)
  * face NEAREST, decoded by GetNormalFromID (:227-238, ids above 5 -> 1,1,1).
  *   I = normalize(P - eye)                                   (:783)
  *   o = P + N * 0.035                                        (:785)
- *   R = reflect(I, N) = I - 2 dot(N, I) N                    (:828-829)
- *   T = VoxelTraversalDF(o, R), max_iter steps (TRACE_LENGTH = 64)
- *   on a hit: the sun shadow GetShadowAt(o + R T + n * 0.055, L) with
- *   shadow_iter steps (150 in the shader, :1098) -> 1 when it hits (:1327-1345)
- *   hit distance: sky -1 (:767-773); else 0.001 + T on a hit, -1 on a miss
- *   (AveragedHitDistance :808, 1024-1031), clamped to [-10, 200] (:1036),
- *   stored as binary16 (R16F, Pipeline.cpp:1141).
+ *   per sample s < spp (:828-1023):
+ *     H = rough ? reflection_normal(N, clamp(roughness, 0.01, 1)) : N
+ *     R = reflect(I, H) = I - 2 dot(H, I) H
+ *     T = VoxelTraversalDF(o, R), max_iter steps (TRACE_LENGTH = 64)
+ *     on a hit (T > 0): the first max(spp / 4, 1) hits cast the sun shadow
+ *     GetShadowAt(o + R T + n * 0.055, L) with shadow_iter steps (150 in
+ *     the shader, :1098; 1 when it hits, :1327-1345); sum += T
+ *   hit distance: sky -1 (:767-773); else (0.001 + sum) / hits, -1 without
+ *   hits (:808, 1024-1031), clamped to [-10, 200] (:1036), stored as
+ *   binary16 (R16F, Pipeline.cpp:1141).
+ * Mirror samples (rough = 0) repeat one ray: it is traced once and its
+ * result counted spp times.  Randoms: the chain state starts at
+ * pcg_hash(pcg_hash(px + Wr*py + Wr*Hr*frame) + 0x9E3779B9).
  * eye is the inverse view's translation (the reference's u_ViewerPosition,
  * the position its view matrix was built from).  hit_out (optional):
- * face | block << 8 | shadowed << 16 of the reflected hit, 255 | 0 without
- * one.  samples / rays (optional): DF samples and traced rays per pixel. */
+ * face | block << 8 of the first hit | shadow << 16 | hits << 24, 255
+ * without a hit.  samples / rays (optional): DF samples and traced rays. */
 OAPI void oracle_trace_reflect(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
                                const float* inv_view, const float* inv_proj, int W, int H,
                                const float* gb_t, const uint8_t* gb_face, const float* light_dir, int div,
-                               int max_iter, int shadow_iter, uint16_t* dist_out, uint32_t* hit_out,
-                               int32_t* samples, int32_t* rays)
+                               int max_iter, int shadow_iter, int spp, int rough, float roughness, uint32_t frame,
+                               uint16_t* dist_out, uint32_t* hit_out, int32_t* samples, int32_t* rays)
 {
     const Dims d = {nx, ny, nz};
     const int Wr = W / div, Hr = H / div;
@@ -1573,6 +1620,8 @@ OAPI void oracle_trace_reflect(const uint8_t* grid, const uint8_t* df, int nx, i
     const Tex tt = {gb_t, W, H, 1};
     const V3 L = v3(light_dir[0], light_dir[1], light_dir[2]);
     const V3 eye = f.origin();
+    const int shadow_rays = std::max(spp / 4, 1);
+    const float rr = fclamp(roughness, 0.01f, 1.0f);
 #pragma omp parallel for schedule(dynamic, 4)
     for (int ry = 0; ry < Hr; ry++) {
         for (int rx = 0; rx < Wr; rx++) {
@@ -1595,24 +1644,48 @@ OAPI void oracle_trace_reflect(const uint8_t* grid, const uint8_t* df, int nx, i
                 const V3 N = face_normal(face, 1.0f);
                 const V3 I = normalize3(v3(P.x - eye.x, P.y - eye.y, P.z - eye.z));
                 const V3 o = v3(P.x + N.x * 0.035f, P.y + N.y * 0.035f, P.z + N.z * 0.035f);
-                const float k = 2.0f * dot3(N, I);
-                const V3 R = v3(I.x - k * N.x, I.y - k * N.y, I.z - k * N.z);
-                const TraceResult tr = voxel_traversal_df(grid, df, d, o, R, max_iter);
-                smp = tr.samples;
-                nr = 1;
-                if (tr.t > 0.0f) {
-                    const V3 hp = v3(o.x + R.x * tr.t, o.y + R.y * tr.t, o.z + R.z * tr.t);
-                    /* the traversal's normal vector (normal[MinIdx] = -RaySign[MinIdx]).
-                     * It is never zero on a hit: a step along an axis whose
-                     * direction component is 0 moves the other coordinates
-                     * by d * inf, and the ray leaves the volume. */
-                    const V3 hn = face_normal(tr.face, 0.0f);
-                    const V3 so = v3(hp.x + hn.x * 0.055f, hp.y + hn.y * 0.055f, hp.z + hn.z * 0.055f);
-                    const TraceResult sr = voxel_traversal_df(grid, df, d, so, L, shadow_iter);
-                    smp += sr.samples;
-                    nr = 2;
-                    hd = 0.001f + tr.t;
-                    rec = (uint32_t)tr.face | ((uint32_t)tr.block << 8) | ((uint32_t)(sr.t > 0.0f) << 16);
+                uint32_t state = vpt_pcg_hash(vpt_pcg_hash((uint32_t)rx + (uint32_t)Wr * (uint32_t)ry +
+                                                           (uint32_t)Wr * (uint32_t)Hr * frame) + 0x9E3779B9u);
+                float sum = 0.001f;
+                int hits = 0, shadow_itr = 0, shadow = 0;
+                TraceResult tr = {};
+                tr.t = -1.0f;
+                V3 R = N;
+                for (int s = 0; s < spp; s++) {
+                    if (rough || s == 0) {
+                        const V3 Hn = rough ? reflection_normal(N, rr, &state) : N;
+                        const float k = 2.0f * dot3(Hn, I);
+                        R = v3(I.x - k * Hn.x, I.y - k * Hn.y, I.z - k * Hn.z);
+                        tr = voxel_traversal_df(grid, df, d, o, R, max_iter);
+                        smp += tr.samples;
+                        nr++;
+                    }
+                    if (tr.t > 0.0f) {
+                        if (shadow_itr < shadow_rays) {
+                            if (rough || shadow_itr == 0) {
+                                /* the traversal's normal vector (normal[MinIdx] =
+                                 * -RaySign[MinIdx]), never zero on a hit: a step
+                                 * along an axis whose direction component is 0
+                                 * moves the other coordinates by d * inf, and the
+                                 * ray leaves the volume */
+                                const V3 hn = face_normal(tr.face, 0.0f);
+                                const V3 hp = v3(o.x + R.x * tr.t, o.y + R.y * tr.t, o.z + R.z * tr.t);
+                                const V3 so = v3(hp.x + hn.x * 0.055f, hp.y + hn.y * 0.055f, hp.z + hn.z * 0.055f);
+                                const TraceResult sr = voxel_traversal_df(grid, df, d, so, L, shadow_iter);
+                                smp += sr.samples;
+                                nr++;
+                                shadow = sr.t > 0.0f ? 1 : 0;
+                            }
+                            shadow_itr++;
+                        }
+                        if (hits == 0) rec = (uint32_t)tr.face | ((uint32_t)tr.block << 8);
+                        sum += tr.t;
+                        hits++;
+                    }
+                }
+                if (hits > 0) {
+                    hd = sum / fmaxf((float)hits, 0.01f);
+                    rec |= ((uint32_t)shadow << 16) | ((uint32_t)hits << 24);
                 }
                 hd = fclamp(hd, -10.0f, 200.0f);
             }

@@ -37,7 +37,8 @@ def _world(ctx, g):
     return w
 
 
-def _check(w, g, df, W, H, div, light, pos=None, yaw=None, pitch=None, max_iter=64, shadow_iter=150):
+def _check(w, g, df, W, H, div, light, pos=None, yaw=None, pitch=None, max_iter=64, shadow_iter=150, spp=1, rough=0,
+           roughness=0.425, frame=0):
     import torch
     kw = {k: v for k, v in (("pos", pos), ("yaw", yaw), ("pitch", pitch)) if v is not None}
     cam, view, proj, iv, ip = camera(W, H, **kw)
@@ -45,16 +46,17 @@ def _check(w, g, df, W, H, div, light, pos=None, yaw=None, pitch=None, max_iter=
     Wr, Hr = W // div, H // div
     dist = torch.full((Hr, Wr), 0x7E00, dtype=torch.int16, device="cuda")
     hit = torch.full((Hr, Wr), 7, dtype=torch.int32, device="cuda")
-    print(f"reflect {W}x{H} div {div} iter {max_iter}/{shadow_iter}", flush=True)
+    print(f"reflect {W}x{H} div {div} iter {max_iter}/{shadow_iter} spp {spp} rough {rough} {roughness}", flush=True)
     w.trace_primary(cam, gb, 350)
     ctr = torch.zeros(2, dtype=torch.int64, device="cuda")
     w.set_sample_counter(ctr)
-    w.trace_reflect(cam, gb, light, dist, hit, div=div, max_iter=max_iter, shadow_iter=shadow_iter)
+    w.trace_reflect(cam, gb, light, dist, hit, div=div, max_iter=max_iter, shadow_iter=shadow_iter, spp=spp,
+                    rough=rough, roughness=roughness, frame=frame)
     torch.cuda.synchronize()
     w.set_sample_counter(None)
     host = gb.numpy()
     ref = oracle.trace_reflect(g, df, iv, ip, host["t"], host["face"], light, div=div, max_iter=max_iter,
-                               shadow_iter=shadow_iter)
+                               shadow_iter=shadow_iter, spp=spp, rough=rough, roughness=roughness, frame=frame)
     np.testing.assert_array_equal(dist.cpu().numpy().view(np.uint16), ref["dist"], err_msg=f"dist {W}x{H}/{div}")
     np.testing.assert_array_equal(hit.cpu().numpy().view(np.uint32), ref["hit"], err_msg=f"hit {W}x{H}/{div}")
     c = ctr.cpu().numpy()
@@ -76,6 +78,28 @@ def test_reflect_terrain_1080p(ctx, terrain_world):
     sun = np.float32(list(gi_params(50.0, 0).sun_dir))
     _check(w, g, df, 1920, 1080, 4, sun)
     _check(w, g, df, 1920, 1080, 1, sun)
+
+
+def test_reflect_reference_defaults_1080p(ctx, terrain_world):
+    """The reference's settings: 1/4 resolution, 2 rough samples, 64 / 150
+    steps (vpt_reflect_params_default)."""
+    from voxelpathtracer_amd.world import reflect_params
+    g, df, w = terrain_world
+    sun = np.float32(list(gi_params(50.0, 0).sun_dir))
+    kw = reflect_params(sun, frame=7)
+    assert (kw["div"], kw["spp"], kw["rough"], kw["max_iter"], kw["shadow_iter"]) == (4, 2, 1, 64, 150)
+    ref = _check(w, g, df, 1920, 1080, kw["div"], sun, max_iter=kw["max_iter"], shadow_iter=kw["shadow_iter"],
+                 spp=kw["spp"], rough=kw["rough"], roughness=kw["roughness"], frame=kw["frame"])
+    hits = ref["hit"] >> 24
+    assert (hits == 1).any() and (hits == 2).any()      # samples of one pixel hit and miss independently
+
+
+def test_reflect_rough_sweep(ctx, terrain_world):
+    g, df, w = terrain_world
+    sun = np.float32(list(gi_params(50.0, 0).sun_dir))
+    for spp, rough, roughness, div, frame in ((1, 1, 0.3, 1, 0), (2, 0, 0.5, 1, 0), (8, 1, 0.9, 2, 3),
+                                              (16, 1, 0.01, 4, 11), (5, 1, 1.7, 1, 2)):
+        _check(w, g, df, 320, 180, div, sun, spp=spp, rough=rough, roughness=roughness, frame=frame)
 
 
 def test_reflect_iteration_caps(ctx, terrain_world):
@@ -128,7 +152,8 @@ def test_reflect_argument_errors(ctx, terrain_world):
     gb = DeviceGBuffer(64, 32)
     d = torch.zeros((32, 64), dtype=torch.int16, device="cuda")
     sun = (0.0, 1.0, 0.0)
-    for kw in (dict(div=0), dict(div=17), dict(div=33), dict(max_iter=-1), dict(shadow_iter=-1)):
+    for kw in (dict(div=0), dict(div=17), dict(div=33), dict(max_iter=-1), dict(shadow_iter=-1), dict(spp=0),
+               dict(spp=17), dict(rough=2), dict(roughness=float("nan"))):
         with pytest.raises(VptError):
             w.trace_reflect(cam, gb, sun, d, **kw)
     with pytest.raises(VptError):

@@ -254,3 +254,143 @@ def test_hit_normal_is_never_zero():
             axis, sgn = comp[int(f)]
             assert dd[axis] * sgn > 0, (f, dd)
     assert total > 1000
+
+
+def test_mirror_samples_average():
+    """Mirror samples repeat one ray: spp = 2 reports (0.001 + 2 T) / 2 and
+    two hits, with one reflected and one shadow ray traced."""
+    g, df, cam, p = _setup()
+    sun = np.float32([0.0, 1.0, 0.0])
+    r1 = oracle.trace_reflect(g, df, cam["inv_view"], cam["inv_proj"], p["t"], p["face"], sun, div=1)
+    r2 = oracle.trace_reflect(g, df, cam["inv_view"], cam["inv_proj"], p["t"], p["face"], sun, div=1, spp=2)
+    hit = r1["hit"] != 255
+    assert hit.sum() > 500
+    np.testing.assert_array_equal(r2["hit"][hit] & 0xFFFFFF, r1["hit"][hit] & 0xFFFFFF)
+    assert ((r2["hit"][hit] >> 24) == 2).all() and ((r1["hit"][hit] >> 24) == 1).all()
+    np.testing.assert_array_equal(r2["rays"], r1["rays"])
+    d1, d2 = h2f(r1["dist"])[hit], h2f(r2["dist"])[hit]
+    T = d1 - 0.001                                       # binary16-rounded
+    ulp = np.spacing(np.float16(d2)).astype(np.float64)
+    assert (np.abs(d2 - (0.0005 + T)) <= ulp / 2 + np.spacing(np.float16(d1)).astype(np.float64) / 2 + 1e-6).all()
+
+
+def _down_camera(pos, W=64, H=64):
+    """View rays within 0.11 degrees of straight down."""
+    return oracle.camera(pos, 0.0, -89.99, fov=0.2, aspect=W / H)
+
+
+def _ggx_cos2(dist_t, oy, ceiling):
+    """cos^2 of the half vector's angle from N = +y for a reflected ray of
+    length T from height oy to the ceiling, the view ray pointing down:
+    R.y = (ceiling - oy) / T = 2 cos^2 - 1."""
+    ry = (ceiling - oy) / dist_t
+    return (1.0 + ry) / 2.0
+
+
+def test_ggx_polar_distribution():
+    """Rough reflections off a floor under a flat ceiling, view rays within
+    0.11 degrees of straight down: each ray's length gives its half vector's
+    angle, and xi = (1 - c^2) / (1 + c^2 (alpha^2 - 1)) of the chosen one of
+    three samples is the smallest of three uniforms on [0, 0.65): u = xi /
+    0.65 has the CDF 1 - (1 - u)^3 (ImportanceSampleGGX with Xi.y scaled by
+    0.65, GetReflectionDirection keeping the sample nearest N)."""
+    g = np.zeros((256, 48, 256), np.uint8)
+    g[:, :8, :] = 1
+    g[:, 40:, :] = 2
+    df = oracle.df_build(g)
+    W = H = 64
+    cam = _down_camera((128.5, 30.5, 128.5), W, H)
+    p = oracle.trace_primary(g, df, cam["inv_view"], cam["inv_proj"], W, H, 350)
+    assert (p["face"] == 2).all()
+    us = []
+    # roughness 0.6-0.8: 1 - R.y stays well above the binary16 rounding of T
+    # (relative 2^-11), which would smear the small-angle samples
+    for rough_in in (0.6, 0.7, 0.8):
+        for frame in range(3):
+            r = oracle.trace_reflect(g, df, cam["inv_view"], cam["inv_proj"], p["t"], p["face"],
+                                     np.float32([0, 1, 0]), div=1, max_iter=350, spp=1, rough=1,
+                                     roughness=rough_in, frame=frame)
+            on = (r["hit"] & 0xFF) == 3                  # the ceiling's underside (-y face)
+            assert on.mean() > 0.999
+            T = h2f(r["dist"])[on] - 0.001
+            c2 = np.clip(_ggx_cos2(T, 8.0 + 0.035, 40.0), 0.0, 1.0)
+            a2 = (rough_in ** 2) ** 2
+            xi = (1.0 - c2) / (1.0 + c2 * (a2 - 1.0))
+            us.append(np.clip(xi / 0.65, 0.0, 1.0))
+    u = np.sort(np.concatenate(us))
+    cdf = 1.0 - (1.0 - u) ** 3
+    emp = np.arange(1, u.size + 1) / u.size
+    ks = np.max(np.abs(emp - cdf))
+    assert ks < 1.63 / np.sqrt(u.size) + 0.01, ks         # 99% KS bound plus the binary16 rounding of T
+    assert u.max() <= 1.0 + 1e-3
+
+
+def test_ggx_azimuth_sectors():
+    """In a tall square room, rough reflections of straight-down view rays
+    leave along psi = pi - phi with phi uniform on [0, 0.9 * 2 pi): the
+    walls at +x, +z and -z take 90 / 324 of the hits each and the wall at -x
+    54 / 324 (the tangent frame of ImportanceSampleGGX around N = +y is
+    t = (-1, 0, 0), b = (0, 0, 1))."""
+    g = np.zeros((64, 128, 64), np.uint8)
+    g[:, :8, :] = 1
+    g[:, :, :16] = 3
+    g[:, :, 48:] = 3
+    g[:16, :, :] = 3
+    g[48:, :, :] = 3
+    df = oracle.df_build(g)
+    W = H = 64
+    cam = _down_camera((32.0, 30.5, 32.0), W, H)
+    p = oracle.trace_primary(g, df, cam["inv_view"], cam["inv_proj"], W, H, 350)
+    faces = []
+    for frame in range(4):
+        r = oracle.trace_reflect(g, df, cam["inv_view"], cam["inv_proj"], p["t"], p["face"], np.float32([0, 1, 0]),
+                                 div=1, max_iter=350, spp=1, rough=1, roughness=0.6, frame=frame)
+        f = (r["hit"] & 0xFF)[r["hit"] != 255]
+        faces.append(f)
+    f = np.concatenate(faces)
+    assert f.size > 8000
+    frac = {k: float((f == k).mean()) for k in (0, 1, 4, 5)}
+    assert set(np.unique(f)) <= {0, 1, 4, 5}
+    # wall at +x: face 4; -x: face 5; +z: face 1; -z: face 0
+    want = {4: 90 / 324, 1: 90 / 324, 0: 90 / 324, 5: 54 / 324}
+    for k in want:
+        assert abs(frac[k] - want[k]) < 0.03, (frac, want)
+
+
+def test_rough_sample_counts():
+    """spp reflected rays per pixel, and shadow rays for the first
+    max(spp / 4, 1) hits; the hit count field equals the hits."""
+    g = np.zeros((128, 48, 128), np.uint8)
+    g[:, :8, :] = 1
+    g[:, 40:, :] = 2
+    df = oracle.df_build(g)
+    cam = _down_camera((64.5, 30.5, 64.5))
+    p = oracle.trace_primary(g, df, cam["inv_view"], cam["inv_proj"], 64, 64, 350)
+    for spp in (1, 2, 4, 8, 16):
+        r = oracle.trace_reflect(g, df, cam["inv_view"], cam["inv_proj"], p["t"], p["face"], np.float32([0, 1, 0]),
+                                 div=1, max_iter=350, spp=spp, rough=1, roughness=0.5, frame=3)
+        assert ((r["hit"] >> 24) == spp).all()
+        assert (r["rays"] == spp + max(spp // 4, 1)).all()
+        assert (((r["hit"] >> 16) & 1) == 1).all()        # the sun straight up: the ceiling shadows every hit
+
+
+def test_roughness_floor():
+    """GetReflectionDirection samples at max(roughness, 0.05) after the
+    caller's clamp to [0.01, 1]: roughness 0.01 and 0.04 reproduce 0.05 bit
+    for bit, while 0.2 and 0.25 differ."""
+    g = np.zeros((128, 48, 128), np.uint8)
+    g[:, :8, :] = 1
+    g[:, 40:, :] = 2
+    df = oracle.df_build(g)
+    cam = _down_camera((64.5, 30.5, 64.5))
+    p = oracle.trace_primary(g, df, cam["inv_view"], cam["inv_proj"], 64, 64, 350)
+
+    def run(rough_in):
+        return oracle.trace_reflect(g, df, cam["inv_view"], cam["inv_proj"], p["t"], p["face"], np.float32([0, 1, 0]),
+                                    div=1, max_iter=350, spp=2, rough=1, roughness=rough_in, frame=5)["dist"]
+
+    base = run(0.05)
+    np.testing.assert_array_equal(run(0.01), base)
+    np.testing.assert_array_equal(run(0.04), base)
+    np.testing.assert_array_equal(run(-3.0), base)
+    assert (run(0.2) != run(0.25)).mean() > 0.2

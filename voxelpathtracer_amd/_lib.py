@@ -56,6 +56,11 @@ class SvgfParams(C.Structure):
                 ("steps", C.c_int32 * 5)]
 
 
+class ReflectParams(C.Structure):
+    _fields_ = [("light_dir", C.c_float * 3), ("div", C.c_int32), ("max_iter", C.c_int32), ("shadow_iter", C.c_int32),
+                ("spp", C.c_int32), ("rough", C.c_int32), ("roughness", C.c_float), ("frame", C.c_uint32)]
+
+
 class McBlock(C.Structure):
     _fields_ = [("name", C.c_char_p), ("id", C.c_uint8), ("data", C.c_uint8)]
 
@@ -131,7 +136,8 @@ _PROTOS = {
     "vpt_trace_primary": (_i, [_vp, C.POINTER(Camera), _i, C.POINTER(GBuffer), _vp]),
     "vpt_trace_shadow": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _vp, _vp]),
     "vpt_trace_shadow2": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _vp, _vp, _vp]),
-    "vpt_trace_reflect": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _i, _i, _vp, _vp, _vp]),
+    "vpt_trace_reflect": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(ReflectParams), _vp, _vp, _vp]),
+    "vpt_reflect_params_default": (_i, [C.POINTER(_f), C.c_uint32, C.POINTER(ReflectParams)]),
     "vpt_trace_gi": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(GiParams), _vp, _vp, _vp]),
     "vpt_trace_frame": (_i, [_vp, C.POINTER(Camera), C.POINTER(GiParams), _i, _i, C.POINTER(GBuffer), _vp, _vp, _vp,
                              _vp]),

@@ -155,6 +155,21 @@ extern "C" VPT_API int vpt_camera_make(const float pos[3], float yaw_deg, float 
 // (0.35, 0.55, 0.90).
 // ---------------------------------------------------------------------------
 
+extern "C" VPT_API int vpt_reflect_params_default(const float light_dir[3], uint32_t frame, vpt_reflect_params* out)
+{
+    if (!light_dir || !out) return set_error(VPT_EINVAL, "vpt_reflect_params_default: null argument");
+    memset(out, 0, sizeof(*out));
+    for (int i = 0; i < 3; i++) out->light_dir[i] = light_dir[i];
+    out->div = 4;              // ReflectionTraceResolution 0.25 (Pipeline.cpp:101)
+    out->max_iter = 64;        // ReflectionTraceLength (Pipeline.cpp:100)
+    out->shadow_iter = 150;    // VoxelTraversalDF's shadow cap (ReflectionTraceFrag.glsl:1098)
+    out->spp = 2;              // ReflectionSPP (Pipeline.cpp:103)
+    out->rough = 1;            // RoughReflections (Pipeline.cpp:104)
+    out->roughness = 0.5f * 0.85f;   // a flat material's 0.5 times RoughnessBias (ReflectionTraceFrag.glsl:814-815)
+    out->frame = frame;
+    return VPT_OK;
+}
+
 extern "C" VPT_API int vpt_gi_params_default(float sun_tick, uint32_t frame, uint32_t albedo_seed, vpt_gi_params* prm)
 {
     if (!prm) return set_error(VPT_EINVAL, "vpt_gi_params_default: null output");

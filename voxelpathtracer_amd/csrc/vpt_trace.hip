@@ -498,17 +498,84 @@ __device__ inline int gl_nearest(const uint8_t* __restrict__ p, int W, int H, fl
     return p[(size_t)y * W + x];
 }
 
-// One lane per target pixel: the reflected ray (max_iter steps) and, on a
-// hit, its sun shadow ray (shadow_iter steps) in the same lane.  Mirror
-// reflections of neighbouring pixels stay coherent, so the lanes of a
-// wavefront walk similar paths.
-//   dist_out: binary16 of the hit distance (sky / miss -1, else 0.001 + T,
-//             clamped to [-10, 200]);
-//   hit_out:  face | block << 8 | shadowed << 16, 255 without a hit.
-__global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, int gW, int gH, int div,
+struct ReflectConst {
+    float lx, ly, lz;
+    int div, max_iter, shadow_iter, spp, rough;
+    float roughness;   // clamp(roughness, 0.01, 1)
+    uint32_t frame;
+};
+
+// ImportanceSampleGGX (ReflectionTraceFrag.glsl:345-366): the GGX half
+// vector around N for roughness r and randoms (x0, x1).
+__device__ inline Vec3 ggx_half(Vec3 N, float r, float x0, float x1)
+{
+    const float alpha = r * r;
+    const float alpha2 = alpha * alpha;
+    float sp, cp;
+    vpt_sincos_2pi(x0, &sp, &cp);
+    const float cos_t = sqrtf((1.0f - x1) / (1.0f + (alpha2 - 1.0f) * x1));
+    const float sin_t = sqrtf(1.0f - cos_t * cos_t);
+    const float hx = cp * sin_t, hy = sp * sin_t, hz = cos_t;
+    const Vec3 up = fabsf(N.z) < 0.999f ? vpt::mk3(0.0f, 0.0f, 1.0f) : vpt::mk3(1.0f, 0.0f, 0.0f);
+    const Vec3 t = vpt::normalize3(vpt::cross3(up, N));
+    const Vec3 b = vpt::cross3(N, t);
+    return vpt::normalize3(vpt::mk3((t.x * hx + b.x * hy) + N.x * hz, (t.y * hx + b.y * hy) + N.y * hz,
+                                    (t.z * hx + b.z * hy) + N.z * hz));
+}
+
+// GetReflectionDirection (:617-640): of three GGX half vectors, the first
+// with the largest dot with N, randoms from the pcg_hash chain.
+__device__ inline Vec3 reflection_normal(Vec3 N, float r, uint32_t& state)
+{
+    r = fmaxf(r, 0.05f);
+    float nearest = -100.0f;
+    Vec3 best = N;
+#pragma unroll 1
+    for (int i = 0; i < 3; i++) {
+        state = vpt_pcg_hash(state);
+        const float x0 = vpt_u32_to_unit(state) * 0.9f;
+        state = vpt_pcg_hash(state);
+        const float x1 = vpt_u32_to_unit(state) * 0.65f;
+        const Vec3 h = ggx_half(N, r, x0, x1);
+        const float d = vpt::dot3(h, N);
+        if (d > nearest) {
+            best = h;
+            nearest = d;
+        }
+    }
+    return best;
+}
+
+// GetShadowAt(HitPosition + Normal * 0.055) toward L: 1 when it hits.  A
+// hit's normal is never zero (a step along an axis with a zero direction
+// component moves the other coordinates by d * inf, and the ray leaves the
+// volume), so the face normal is the traversal's normal vector.
+__device__ inline int reflect_shadow(const WorldView& w, Vec3 o, Vec3 R, const vpt::Hit& h, Vec3 L, int shadow_iter,
+                                     int& samples, int& rays)
+{
+    const Vec3 hn = vpt::face_normal(h.face, 0.0f);
+    const Vec3 hp = vpt::mk3(o.x + R.x * h.t, o.y + R.y * h.t, o.z + R.z * h.t);
+    const Vec3 so = vpt::mk3(hp.x + hn.x * 0.055f, hp.y + hn.y * 0.055f, hp.z + hn.z * 0.055f);
+    const vpt::Hit sh = vpt::traverse(w, so, L, shadow_iter);
+    samples += sh.samples;
+    rays++;
+    return sh.t > 0.0f ? 1 : 0;
+}
+
+// One lane per target pixel: spp reflected rays (max_iter steps each) and,
+// for the first max(spp / 4, 1) that hit, the sun shadow ray (shadow_iter
+// steps), all in the lane.  kRough: GGX-sampled rays; otherwise mirror
+// rays, of which the one ray is traced once and its result summed spp
+// times in the same order.  Mirror reflections of neighbouring pixels stay
+// coherent, so the lanes of a wavefront walk similar paths.
+//   dist_out: binary16 of the hit distance (sky / no hit -1, else
+//             (0.001 + sum of T) / hits, clamped to [-10, 200]);
+//   hit_out:  face | block << 8 of the first hit | shadow << 16 |
+//             hits << 24, 255 without a hit.
+template <bool kRough>
+__global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, int gW, int gH,
                                                       const float* __restrict__ gb_t,
-                                                      const uint8_t* __restrict__ gb_face, float lx, float ly,
-                                                      float lz, int max_iter, int shadow_iter,
+                                                      const uint8_t* __restrict__ gb_face, ReflectConst rc,
                                                       uint16_t* __restrict__ dist_out, uint32_t* __restrict__ hit_out,
                                                       unsigned long long* __restrict__ counter)
 {
@@ -519,7 +586,7 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
         const size_t i = (size_t)py * cam.W + px;
         float dist;
         int face;
-        if (div == 1) {
+        if (rc.div == 1) {
             dist = gb_t[i];
             face = gb_face[i];
         } else {
@@ -534,25 +601,48 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
             const Vec3 N = vpt::face_normal(face, 1.0f);   // GetNormalFromID: ids above 5 -> (1, 1, 1)
             const Vec3 I = vpt::normalize3(vpt::mk3(P.x - cam.iv[12], P.y - cam.iv[13], P.z - cam.iv[14]));
             const Vec3 o = vpt::mk3(P.x + N.x * 0.035f, P.y + N.y * 0.035f, P.z + N.z * 0.035f);
-            const float k = 2.0f * vpt::dot3(N, I);
-            const Vec3 R = vpt::mk3(I.x - k * N.x, I.y - k * N.y, I.z - k * N.z);   // reflect(I, N)
-            const vpt::Hit h = vpt::traverse(w, o, R, max_iter);
-            samples = h.samples;
-            rays = 1;
-            if (h.t > 0.0f) {
-                // GetShadowAt(HitPosition + Normal * 0.055); a hit's normal is
-                // never zero (a step along an axis with a zero direction
-                // component moves the other coordinates by d * inf, and the
-                // ray leaves the volume)
-                const Vec3 hn = vpt::face_normal(h.face, 0.0f);
-                const Vec3 hp = vpt::mk3(o.x + R.x * h.t, o.y + R.y * h.t, o.z + R.z * h.t);
-                const Vec3 so = vpt::mk3(hp.x + hn.x * 0.055f, hp.y + hn.y * 0.055f, hp.z + hn.z * 0.055f);
-                const vpt::Hit sh = vpt::traverse(w, so, vpt::mk3(lx, ly, lz), shadow_iter);
-                samples += sh.samples;
-                rays = 2;
-                hd = 0.001f + h.t;
-                rec = (uint32_t)(h.face & 0xff) | ((uint32_t)(h.block & 0xff) << 8) |
-                      ((uint32_t)(sh.t > 0.0f) << 16);
+            const Vec3 L = vpt::mk3(rc.lx, rc.ly, rc.lz);
+            const int shadow_rays = max(rc.spp / 4, 1);
+            float sum = 0.001f;
+            int hits = 0, shadow = 0;
+            if (kRough) {
+                uint32_t state = vpt_pcg_hash(vpt_pcg_hash((uint32_t)px + (uint32_t)cam.W * (uint32_t)py +
+                                                           (uint32_t)cam.W * (uint32_t)cam.H * rc.frame) +
+                                              0x9E3779B9u);
+                int shadow_itr = 0;
+                for (int s = 0; s < rc.spp; s++) {
+                    const Vec3 Hn = reflection_normal(N, rc.roughness, state);
+                    const float k = 2.0f * vpt::dot3(Hn, I);
+                    const Vec3 R = vpt::mk3(I.x - k * Hn.x, I.y - k * Hn.y, I.z - k * Hn.z);   // reflect(I, H)
+                    const vpt::Hit h = vpt::traverse(w, o, R, rc.max_iter);
+                    samples += h.samples;
+                    rays++;
+                    if (h.t > 0.0f) {
+                        if (shadow_itr < shadow_rays) {
+                            shadow = reflect_shadow(w, o, R, h, L, rc.shadow_iter, samples, rays);
+                            shadow_itr++;
+                        }
+                        if (hits == 0) rec = (uint32_t)(h.face & 0xff) | ((uint32_t)(h.block & 0xff) << 8);
+                        sum = sum + h.t;
+                        hits++;
+                    }
+                }
+            } else {
+                const float k = 2.0f * vpt::dot3(N, I);
+                const Vec3 R = vpt::mk3(I.x - k * N.x, I.y - k * N.y, I.z - k * N.z);   // reflect(I, N)
+                const vpt::Hit h = vpt::traverse(w, o, R, rc.max_iter);
+                samples += h.samples;
+                rays++;
+                if (h.t > 0.0f) {
+                    shadow = reflect_shadow(w, o, R, h, L, rc.shadow_iter, samples, rays);
+                    rec = (uint32_t)(h.face & 0xff) | ((uint32_t)(h.block & 0xff) << 8);
+                    for (int s = 0; s < rc.spp; s++) sum = sum + h.t;
+                    hits = rc.spp;
+                }
+            }
+            if (hits > 0) {
+                hd = sum / fmaxf((float)hits, 0.01f);
+                rec |= ((uint32_t)shadow << 16) | ((uint32_t)hits << 24);
             }
             hd = vpt::fclamp(hd, -10.0f, 200.0f);
         }
@@ -613,23 +703,35 @@ extern "C" VPT_API int vpt_trace_shadow2(vpt_world* w, const vpt_camera* cam, co
 }
 
 extern "C" VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
-                                         const float light_dir[3], int div, int max_iter, int shadow_iter,
-                                         uint16_t* d_dist, uint32_t* d_hit, void* stream)
+                                         const vpt_reflect_params* prm, uint16_t* d_dist, uint32_t* d_hit,
+                                         void* stream)
 {
-    if (!w || !light_dir || !d_dist) return set_error(VPT_EINVAL, "vpt_trace_reflect: null argument");
+    if (!w || !prm || !d_dist) return set_error(VPT_EINVAL, "vpt_trace_reflect: null argument");
     if (int rc = vpt::check_cam(cam, "vpt_trace_reflect")) return rc;
     if (int rc = vpt::check_gbuffer(gb, cam, "vpt_trace_reflect", false)) return rc;
-    if (int rc = vpt::check_iter(max_iter, "vpt_trace_reflect")) return rc;
-    if (int rc = vpt::check_iter(shadow_iter, "vpt_trace_reflect")) return rc;
+    if (int rc = vpt::check_iter(prm->max_iter, "vpt_trace_reflect")) return rc;
+    if (int rc = vpt::check_iter(prm->shadow_iter, "vpt_trace_reflect")) return rc;
+    const int div = prm->div;
     if (div < 1 || div > 16 || cam->width / div < 1 || cam->height / div < 1)
         return set_error(VPT_EINVAL, "vpt_trace_reflect: div must be 1..16 and leave at least one pixel");
+    if (prm->spp < 1 || prm->spp > 16) return set_error(VPT_EINVAL, "vpt_trace_reflect: spp %d not in 1..16", prm->spp);
+    if (prm->rough != 0 && prm->rough != 1) return set_error(VPT_EINVAL, "vpt_trace_reflect: rough must be 0 or 1");
+    if (!(prm->roughness == prm->roughness)) return set_error(VPT_EINVAL, "vpt_trace_reflect: roughness is NaN");
     DeviceGuard g(w->device);
-    CamView rc = vpt::make_cam(cam);
-    rc.W = cam->width / div;
-    rc.H = cam->height / div;
-    hipLaunchKernelGGL(reflect_kernel, tile_grid(rc.W, rc.H), dim3(256), 0, (hipStream_t)stream, vpt::make_view(w), rc,
-                       cam->width, cam->height, div, gb->d_t, gb->d_face, light_dir[0], light_dir[1], light_dir[2],
-                       max_iter, shadow_iter, d_dist, d_hit, w->d_counter);
+    CamView cv = vpt::make_cam(cam);
+    cv.W = cam->width / div;
+    cv.H = cam->height / div;
+    const ReflectConst rc = {prm->light_dir[0], prm->light_dir[1], prm->light_dir[2], div, prm->max_iter,
+                             prm->shadow_iter, prm->spp, prm->rough, fminf(fmaxf(prm->roughness, 0.01f), 1.0f),
+                             prm->frame};
+    if (rc.rough)
+        hipLaunchKernelGGL(reflect_kernel<true>, tile_grid(cv.W, cv.H), dim3(256), 0, (hipStream_t)stream,
+                           vpt::make_view(w), cv, cam->width, cam->height, gb->d_t, gb->d_face, rc, d_dist, d_hit,
+                           w->d_counter);
+    else
+        hipLaunchKernelGGL(reflect_kernel<false>, tile_grid(cv.W, cv.H), dim3(256), 0, (hipStream_t)stream,
+                           vpt::make_view(w), cv, cam->width, cam->height, gb->d_t, gb->d_face, rc, d_dist, d_hit,
+                           w->d_counter);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }

@@ -158,14 +158,18 @@ class World:
              _ptr(transversal), _stream(stream))
 
     def trace_reflect(self, cam: Camera, gbuf: "DeviceGBuffer", light_dir, dist, hit=None, div: int = 4,
-                      max_iter: int = 64, shadow_iter: int = 150, stream=None) -> None:
-        """Mirror reflections of the G-buffer (ReflectionTraceFrag.glsl:764-1035,
-        traversal core) at (W // div) x (H // div): `dist` receives the
-        binary16 bits of o_HitDistance (int16/uint16 tensor), `hit` (optional,
-        int32/uint32) face | block << 8 | shadowed << 16, 255 without a hit."""
-        ld = (C.c_float * 3)(*map(float, light_dir))
-        call("vpt_trace_reflect", self.handle, C.byref(cam), C.byref(gbuf.struct()), ld, div, max_iter, shadow_iter,
-             _ptr(dist), _ptr(hit), _stream(stream))
+                      max_iter: int = 64, shadow_iter: int = 150, spp: int = 1, rough: int = 0,
+                      roughness: float = 0.425, frame: int = 0, stream=None) -> None:
+        """Reflections off the G-buffer (ReflectionTraceFrag.glsl:764-1035,
+        traversal core) at (W // div) x (H // div): mirror (rough 0) or GGX
+        sampled (rough 1), spp samples.  `dist` receives the binary16 bits of
+        o_HitDistance (int16/uint16 tensor), `hit` (optional, int32/uint32)
+        face | block << 8 | shadow << 16 | hits << 24, 255 without a hit.
+        reflect_params() gives the reference's defaults."""
+        prm = _lib.ReflectParams((C.c_float * 3)(*map(float, light_dir)), div, max_iter, shadow_iter, spp, rough,
+                                 float(roughness), int(frame))
+        call("vpt_trace_reflect", self.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), _ptr(dist),
+             _ptr(hit), _stream(stream))
 
     def trace_frame(self, cam: Camera, gbuf: "DeviceGBuffer", prm, mask, rgb, texels=None, primary_iter: int = 350,
                     shadow_iter: int = 350, stream=None) -> None:
@@ -298,3 +302,12 @@ def mc_import(directory: str, dims, origin=(0.0, 0.0, 0.0), mc_to_block=None, na
     call("vpt_mc_import", str(directory).encode(), nx, ny, nz, org, lut.ctypes.data_as(C.c_void_p),
          C.cast(table, C.c_void_p), len(items), grid.ctypes.data_as(C.c_void_p), stats)
     return grid, list(stats)
+
+
+def reflect_params(light_dir, frame: int = 0) -> dict:
+    """The reference's reflection settings (vpt_reflect_params_default) as
+    World.trace_reflect keyword arguments."""
+    p = _lib.ReflectParams()
+    call("vpt_reflect_params_default", (C.c_float * 3)(*map(float, light_dir)), int(frame), C.byref(p))
+    return dict(div=p.div, max_iter=p.max_iter, shadow_iter=p.shadow_iter, spp=p.spp, rough=p.rough,
+                roughness=p.roughness, frame=p.frame)
