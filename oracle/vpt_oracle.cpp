@@ -52,9 +52,12 @@
  *   vpt_logf / vpt_powf    error bounds against double libm        -> pinned
  *   oracle_trace_reflect   corridor closed forms (first hit, face,
  *                          block, distance incl. the 0.001 offset,
- *                          shadow at the hit), an independent GL
- *                          LINEAR / NEAREST resampler, the 200
- *                          clamp                                   -> pinned
+ *                          shadow at the hit, the two-sample
+ *                          average), an independent GL LINEAR /
+ *                          NEAREST resampler, the 200 clamp, the
+ *                          GGX half-vector polar law of the best of
+ *                          three (KS) and its azimuth sectors, the
+ *                          0.05 roughness floor                    -> pinned
  */
 #include <stdint.h>
 #include <string.h>
