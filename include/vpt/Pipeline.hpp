@@ -1,10 +1,11 @@
 // C++ façade for the hot-path slice of the reference's frame loop
-// (MainPipeline::StartPipeline, Core/Pipeline.cpp:1960-2852): primary trace
+// (MainPipeline::StartPipeline, Core/Pipeline.cpp:1960-3060): primary trace
 // into a ping-pong G-buffer (InitialTraceFBO / InitialTraceFBOPrev,
-// :1970-1971), diffuse GI trace (:2171-2281), SVGF (:2284-2615) and the sun
-// shadow trace (:2790-2852).  Device buffers are plain hipMalloc
-// allocations owned by the pipeline.  Header-only; links against libvpt.so
-// and the HIP runtime.
+// :1970-1971), diffuse GI trace (:2171-2281), SVGF (:2284-2615), the sun
+// shadow trace with its transversal (:2790-2852), the sun-shadow denoise
+// (:2854-2948) and the reflection trace's core at the reference's settings
+// (:3000-3060).  Device buffers are plain hipMalloc allocations owned by the
+// pipeline.  Header-only; links against libvpt.so and the HIP runtime.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -31,18 +32,26 @@ public:
             g.height = height;
         }
         Alloc(&m_Mask, px);
+        Alloc(&m_Transversal, px * sizeof(uint16_t));
+        Alloc(&m_ShadowOut, px);
+        m_RW = width / 4;    // ReflectionTraceResolution 0.25
+        m_RH = height / 4;
+        Alloc(&m_ReflDist, (size_t)m_RW * m_RH * sizeof(uint16_t) + 16);
+        Alloc(&m_ReflHit, (size_t)m_RW * m_RH * sizeof(uint32_t) + 16);
         Alloc(&m_RGB, px * 3 * sizeof(float));
         Alloc(&m_Denoised, px * 3 * sizeof(float));
         Alloc(&m_Texels, px * sizeof(vpt_svgf_texel));
         vpt_svgf_params sp;
         vpt_check(vpt_svgf_params_default(&sp));
         vpt_check(vpt_svgf_create(ctx.Handle(), width, height, &sp, &m_Svgf));
+        vpt_check(vpt_shadow_denoise_create(ctx.Handle(), width, height, &m_ShadowDn));
         if (persistent) vpt_check(vpt_frame_create(ctx.Handle(), width, height, &m_Frame));
     }
 
     ~Pipeline()
     {
         if (m_Frame) vpt_frame_destroy(m_Frame);
+        vpt_shadow_denoise_destroy(m_ShadowDn);
         vpt_svgf_destroy(m_Svgf);
         for (auto& g : m_GBuf) {
             (void)hipFree(g.d_t);
@@ -51,6 +60,10 @@ public:
             (void)hipFree(g.d_voxel);
         }
         (void)hipFree(m_Mask);
+        (void)hipFree(m_Transversal);
+        (void)hipFree(m_ShadowOut);
+        (void)hipFree(m_ReflDist);
+        (void)hipFree(m_ReflHit);
         (void)hipFree(m_RGB);
         (void)hipFree(m_Denoised);
         (void)hipFree(m_Texels);
@@ -78,13 +91,23 @@ public:
         if (m_Persistent) {
             vpt_check(vpt_frame_trace(m_Frame, m_World.Handle(), &cam, &prm, 350, 350, &cur, m_Mask, m_RGB, m_Texels,
                                       1, stream));
+            // the shadow pass again for its transversal (the mask is the same)
+            vpt_check(vpt_trace_shadow2(m_World.Handle(), &cam, &cur, prm.sun_dir, 350, m_Mask, m_Transversal, stream));
         } else {
-            vpt_check(vpt_trace_frame(m_World.Handle(), &cam, &prm, 350, 350, &cur, m_Mask, m_RGB, m_Texels, stream));
+            vpt_check(vpt_trace_frame(m_World.Handle(), &cam, &prm, 350, 350, &cur, m_Mask, m_Transversal, m_RGB,
+                                      m_Texels, stream));
         }
         float prev_vp[16];
         ViewProj(m_PrevView, m_PrevProj, prev_vp);
         vpt_check(vpt_svgf_frame(m_Svgf, &cam, prev_vp, &cur, &prev, m_Texels, (float)m_Frame_ / 60.0f, m_Denoised,
                                  nullptr, stream));
+        vpt_check(vpt_shadow_denoise_frame(m_ShadowDn, &cam, prev_vp, &cur, &prev, m_Mask, m_Transversal, 1.0f,
+                                           m_ShadowOut, stream));
+        if (m_RW > 0 && m_RH > 0) {
+            vpt_reflect_params rp;
+            vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
+            vpt_check(vpt_trace_reflect(m_World.Handle(), &cam, &cur, &rp, m_ReflDist, m_ReflHit, stream));
+        }
         for (int i = 0; i < 16; i++) {
             m_PrevView[i] = view[i];
             m_PrevProj[i] = proj[i];
@@ -96,6 +119,14 @@ public:
     const float* DenoisedRGB() const { return m_Denoised; }
     const float* RawRGB() const { return m_RGB; }
     const uint8_t* ShadowMask() const { return m_Mask; }
+    // the denoised sun shadow (unorm8 per pixel)
+    const uint8_t* DenoisedShadow() const { return m_ShadowOut; }
+    // reflections at (width / 4) x (height / 4): o_HitDistance (binary16) and
+    // the hit record of vpt_trace_reflect
+    const uint16_t* ReflectionHitDistance() const { return m_ReflDist; }
+    const uint32_t* ReflectionHits() const { return m_ReflHit; }
+    int ReflectionWidth() const { return m_RW; }
+    int ReflectionHeight() const { return m_RH; }
     int FrameIndex() const { return m_Frame_; }
 
 private:
@@ -123,6 +154,12 @@ private:
     int m_Frame_ = 0;
     vpt_gbuffer m_GBuf[2] = {};
     uint8_t* m_Mask = nullptr;
+    uint16_t* m_Transversal = nullptr;
+    uint8_t* m_ShadowOut = nullptr;
+    uint16_t* m_ReflDist = nullptr;
+    uint32_t* m_ReflHit = nullptr;
+    int m_RW = 0, m_RH = 0;
+    vpt_shadow_denoise* m_ShadowDn = nullptr;
     float* m_RGB = nullptr;
     float* m_Denoised = nullptr;
     vpt_svgf_texel* m_Texels = nullptr;

@@ -284,13 +284,14 @@ VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer*
  * SHADOW_TRACE, DIFFUSE_TRACE, Pipeline.cpp:1963-2852): the primary pass on
  * `stream` fills gb, then the shadow pass runs on a stream of the context
  * beside the GI pass on `stream`, and `stream` waits for both.  Outputs
- * equal vpt_trace_primary + vpt_trace_shadow + vpt_trace_gi; the overlap
- * fills the tail of each pass with the other's work.  Calls sharing a
- * context must not overlap (one side stream and the GI scratch per
- * context). */
+ * equal vpt_trace_primary + vpt_trace_shadow + vpt_trace_gi, or, with
+ * d_transversal (optional), vpt_trace_shadow2's two outputs in place of
+ * vpt_trace_shadow's; the overlap fills the tail of each pass with the
+ * other's work.  Calls sharing a context must not overlap (one side stream
+ * and the GI scratch per context). */
 VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, const vpt_gi_params* prm, int primary_iter,
-                            int shadow_iter, const vpt_gbuffer* gb, uint8_t* d_mask, float* d_rgb,
-                            vpt_svgf_texel* d_svgf_in, void* stream);
+                            int shadow_iter, const vpt_gbuffer* gb, uint8_t* d_mask, uint16_t* d_transversal,
+                            float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream);
 
 /* Ray queries (serving API): n rays, traversal cap max_iter. */
 VPT_API int vpt_trace_rays(vpt_world* w, const vpt_ray* d_rays, int n, int max_iter, vpt_hit* d_hits, void* stream);

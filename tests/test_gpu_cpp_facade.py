@@ -1,7 +1,7 @@
 """The C++ host façades (include/vpt/World.hpp, Pipeline.hpp) drive the same
-passes as the Python host: the C++ example's denoised frame equals the
-Python Pipeline's bit for bit (persistent + graph path in C++, one launch
-per pass in Python)."""
+passes as the Python host: the C++ example's denoised frame, denoised sun
+shadow and reflection hit distances equal the Python Pipeline's bit for bit
+(persistent + graph path in C++, one launch per pass in Python)."""
 from __future__ import annotations
 
 import json
@@ -27,7 +27,11 @@ def test_cpp_facade_matches_python_pipeline(ctx, tmp_path):
     assert r.returncode == 0, r.stderr
     info = json.loads(r.stdout.strip().splitlines()[-1])
     assert info["path"] == "persistent" and info["frames"] == frames
-    cpp = np.fromfile(dump, np.float32).reshape(H, W, 3)
+    raw = np.fromfile(dump, np.uint8)
+    n_rgb, n_sh = H * W * 3 * 4, H * W
+    cpp = raw[:n_rgb].view(np.float32).reshape(H, W, 3)
+    cpp_shadow = raw[n_rgb:n_rgb + n_sh].reshape(H, W)
+    cpp_refl = raw[n_rgb + n_sh:].view(np.uint16).reshape(H // 4, W // 4)
 
     world = World(ctx, 384, 128, 384)
     world.m_WorldData[...] = generate_terrain(384, 128, 384)
@@ -38,5 +42,8 @@ def test_cpp_facade_matches_python_pipeline(ctx, tmp_path):
         pipe.render((192.5 + 0.1 * f, 100.5, 192.5), 45.0 + 0.2 * f, -20.0)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(cpp, pipe.denoised.cpu().numpy())
+    np.testing.assert_array_equal(cpp_shadow, pipe.shadow_denoised.cpu().numpy())
+    np.testing.assert_array_equal(cpp_refl, pipe.reflect_dist.cpu().numpy().view(np.uint16))
+    assert (cpp_refl != 0xBC00).any()                    # some reflections hit (-1 is 0xBC00)
     # RaycastDetect through the façade agrees with the Python host
     assert tuple(info["pick"]) == world.RaycastDetect((192.5, 100.5, 192.5), (0.664463, -0.342020, 0.664463))

@@ -218,6 +218,14 @@ def test_trace_frame_parity(terrain_world):
         torch.cuda.synchronize()
         np.testing.assert_array_equal(c, counter.cpu().numpy())
         assert c[1] > W * H
+    # with the transversal: the shadow pass is vpt_trace_shadow2's
+    tv = torch.empty((H, W), dtype=torch.int16, device="cuda")
+    w.trace_frame(cam, gb, prm, mask, rgb, tex, transversal=tv)
+    torch.cuda.synchronize()
+    host = gb.numpy()
+    sh2 = oracle.trace_shadow2(g, df, iv, ip, host["t"], host["face"], np.array(prm.sun_dir, np.float32), 350)
+    np.testing.assert_array_equal(mask.cpu().numpy(), sh2["mask"])
+    np.testing.assert_array_equal(tv.cpu().numpy().view(np.uint16), sh2["transversal"])
     from voxelpathtracer_amd._lib import VptError
     with pytest.raises(VptError):
         w.trace_frame(cam, gb, prm, None, rgb, tex)

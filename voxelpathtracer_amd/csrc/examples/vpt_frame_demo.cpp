@@ -1,8 +1,10 @@
 // A C++ host driving the hot path through the VoxelRT::World /
 // VoxelRT::Pipeline façades (include/vpt/World.hpp, Pipeline.hpp): terrain
-// world, GPU distance field, N frames of primary + GI + SVGF + shadow along
-// the SVGF workload's camera path.  Prints one JSON line; with --dump FILE
-// writes the last frame's denoised RGB (float32, H x W x 3).
+// world, GPU distance field, N frames of primary + shadow + GI + SVGF +
+// shadow denoise + reflections along the SVGF workload's camera path.
+// Prints one JSON line; with --dump FILE writes the last frame's denoised
+// RGB (float32, H x W x 3), then its denoised shadow (uint8, H x W) and its
+// reflection hit distances (binary16 bits, H/4 x W/4).
 //
 //   vpt_frame_demo [W H FRAMES [--plain] [--dump FILE]]
 #include <hip/hip_runtime_api.h>
@@ -58,9 +60,16 @@ int main(int argc, char** argv)
         const float d0[3] = {0.664463f, -0.342020f, 0.664463f};
         auto pick = world.RaycastDetect(p0, d0);
         if (dump) {
+            std::vector<uint8_t> shadow((size_t)W * H);
+            std::vector<uint16_t> refl((size_t)pipe.ReflectionWidth() * pipe.ReflectionHeight());
+            (void)hipMemcpy(shadow.data(), pipe.DenoisedShadow(), shadow.size(), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(refl.data(), pipe.ReflectionHitDistance(), refl.size() * sizeof(uint16_t),
+                            hipMemcpyDeviceToHost);
             FILE* fp = fopen(dump, "wb");
             if (!fp) throw std::runtime_error("cannot open dump file");
             fwrite(rgb.data(), sizeof(float), rgb.size(), fp);
+            fwrite(shadow.data(), 1, shadow.size(), fp);
+            fwrite(refl.data(), sizeof(uint16_t), refl.size(), fp);
             fclose(fp);
         }
         printf("{\"frames\": %d, \"width\": %d, \"height\": %d, \"path\": \"%s\", \"ms_per_frame\": %.4f, "

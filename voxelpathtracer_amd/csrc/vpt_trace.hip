@@ -738,7 +738,8 @@ extern "C" VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, co
 
 extern "C" VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, const vpt_gi_params* prm,
                                        int primary_iter, int shadow_iter, const vpt_gbuffer* gb, uint8_t* d_mask,
-                                       float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream)
+                                       uint16_t* d_transversal, float* d_rgb, vpt_svgf_texel* d_svgf_in,
+                                       void* stream)
 {
     if (!w || !prm || !d_mask || !d_rgb) return set_error(VPT_EINVAL, "vpt_trace_frame: null argument");
     if (int rc = vpt::check_cam(cam, "vpt_trace_frame")) return rc;
@@ -770,7 +771,9 @@ extern "C" VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, cons
     // it); join before anything later on `stream`
     HIP_TRY(hipEventRecord((hipEvent_t)ctx->ev_fork, s));
     HIP_TRY(hipStreamWaitEvent(side, (hipEvent_t)ctx->ev_fork, 0));
-    const int rc_shadow = vpt_trace_shadow(w, cam, gb, prm->sun_dir, shadow_iter, d_mask, side);
+    const int rc_shadow = d_transversal
+                              ? vpt_trace_shadow2(w, cam, gb, prm->sun_dir, shadow_iter, d_mask, d_transversal, side)
+                              : vpt_trace_shadow(w, cam, gb, prm->sun_dir, shadow_iter, d_mask, side);
     HIP_TRY(hipEventRecord((hipEvent_t)ctx->ev_join, side));
     const int rc_gi = vpt_trace_gi(w, cam, gb, prm, d_rgb, d_svgf_in, stream);
     HIP_TRY(hipStreamWaitEvent(s, (hipEvent_t)ctx->ev_join, 0));

@@ -121,7 +121,11 @@ def viewproj(view: np.ndarray, proj: np.ndarray) -> np.ndarray:
 
 
 class Pipeline:
-    """Per-frame pass chain for one view of one world."""
+    """Per-frame pass chain for one view of one world, in the reference's
+    order (MainPipeline::StartPipeline, Pipeline.cpp:1960-3060): primary,
+    sun shadow (with its transversal) beside diffuse GI, SVGF, sun-shadow
+    denoise, and the reflection trace's core at the reference's settings
+    (1/4 resolution, 2 rough samples)."""
 
     def __init__(self, ctx: Context, world: World, width: int, height: int, sun_tick: float = 50.0,
                  max_iter: int = 350, shadow_iter: int = 350):
@@ -133,7 +137,12 @@ class Pipeline:
         self.texels = torch.empty((height, width, 8), dtype=torch.int16, device="cuda")
         self.denoised = torch.empty((height, width, 3), dtype=torch.float32, device="cuda")
         self.shadow = torch.empty((height, width), dtype=torch.uint8, device="cuda")
+        self.transversal = torch.empty((height, width), dtype=torch.int16, device="cuda")
+        self.shadow_denoised = torch.empty((height, width), dtype=torch.uint8, device="cuda")
+        self.reflect_dist = torch.empty((height // 4, width // 4), dtype=torch.int16, device="cuda")
+        self.reflect_hit = torch.empty((height // 4, width // 4), dtype=torch.int32, device="cuda")
         self.svgf = SVGF(ctx, width, height)
+        self.shadow_denoiser = ShadowDenoiser(ctx, width, height)
         self.sun_tick = sun_tick
         self.max_iter, self.shadow_iter = max_iter, shadow_iter
         self.frame_index = 0
@@ -141,7 +150,8 @@ class Pipeline:
         self.prev_proj = None
 
     def render(self, pos, yaw: float, pitch: float, stream=None):
-        """One frame: primary -> shadow beside GI -> SVGF.  Returns the camera."""
+        """One frame: primary -> shadow beside GI -> SVGF -> shadow denoise
+        -> reflections.  Returns the camera."""
         cam, view, proj = make_camera(pos, yaw, pitch, self.width, self.height)
         cur = self.gbuf[self.frame_index % 2]
         prev = self.gbuf[(self.frame_index + 1) % 2]
@@ -149,11 +159,18 @@ class Pipeline:
             self.prev_view, self.prev_proj = view, proj
         prm = gi_params(self.sun_tick, self.frame_index)
         self.world.trace_frame(cam, cur, prm, self.shadow, self.rgb, self.texels, self.max_iter, self.shadow_iter,
-                               stream)
+                               self.transversal, stream)
         if self.frame_index == 0:
             prev = cur
-        self.svgf.frame(cam, viewproj(self.prev_view, self.prev_proj), cur, prev, self.texels, self.denoised,
-                        time=self.frame_index / 60.0, stream=stream)
+        prev_vp = viewproj(self.prev_view, self.prev_proj)
+        self.svgf.frame(cam, prev_vp, cur, prev, self.texels, self.denoised, time=self.frame_index / 60.0,
+                        stream=stream)
+        self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal, self.shadow_denoised,
+                                   stream=stream)
+        if self.width >= 4 and self.height >= 4:
+            from .world import reflect_params
+            self.world.trace_reflect(cam, cur, list(prm.sun_dir), self.reflect_dist, self.reflect_hit,
+                                     stream=stream, **reflect_params(list(prm.sun_dir), self.frame_index))
         self.prev_view, self.prev_proj = view, proj
         self.frame_index += 1
         return cam

@@ -172,11 +172,13 @@ class World:
              _ptr(hit), _stream(stream))
 
     def trace_frame(self, cam: Camera, gbuf: "DeviceGBuffer", prm, mask, rgb, texels=None, primary_iter: int = 350,
-                    shadow_iter: int = 350, stream=None) -> None:
+                    shadow_iter: int = 350, transversal=None, stream=None) -> None:
         """Primary, shadow and GI passes of one frame; the shadow pass runs
-        beside GI on a stream of the context (vpt_trace_frame)."""
+        beside GI on a stream of the context (vpt_trace_frame).  With
+        `transversal` the shadow pass also writes its binary16 transversal
+        (vpt_trace_shadow2)."""
         call("vpt_trace_frame", self.handle, C.byref(cam), C.byref(prm), primary_iter, shadow_iter,
-             C.byref(gbuf.struct()), _ptr(mask), _ptr(rgb), _ptr(texels), _stream(stream))
+             C.byref(gbuf.struct()), _ptr(mask), _ptr(transversal), _ptr(rgb), _ptr(texels), _stream(stream))
 
     def trace_gi(self, cam: Camera, gbuf: "DeviceGBuffer", prm, rgb, texels=None, stream=None) -> None:
         call("vpt_trace_gi", self.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), _ptr(rgb),
