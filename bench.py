@@ -15,7 +15,8 @@ Workload (one step = one frame of rays, inputs resident in HBM):
   sequence, and the reflection pass (reflected rays + sun shadow at their
   hits, 64 / 150 steps) off the frame's G-buffer with the reference's
   settings (1/4 resolution, 2 rough samples) and as full-resolution
-  mirrors.
+  mirrors; and the whole pipeline frame (all of the above that a frame
+  runs, as Pipeline.render runs them).
 
 Prints one JSON line (see the "roofline" and "cpu_baseline" objects).
 """
@@ -322,6 +323,23 @@ def main():
         reflect[name] = {"target": [Wr, Hr], "spp": kw["spp"], "rough": kw["rough"], "ms": round(r_ms, 4),
                          "rays": r_rays, "samples": int(r_ctr[0].item()), "grays": round(r_rays / r_ms / 1e6, 4)}
 
+    # ---- the whole frame as the pipelines run it (Pipeline.render: primary,
+    # shadow + transversal beside GI, SVGF, shadow denoise, reflections at the
+    # reference's settings) along the SVGF camera path, 16 frames after 4
+    from voxelpathtracer_amd import Pipeline
+    pipe = Pipeline(ctx, world, W, H)
+    for f in range(4):
+        pipe.render((CAM_POS[0] + 0.1 * f, CAM_POS[1], CAM_POS[2]), CAM_YAW + 0.2 * f, CAM_PITCH)
+    torch.cuda.synchronize()
+    p0, p1 = ev(), ev()
+    p0.record(stream)
+    for f in range(4, 20):
+        pipe.render((CAM_POS[0] + 0.1 * f, CAM_POS[1], CAM_POS[2]), CAM_YAW + 0.2 * f, CAM_PITCH)
+    p1.record(stream)
+    torch.cuda.synchronize()
+    pipeline_ms = p0.elapsed_time(p1) / 16
+    del pipe
+
     result = {
         "metric": "Grays/s DDA+GI @1080p (bit-exact hits); SVGF ms/frame; served p50/p99 latency",
         "value": round(grays, 4),
@@ -350,6 +368,7 @@ def main():
         "shadow_denoise": {"ms_per_frame": round(shadow_denoise_ms, 4), "shadow_pass_with_transversal_ms":
                            round(shadow2_ms, 4), "frames": nf},
         "reflect": reflect,
+        "pipeline_ms_per_frame": round(pipeline_ms, 4),
         "roofline": {"bound": "hbm", "kernel": PASS_KERNELS[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": profiled_traffic(f"{dom} pass"), "algorithmic_bytes": alg[dom]},
