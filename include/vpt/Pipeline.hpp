@@ -19,7 +19,9 @@ namespace VoxelRT {
 
 class Pipeline {
 public:
-    Pipeline(Context& ctx, World& world, int width, int height, float sun_tick = 50.0f, bool persistent = true)
+    // persistent: trace with the persistent engine (vpt_frame_trace) instead
+    // of one launch per pass (vpt_trace_frame, the faster path)
+    Pipeline(Context& ctx, World& world, int width, int height, float sun_tick = 50.0f, bool persistent = false)
         : m_World(world), m_W(width), m_H(height), m_SunTick(sun_tick), m_Persistent(persistent)
     {
         const size_t px = (size_t)width * height;
