@@ -47,13 +47,21 @@ public:
         vpt_check(vpt_svgf_params_default(&sp));
         vpt_check(vpt_svgf_create(ctx.Handle(), width, height, &sp, &m_Svgf));
         vpt_check(vpt_shadow_denoise_create(ctx.Handle(), width, height, &m_ShadowDn));
+        if (hipStreamCreateWithFlags(&m_Side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&m_Fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&m_Join, hipEventDisableTiming) != hipSuccess)
+            throw std::runtime_error("Pipeline: side stream");
         if (persistent) vpt_check(vpt_frame_create(ctx.Handle(), width, height, &m_Frame));
     }
 
     ~Pipeline()
     {
+        if (m_Side) (void)hipStreamSynchronize(m_Side);
         if (m_Frame) vpt_frame_destroy(m_Frame);
         vpt_shadow_denoise_destroy(m_ShadowDn);
+        if (m_Side) (void)hipStreamDestroy(m_Side);
+        if (m_Fork) (void)hipEventDestroy(m_Fork);
+        if (m_Join) (void)hipEventDestroy(m_Join);
         vpt_svgf_destroy(m_Svgf);
         for (auto& g : m_GBuf) {
             (void)hipFree(g.d_t);
@@ -101,15 +109,23 @@ public:
         }
         float prev_vp[16];
         ViewProj(m_PrevView, m_PrevProj, prev_vp);
+        // reflections read only the G-buffer: they run on the side stream
+        // beside the denoisers (the trace kernel leaves most of the chip idle)
+        hipStream_t s = (hipStream_t)stream;
+        const bool reflect = m_RW > 0 && m_RH > 0;
+        if (reflect) {
+            if (hipEventRecord(m_Fork, s) != hipSuccess || hipStreamWaitEvent(m_Side, m_Fork, 0) != hipSuccess)
+                throw std::runtime_error("Pipeline: fork");
+            vpt_reflect_params rp;
+            vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
+            vpt_check(vpt_trace_reflect(m_World.Handle(), &cam, &cur, &rp, m_ReflDist, m_ReflHit, m_Side));
+            if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
+        }
         vpt_check(vpt_svgf_frame(m_Svgf, &cam, prev_vp, &cur, &prev, m_Texels, (float)m_Frame_ / 60.0f, m_Denoised,
                                  nullptr, stream));
         vpt_check(vpt_shadow_denoise_frame(m_ShadowDn, &cam, prev_vp, &cur, &prev, m_Mask, m_Transversal, 1.0f,
                                            m_ShadowOut, stream));
-        if (m_RW > 0 && m_RH > 0) {
-            vpt_reflect_params rp;
-            vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
-            vpt_check(vpt_trace_reflect(m_World.Handle(), &cam, &cur, &rp, m_ReflDist, m_ReflHit, stream));
-        }
+        if (reflect && hipStreamWaitEvent(s, m_Join, 0) != hipSuccess) throw std::runtime_error("Pipeline: join");
         for (int i = 0; i < 16; i++) {
             m_PrevView[i] = view[i];
             m_PrevProj[i] = proj[i];
@@ -162,6 +178,8 @@ private:
     uint32_t* m_ReflHit = nullptr;
     int m_RW = 0, m_RH = 0;
     vpt_shadow_denoise* m_ShadowDn = nullptr;
+    hipStream_t m_Side = nullptr;   // reflections beside the denoisers (default priority)
+    hipEvent_t m_Fork = nullptr, m_Join = nullptr;
     float* m_RGB = nullptr;
     float* m_Denoised = nullptr;
     vpt_svgf_texel* m_Texels = nullptr;

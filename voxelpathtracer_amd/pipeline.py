@@ -143,6 +143,7 @@ class Pipeline:
         self.reflect_hit = torch.empty((height // 4, width // 4), dtype=torch.int32, device="cuda")
         self.svgf = SVGF(ctx, width, height)
         self.shadow_denoiser = ShadowDenoiser(ctx, width, height)
+        self.side = torch.cuda.Stream()   # reflections beside the denoisers (default priority)
         self.sun_tick = sun_tick
         self.max_iter, self.shadow_iter = max_iter, shadow_iter
         self.frame_index = 0
@@ -163,14 +164,27 @@ class Pipeline:
         if self.frame_index == 0:
             prev = cur
         prev_vp = viewproj(self.prev_view, self.prev_proj)
+        # reflections read only the G-buffer: they run on the side stream
+        # beside the denoisers (the trace kernel leaves most of the chip idle)
+        import torch
+        main = torch.cuda.current_stream() if stream is None else torch.cuda.ExternalStream(int(stream))
+        join = None
+        if self.width >= 4 and self.height >= 4:
+            from .world import reflect_params
+            fork = torch.cuda.Event()
+            fork.record(main)
+            self.side.wait_event(fork)
+            with torch.cuda.stream(self.side):
+                self.world.trace_reflect(cam, cur, list(prm.sun_dir), self.reflect_dist, self.reflect_hit,
+                                         **reflect_params(list(prm.sun_dir), self.frame_index))
+            join = torch.cuda.Event()
+            join.record(self.side)
         self.svgf.frame(cam, prev_vp, cur, prev, self.texels, self.denoised, time=self.frame_index / 60.0,
                         stream=stream)
         self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal, self.shadow_denoised,
                                    stream=stream)
-        if self.width >= 4 and self.height >= 4:
-            from .world import reflect_params
-            self.world.trace_reflect(cam, cur, list(prm.sun_dir), self.reflect_dist, self.reflect_hit,
-                                     stream=stream, **reflect_params(list(prm.sun_dir), self.frame_index))
+        if join is not None:
+            main.wait_event(join)
         self.prev_view, self.prev_proj = view, proj
         self.frame_index += 1
         return cam
