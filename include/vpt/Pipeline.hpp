@@ -109,23 +109,23 @@ public:
         }
         float prev_vp[16];
         ViewProj(m_PrevView, m_PrevProj, prev_vp);
-        // reflections read only the G-buffer: they run on the side stream
-        // beside the denoisers (the trace kernel leaves most of the chip idle)
+        // the reflections and the shadow denoise read only this frame's
+        // traces: they run on the side stream beside SVGF (the reflection
+        // kernel leaves most of the chip idle)
         hipStream_t s = (hipStream_t)stream;
-        const bool reflect = m_RW > 0 && m_RH > 0;
-        if (reflect) {
-            if (hipEventRecord(m_Fork, s) != hipSuccess || hipStreamWaitEvent(m_Side, m_Fork, 0) != hipSuccess)
-                throw std::runtime_error("Pipeline: fork");
+        if (hipEventRecord(m_Fork, s) != hipSuccess || hipStreamWaitEvent(m_Side, m_Fork, 0) != hipSuccess)
+            throw std::runtime_error("Pipeline: fork");
+        if (m_RW > 0 && m_RH > 0) {
             vpt_reflect_params rp;
             vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
             vpt_check(vpt_trace_reflect(m_World.Handle(), &cam, &cur, &rp, m_ReflDist, m_ReflHit, m_Side));
-            if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
         }
+        vpt_check(vpt_shadow_denoise_frame(m_ShadowDn, &cam, prev_vp, &cur, &prev, m_Mask, m_Transversal, 1.0f,
+                                           m_ShadowOut, m_Side));
+        if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
         vpt_check(vpt_svgf_frame(m_Svgf, &cam, prev_vp, &cur, &prev, m_Texels, (float)m_Frame_ / 60.0f, m_Denoised,
                                  nullptr, stream));
-        vpt_check(vpt_shadow_denoise_frame(m_ShadowDn, &cam, prev_vp, &cur, &prev, m_Mask, m_Transversal, 1.0f,
-                                           m_ShadowOut, stream));
-        if (reflect && hipStreamWaitEvent(s, m_Join, 0) != hipSuccess) throw std::runtime_error("Pipeline: join");
+        if (hipStreamWaitEvent(s, m_Join, 0) != hipSuccess) throw std::runtime_error("Pipeline: join");
         for (int i = 0; i < 16; i++) {
             m_PrevView[i] = view[i];
             m_PrevProj[i] = proj[i];

@@ -164,8 +164,9 @@ class Pipeline:
         if self.frame_index == 0:
             prev = cur
         prev_vp = viewproj(self.prev_view, self.prev_proj)
-        # reflections read only the G-buffer: they run on the side stream
-        # beside the denoisers (the trace kernel leaves most of the chip idle)
+        # the reflections and the shadow denoise read only this frame's
+        # traces: they run on the side stream beside SVGF (the reflection
+        # kernel leaves most of the chip idle)
         import torch
         main = torch.cuda.current_stream() if stream is None else torch.cuda.ExternalStream(int(stream))
         join = None
@@ -177,12 +178,15 @@ class Pipeline:
             with torch.cuda.stream(self.side):
                 self.world.trace_reflect(cam, cur, list(prm.sun_dir), self.reflect_dist, self.reflect_hit,
                                          **reflect_params(list(prm.sun_dir), self.frame_index))
+                self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal,
+                                           self.shadow_denoised)
             join = torch.cuda.Event()
             join.record(self.side)
+        else:
+            self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal, self.shadow_denoised,
+                                       stream=stream)
         self.svgf.frame(cam, prev_vp, cur, prev, self.texels, self.denoised, time=self.frame_index / 60.0,
                         stream=stream)
-        self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal, self.shadow_denoised,
-                                   stream=stream)
         if join is not None:
             main.wait_event(join)
         self.prev_view, self.prev_proj = view, proj
