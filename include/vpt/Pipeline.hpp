@@ -19,8 +19,11 @@ namespace VoxelRT {
 
 class Pipeline {
 public:
-    // persistent: trace with the persistent engine (vpt_frame_trace) instead
-    // of one launch per pass (vpt_trace_frame, the faster path)
+    // persistent: trace primary, sun shadow and GI with the persistent engine
+    // (vpt_frame_trace) instead of one launch per pass (vpt_trace_primary,
+    // vpt_trace_gi; the faster path).  Either way the shadow pass with its
+    // transversal, the shadow denoise and the reflections run on a side
+    // stream beside GI and SVGF.
     Pipeline(Context& ctx, World& world, int width, int height, float sun_tick = 50.0f, bool persistent = false)
         : m_World(world), m_W(width), m_H(height), m_SunTick(sun_tick), m_Persistent(persistent)
     {
@@ -98,31 +101,29 @@ public:
         }
         vpt_gi_params prm;
         vpt_check(vpt_gi_params_default(m_SunTick, (uint32_t)m_Frame_, 0x5EED0003u, &prm));
-        if (m_Persistent) {
+        hipStream_t s = (hipStream_t)stream;
+        if (m_Persistent)
             vpt_check(vpt_frame_trace(m_Frame, m_World.Handle(), &cam, &prm, 350, 350, &cur, m_Mask, m_RGB, m_Texels,
                                       1, stream));
-            // the shadow pass again for its transversal (the mask is the same)
-            vpt_check(vpt_trace_shadow2(m_World.Handle(), &cam, &cur, prm.sun_dir, 350, m_Mask, m_Transversal, stream));
-        } else {
-            vpt_check(vpt_trace_frame(m_World.Handle(), &cam, &prm, 350, 350, &cur, m_Mask, m_Transversal, m_RGB,
-                                      m_Texels, stream));
-        }
+        else
+            vpt_check(vpt_trace_primary(m_World.Handle(), &cam, 350, &cur, stream));
         float prev_vp[16];
         ViewProj(m_PrevView, m_PrevProj, prev_vp);
-        // the reflections and the shadow denoise read only this frame's
-        // traces: they run on the side stream beside SVGF (the reflection
-        // kernel leaves most of the chip idle)
-        hipStream_t s = (hipStream_t)stream;
+        // fork: the side stream runs the shadow pass with its transversal,
+        // the shadow denoise and the reflections, which read only the
+        // G-buffer and the shadow pass; GI and SVGF stay on `stream`
         if (hipEventRecord(m_Fork, s) != hipSuccess || hipStreamWaitEvent(m_Side, m_Fork, 0) != hipSuccess)
             throw std::runtime_error("Pipeline: fork");
+        vpt_check(vpt_trace_shadow2(m_World.Handle(), &cam, &cur, prm.sun_dir, 350, m_Mask, m_Transversal, m_Side));
+        vpt_check(vpt_shadow_denoise_frame(m_ShadowDn, &cam, prev_vp, &cur, &prev, m_Mask, m_Transversal, 1.0f,
+                                           m_ShadowOut, m_Side));
         if (m_RW > 0 && m_RH > 0) {
             vpt_reflect_params rp;
             vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
             vpt_check(vpt_trace_reflect(m_World.Handle(), &cam, &cur, &rp, m_ReflDist, m_ReflHit, m_Side));
         }
-        vpt_check(vpt_shadow_denoise_frame(m_ShadowDn, &cam, prev_vp, &cur, &prev, m_Mask, m_Transversal, 1.0f,
-                                           m_ShadowOut, m_Side));
         if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
+        if (!m_Persistent) vpt_check(vpt_trace_gi(m_World.Handle(), &cam, &cur, &prm, m_RGB, m_Texels, stream));
         vpt_check(vpt_svgf_frame(m_Svgf, &cam, prev_vp, &cur, &prev, m_Texels, (float)m_Frame_ / 60.0f, m_Denoised,
                                  nullptr, stream));
         if (hipStreamWaitEvent(s, m_Join, 0) != hipSuccess) throw std::runtime_error("Pipeline: join");
@@ -178,7 +179,7 @@ private:
     uint32_t* m_ReflHit = nullptr;
     int m_RW = 0, m_RH = 0;
     vpt_shadow_denoise* m_ShadowDn = nullptr;
-    hipStream_t m_Side = nullptr;   // reflections beside the denoisers (default priority)
+    hipStream_t m_Side = nullptr;   // shadow, shadow denoise, reflections (default priority)
     hipEvent_t m_Fork = nullptr, m_Join = nullptr;
     float* m_RGB = nullptr;
     float* m_Denoised = nullptr;

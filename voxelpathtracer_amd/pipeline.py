@@ -143,7 +143,7 @@ class Pipeline:
         self.reflect_hit = torch.empty((height // 4, width // 4), dtype=torch.int32, device="cuda")
         self.svgf = SVGF(ctx, width, height)
         self.shadow_denoiser = ShadowDenoiser(ctx, width, height)
-        self.side = torch.cuda.Stream()   # reflections beside the denoisers (default priority)
+        self.side = torch.cuda.Stream()   # shadow, shadow denoise, reflections (default priority)
         self.sun_tick = sun_tick
         self.max_iter, self.shadow_iter = max_iter, shadow_iter
         self.frame_index = 0
@@ -151,44 +151,39 @@ class Pipeline:
         self.prev_proj = None
 
     def render(self, pos, yaw: float, pitch: float, stream=None):
-        """One frame: primary -> shadow beside GI -> SVGF -> shadow denoise
-        -> reflections.  Returns the camera."""
+        """One frame: primary; then the sun shadow with its transversal, the
+        shadow denoise and the reflections on a side stream, beside GI and
+        SVGF on `stream`.  Returns the camera."""
+        import torch
         cam, view, proj = make_camera(pos, yaw, pitch, self.width, self.height)
         cur = self.gbuf[self.frame_index % 2]
         prev = self.gbuf[(self.frame_index + 1) % 2]
         if self.prev_view is None:
             self.prev_view, self.prev_proj = view, proj
         prm = gi_params(self.sun_tick, self.frame_index)
-        self.world.trace_frame(cam, cur, prm, self.shadow, self.rgb, self.texels, self.max_iter, self.shadow_iter,
-                               self.transversal, stream)
+        sun = list(prm.sun_dir)
         if self.frame_index == 0:
             prev = cur
         prev_vp = viewproj(self.prev_view, self.prev_proj)
-        # the reflections and the shadow denoise read only this frame's
-        # traces: they run on the side stream beside SVGF (the reflection
-        # kernel leaves most of the chip idle)
-        import torch
         main = torch.cuda.current_stream() if stream is None else torch.cuda.ExternalStream(int(stream))
-        join = None
-        if self.width >= 4 and self.height >= 4:
-            from .world import reflect_params
-            fork = torch.cuda.Event()
-            fork.record(main)
-            self.side.wait_event(fork)
-            with torch.cuda.stream(self.side):
-                self.world.trace_reflect(cam, cur, list(prm.sun_dir), self.reflect_dist, self.reflect_hit,
-                                         **reflect_params(list(prm.sun_dir), self.frame_index))
-                self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal,
-                                           self.shadow_denoised)
-            join = torch.cuda.Event()
-            join.record(self.side)
-        else:
-            self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal, self.shadow_denoised,
-                                       stream=stream)
+        self.world.trace_primary(cam, cur, self.max_iter, stream=main.cuda_stream)
+        # the side passes read only the G-buffer and the shadow pass
+        fork = torch.cuda.Event()
+        fork.record(main)
+        self.side.wait_event(fork)
+        with torch.cuda.stream(self.side):
+            self.world.trace_shadow2(cam, cur, sun, self.shadow, self.transversal, self.shadow_iter)
+            self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal, self.shadow_denoised)
+            if self.width >= 4 and self.height >= 4:
+                from .world import reflect_params
+                self.world.trace_reflect(cam, cur, sun, self.reflect_dist, self.reflect_hit,
+                                         **reflect_params(sun, self.frame_index))
+        join = torch.cuda.Event()
+        join.record(self.side)
+        self.world.trace_gi(cam, cur, prm, self.rgb, self.texels, stream=main.cuda_stream)
         self.svgf.frame(cam, prev_vp, cur, prev, self.texels, self.denoised, time=self.frame_index / 60.0,
-                        stream=stream)
-        if join is not None:
-            main.wait_event(join)
+                        stream=main.cuda_stream)
+        main.wait_event(join)
         self.prev_view, self.prev_proj = view, proj
         self.frame_index += 1
         return cam
