@@ -125,3 +125,65 @@ def test_import_errors(tmp_path):
         mc_import(tmp_path, (16, 16, 16), (0, 0, 0), lut)
     grid, stats = mc_import(tmp_path, (16, 16, 16))            # no region files: an empty world
     assert stats == [0, 0, 0] and not grid.any()
+
+
+def _chunks_of(region: bytes) -> dict:
+    """{index: decompressed NBT} of a region written by mca_synth.region."""
+    import gzip
+    import struct
+    import zlib
+    out = {}
+    for idx in range(1024):
+        loc = struct.unpack(">I", region[4 * idx:4 * idx + 4])[0]
+        if loc:
+            off = (loc >> 8) * 4096
+            n, ctype = struct.unpack(">iB", region[off:off + 5])
+            data = region[off + 5:off + 4 + n]
+            if ctype == 2:
+                try:
+                    data = zlib.decompress(data)
+                except zlib.error:
+                    continue
+            elif ctype == 1:
+                data = gzip.decompress(data)
+            out[idx] = data
+    return out
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_mutated_files_import_without_error(tmp_path, seed):
+    """Corrupted region files (flipped bytes, huge or negative lengths and
+    counts, truncated NBT, header sectors past the end of the file) import
+    without an error: a chunk the reader cannot walk is skipped, as the
+    reference's reader skips it, and every read stays inside its buffer."""
+    import struct
+    rng = np.random.default_rng(seed)
+    for it in range(25):
+        case = M.CASES[it % len(M.CASES)]
+        d = tmp_path / f"c{it}"
+        d.mkdir()
+        for name, reg in M.build_case(case, 1000 * seed + it).items():
+            chunks = {}
+            for i, nbt in _chunks_of(reg).items():
+                b = bytearray(nbt)
+                for _ in range(int(rng.integers(1, 5))):
+                    p = int(rng.integers(0, len(b)))
+                    k = int(rng.integers(0, 4))
+                    if k == 0:
+                        b[p] ^= 1 << int(rng.integers(0, 8))
+                    elif k == 1 and p + 4 <= len(b):
+                        b[p:p + 4] = struct.pack(">I", int(rng.choice([0x7fffffff, 0x80000000, 0xffffffff, 4096])))
+                    elif k == 2:
+                        del b[max(p, 1):]
+                    else:
+                        b[p] = int(rng.integers(0, 13))
+                chunks[i] = bytes(b)
+            reg = bytearray(M.region(chunks, {i: str(rng.choice(["zlib", "gzip", "raw"])) for i in chunks}))
+            if rng.random() < 0.4:   # a location far past the end of the file, or a cut file
+                j = int(rng.integers(0, 1024))
+                reg[4 * j:4 * j + 4] = struct.pack(">I", int(rng.integers(0, 2**32)))
+            if rng.random() < 0.2:
+                del reg[int(rng.integers(0, len(reg))):]
+            (d / name).write_bytes(bytes(reg))
+        grid, stats = mc_import(d, M.DIMS, M.ORIGIN, M.block_lut(), M.NAMES)
+        assert stats[0] == len(list(d.iterdir())) and stats[1] >= 0 and stats[2] >= 0
