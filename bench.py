@@ -72,8 +72,8 @@ def copy_probe(mib: int) -> None:
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1 "
-                          "(the path shards by pixel tiles / requests with no collective, see DESIGN.md)"}))
+        print(json.dumps({"error": "bench.py measures one GPU only; run with --gpus 1 "
+                          "(the path would shard by pixel tiles / requests with no collective)"}))
         sys.exit(2)
     import torch
     from voxelpathtracer_amd import (SVGF, Context, ShadowDenoiser, World, generate_terrain, gi_params, make_camera,
