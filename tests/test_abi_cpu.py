@@ -4,6 +4,7 @@ compute is called here."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 import re
 import subprocess
 import sys
@@ -180,3 +181,18 @@ def test_argument_errors_without_a_device():
     assert L.vpt_raycast_detect(None, 16, 8, 8, pos, d, res) == _lib.VPT_EINVAL
     # no device work was needed for any of the above: the error text names the call
     assert b"vpt_raycast_detect" in L.vpt_last_error()
+
+
+def test_product_fails_loudly_without_the_library(tmp_path):
+    """With no libvpt.so at the configured path, the first call into the
+    package raises; nothing falls back to another implementation."""
+    code = ("from voxelpathtracer_amd.world import mc_import\n"
+            "try:\n"
+            "    mc_import('.', (16, 16, 16))\n"
+            "except RuntimeError as e:\n"
+            "    print('raised', e)\n")
+    env = dict(os.environ, VPT_LIB=str(tmp_path / "absent" / "libvpt.so"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120, cwd=str(ROOT),
+                       env=env)
+    assert r.returncode == 0, r.stderr
+    assert "raised" in r.stdout and "not found" in r.stdout
