@@ -1,7 +1,8 @@
 """The C++ host façades (include/vpt/World.hpp, Pipeline.hpp) drive the same
 passes as the Python host: the C++ example's denoised frame, denoised sun
-shadow and reflection hit distances equal the Python Pipeline's bit for bit
-(persistent + graph path in C++, one launch per pass in Python)."""
+shadow and reflection hit distances equal the Python Pipeline's bit for bit,
+with the C++ pipeline tracing through the persistent + graph engine or one
+launch per pass (the Python pipeline: one launch per pass)."""
 from __future__ import annotations
 
 import json
@@ -17,16 +18,18 @@ from voxelpathtracer_amd.world import generate_terrain
 pytestmark = pytest.mark.gpu
 
 
-def test_cpp_facade_matches_python_pipeline(ctx, tmp_path):
+@pytest.mark.parametrize("path", ["persistent", "plain"])
+def test_cpp_facade_matches_python_pipeline(ctx, tmp_path, path):
     import torch
     W, H, frames = 320, 180, 3
     dump = tmp_path / "rgb.bin"
-    print("running the C++ façade example", flush=True)
-    r = subprocess.run([str(DEMO), str(W), str(H), str(frames), "--dump", str(dump)], capture_output=True, text=True,
-                       timeout=120)
+    print(f"running the C++ façade example ({path})", flush=True)
+    extra = ["--plain"] if path == "plain" else []
+    r = subprocess.run([str(DEMO), str(W), str(H), str(frames), *extra, "--dump", str(dump)], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     info = json.loads(r.stdout.strip().splitlines()[-1])
-    assert info["path"] == "persistent" and info["frames"] == frames
+    assert info["path"] == path and info["frames"] == frames
     raw = np.fromfile(dump, np.uint8)
     n_rgb, n_sh = H * W * 3 * 4, H * W
     cpp = raw[:n_rgb].view(np.float32).reshape(H, W, 3)
