@@ -176,6 +176,8 @@ __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restri
     const int z = blockIdx.x;
     const size_t base = (size_t)z * nx * ny;
     const int bytes = nx * ny;
+    // unrolled so that a wavefront has several loads in flight per wait
+#pragma unroll 4
     for (int i = threadIdx.x * 16; i < bytes; i += blockDim.x * 16)
         *reinterpret_cast<uint4*>(slab + i) = *reinterpret_cast<const uint4*>(grid + base + i);
     __syncthreads();
@@ -277,9 +279,14 @@ __global__ void __launch_bounds__(256) df_z_tile_kernel(const uint8_t* __restric
     if (gate && !*gate) return;
     extern __shared__ uint8_t tile[];   // [z][y 0..7][x 0..31]
     const int tiles_x = nx / 32;
-    const int x0 = (blockIdx.x % tiles_x) * 32, y0 = (blockIdx.x / tiles_x) * 8;
+    // consecutive tiles on one XCD: the four 32-byte-wide tiles that share a
+    // 128-byte line of a row meet in the same L2
+    const uint32_t t = vpt::xcd_remap(blockIdx.x, gridDim.x);
+    const int x0 = (int)(t % (uint32_t)tiles_x) * 32, y0 = (int)(t / (uint32_t)tiles_x) * 8;
     const size_t plane = (size_t)nx * ny;
-    // stage: 8 rows of 32 bytes per z, two 16-byte loads per row
+    // stage: 8 rows of 32 bytes per z, two 16-byte loads per row; unrolled
+    // so that a wavefront has several loads in flight per wait
+#pragma unroll 8
     for (int i = threadIdx.x; i < nz * 16; i += blockDim.x) {
         const int z = i >> 4, r = (i >> 1) & 7, h = i & 1;
         *reinterpret_cast<uint4*>(tile + z * 256 + r * 32 + h * 16) =
