@@ -79,9 +79,22 @@ def test_shadow_denoise_sequence_small(ctx, terrain_world):
     assert (frames.view(np.float16) > 0).mean() > 0.3   # history accumulates on most lit pixels
 
 
+def test_shadow_denoise_steady_state(ctx, terrain_world):
+    """24 frames at 640x360: the frame count passes 7.5 on most pixels, so
+    ShadowFilter.glsl:70 (ApplyLumaWeight) runs with the full luminance
+    exponent there, beside pixels still below it; every plane of every frame
+    bit-exact."""
+    g, df, w = terrain_world
+    frames = _run(ctx, w, g, df, 640, 360, 24).view(np.float16).astype(np.float32)
+    assert (frames > 7.5).mean() > 0.2, "ApplyLumaWeight branch not reached"
+    assert ((frames > 0) & (frames <= 7.5)).mean() > 0.01, "short-history weight branch not exercised"
+
+
+@pytest.mark.timeout(600)
 def test_shadow_denoise_1080p(ctx, terrain_world):
     g, df, w = terrain_world
-    _run(ctx, w, g, df, 1920, 1080, 2)
+    frames = _run(ctx, w, g, df, 1920, 1080, 10).view(np.float16).astype(np.float32)
+    assert (frames > 7.5).mean() > 0.2
 
 
 def test_shadow_denoise_filter_scale(ctx, terrain_world):
