@@ -300,6 +300,25 @@ def threads() -> int:
     return int(lib().oracle_threads())
 
 
+# ---- planted mistakes (the oracle's own pinning tests) ----------------------
+
+def mutation_count() -> int:
+    """Number of planted-mistake ids (0 = none)."""
+    return int(lib().oracle_mutation_count())
+
+
+def mutation_name(mid: int) -> str:
+    f = lib().oracle_mutation_name
+    f.restype = C.c_char_p
+    return f(C.c_int(mid)).decode()
+
+
+def set_mutation(mid: int) -> None:
+    """Switches on planted mistake `mid` in the SVGF and shadow-filter
+    passes (0 switches them off).  For the pinning tests only."""
+    lib().oracle_set_mutation(C.c_int(mid))
+
+
 # ---- shared numerics (array wrappers) ---------------------------------------
 
 def expf(x):

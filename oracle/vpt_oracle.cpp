@@ -37,7 +37,12 @@
  *                          cosine-lobe statistics)                 -> pinned
  *   oracle_svgf_*          constant-image identity, weight
  *                          normalisation, running-mean closed form,
- *                          disocclusion passthrough                -> pinned
+ *                          disocclusion passthrough; per-tap weights
+ *                          and constants by closed forms on
+ *                          structured inputs (impulses, edges, linear
+ *                          and quadratic ramps, the distance ladder),
+ *                          each of 43 planted mistakes failing one
+ *                          (tests/test_svgf_pins_cpu.py)           -> pinned
  *   oracle_camera          closed forms of glm lookAt/perspective  -> pinned
  *   oracle_trace_shadow2   transversal: a known slab distance, the
  *                          miss / N.L / sky constants; mask equal to
@@ -46,9 +51,13 @@
  *                          form blend of uniform images, sky
  *                          passthrough                             -> pinned
  *   oracle_shadow_filter   uniform-image identity, sharp and sky
- *                          passthrough                             -> pinned
- *                          (per-tap weights of both follow the
- *                          transcription only)
+ *                          passthrough; per-tap weights, kernel-size,
+ *                          spacing and luminance switches by edge
+ *                          responses, 10 planted mistakes each
+ *                          failing one (tests/test_svgf_pins_cpu.py)
+ *                                                                  -> pinned
+ *                          (the shadow temporal filter's per-tap
+ *                          weights follow the transcription only)
  *   vpt_logf / vpt_powf    error bounds against double libm        -> pinned
  *   oracle_trace_reflect   corridor closed forms (first hit, face,
  *                          block, distance incl. the 0.001 offset,
@@ -650,6 +659,85 @@ static inline bool in_screen(float x, float y) { return x > 0.0f && x < 1.0f && 
 
 static inline float sh_to_y(float w) { return fmaxf(0.0f, 3.544905f * w); }
 
+/* Planted mistakes, for the oracle's own pinning tests only: each id swaps
+ * one constant or operation of the SVGF and shadow-filter passes for a
+ * plausible wrong one, so that a test can show that its closed-form checks
+ * fail on it.  0 (the default) is the transcription as written. */
+enum Mutation {
+    MUT_NONE = 0,
+    MUT_PRE_K1, MUT_PRE_PHI, MUT_PRE_NW_SIGN, MUT_PRE_NW_DROP, MUT_PRE_FLOOR, MUT_PRE_ACCEPT,
+    MUT_TMP_CENTRE, MUT_TMP_FLOOR, MUT_TMP_TOL_8_16, MUT_TMP_TOL_48_64, MUT_TMP_LAST_BAND, MUT_TMP_BLOCK_32,
+    MUT_TMP_NORMAL_64, MUT_TMP_EDGE, MUT_TMP_MOMENT, MUT_TMP_CLAMP_COCG, MUT_TMP_SPP_CLAMP, MUT_TMP_SKY,
+    MUT_VAR_THRESH, MUT_VAR_FACTOR3, MUT_VAR_PHI, MUT_VAR_DEPTH, MUT_VAR_WINDOW, MUT_VAR_SCALE, MUT_VAR_CLAMP,
+    MUT_VAR_NW,
+    MUT_AT_K1, MUT_AT_SCALE, MUT_AT_JIT_HALF, MUT_AT_JIT_AMP, MUT_AT_STRONG_FRAMES, MUT_AT_STRONG_STEP,
+    MUT_AT_PHI_BIAS, MUT_AT_TWEAK, MUT_AT_GAUSS, MUT_AT_DEPTH, MUT_AT_DEPTH_CLAMP, MUT_AT_NW_CLAMP, MUT_AT_W_CLAMP,
+    MUT_AT_VAR_PROP, MUT_AT_AO_STEP, MUT_AT_SKY, MUT_AT_LUMA_PHI,
+    MUT_SF_SIZE, MUT_SF_SPACING, MUT_SF_SC24, MUT_SF_LUMA_SWITCH, MUT_SF_LM, MUT_SF_EXPO, MUT_SF_VE, MUT_SF_LE,
+    MUT_SF_DEPTH, MUT_SF_PASS,
+    MUT_COUNT
+};
+
+static const char* const kMutationNames[MUT_COUNT] = {
+    "none",
+    "prespatial: K(1) = 1/2 instead of 2/3",
+    "prespatial: colour phi 5 instead of 4",
+    "prespatial: exp(-lw + nw) instead of exp(-lw - nw)",
+    "prespatial: normal term dropped, exp(-lw)",
+    "prespatial: weight floor 0.001 instead of 0.01",
+    "prespatial: position test |dP|^2 < 2 instead of < 1",
+    "temporal: centre tap weight 3/32 instead of 9/64",
+    "temporal: blend floor 0.1 instead of 0.05",
+    "temporal: tolerance 2.0 instead of 1.414 for 8 <= d < 16",
+    "temporal: tolerance 3.5 instead of 4.2 for 48 <= d < 64",
+    "temporal: last band (tolerance 14, no block / normal test) extends past 200",
+    "temporal: block test kept up to d < 48 instead of 32",
+    "temporal: normal test kept up to d < 96 instead of 64",
+    "temporal: screen-edge threshold 0.001 instead of 0.0035",
+    "temporal: moment blends luma instead of luma^2",
+    "temporal: CoCg lower clamp -100 instead of -10",
+    "temporal: history length clamp 255 instead of 150",
+    "temporal: sky / non-sky mismatch not rejected",
+    "variance: short-history threshold 8 instead of 12",
+    "variance: short-history factor 1 instead of 3",
+    "variance: colour phi 4 instead of 5",
+    "variance: depth weight exp(-|dt|) not squared",
+    "variance: 7x7 window instead of 9x9",
+    "variance: no 12 / frames scaling",
+    "variance: upper clamp 10 instead of 50",
+    "variance: normal weight ignored",
+    "a-trous: K(1) = 1/2 instead of 2/3",
+    "a-trous: tap spacing without the resolution scale",
+    "a-trous: jitter not halved",
+    "a-trous: jitter amplitude step * 1.0 instead of step * 0.8",
+    "a-trous: strong filtering while history <= 12 instead of <= 8",
+    "a-trous: strong filtering at step 16 too",
+    "a-trous: phi multiplied by the bias instead of divided",
+    "a-trous: TweakVariance curve 96 instead of 128 below 0.01",
+    "a-trous: variance pre-blur kernel 0.5 / 0.25",
+    "a-trous: depth weight exp(-d) not squared",
+    "a-trous: depth weight floor 1e-3 instead of 1e-4",
+    "a-trous: normal weight floor 0.01 instead of 0.001",
+    "a-trous: weight floor 0.01 instead of 0.001",
+    "a-trous: variance propagated with w instead of w^2",
+    "a-trous: AO filtered at step 8",
+    "a-trous: sky pixels filtered",
+    "a-trous: luminance weight without the sqrt of the variance",
+    "shadow filter: 7x7 switch at sqrt(2) instead of 1.414 sqrt(2)",
+    "shadow filter: tap spacing 1.0 instead of 1.2",
+    "shadow filter: scale 2.0 instead of 2.4 above 16",
+    "shadow filter: luminance switch at 4 instead of 7.5",
+    "shadow filter: short-history exponent factor from 0.2 instead of 0.1",
+    "shadow filter: exponent factor 1.0 instead of 0.9",
+    "shadow filter: variance estimate +2 / +5 swapped",
+    "shadow filter: luminance error / 2 instead of / 3",
+    "shadow filter: depth weight squared instead of cubed",
+    "shadow filter: pass-through below sqrt(2) dropped",
+};
+
+static int g_mutation = MUT_NONE;
+static inline bool mut(int id) { return g_mutation == id; }
+
 struct Frame {
     const float* inv_view;
     const float* inv_proj;
@@ -670,6 +758,10 @@ static inline float r8(float x) { return vpt_unorm8_decode(vpt_unorm8_encode(x))
 
 } /* namespace */
 
+OAPI int oracle_mutation_count(void) { return MUT_COUNT; }
+OAPI const char* oracle_mutation_name(int id) { return id >= 0 && id < MUT_COUNT ? kMutationNames[id] : ""; }
+OAPI void oracle_set_mutation(int id) { g_mutation = id >= 0 && id < MUT_COUNT ? id : MUT_NONE; }
+
 /* Spatial3x3Initial.main (Spatial3x3Initial.glsl:96-176).
  * in:  sh[4], cocg[2], utility[1], ao[2]  (raw GI planes)
  * out: sh[4], cocg[2], utility[1], ao[2]  (DiffusePreTemporal_SpatialFBO) */
@@ -679,7 +771,7 @@ OAPI void oracle_svgf_prespatial(const float* inv_view, const float* inv_proj, i
                                  float* o_sh, float* o_cocg, float* o_util, float* o_ao)
 {
     Frame fr = {inv_view, inv_proj, W, H};
-    const float atrous[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+    const float atrous[3] = {1.0f, mut(MUT_PRE_K1) ? 0.5f : 2.0f / 3.0f, 1.0f / 6.0f};
     const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
 #pragma omp parallel for schedule(static)
     for (int py = 0; py < H; py++) {
@@ -695,7 +787,7 @@ OAPI void oracle_svgf_prespatial(const float* inv_view, const float* inv_proj, i
             float tw = 1.0f;
             float tao[2] = {ao[2 * i], ao[2 * i + 1]};
             float taow = 1.0f;
-            const float phi_color = 4.0f;
+            const float phi_color = mut(MUT_PRE_PHI) ? 5.0f : 4.0f;
             for (int x = -1; x <= 1; x++) {
                 for (int y = -1; y <= 1; y++) {
                     if (x == 0 && y == 0) continue;
@@ -706,16 +798,17 @@ OAPI void oracle_svgf_prespatial(const float* inv_view, const float* inv_proj, i
                     V3 sp = position_from(fr, sx, sy, gb_t[j]);
                     V3 pd = v3(fabsf(sp.x - base_pos.x), fabsf(sp.y - base_pos.y), fabsf(sp.z - base_pos.z));
                     float dist_sqr = dot3(pd, pd);
-                    if (dist_sqr < 1.0f) {
+                    if (dist_sqr < (mut(MUT_PRE_ACCEPT) ? 2.0f : 1.0f)) {
                         float ssh[4] = {sh[4 * j], sh[4 * j + 1], sh[4 * j + 2], sh[4 * j + 3]};
                         V3 sn = face_normal(gb_face[j], 1.0f);
                         float sluma = sh_to_y(ssh[3]);
                         float nw = vpt_powi(fmaxf(dot3(base_n, sn), 0.0f), 16);
                         float lw = fabsf(sluma - base_luma) / phi_color;
-                        float w = vpt_expf(-lw - nw);
-                        w = fmaxf(w, 0.01f);
+                        float w = vpt_expf(mut(MUT_PRE_NW_SIGN) ? -lw + nw : (mut(MUT_PRE_NW_DROP) ? -lw : -lw - nw));
+                        const float floor_w = mut(MUT_PRE_FLOOR) ? 0.001f : 0.01f;
+                        w = fmaxf(w, floor_w);
                         w = (atrous[abs(x)] * atrous[abs(y)]) * w;
-                        w = fmaxf(w, 0.01f);
+                        w = fmaxf(w, floor_w);
                         w = fclamp(w, 0.0f, 1.0f);
                         for (int k = 0; k < 4; k++) tsh[k] = tsh[k] + ssh[k] * w;
                         tcocg[0] = tcocg[0] + cocg[2 * j] * w;
@@ -756,10 +849,11 @@ OAPI void oracle_svgf_temporal(const float* inv_view, const float* inv_proj, con
     Frame fr = {inv_view, inv_proj, W, H};
     Tex Tpt = {prev_t, W, H, 1};
     Tex Tsh = {prev_sh, W, H, 4}, Tcc = {prev_cocg, W, H, 2}, Tut = {prev_util, W, H, 3}, Tao = {prev_ao, W, H, 2};
-    const float weights[5] = {3.0f / 32.0f, 3.0f / 32.0f, 9.0f / 64.0f, 3.0f / 32.0f, 3.0f / 32.0f};
+    const float weights[5] = {3.0f / 32.0f, 3.0f / 32.0f, mut(MUT_TMP_CENTRE) ? 3.0f / 32.0f : 9.0f / 64.0f,
+                              3.0f / 32.0f, 3.0f / 32.0f};
     const float offs[5][2] = {{1, 0}, {0, 1}, {0, 0}, {-1, 0}, {0, -1}};
     const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
-    const float b = 0.0035f;
+    const float b = mut(MUT_TMP_EDGE) ? 0.001f : 0.0035f;
 #pragma omp parallel for schedule(static)
     for (int py = 0; py < H; py++) {
         for (int px = 0; px < W; px++) {
@@ -791,13 +885,13 @@ OAPI void oracle_svgf_temporal(const float* inv_view, const float* inv_proj, con
             if (dtp < 4.0f) { tol = 0.3f; do_block = true; do_normal = true; }
             else if (dtp < 6.0f) { tol = 0.65f; do_block = true; do_normal = true; }
             else if (dtp < 8.0f) { tol = 0.85f; do_block = true; do_normal = true; }
-            else if (dtp < 16.0f) { tol = 1.414f; do_block = true; do_normal = true; }
+            else if (dtp < 16.0f) { tol = mut(MUT_TMP_TOL_8_16) ? 2.0f : 1.414f; do_block = true; do_normal = true; }
             else if (dtp < 32.0f) { tol = 2.4f; do_block = true; do_normal = true; }
-            else if (dtp < 48.0f) { tol = 3.5f; do_block = false; do_normal = true; }
-            else if (dtp < 64.0f) { tol = 4.2f; do_block = false; do_normal = true; }
-            else if (dtp < 96.0f) { tol = 6.25f; do_block = false; do_normal = false; }
+            else if (dtp < 48.0f) { tol = 3.5f; do_block = mut(MUT_TMP_BLOCK_32); do_normal = true; }
+            else if (dtp < 64.0f) { tol = mut(MUT_TMP_TOL_48_64) ? 3.5f : 4.2f; do_block = false; do_normal = true; }
+            else if (dtp < 96.0f) { tol = 6.25f; do_block = false; do_normal = mut(MUT_TMP_NORMAL_64); }
             else if (dtp < 128.0f) { tol = 9.0f; do_block = false; do_normal = false; }
-            else if (dtp < 200.0f) { tol = 14.0f; do_block = false; do_normal = false; }
+            else if (dtp < 200.0f || mut(MUT_TMP_LAST_BAND)) { tol = 14.0f; do_block = false; do_normal = false; }
 
             float tw = 0.0f;
             float ssh[4] = {0, 0, 0, 0}, scc[2] = {0, 0}, sao[2] = {0, 0};
@@ -814,7 +908,7 @@ OAPI void oracle_svgf_temporal(const float* inv_view, const float* inv_proj, con
                 float perr = dot3(pd, pd);
                 int sblock = std::min(std::max(nearest_u8(prev_block, W, H, sx, sy), 0), 127);
                 bool valid = false;
-                if (perr < tol && ((pw < 0.0f) == (base_w < 0.0f))) {
+                if (perr < tol && (((pw < 0.0f) == (base_w < 0.0f)) || mut(MUT_TMP_SKY))) {
                     valid = true;
                     if (do_normal && !(pn.x == base_n.x && pn.y == base_n.y && pn.z == base_n.z)) valid = false;
                     if (do_block && base_block != sblock) valid = false;
@@ -846,11 +940,13 @@ OAPI void oracle_svgf_temporal(const float* inv_view, const float* inv_proj, con
             float increment = 1.0f;
             float spp_inc = sspp + increment;
             if (successful <= 0) spp_inc = 0.01f;
-            float blend = fmaxf(1.0f / spp_inc, 0.05f);
-            float moment_factor = fmaxf(1.0f / spp_inc, 0.05f);
+            const float blend_floor = mut(MUT_TMP_FLOOR) ? 0.1f : 0.05f;
+            float blend = fmaxf(1.0f / spp_inc, blend_floor);
+            float moment_factor = fmaxf(1.0f / spp_inc, blend_floor);
             float util_spp = spp_inc;
             if (successful <= 0) util_spp = 0.0f;
-            float util_moment = (1.0f - moment_factor) * smom + moment_factor * (base_lum * base_lum);
+            float util_moment = (1.0f - moment_factor) * smom +
+                                moment_factor * (mut(MUT_TMP_MOMENT) ? base_lum : base_lum * base_lum);
             float store_luma = slum * (1.0f - blend) + base_lum * blend;   /* mix(x, y, a) */
             float osh[4], occ[2], oao[2];
             for (int c = 0; c < 4; c++) osh[c] = ssh[c] * (1.0f - blend) + base_sh[c] * blend;
@@ -861,9 +957,9 @@ OAPI void oracle_svgf_temporal(const float* inv_view, const float* inv_proj, con
                 for (int c = 0; c < 2; c++) { occ[c] = base_cocg[c]; oao[c] = base_ao[c]; }
             }
             for (int c = 0; c < 4; c++) o_sh[4 * i + c] = r16(fclamp(osh[c], -100.0f, 100.0f));
-            for (int c = 0; c < 2; c++) o_cocg[2 * i + c] = r16(fclamp(occ[c], -10.0f, 100.0f));
+            for (int c = 0; c < 2; c++) o_cocg[2 * i + c] = r16(fclamp(occ[c], mut(MUT_TMP_CLAMP_COCG) ? -100.0f : -10.0f, 100.0f));
             for (int c = 0; c < 2; c++) o_ao[2 * i + c] = r8(fclamp(oao[c], 0.0f, 1.0f));
-            o_util[3 * i] = r16(fclamp(util_spp, -150.0f, 150.0f));
+            o_util[3 * i] = r16(fclamp(util_spp, -150.0f, mut(MUT_TMP_SPP_CLAMP) ? 255.0f : 150.0f));
             o_util[3 * i + 1] = r16(fclamp(util_moment, -150.0f, 150.0f));
             o_util[3 * i + 2] = r16(fclamp(store_luma, -150.0f, 150.0f));
         }
@@ -878,8 +974,9 @@ OAPI void oracle_svgf_variance(const float* inv_view, const float* inv_proj, int
                                float* o_sh, float* o_cocg, float* o_var)
 {
     const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
-    const float frame_bias = 4.0f;
+    const float frame_bias = mut(MUT_VAR_THRESH) ? 0.0f : 4.0f;
     const float thresh = 4.0f + 4.0f + frame_bias;
+    const int reach = mut(MUT_VAR_WINDOW) ? 3 : 4;
 #pragma omp parallel for schedule(dynamic, 8)
     for (int py = 0; py < H; py++) {
         for (int px = 0; px < W; px++) {
@@ -895,11 +992,11 @@ OAPI void oracle_svgf_variance(const float* inv_view, const float* inv_proj, int
             float osh[4], occ[2];
             float variance;
             if (frames < thresh) {
-                const float color_phi = 5.0f;
+                const float color_phi = mut(MUT_VAR_PHI) ? 4.0f : 5.0f;
                 float tw = 0.0f, tmom = 0.0f, tlum = 0.0f, tw2 = 0.0f;
                 float tsh[4] = {0, 0, 0, 0}, tcc[2] = {0, 0};
-                for (int x = -4; x <= 4; x++) {
-                    for (int y = -4; y <= 4; y++) {
+                for (int x = -reach; x <= reach; x++) {
+                    for (int y = -reach; y <= reach; y++) {
                         float sx = uvx + (float)x * tex_w;
                         float sy = uvy + (float)y * tex_h;
                         if (!in_screen(sx, sy)) continue;
@@ -909,9 +1006,9 @@ OAPI void oracle_svgf_variance(const float* inv_view, const float* inv_proj, int
                         float smom = util[3 * j + 1];
                         float s_sh[4] = {sh[4 * j], sh[4 * j + 1], sh[4 * j + 2], sh[4 * j + 3]};
                         float slum = sh_to_y(s_sh[3]);
-                        float nw = vpt_powi(fmaxf(dot3(base_n, sn), 0.0f), 16);
+                        float nw = mut(MUT_VAR_NW) ? 1.0f : vpt_powi(fmaxf(dot3(base_n, sn), 0.0f), 16);
                         float e = vpt_expf(-fabsf(sw - base_w));
-                        float dw = e * e;
+                        float dw = mut(MUT_VAR_DEPTH) ? e : e * e;
                         float lw = fabsf(slum - base_lum) / color_phi;
                         float w = (vpt_expf(-lw) * nw) * dw;
                         float w2 = w;
@@ -936,17 +1033,17 @@ OAPI void oracle_svgf_variance(const float* inv_view, const float* inv_proj, int
                 occ[0] = tcc[0]; occ[1] = tcc[1];
                 float acc = tlum * tlum;
                 variance = tmom - acc;
-                variance = variance * 3.0f;
+                variance = variance * (mut(MUT_VAR_FACTOR3) ? 1.0f : 3.0f);
             } else {
                 for (int k = 0; k < 4; k++) osh[k] = base_sh[k];
                 occ[0] = base_cocg[0]; occ[1] = base_cocg[1];
                 variance = base_moment - base_lum * base_lum;
             }
-            variance = variance * (thresh / frames);
+            if (!mut(MUT_VAR_SCALE)) variance = variance * (thresh / frames);
             for (int k = 0; k < 4; k++) o_sh[4 * i + k] = r16(fclamp(osh[k], -100.0f, 100.0f));
             o_cocg[2 * i] = r16(fclamp(occ[0], -10.0f, 100.0f));
             o_cocg[2 * i + 1] = r16(fclamp(occ[1], -10.0f, 100.0f));
-            o_var[i] = r16(fclamp(variance, -1.0f, 50.0f));
+            o_var[i] = r16(fclamp(variance, -1.0f, mut(MUT_VAR_CLAMP) ? 10.0f : 50.0f));
         }
     }
 }
@@ -974,10 +1071,11 @@ OAPI void oracle_svgf_atrous(int W, int H, const float* gb_t, const uint8_t* gb_
                              float* o_sh, float* o_cocg, float* o_var, float* o_ao)
 {
     Tex Tt = {gb_t, W, H, 1}, Tsh = {sh, W, H, 4}, Tcc = {cocg, W, H, 2}, Tv = {var, W, H, 1}, Tao = {ao, W, H, 2};
-    const float atrous[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
-    const float gauss[2] = {0.60283f, 0.198585f};
+    const float atrous[3] = {1.0f, mut(MUT_AT_K1) ? 0.5f : 2.0f / 3.0f, 1.0f / 6.0f};
+    const float gauss[2] = {mut(MUT_AT_GAUSS) ? 0.5f : 0.60283f, mut(MUT_AT_GAUSS) ? 0.25f : 0.198585f};
     const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
-    bool filter_ao = step <= 4;
+    if (mut(MUT_AT_SCALE)) scale = 1.0f;
+    bool filter_ao = step <= (mut(MUT_AT_AO_STEP) ? 8 : 4);
     bool filter_sky = step <= 6;
     filter_sky = filter_sky || filter_ao;
 #pragma omp parallel for schedule(static)
@@ -985,7 +1083,7 @@ OAPI void oracle_svgf_atrous(int W, int H, const float* gb_t, const uint8_t* gb_
         for (int px = 0; px < W; px++) {
             size_t i = (size_t)py * W + px;
             float uvx = texcoord(px, W), uvy = texcoord(py, H);
-            int jit = (int)((gradient_noise(px, py, time) - 0.5f) * ((float)step * 0.8f));
+            int jit = (int)((gradient_noise(px, py, time) - 0.5f) * ((float)step * (mut(MUT_AT_JIT_AMP) ? 1.0f : 0.8f)));
             float base_depth = gb_t[i];
             V3 base_n = face_normal(gb_face[i], 1.0f);
             float base_sh[4] = {sh[4 * i], sh[4 * i + 1], sh[4 * i + 2], sh[4 * i + 3]};
@@ -1016,10 +1114,10 @@ OAPI void oracle_svgf_atrous(int W, int H, const float* gb_t, const uint8_t* gb_
             float taow = 1.0f;
 
             float frames = temporal_util[3 * i];
-            bool strong = frames <= 8.0f && step <= 8;
+            bool strong = frames <= (mut(MUT_AT_STRONG_FRAMES) ? 12.0f : 8.0f) && step <= (mut(MUT_AT_STRONG_STEP) ? 16 : 8);
 
             float curve = 0.0f;
-            if (var_est < 0.01f) curve = 128.0f;
+            if (var_est < 0.01f) curve = mut(MUT_AT_TWEAK) ? 96.0f : 128.0f;
             else if (var_est < 0.025f) curve = 112.0f;
             else if (var_est < 0.05f) curve = 96.0f;
             else if (var_est < 0.075f) curve = 84.0f;
@@ -1033,15 +1131,16 @@ OAPI void oracle_svgf_atrous(int W, int H, const float* gb_t, const uint8_t* gb_
             } else {
                 tweaked = var_est;
             }
-            float phi = sqrtf(fmaxf(0.0f, 0.000001f + tweaked));
-            phi = phi / fmaxf(color_phi_bias, 0.1f);
+            float phi = mut(MUT_AT_LUMA_PHI) ? fmaxf(0.0f, 0.000001f + tweaked) : sqrtf(fmaxf(0.0f, 0.000001f + tweaked));
+            phi = mut(MUT_AT_PHI_BIAS) ? phi * fmaxf(color_phi_bias, 0.1f) : phi / fmaxf(color_phi_bias, 0.1f);
 
-            if (!(base_depth < 0.0f)) {   /* (BaseDepth < 0) == (DepthDiff < 0), DepthDiff = abs(..) >= 0 */
+            if (!(base_depth < 0.0f) || mut(MUT_AT_SKY)) {   /* (BaseDepth < 0) == (DepthDiff < 0), DepthDiff = abs(..) >= 0 */
                 for (int x = -1; x <= 1; x++) {
                     for (int y = -1; y <= 1; y++) {
                         if (x == 0 && y == 0) continue;
-                        float ox = ((float)x * (float)step) * scale + (float)jit * 0.5f;
-                        float oy = ((float)y * (float)step) * scale + (float)jit * 0.5f;
+                        const float jh = mut(MUT_AT_JIT_HALF) ? 1.0f : 0.5f;
+                        float ox = ((float)x * (float)step) * scale + (float)jit * jh;
+                        float oy = ((float)y * (float)step) * scale + (float)jit * jh;
                         float sx = uvx + ox * tex_w;
                         float sy = uvy + oy * tex_h;
                         if (!in_screen(sx, sy)) continue;
@@ -1054,19 +1153,19 @@ OAPI void oracle_svgf_atrous(int W, int H, const float* gb_t, const uint8_t* gb_
                         float slum = sh_to_y(s_sh[3]);
                         float svar = bilinear(Tv, sx, sy, 0);
                         float nw = vpt_powi(fmaxf(dot3(base_n, sn), 0.0f), 32);
-                        nw = fclamp(nw, 0.001f, 1.0f);
+                        nw = fclamp(nw, mut(MUT_AT_NW_CLAMP) ? 0.01f : 0.001f, 1.0f);
                         float lw = fabsf(slum - base_lum) / phi;
                         float e = vpt_expf(-fmaxf(ddiff, 0.00001f));
-                        float dw = fclamp(e * e, 0.0001f, 1.0f);
+                        float dw = fclamp(mut(MUT_AT_DEPTH) ? e : e * e, mut(MUT_AT_DEPTH_CLAMP) ? 0.001f : 0.0001f, 1.0f);
                         float w = strong ? (nw * dw) : ((vpt_expf(-lw) * nw) * dw);
-                        w = fclamp(w, 0.001f, 1.0f);
+                        w = fclamp(w, mut(MUT_AT_W_CLAMP) ? 0.01f : 0.001f, 1.0f);
                         float xw = atrous[abs(x)], yw = atrous[abs(y)];
                         w = (xw * yw) * w;
                         w = fmaxf(w, 0.00000001f);
                         for (int c = 0; c < 4; c++) tsh[c] = tsh[c] + s_sh[c] * w;
                         tcc[0] = tcc[0] + s_cc0 * w;
                         tcc[1] = tcc[1] + s_cc1 * w;
-                        tvar = tvar + (w * w) * svar;
+                        tvar = tvar + (mut(MUT_AT_VAR_PROP) ? w : w * w) * svar;
                         tw = tw + w;
                         if (filter_sky || filter_ao) {
                             float aow = fclamp(((xw * yw) * nw) * dw, 0.000001f, 1.0f);
@@ -1428,43 +1527,44 @@ OAPI void oracle_shadow_filter(const float* inv_view, const float* inv_proj, int
             const size_t i = (size_t)py * W + px;
             const float uvx = texcoord(px, W), uvy = texcoord(py, H);
             const float fr = vpt_f16_to_f32(frames[i]);
-            const bool apply_luma = fr > 7.5f;
+            const bool apply_luma = fr > (mut(MUT_SF_LUMA_SWITCH) ? 4.0f : 7.5f);
             const float dist = gb_t[i];
             const bool sky = dist < 0.0f;
             const V3 cn = face_normal(gb_face[i], 1.0f);
             const float cs = cf[i];
             const float T = vpt_f16_to_f32(transversal[i]) * 100.0f;
             const float cutoff = sqrtf(2.0f);
-            if ((T > 0.0f && T < cutoff) || sky) {
+            if (((T > 0.0f && T < cutoff) && !mut(MUT_SF_PASS)) || sky) {
                 out[i] = vpt_unorm8_encode(cs);
                 continue;
             }
-            const int size = T < cutoff * 1.414f ? 1 : 3;
+            const int size = T < cutoff * (mut(MUT_SF_SIZE) ? 1.0f : 1.414f) ? 1 : 3;
             float sc = 1.0f;
             if (T > 6.0f) sc = 2.0f;
-            if (T > 16.0f) sc = 2.4f;
+            if (T > 16.0f) sc = mut(MUT_SF_SC24) ? 2.0f : 2.4f;
             if (T > 32.0f) sc = 2.6f;
             const float ct = fclamp(T, 0.0f, 10.0f);
             const float a = ct / 10.0f;
-            float ve = (20.0f * (1.0f - a) + 6.0f * a) + (T < 6.0f ? 5.0f : 2.0f);
+            float ve = (20.0f * (1.0f - a) + 6.0f * a) + (T < 6.0f ? (mut(MUT_SF_VE) ? 2.0f : 5.0f) : (mut(MUT_SF_VE) ? 5.0f : 2.0f));
             ve = fclamp(ve - 1.75f, 0.0000001f, 64.0f);
             float lm = 1.0f;
             if (!apply_luma) {
                 const float fn = fr / 7.5f;
-                lm = 0.1f * (1.0f - fn) + 0.5f * fn;
+                lm = (mut(MUT_SF_LM) ? 0.2f : 0.1f) * (1.0f - fn) + 0.5f * fn;
             }
-            const float expo = (ve * lm) * 0.9f;
+            const float expo = (ve * lm) * (mut(MUT_SF_EXPO) ? 1.0f : 0.9f);
             float tw = 0.0f, ts = 0.0f;
             for (int x = -size; x <= size; x++) {
                 for (int y = -size; y <= size; y++) {
-                    const float sx = uvx + (((float)x * tex_w) * 1.2f * sc) * scale;
-                    const float sy = uvy + (((float)y * tex_h) * 1.2f * sc) * scale;
+                    const float sp = mut(MUT_SF_SPACING) ? 1.0f : 1.2f;
+                    const float sx = uvx + (((float)x * tex_w) * sp * sc) * scale;
+                    const float sy = uvy + (((float)y * tex_h) * sp * sc) * scale;
                     const float sd = bilinear(Tt, sx, sy, 0);
                     const V3 sn = face_normal(nearest_u8(gb_face, W, H, sx, sy), 1.0f);
-                    const float dw = vpt_powi(vpt_expf(-fabsf(dist - sd)), 3);
+                    const float dw = vpt_powi(vpt_expf(-fabsf(dist - sd)), mut(MUT_SF_DEPTH) ? 2 : 3);
                     const float nw = vpt_powi(fmaxf(dot3(cn, sn), 0.000000001f), 32);
                     const float sa = bilinear(Tc, sx, sy, 0);
-                    const float le = fclamp(1.0f - fclamp(fabsf(sa - cs) / 3.0f, 0.0f, 1.0f), 0.0f, 1.0f);
+                    const float le = fclamp(1.0f - fclamp(fabsf(sa - cs) / (mut(MUT_SF_LE) ? 2.0f : 3.0f), 0.0f, 1.0f), 0.0f, 1.0f);
                     float w = 1.0f;
                     w = w * fclamp(vpt_powf(le, expo), 0.0f, 1.0f);
                     w = w * dw;
