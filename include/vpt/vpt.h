@@ -25,6 +25,11 @@
  *     Work is enqueued asynchronously on it; host-pointer outputs are
  *     complete when the call returns.
  *   - One host thread per object at a time; concurrency comes from streams.
+ *     Calls on different streams may share a context and its worlds: the
+ *     scratch a call needs (GI hit records and shadow jobs) is kept per
+ *     stream, and its constants (camera, GI parameters, the albedo table)
+ *     travel with its launches, so no call reads what another stream's call
+ *     writes.  The caller orders its own buffers (G-buffers, outputs).
  *   - Matrices are 4x4 float, column-major (GLSL / glm layout).
  *   - Images are row-major W x H; row 0 is the bottom row (v_TexCoords.y
  *     smallest), pixel centre uv = ((px + 0.5) / W, (py + 0.5) / H).
@@ -148,7 +153,8 @@ VPT_API int vpt_world_upload_device(vpt_world* w, const uint8_t* d_grid, void* s
 
 /* Batched World::SetBlock (World.h:51-70) followed by the single-voxel
  * glTexSubImage3D of World.cpp:367-374.  edits[i] = x | y<<10 | z<<20 and
- * ids[i] the new block id.  Does not rebuild the DF. */
+ * ids[i] the new block id; a voxel named more than once takes its last id,
+ * as sequential SetBlock calls would.  Does not rebuild the DF. */
 VPT_API int vpt_world_set_blocks(vpt_world* w, const uint32_t* edits, const uint8_t* ids, int n, void* stream);
 
 /* World::GenerateDistanceField (World.cpp:69-113):
@@ -276,7 +282,11 @@ VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbu
 
 /* DIFFUSE_TRACE (DiffuseRayTraceFrag.glsl:822-935), 1 spp, 1 bounce with a
  * shadow ray at the bounce hit.  d_rgb: 3 floats per pixel (clamped to
- * [0, 8]).  d_svgf_in (optional): SVGF input texels. */
+ * [0, 8]).  d_svgf_in (optional): SVGF input texels.  The hit records and
+ * shadow jobs between its three kernels live in scratch the context keeps
+ * for `stream` (24 B per pixel, grown on demand; growing it waits for that
+ * scratch's last use).  The albedo table in prm is passed with the launch
+ * and also becomes the context's "last table" for vpt_unet_assemble. */
 VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
                          float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream);
 
@@ -287,8 +297,9 @@ VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer*
  * equal vpt_trace_primary + vpt_trace_shadow + vpt_trace_gi, or, with
  * d_transversal (optional), vpt_trace_shadow2's two outputs in place of
  * vpt_trace_shadow's; the overlap fills the tail of each pass with the
- * other's work.  Calls sharing a context must not overlap (one side stream
- * and the GI scratch per context). */
+ * other's work.  The side stream and its fork / join events are the
+ * context's: calls from one host thread on any streams are ordered
+ * correctly, calls from several host threads on one context are not. */
 VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, const vpt_gi_params* prm, int primary_iter,
                             int shadow_iter, const vpt_gbuffer* gb, uint8_t* d_mask, uint16_t* d_transversal,
                             float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream);
@@ -387,10 +398,11 @@ VPT_API int vpt_frame_autotune(vpt_frame* f, vpt_world* w, const vpt_camera* cam
  * mode VPT_SERVE_PLAIN runs one request at a time with the one-launch
  * kernels (vpt_trace_rays / vpt_trace_gi_rays / vpt_trace_primary +
  * vpt_trace_shadow + vpt_trace_gi) and waits for each.  VPT_SERVE_BATCHED
- * does continuous batching: every launch of the persistent kernel drains
- * the rays of all admitted requests together (a request larger than a launch
- * is split over several), GI bounces continue as their shadow rays in the
- * same lane, up to max_inflight launches are queued on the device at once,
+ * does continuous batching: every launch traces the rays of all admitted
+ * requests together in one kernel (a request larger than a launch is split
+ * over several) — one lane per ray by default, or the persistent refill
+ * kernel when the launch shape's bit 1 is clear (vpt_server_set_launch_cfg)
+ * — GI bounces continue as their shadow rays in the same lane, up to max_inflight launches are queued on the device at once,
  * and requests that arrive meanwhile join the next launch.  Results are
  * bit-identical in both modes and do not depend on what else is in flight.
  *

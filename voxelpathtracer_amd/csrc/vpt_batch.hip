@@ -54,6 +54,7 @@ struct FrameConst {
     GiConst gi;
     int primary_iter;
     int shadow_iter;
+    vpt::AlbedoTable albedo;   // this frame's, uploaded with the other constants
 };
 
 // Where finished rays write.
@@ -441,8 +442,7 @@ __global__ void __launch_bounds__(256) shade_kernel(const FrameConst* __restrict
                                                     const uint8_t* __restrict__ gb_face,
                                                     const float* __restrict__ gi_t,
                                                     const uint16_t* __restrict__ gi_fb,
-                                                    const uint8_t* __restrict__ gi_vis,
-                                                    const float* __restrict__ albedo, float* __restrict__ rgb,
+                                                    const uint8_t* __restrict__ gi_vis, float* __restrict__ rgb,
                                                     vpt_svgf_texel* __restrict__ texels)
 {
     const CamView& cam = fc->cam;
@@ -468,7 +468,8 @@ __global__ void __launch_bounds__(256) shade_kernel(const FrameConst* __restrict
     const int hit_face = (int)(fb & 0xffu), hit_block = (int)(fb >> 8);
     const bool hit = T > 0.0f && hit_block > 0;
     const float shadow_at = hit ? (float)gi_vis[i] : 0.0f;
-    vpt::gi_write_hit_pixel(fc->gi, albedo, dir, T, hit, hit_face, hit_block, shadow_at, rgb + 3 * (size_t)i, tx);
+    vpt::gi_write_hit_pixel(fc->gi, fc->albedo.v, dir, T, hit, hit_face, hit_block, shadow_at, rgb + 3 * (size_t)i,
+                            tx);
 }
 
 struct DeviceGuard {
@@ -761,7 +762,7 @@ struct vpt_frame {
     // captured graph and the buffers it was captured with
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    uintptr_t key[11] = {};
+    uintptr_t key[10] = {};
     // graphs are captured and replayed on an internal stream (the caller's
     // may be the legacy null stream, which cannot be captured), ordered
     // against the caller's stream with events
@@ -871,7 +872,7 @@ static int frame_enqueue(vpt_frame* f, const vpt_launch_cfg& cfg, vpt_world* w, 
                                              f->d_counters + 1, o2, f->d_fc, w->d_counter))
         return rc;
     hipLaunchKernelGGL(shade_kernel, tiles, dim3(256), 0, s, f->d_fc, gb->d_t, gb->d_face, f->d_gi_t, f->d_gi_fb,
-                       f->d_gi_vis, w->ctx->d_albedo, d_rgb, d_tex);
+                       f->d_gi_vis, d_rgb, d_tex);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }
@@ -901,7 +902,7 @@ static int frame_upload_consts(vpt_frame* f, vpt_world* w, const vpt_camera* cam
     fc.gi = vpt::make_gi(prm);
     fc.primary_iter = primary_iter;
     fc.shadow_iter = shadow_iter;
-    if (int rc = vpt::upload_albedo(w->ctx, prm, s)) return rc;
+    fc.albedo = vpt::take_albedo(w->ctx, prm);
     // pageable source: the copy is staged before the call returns
     HIP_TRY(hipMemcpyAsync(f->d_fc, &fc, sizeof(fc), hipMemcpyHostToDevice, s));
     return VPT_OK;
@@ -918,10 +919,10 @@ extern "C" VPT_API int vpt_frame_trace(vpt_frame* f, vpt_world* w, const vpt_cam
     if (int rc = frame_upload_consts(f, w, cam, prm, primary_iter, shadow_iter, s)) return rc;
     if (!use_graph) return frame_enqueue(f, f->cfg, w, gb, d_mask, d_rgb, d_svgf_in, s);
 
-    const uintptr_t key[11] = {(uintptr_t)w,          (uintptr_t)w->d_df,    (uintptr_t)w->d_counter,
+    const uintptr_t key[10] = {(uintptr_t)w,          (uintptr_t)w->d_df,    (uintptr_t)w->d_counter,
                                (uintptr_t)gb->d_t,    (uintptr_t)gb->d_face, (uintptr_t)gb->d_block,
                                (uintptr_t)gb->d_voxel, (uintptr_t)d_mask,    (uintptr_t)d_rgb,
-                               (uintptr_t)d_svgf_in,  (uintptr_t)w->ctx->d_albedo};
+                               (uintptr_t)d_svgf_in};
     if (!f->cap) {
         HIP_TRY(hipStreamCreateWithFlags(&f->cap, hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&f->ev_in, hipEventDisableTiming));

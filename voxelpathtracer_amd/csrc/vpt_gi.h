@@ -191,11 +191,18 @@ __device__ inline void gi_write_hit_pixel(const GiConst& g, const float* albedo,
     if (texel) store_texel(texel, sh, luma, fclamp(ao, 0.0f, 1.0f), skyhit ? 1.0f : 0.0f);
 }
 
+// The GI albedo table (128 block ids x RGB).  Kernels receive it by value,
+// so every launch carries the table it was enqueued with, in stream order.
+struct AlbedoTable {
+    float v[128 * 3];
+};
+
 // Host helpers shared by the launchers (vpt_trace.hip).
 WorldView make_view(const vpt_world* w);
 CamView make_cam(const vpt_camera* c);
 GiConst make_gi(const vpt_gi_params* prm);
-int upload_albedo(vpt_ctx* ctx, const vpt_gi_params* prm, hipStream_t s);
+// The call's albedo table; also records it as the context's last table.
+AlbedoTable take_albedo(vpt_ctx* ctx, const vpt_gi_params* prm);
 int check_cam(const vpt_camera* cam, const char* fn);
 int check_gbuffer(const vpt_gbuffer* gb, const vpt_camera* cam, const char* fn, bool need_block);
 int check_iter(int max_iter, const char* fn);

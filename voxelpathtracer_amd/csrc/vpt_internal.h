@@ -5,6 +5,9 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <vector>
+
 #include "vpt/vpt.h"
 
 namespace vpt {
@@ -60,17 +63,42 @@ bool launch_cfg_valid(const vpt_launch_cfg& cfg);
 
 }  // namespace vpt
 
+namespace vpt {
+
+// Scratch of the GI pass for one stream: per-pixel hit records (2 words),
+// shadow-ray jobs (4 words each) and the job counter.  Calls on one stream
+// run in order, so they share it; calls on other streams get their own.
+// `done` is recorded after a call's last kernel and waited for (on the
+// device) before the next call's first, so an entry handed to another
+// stream (eviction, or a stream handle reused after destruction) is never
+// written while earlier work still reads it.
+struct GiScratch {
+    void* stream = nullptr;
+    void* done = nullptr;    // hipEvent_t
+    uint32_t* d_hits = nullptr;
+    uint32_t* d_jobs = nullptr;
+    uint32_t* d_njobs = nullptr;
+    size_t cap = 0;          // pixels
+    uint64_t last_use = 0;
+};
+
+// Most streams a context keeps GI scratch for; the least recently used
+// one is freed beyond that.
+constexpr size_t kMaxGiScratch = 8;
+
+}  // namespace vpt
+
 struct vpt_ctx {
     int device = 0;
-    float* d_albedo = nullptr;       // GI albedo table (128 ids x RGB)
+    // guards the members below: the serving worker thread and the caller's
+    // threads may trace on one context
+    std::mutex mu;
+    // the albedo table of the last GI trace (the learned denoiser's input
+    // assembly reads it); kernels receive their table by value
     float albedo_host[128 * 3] = {};
     bool albedo_valid = false;
-    // GI pass scratch, grown on demand: per-pixel hit records (2 words),
-    // shadow-ray jobs (4 words each) and the job counter
-    uint32_t* d_gi_hits = nullptr;
-    uint32_t* d_gi_jobs = nullptr;
-    uint32_t* d_gi_njobs = nullptr;
-    size_t gi_cap = 0;   // pixels
+    std::vector<vpt::GiScratch> gi_scratch;
+    uint64_t gi_clock = 0;
     // vpt_trace_frame: the stream the shadow pass runs on beside GI, and the
     // fork / join events (created on first use, default priority)
     void* side_stream = nullptr;

@@ -9,10 +9,11 @@
 // Plain mode: one request at a time through the one-launch kernels, each
 // waited for before the next starts.
 //
-// Batched mode (continuous batching): each launch of the persistent kernel
-// takes rays from every admitted request at once (a fair share each, then
-// whatever room is left in admission order; a request larger than a launch
-// spans several).  Up to `max_inflight` launches are queued on the device;
+// Batched mode (continuous batching): each launch takes rays from every
+// admitted request at once (a fair share each, then whatever room is left in
+// admission order; a request larger than a launch spans several) and traces
+// them in one kernel: one lane per ray (serve_direct_kernel, the default) or
+// the persistent refill kernel (launch shape bit 1 clear).  Up to `max_inflight` launches are queued on the device;
 // while they run the worker admits new arrivals and builds the next launch,
 // so requests that arrive mid-launch join the following one.  Frame requests
 // run through the persistent frame chain after the launch's ray kernel.
@@ -307,8 +308,17 @@ struct vpt_server {
             if (q->scheduled >= q->units) continue;
             if (q->r.kind == VPT_REQ_FRAME) {
                 if (frame_taken) continue;   // one frame per launch
+                const auto fit = frames.find(frame_key(slot, q->cam));
+                if (fit == frames.end()) {
+                    // cannot happen (submit creates every slot's scratch or
+                    // refuses the request); fail the request, not the process
+                    q->scheduled = q->units;
+                    q->completed = q->units;
+                    complete(q, set_error(VPT_ESTATE, "vpt_server: no frame scratch for slot %d", slot));
+                    return build(L);
+                }
                 frame_taken = true;
-                L.parts.push_back(Part{q, 0, q->units, frames.at(frame_key(slot, q->cam)), worlds[q->r.world]});
+                L.parts.push_back(Part{q, 0, q->units, fit->second, worlds[q->r.world]});
                 q->scheduled = q->units;
                 q->res.t_start_ns = clock_ns();
             } else {
@@ -722,14 +732,19 @@ extern "C" VPT_API int vpt_server_submit(vpt_server* s, const vpt_request* req, 
             delete q;
             return set_error(VPT_ENOMEM, "vpt_server_submit: frame scratch exceeds the world budget");
         }
+        // all slots or none: a partial set would leave later requests of this
+        // size without scratch in some slot
+        std::vector<vpt_frame*> made;
         for (int slot = 0; slot < s->cfg.max_inflight; slot++) {
             vpt_frame* f = nullptr;
             if (int rc = vpt_frame_create(s->ctx, q->cam.width, q->cam.height, &f)) {
+                for (vpt_frame* m : made) vpt_frame_destroy(m);
                 delete q;
                 return rc;
             }
-            s->frames[vpt_server::frame_key(slot, q->cam)] = f;
+            made.push_back(f);
         }
+        for (int slot = 0; slot < s->cfg.max_inflight; slot++) s->frames[vpt_server::frame_key(slot, q->cam)] = made[slot];
         s->world_bytes += scratch;
         s->st.world_bytes = s->world_bytes;
     }

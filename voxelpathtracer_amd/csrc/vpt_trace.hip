@@ -70,17 +70,55 @@ GiConst make_gi(const vpt_gi_params* prm)
     return c;
 }
 
-// The GI albedo table lives with the context and is re-uploaded only when
-// it changes.
-int upload_albedo(vpt_ctx* ctx, const vpt_gi_params* prm, hipStream_t s)
+AlbedoTable take_albedo(vpt_ctx* ctx, const vpt_gi_params* prm)
 {
-    if (!ctx->d_albedo) HIP_TRY(hipMalloc(&ctx->d_albedo, sizeof(ctx->albedo_host)));
-    if (!ctx->albedo_valid || memcmp(ctx->albedo_host, prm->albedo, sizeof(ctx->albedo_host)) != 0) {
-        memcpy(ctx->albedo_host, prm->albedo, sizeof(ctx->albedo_host));
-        HIP_TRY(hipMemcpyAsync(ctx->d_albedo, ctx->albedo_host, sizeof(ctx->albedo_host), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        ctx->albedo_valid = true;
+    AlbedoTable t;
+    static_assert(sizeof(t.v) == sizeof(prm->albedo), "albedo table size");
+    memcpy(t.v, prm->albedo, sizeof(t.v));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    memcpy(ctx->albedo_host, t.v, sizeof(t.v));
+    ctx->albedo_valid = true;
+    return t;
+}
+
+// The GI scratch of `stream`, grown to `npx` pixels, with `stream` made to
+// wait for the entry's previous use.  Called with ctx->mu held.
+static int gi_scratch_for(vpt_ctx* ctx, hipStream_t stream, size_t npx, GiScratch** out)
+{
+    GiScratch* sc = nullptr;
+    for (GiScratch& e : ctx->gi_scratch)
+        if (e.stream == (void*)stream) sc = &e;
+    if (!sc) {
+        if (ctx->gi_scratch.size() < kMaxGiScratch) {
+            ctx->gi_scratch.emplace_back();
+            sc = &ctx->gi_scratch.back();
+        } else {
+            sc = &ctx->gi_scratch[0];
+            for (GiScratch& e : ctx->gi_scratch)
+                if (e.last_use < sc->last_use) sc = &e;
+        }
+        sc->stream = (void*)stream;
     }
+    if (!sc->done) {
+        hipEvent_t ev = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        sc->done = ev;
+    }
+    if (sc->cap < npx) {
+        // the old buffers may still be read by the entry's last use
+        HIP_TRY(hipEventSynchronize((hipEvent_t)sc->done));
+        (void)hipFree(sc->d_hits);
+        (void)hipFree(sc->d_jobs);
+        sc->d_hits = sc->d_jobs = nullptr;
+        sc->cap = 0;
+        HIP_TRY(hipMalloc(&sc->d_hits, npx * 2 * sizeof(uint32_t)));
+        HIP_TRY(hipMalloc(&sc->d_jobs, npx * 4 * sizeof(uint32_t)));
+        if (!sc->d_njobs) HIP_TRY(hipMalloc(&sc->d_njobs, sizeof(uint32_t)));
+        sc->cap = npx;
+    }
+    HIP_TRY(hipStreamWaitEvent(stream, (hipEvent_t)sc->done, 0));
+    sc->last_use = ++ctx->gi_clock;
+    *out = sc;
     return VPT_OK;
 }
 
@@ -391,7 +429,7 @@ __global__ void __launch_bounds__(256) gi_shadow_kernel(WorldView w, GiConst g, 
 
 __global__ void __launch_bounds__(256) gi_shade_kernel(CamView cam, const float* __restrict__ gb_t,
                                                        const uint8_t* __restrict__ gb_face, GiConst g,
-                                                       const uint2* __restrict__ hits, const float* __restrict__ albedo,
+                                                       const uint2* __restrict__ hits, const vpt::AlbedoTable albedo,
                                                        float* __restrict__ rgb, vpt_svgf_texel* __restrict__ texels)
 {
     int px, py;
@@ -408,7 +446,7 @@ __global__ void __launch_bounds__(256) gi_shade_kernel(CamView cam, const float*
     Vec3 o, dir;
     vpt::gi_ray(cam, px, py, dist, N, g.frame, o, dir);
     const uint2 rec = hits[i];
-    vpt::gi_write_hit_pixel(g, albedo, dir, __uint_as_float(rec.x), (rec.y >> 16) & 1u, (int)(rec.y & 0xffu),
+    vpt::gi_write_hit_pixel(g, albedo.v, dir, __uint_as_float(rec.x), (rec.y >> 16) & 1u, (int)(rec.y & 0xffu),
                             (int)((rec.y >> 8) & 0xffu), ((rec.y >> 17) & 1u) ? 1.0f : 0.0f, rgb + 3 * i, tx);
 }
 
@@ -790,33 +828,28 @@ extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const v
     if (int rc = vpt::check_iter(prm->shadow_iter, "vpt_trace_gi")) return rc;
     DeviceGuard g(w->device);
     hipStream_t s = (hipStream_t)stream;
-    if (int rc = vpt::upload_albedo(w->ctx, prm, s)) return rc;
     vpt_ctx* ctx = w->ctx;
+    const vpt::AlbedoTable albedo = vpt::take_albedo(ctx, prm);
     const size_t npx = (size_t)cam->width * cam->height;
-    if (ctx->gi_cap < npx) {
-        (void)hipFree(ctx->d_gi_hits);   // hipFree waits for work that may still use them
-        (void)hipFree(ctx->d_gi_jobs);
-        ctx->d_gi_hits = ctx->d_gi_jobs = nullptr;
-        ctx->gi_cap = 0;
-        HIP_TRY(hipMalloc(&ctx->d_gi_hits, npx * 2 * sizeof(uint32_t)));
-        HIP_TRY(hipMalloc(&ctx->d_gi_jobs, npx * sizeof(GiJob)));
-        if (!ctx->d_gi_njobs) HIP_TRY(hipMalloc(&ctx->d_gi_njobs, sizeof(uint32_t)));
-        ctx->gi_cap = npx;
-    }
-    uint2* hits = reinterpret_cast<uint2*>(ctx->d_gi_hits);
-    GiJob* jobs = reinterpret_cast<GiJob*>(ctx->d_gi_jobs);
+    // the scratch stays this stream's until the launches below are enqueued
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    vpt::GiScratch* sc = nullptr;
+    if (int rc = gi_scratch_for(ctx, s, npx, &sc)) return rc;
+    uint2* hits = reinterpret_cast<uint2*>(sc->d_hits);
+    GiJob* jobs = reinterpret_cast<GiJob*>(sc->d_jobs);
     const GiConst gc = vpt::make_gi(prm);
-    hipLaunchKernelGGL(gi_jobs_clear_kernel, dim3(1), dim3(1), 0, s, ctx->d_gi_njobs);
+    hipLaunchKernelGGL(gi_jobs_clear_kernel, dim3(1), dim3(1), 0, s, sc->d_njobs);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(gi_bounce_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w),
-                       vpt::make_cam(cam), gb->d_t, gb->d_face, gc, hits, jobs, ctx->d_gi_njobs, w->d_counter);
+                       vpt::make_cam(cam), gb->d_t, gb->d_face, gc, hits, jobs, sc->d_njobs, w->d_counter);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(gi_shadow_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, vpt::make_view(w), gc,
-                       jobs, ctx->d_gi_njobs, hits, w->d_counter);
+                       jobs, sc->d_njobs, hits, w->d_counter);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(gi_shade_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_cam(cam),
-                       gb->d_t, gb->d_face, gc, hits, ctx->d_albedo, d_rgb, d_svgf_in);
+                       gb->d_t, gb->d_face, gc, hits, albedo, d_rgb, d_svgf_in);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord((hipEvent_t)sc->done, s));
     return VPT_OK;
 }
 

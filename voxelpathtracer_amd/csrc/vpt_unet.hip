@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "vpt_device.h"
+#include "vpt_gi.h"
 #include "vpt_internal.h"
 
 using vpt::set_error;
@@ -588,7 +589,7 @@ __device__ inline void texel_values(const vpt_svgf_texel& t, float v[6])
 // pixels: irradiance of their texel, zero albedo / normal / depth.
 __global__ void assemble_kernel(int npx, const float* __restrict__ gb_t, const uint8_t* __restrict__ gb_face,
                                 const uint8_t* __restrict__ gb_block, const vpt_svgf_texel* __restrict__ tex,
-                                const float* __restrict__ albedo, uint16_t* __restrict__ x)
+                                const vpt::AlbedoTable albedo, uint16_t* __restrict__ x)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npx) return;
@@ -605,9 +606,9 @@ __global__ void assemble_kernel(int npx, const float* __restrict__ gb_t, const u
     const float t = gb_t[i];
     if (t >= 0.0f && face < 6) {
         const int blk = gb_block[i] & 127;
-        o[3] = f2bf(albedo[blk * 3]);
-        o[4] = f2bf(albedo[blk * 3 + 1]);
-        o[5] = f2bf(albedo[blk * 3 + 2]);
+        o[3] = f2bf(albedo.v[blk * 3]);
+        o[4] = f2bf(albedo.v[blk * 3 + 1]);
+        o[5] = f2bf(albedo.v[blk * 3 + 2]);
         const vpt::Vec3 nrm = vpt::face_normal(face, 0.0f);
         o[6] = f2bf(nrm.x);
         o[7] = f2bf(nrm.y);
@@ -1119,11 +1120,17 @@ extern "C" VPT_API int vpt_unet_assemble(vpt_unet* u, const vpt_gbuffer* gb, con
 {
     if (!u || !gb || !d_tex || !d_x || !gb->d_t || !gb->d_face || !gb->d_block)
         return set_error(VPT_EINVAL, "vpt_unet_assemble: null argument");
-    if (!u->ctx->albedo_valid) return set_error(VPT_ESTATE, "vpt_unet_assemble: no albedo table yet (trace GI first)");
+    vpt::AlbedoTable albedo;
+    {
+        std::lock_guard<std::mutex> lk(u->ctx->mu);
+        if (!u->ctx->albedo_valid)
+            return set_error(VPT_ESTATE, "vpt_unet_assemble: no albedo table yet (trace GI first)");
+        memcpy(albedo.v, u->ctx->albedo_host, sizeof(albedo.v));
+    }
     DeviceGuard g(u->device);
     const int npx = gb->width * gb->height;
     hipLaunchKernelGGL(assemble_kernel, dim3(blocks_for((size_t)npx)), dim3(256), 0, (hipStream_t)stream, npx,
-                       gb->d_t, gb->d_face, gb->d_block, d_tex, u->ctx->d_albedo, d_x);
+                       gb->d_t, gb->d_face, gb->d_block, d_tex, albedo, d_x);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <unordered_set>
 #include <vector>
 
 #include "vpt_device.h"
@@ -443,10 +444,12 @@ extern "C" VPT_API int vpt_ctx_destroy(vpt_ctx* ctx)
 {
     if (!ctx) return VPT_OK;
     DeviceGuard g(ctx->device);
-    if (ctx->d_albedo) (void)hipFree(ctx->d_albedo);
-    (void)hipFree(ctx->d_gi_hits);
-    (void)hipFree(ctx->d_gi_jobs);
-    (void)hipFree(ctx->d_gi_njobs);
+    for (vpt::GiScratch& sc : ctx->gi_scratch) {
+        if (sc.done) (void)hipEventDestroy((hipEvent_t)sc.done);
+        (void)hipFree(sc.d_hits);
+        (void)hipFree(sc.d_jobs);
+        (void)hipFree(sc.d_njobs);
+    }
     if (ctx->side_stream) (void)hipStreamDestroy((hipStream_t)ctx->side_stream);
     if (ctx->ev_fork) (void)hipEventDestroy((hipEvent_t)ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy((hipEvent_t)ctx->ev_join);
@@ -574,6 +577,24 @@ extern "C" VPT_API int vpt_world_set_blocks(vpt_world* w, const uint32_t* edits,
         int x = (int)(e & 1023u), y = (int)((e >> 10) & 1023u), z = (int)((e >> 20) & 1023u);
         if (x >= w->nx || y >= w->ny || z >= w->nz || (e >> 30))
             return set_error(VPT_EINVAL, "vpt_world_set_blocks: edit %d at (%d,%d,%d) outside the world", i, x, y, z);
+    }
+    // The kernels write all edits at once; a voxel named twice keeps the
+    // last id, as sequential SetBlock calls do.
+    std::vector<uint32_t> uniq_e;
+    std::vector<uint8_t> uniq_i;
+    {
+        std::unordered_set<uint32_t> seen;
+        seen.reserve((size_t)n * 2);
+        for (int i = n - 1; i >= 0; i--)
+            if (seen.insert(edits[i]).second) {
+                uniq_e.push_back(edits[i]);
+                uniq_i.push_back(ids[i]);
+            }
+        if ((int)uniq_e.size() != n) {
+            edits = uniq_e.data();
+            ids = uniq_i.data();
+            n = (int)uniq_e.size();
+        }
     }
     DeviceGuard g(w->device);
     hipStream_t s = as_stream(stream);

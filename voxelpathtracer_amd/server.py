@@ -4,7 +4,9 @@ Requests are ray queries (primary, shadow or GI rays already in device
 memory) or frames (primary + shadow + GI passes of one camera).  The server
 admits them while their result memory fits in its arena and runs them one at
 a time (``mode="plain"``, one launch per request) or continuously batched
-through the persistent kernel (``mode="batched"``).  Results stay in server
+(``mode="batched"``): each launch traces the rays of every admitted request
+in one kernel, one lane per ray by default (the persistent refill kernel is
+a launch-shape option).  Results stay in server
 memory until :meth:`Server.release`; :meth:`Server.fetch` copies them into
 torch tensors.
 """
