@@ -7,6 +7,8 @@ known grid through legacy region files pins the import independently."""
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
+import json
 from pathlib import Path
 
 import numpy as np
@@ -18,6 +20,7 @@ from voxelpathtracer_amd.world import mc_import
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 REF_LIB = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "libmca_ref.so"
+MEDIVAL = Path("/root/reference/Test MC Worlds/Medival")
 
 
 def _write(tmp_path: Path, files: dict) -> Path:
@@ -187,3 +190,31 @@ def test_mutated_files_import_without_error(tmp_path, seed):
             (d / name).write_bytes(bytes(reg))
         grid, stats = mc_import(d, M.DIMS, M.ORIGIN, M.block_lut(), M.NAMES)
         assert stats[0] == len(list(d.iterdir())) and stats[1] >= 0 and stats[2] >= 0
+
+
+@pytest.mark.skipif(not MEDIVAL.exists(), reason="the reference's test world is read where the reference is present")
+def test_reference_test_world():
+    """The reference's own test world (Test MC Worlds/Medival, pre-1.13
+    chunks): at its stated origin 136 0 -94 and at origins whose window
+    covers each region file's saved chunks, the imported grid has the
+    digest, solid count and section count the reference's reader gave
+    (tests/golden/mca_medival.json), and, where that reader is built,
+    equals it live.  At the stated origin the grid is empty: the files hold
+    a ring of chunks around the build, none inside the 384 x 384 window."""
+    from tests.golden.make_mca_golden import MEDIVAL_LUTS, reference_import, reference_names
+    golden = json.loads((GOLDEN / "mca_medival.json").read_text())
+    assert sorted(p.name for p in MEDIVAL.glob("*.mca")) == golden["files"]
+    ref = C.CDLL(str(REF_LIB)) if REF_LIB.exists() else None
+    files = {p.name: p.read_bytes() for p in sorted(MEDIVAL.glob("*.mca"))}
+    for case in golden["cases"]:
+        lut = MEDIVAL_LUTS[case["lut"]]()
+        got, stats = mc_import(MEDIVAL, (384, 128, 384), tuple(map(float, case["origin"])), lut, None)
+        assert stats[0] == 7 and stats[2] == case["sections"], (case, stats)
+        assert int((got > 0).sum()) == case["solid"], case
+        assert hashlib.sha256(got.tobytes()).hexdigest() == case["sha256"], case
+        if ref is not None:
+            want, sections = reference_import(ref, files, (384, 128, 384), case["origin"], lut)
+            np.testing.assert_array_equal(got, want)
+    if ref is not None:
+        assert len(reference_names(ref)) > 700
+    assert sum(c["solid"] > 0 for c in golden["cases"]) == len(golden["cases"]) - 2

@@ -1,3 +1,27 @@
+// This file is an altered C++ port of parts of enkiMI (the region and NBT
+// chunk reader the reference vendors as Dependencies/enkiMI/enkimi.c):
+// rewritten with bounds checks, the system zlib, and tag names matched
+// without rewriting the stream.  It is not the original software.  enkiMI's
+// licence notice follows, as its terms require:
+//
+//   Copyright (c) 2017 Juliette Foucaut & Doug Binks
+//
+//   This software is provided 'as-is', without any express or implied
+//   warranty. In no event will the authors be held liable for any damages
+//   arising from the use of this software.
+//
+//   Permission is granted to anyone to use this software for any purpose,
+//   including commercial applications, and to alter it and redistribute it
+//   freely, subject to the following restrictions:
+//
+//   1. The origin of this software must not be misrepresented; you must not
+//      claim that you wrote the original software. If you use this software
+//      in a product, an acknowledgement in the product documentation would be
+//      appreciated but is not required.
+//   2. Altered source versions must be plainly marked as such, and must not be
+//      misrepresented as being the original software.
+//   3. This notice may not be removed or altered from any source distribution.
+//
 // Minecraft world import: MCWorldImporter::ImportWorld (Core/NBT/Importer.cpp:
 // 86-168) over the enkiMI region reader it links (Dependencies/enkiMI/
 // enkimi.c:1535-1638, 1743-1895, 1905-1966, 1998-2406).  Host code only:
