@@ -290,6 +290,19 @@ VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbu
 VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
                          float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream);
 
+/* No counterpart in the reference: vpt_trace_gi for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results,
+ * same kernels; the hit records and shadow jobs live in a fixed array of
+ * 48 MiB + 256 B that each GPU the library runs on holds from context
+ * creation until the process ends, so images are limited to 2,097,152
+ * pixels (1920 x 1080 fits; larger is VPT_EINVAL).  It calls no hipMalloc
+ * or hipFree.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other contexts, other threads) race; order
+ * them on one stream or wait in between. */
+VPT_API int vpt_trace_gi_noalloc(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                 const vpt_gi_params* prm, float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream);
+
 /* The three trace passes of one frame, one launch each (INITIAL_TRACE,
  * SHADOW_TRACE, DIFFUSE_TRACE, Pipeline.cpp:1963-2852): the primary pass on
  * `stream` fills gb, then the shadow pass runs on a stream of the context

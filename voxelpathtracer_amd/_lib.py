@@ -139,6 +139,8 @@ _PROTOS = {
     "vpt_trace_reflect": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(ReflectParams), _vp, _vp, _vp]),
     "vpt_reflect_params_default": (_i, [C.POINTER(_f), C.c_uint32, C.POINTER(ReflectParams)]),
     "vpt_trace_gi": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(GiParams), _vp, _vp, _vp]),
+    "vpt_trace_gi_noalloc": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(GiParams), _vp, _vp,
+                                  _vp]),
     "vpt_trace_frame": (_i, [_vp, C.POINTER(Camera), C.POINTER(GiParams), _i, _i, C.POINTER(GBuffer), _vp, _vp, _vp,
                              _vp, _vp]),
     "vpt_trace_rays": (_i, [_vp, _vp, _i, _i, _vp, _vp]),

@@ -60,6 +60,9 @@ int serve_launch(int cus, const vpt_launch_cfg& cfg, const void* d_worlds, uint3
 int device_cus(int device);
 // Whether `cfg` is a launch shape the persistent kernels accept.
 bool launch_cfg_valid(const vpt_launch_cfg& cfg);
+// Looks up, with `device` current, the device address of the fixed scratch
+// array of vpt_trace_gi_noalloc on that device (once per device).
+int gi_noalloc_init(int device);
 
 }  // namespace vpt
 

@@ -818,6 +818,28 @@ extern "C" VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, cons
     return rc_shadow ? rc_shadow : rc_gi;
 }
 
+// The GI pass's three kernels over the given scratch (the job counter is
+// cleared by a kernel, so the sequence can be captured in a graph).
+static int gi_enqueue(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
+                      const vpt::AlbedoTable& albedo, float* d_rgb, vpt_svgf_texel* d_svgf_in, uint2* hits,
+                      GiJob* jobs, uint32_t* njobs, hipStream_t s)
+{
+    const size_t npx = (size_t)cam->width * cam->height;
+    const GiConst gc = vpt::make_gi(prm);
+    hipLaunchKernelGGL(gi_jobs_clear_kernel, dim3(1), dim3(1), 0, s, njobs);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(gi_bounce_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w),
+                       vpt::make_cam(cam), gb->d_t, gb->d_face, gc, hits, jobs, njobs, w->d_counter);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(gi_shadow_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, vpt::make_view(w), gc,
+                       jobs, njobs, hits, w->d_counter);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(gi_shade_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_cam(cam),
+                       gb->d_t, gb->d_face, gc, hits, albedo, d_rgb, d_svgf_in);
+    HIP_TRY(hipGetLastError());
+    return VPT_OK;
+}
+
 extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
                                     const vpt_gi_params* prm, float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream)
 {
@@ -835,22 +857,91 @@ extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const v
     std::lock_guard<std::mutex> lk(ctx->mu);
     vpt::GiScratch* sc = nullptr;
     if (int rc = gi_scratch_for(ctx, s, npx, &sc)) return rc;
-    uint2* hits = reinterpret_cast<uint2*>(sc->d_hits);
-    GiJob* jobs = reinterpret_cast<GiJob*>(sc->d_jobs);
-    const GiConst gc = vpt::make_gi(prm);
-    hipLaunchKernelGGL(gi_jobs_clear_kernel, dim3(1), dim3(1), 0, s, sc->d_njobs);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(gi_bounce_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w),
-                       vpt::make_cam(cam), gb->d_t, gb->d_face, gc, hits, jobs, sc->d_njobs, w->d_counter);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(gi_shadow_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, vpt::make_view(w), gc,
-                       jobs, sc->d_njobs, hits, w->d_counter);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(gi_shade_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_cam(cam),
-                       gb->d_t, gb->d_face, gc, hits, albedo, d_rgb, d_svgf_in);
-    HIP_TRY(hipGetLastError());
+    if (int rc = gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<uint2*>(sc->d_hits),
+                            reinterpret_cast<GiJob*>(sc->d_jobs), sc->d_njobs, s))
+        return rc;
     HIP_TRY(hipEventRecord((hipEvent_t)sc->done, s));
     return VPT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free GI pass (vpt_trace_gi_noalloc): the same kernels over a
+// fixed scratch array instead of the context's per-stream scratch.
+// Layout: the job counter in the first 256 bytes, then the hit records
+// (8 B per pixel), then the shadow jobs (16 B per pixel), for images of at
+// most kNoAllocGiMaxPixels pixels: 48 MiB + 256 B.  Each GPU the library
+// runs on holds its own copy until the process ends.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr size_t kNoAllocGiMaxPixels = (size_t)1 << 21;   // 2,097,152: 1920 x 1080 fits
+constexpr size_t kNoAllocGiHitsOff = 256;
+constexpr size_t kNoAllocGiJobsOff = kNoAllocGiHitsOff + kNoAllocGiMaxPixels * sizeof(uint2);
+constexpr size_t kNoAllocGiBytes = kNoAllocGiJobsOff + kNoAllocGiMaxPixels * sizeof(GiJob);
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocGiBytes];
+
+// The device address of `scratch` on each GPU (hipGetSymbolAddress with
+// that GPU current).  In host code the array's name is only a host-side
+// stand-in, never passed to a kernel.
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+}  // namespace
+
+namespace vpt {
+
+int gi_noalloc_init(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices)
+        return set_error(VPT_EINVAL, "vpt_trace_gi_noalloc: device %d beyond %d", device, kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return VPT_OK;
+    int cur = -1;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) return set_error(VPT_EINVAL, "gi_noalloc_init: device %d is not current (%d)", device, cur);
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_error(VPT_EDEVICE, "gi_noalloc_init: no address for the scratch array on device %d", device);
+    g_scratch_ptr[device] = p;
+    return VPT_OK;
+}
+
+}  // namespace vpt
+
+extern "C" VPT_API int vpt_trace_gi_noalloc(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                            const vpt_gi_params* prm, float* d_rgb, vpt_svgf_texel* d_svgf_in,
+                                            void* stream)
+{
+    if (!w || !prm || !d_rgb) return set_error(VPT_EINVAL, "vpt_trace_gi_noalloc: null argument");
+    if (int rc = vpt::check_cam(cam, "vpt_trace_gi_noalloc")) return rc;
+    if (int rc = vpt::check_gbuffer(gb, cam, "vpt_trace_gi_noalloc", false)) return rc;
+    if (int rc = vpt::check_iter(prm->gi_iter, "vpt_trace_gi_noalloc")) return rc;
+    if (int rc = vpt::check_iter(prm->shadow_iter, "vpt_trace_gi_noalloc")) return rc;
+    const size_t npx = (size_t)cam->width * cam->height;
+    if (npx > kNoAllocGiMaxPixels)
+        return set_error(VPT_EINVAL, "vpt_trace_gi_noalloc: %dx%d = %zu pixels exceeds its fixed scratch (%zu pixels)",
+                         cam->width, cam->height, npx, kNoAllocGiMaxPixels);
+    if (w->device < 0 || w->device >= kNoAllocMaxDevices)
+        return set_error(VPT_EINVAL, "vpt_trace_gi_noalloc: device %d beyond %d", w->device, kNoAllocMaxDevices);
+    DeviceGuard g(w->device);
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = g_scratch_ptr[w->device];
+    }
+    if (!base) {   // a context created on this device sets it up; this is the fallback
+        if (int rc = vpt::gi_noalloc_init(w->device)) return rc;
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = g_scratch_ptr[w->device];
+    }
+    uint8_t* b = static_cast<uint8_t*>(base);
+    const vpt::AlbedoTable albedo = vpt::take_albedo(w->ctx, prm);
+    return gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<uint2*>(b + kNoAllocGiHitsOff),
+                      reinterpret_cast<GiJob*>(b + kNoAllocGiJobsOff), reinterpret_cast<uint32_t*>(b),
+                      (hipStream_t)stream);
 }
 
 extern "C" VPT_API int vpt_trace_rays(vpt_world* w, const vpt_ray* d_rays, int n, int max_iter, vpt_hit* d_hits,

@@ -434,6 +434,7 @@ extern "C" VPT_API int vpt_ctx_create(int device, vpt_ctx** out)
         return set_error(VPT_EINVAL, "vpt_ctx_create: device %d out of range (%d devices)", device, count);
     DeviceGuard g(device);
     HIP_TRY(hipFree(nullptr));  // initialise the device context
+    if (int rc = vpt::gi_noalloc_init(device)) return rc;
     vpt_ctx* c = new vpt_ctx();
     c->device = device;
     *out = c;
