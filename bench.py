@@ -176,6 +176,29 @@ def main():
     rays_per_step = sum(c["rays"] for c in counts.values())
     grays_plain = rays_per_step / (ms_per_step * 1e-3) / 1e9
     ms_plain = ms_per_step
+    # the same frame captured each step and replayed as a HIP graph
+    # (vpt_graph: capture, in-place update of the instantiated graph, launch)
+    from voxelpathtracer_amd import Graph
+    graph = Graph(ctx)
+
+    def step_graph():
+        with graph.capture() as s:
+            world.trace_frame(cam, gb, prm, mask, rgb, tex, 350, 350, stream=s)
+
+    for _ in range(args.warmup):
+        step_graph()
+    torch.cuda.synchronize()
+    g0, g1 = ev(), ev()
+    g0.record(stream)
+    t_host = time.perf_counter()
+    for _ in range(args.steps):
+        step_graph()
+    host_graph_ms = (time.perf_counter() - t_host) * 1e3 / args.steps
+    g1.record(stream)
+    torch.cuda.synchronize()
+    ms_graph = g0.elapsed_time(g1) / args.steps
+    graph_stats = graph.stats()
+    graph.close()
 
     # ---- persistent frame path (same outputs bit for bit): launch shape
     # autotuned first (untimed), then the pass chain replayed as a HIP graph
@@ -339,6 +362,20 @@ def main():
     torch.cuda.synchronize()
     pipeline_ms = p0.elapsed_time(p1) / 16
     del pipe
+    # the same frames replayed as a HIP graph each (Pipeline(use_graph=True))
+    pipe = Pipeline(ctx, world, W, H, use_graph=True)
+    for f in range(4):
+        pipe.render((CAM_POS[0] + 0.1 * f, CAM_POS[1], CAM_POS[2]), CAM_YAW + 0.2 * f, CAM_PITCH)
+    torch.cuda.synchronize()
+    p0, p1 = ev(), ev()
+    p0.record(stream)
+    for f in range(4, 20):
+        pipe.render((CAM_POS[0] + 0.1 * f, CAM_POS[1], CAM_POS[2]), CAM_YAW + 0.2 * f, CAM_PITCH)
+    p1.record(stream)
+    torch.cuda.synchronize()
+    pipeline_graph_ms = p0.elapsed_time(p1) / 16
+    pipeline_graph_stats = {"side_chain": pipe.graphs[0].stats(), "main_chain": pipe.graphs[1].stats()}
+    del pipe
 
     result = {
         "metric": "Grays/s DDA+GI @1080p (bit-exact hits); SVGF ms/frame; served p50/p99 latency",
@@ -359,6 +396,8 @@ def main():
                    "parallelism": "single GPU"},
         "path": path,
         "plain": {"ms_per_frame": round(ms_plain, 4), "grays": round(grays_plain, 4),
+                  "ms_per_frame_graph": round(ms_graph, 4), "graph_host_ms_per_frame": round(host_graph_ms, 4),
+                  "graph": graph_stats,
                   "ms_per_frame_passes_in_order": round(ms_sequential, 4),
                   "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()}, "work": counts},
         "persistent": persistent,
@@ -369,6 +408,7 @@ def main():
                            round(shadow2_ms, 4), "frames": nf},
         "reflect": reflect,
         "pipeline_ms_per_frame": round(pipeline_ms, 4),
+        "pipeline_graph": {"ms_per_frame": round(pipeline_graph_ms, 4), "graphs": pipeline_graph_stats},
         "roofline": {"bound": "hbm", "kernel": PASS_KERNELS[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": profiled_traffic(f"{dom} pass"), "algorithmic_bytes": alg[dom]},
@@ -590,10 +630,31 @@ def cpu_baseline(grid, gb, mask, rgb, cam, prm, reps):
                     break
     except OSError:
         pass
+    # the CPU plain-method config: DF build + 640x360 pinhole primary rays on
+    # the same world, 1 warm-up + median of 5 (steady_clock inside the oracle)
+    cam640, _, _ = _make_camera_640()
+    plain = oracle.time_plain(grid, np.array(cam640.inv_view, np.float32), np.array(cam640.inv_proj, np.float32),
+                              640, 360, 350, reps=5)
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
     return {"value": round(rays / ray_s / 1e9, 6), "unit": "Grays/s", "cores": threads, "kind": "port",
-            "sample": f"one {W}x{H} frame (primary+shadow+GI, {rays} rays), median of {reps} after 1 warm-up; "
-                      f"DF build {df_s * 1e3:.1f} ms",
-            "df_build_ms": round(df_s * 1e3, 2), "cpu_model": model, "gpu_matches_oracle": parity}
+            "sample": f"one {W}x{H} frame (primary+shadow+GI, {rays} rays traced by the oracle: it also traces the "
+                      f"shadow rays whose origin cell is solid, which the GPU skips as their mask is 1 either way), "
+                      f"median of {reps} after 1 warm-up; DF build {df_s * 1e3:.1f} ms",
+            "df_build_ms": round(df_s * 1e3, 2), "cpu_model": model, "gpu_matches_oracle": parity,
+            "nproc": os.cpu_count(), "affinity_cpus": affinity, "omp_threads": threads,
+            "cpu_plain_method": {"config": "384x128x384 terrain, DF build + 640x360 primary rays (350 steps), "
+                                           "1 warm-up + median of 5",
+                                 "df_build_ms": round(plain["df_ms"], 3), "primary_ms": round(plain["primary_ms"], 3),
+                                 "primary_grays_s": round(640 * 360 / (plain["primary_ms"] * 1e-3) / 1e9, 6),
+                                 "df_samples": int(plain["samples"]), "omp_threads": plain["threads"]}}
+
+
+def _make_camera_640():
+    from voxelpathtracer_amd import make_camera
+    return make_camera(CAM_POS, CAM_YAW, CAM_PITCH, 640, 360)
 
 
 if __name__ == "__main__":

@@ -23,10 +23,17 @@ public:
     // (vpt_frame_trace) instead of one launch per pass (vpt_trace_primary,
     // vpt_trace_gi; the faster path).  Either way the shadow pass with its
     // transversal, the shadow denoise and the reflections run on a side
-    // stream beside GI and SVGF.
-    Pipeline(Context& ctx, World& world, int width, int height, float sun_tick = 50.0f, bool persistent = false)
+    // stream beside GI and SVGF.  use_graph (one launch per pass only): after
+    // the primary pass, the side chain and the main chain are each captured
+    // and replayed as a HIP graph (vpt_graph_*) every frame, the instantiated
+    // graphs updated in place.
+    Pipeline(Context& ctx, World& world, int width, int height, float sun_tick = 50.0f, bool persistent = false,
+             bool use_graph = false)
         : m_World(world), m_W(width), m_H(height), m_SunTick(sun_tick), m_Persistent(persistent)
     {
+        if (use_graph && persistent)
+            throw std::invalid_argument("Pipeline: the persistent engine replays its own graph; use_graph is for "
+                                        "the one-launch passes");
         const size_t px = (size_t)width * height;
         for (auto& g : m_GBuf) {
             Alloc(&g.d_t, px * sizeof(float));
@@ -55,11 +62,17 @@ public:
             hipEventCreateWithFlags(&m_Join, hipEventDisableTiming) != hipSuccess)
             throw std::runtime_error("Pipeline: side stream");
         if (persistent) vpt_check(vpt_frame_create(ctx.Handle(), width, height, &m_Frame));
+        if (use_graph) {
+            vpt_check(vpt_graph_create(ctx.Handle(), &m_Graphs[0]));
+            vpt_check(vpt_graph_create(ctx.Handle(), &m_Graphs[1]));
+        }
     }
 
     ~Pipeline()
     {
         if (m_Side) (void)hipStreamSynchronize(m_Side);
+        for (vpt_graph* g : m_Graphs)
+            if (g) vpt_graph_destroy(g);
         if (m_Frame) vpt_frame_destroy(m_Frame);
         vpt_shadow_denoise_destroy(m_ShadowDn);
         if (m_Side) (void)hipStreamDestroy(m_Side);
@@ -109,23 +122,38 @@ public:
             vpt_check(vpt_trace_primary(m_World.Handle(), &cam, 350, &cur, stream));
         float prev_vp[16];
         ViewProj(m_PrevView, m_PrevProj, prev_vp);
-        // fork: the side stream runs the shadow pass with its transversal,
-        // the shadow denoise and the reflections, which read only the
-        // G-buffer and the shadow pass; GI and SVGF stay on `stream`
-        if (hipEventRecord(m_Fork, s) != hipSuccess || hipStreamWaitEvent(m_Side, m_Fork, 0) != hipSuccess)
-            throw std::runtime_error("Pipeline: fork");
-        vpt_check(vpt_trace_shadow2(m_World.Handle(), &cam, &cur, prm.sun_dir, 350, m_Mask, m_Transversal, m_Side));
-        vpt_check(vpt_shadow_denoise_frame(m_ShadowDn, &cam, prev_vp, &cur, &prev, m_Mask, m_Transversal, 1.0f,
-                                           m_ShadowOut, m_Side));
-        if (m_RW > 0 && m_RH > 0) {
-            vpt_reflect_params rp;
-            vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
-            vpt_check(vpt_trace_reflect(m_World.Handle(), &cam, &cur, &rp, m_ReflDist, m_ReflHit, m_Side));
+        // the side chain (shadow with its transversal, shadow denoise,
+        // reflections) reads only the G-buffer and the shadow pass and runs
+        // beside the main chain (GI, SVGF)
+        auto side_chain = [&](void* ss) {
+            vpt_check(vpt_trace_shadow2(m_World.Handle(), &cam, &cur, prm.sun_dir, 350, m_Mask, m_Transversal, ss));
+            vpt_check(vpt_shadow_denoise_frame(m_ShadowDn, &cam, prev_vp, &cur, &prev, m_Mask, m_Transversal, 1.0f,
+                                               m_ShadowOut, ss));
+            if (m_RW > 0 && m_RH > 0) {
+                vpt_reflect_params rp;
+                vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
+                vpt_check(vpt_trace_reflect(m_World.Handle(), &cam, &cur, &rp, m_ReflDist, m_ReflHit, ss));
+            }
+        };
+        auto main_chain = [&](void* ms) {
+            if (!m_Persistent) vpt_check(vpt_trace_gi(m_World.Handle(), &cam, &cur, &prm, m_RGB, m_Texels, ms));
+            vpt_check(vpt_svgf_frame(m_Svgf, &cam, prev_vp, &cur, &prev, m_Texels, (float)m_Frame_ / 60.0f,
+                                     m_Denoised, nullptr, ms));
+        };
+        if (!m_Graphs[0]) {
+            if (hipEventRecord(m_Fork, s) != hipSuccess || hipStreamWaitEvent(m_Side, m_Fork, 0) != hipSuccess)
+                throw std::runtime_error("Pipeline: fork");
+            side_chain(m_Side);
+            if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
+            main_chain(stream);
+        } else {
+            // each chain is captured and replayed as a graph of its own
+            // (the two replays start after the primary pass and overlap);
+            // the side chain's completion is joined on the side stream
+            Replay(m_Graphs[0], stream, m_Side, side_chain);
+            if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
+            Replay(m_Graphs[1], stream, stream, main_chain);
         }
-        if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
-        if (!m_Persistent) vpt_check(vpt_trace_gi(m_World.Handle(), &cam, &cur, &prm, m_RGB, m_Texels, stream));
-        vpt_check(vpt_svgf_frame(m_Svgf, &cam, prev_vp, &cur, &prev, m_Texels, (float)m_Frame_ / 60.0f, m_Denoised,
-                                 nullptr, stream));
         if (hipStreamWaitEvent(s, m_Join, 0) != hipSuccess) throw std::runtime_error("Pipeline: join");
         for (int i = 0; i < 16; i++) {
             m_PrevView[i] = view[i];
@@ -134,6 +162,24 @@ public:
         ++m_Frame_;
     }
 
+    const vpt_graph* Graph(int chain) const { return m_Graphs[chain]; }
+
+private:
+    template <typename F>
+    static void Replay(vpt_graph* g, void* after, void* join, F&& chain)
+    {
+        void* cap = nullptr;
+        vpt_check(vpt_graph_begin(g, after, &cap));
+        try {
+            chain(cap);
+        } catch (...) {
+            vpt_graph_abort(g);
+            throw;
+        }
+        vpt_check(vpt_graph_launch(g, join));
+    }
+
+public:
     const vpt_gbuffer& GBuffer() const { return m_GBuf[(m_Frame_ + 1) % 2]; }
     const float* DenoisedRGB() const { return m_Denoised; }
     const float* RawRGB() const { return m_RGB; }
@@ -186,6 +232,7 @@ private:
     vpt_svgf_texel* m_Texels = nullptr;
     vpt_svgf* m_Svgf = nullptr;
     vpt_frame* m_Frame = nullptr;
+    vpt_graph* m_Graphs[2] = {nullptr, nullptr};   // side chain, main chain
     float m_PrevView[16] = {}, m_PrevProj[16] = {};
 };
 

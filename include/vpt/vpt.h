@@ -284,8 +284,9 @@ VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbu
  * shadow ray at the bounce hit.  d_rgb: 3 floats per pixel (clamped to
  * [0, 8]).  d_svgf_in (optional): SVGF input texels.  The hit records and
  * shadow jobs between its three kernels live in scratch the context keeps
- * for `stream` (24 B per pixel, grown on demand; growing it waits for that
- * scratch's last use).  The albedo table in prm is passed with the launch
+ * for `stream` (24 B per pixel, grown on demand; outgrown buffers are kept
+ * until the context is destroyed, since a captured graph may still use
+ * them).  The albedo table in prm is passed with the launch
  * and also becomes the context's "last table" for vpt_unet_assemble. */
 VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
                          float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream);
@@ -316,6 +317,37 @@ VPT_API int vpt_trace_gi_noalloc(vpt_world* w, const vpt_camera* cam, const vpt_
 VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, const vpt_gi_params* prm, int primary_iter,
                             int shadow_iter, const vpt_gbuffer* gb, uint8_t* d_mask, uint16_t* d_transversal,
                             float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream);
+
+/* ---- graph replay of a frame's calls ---------------------------------------
+ * No counterpart in the reference (its frame is a sequence of GL draws).
+ * A host brackets one frame's calls of this library with vpt_graph_begin /
+ * vpt_graph_launch: begin starts capturing an internal stream (ordered
+ * after the work already on `stream`) and returns it in *capture_stream;
+ * the host enqueues the frame's calls on that stream (and on other streams
+ * that wait on events recorded in it and are joined back to it); launch
+ * ends the capture, instantiates the graph the first time — later frames
+ * update the instantiated graph's kernel arguments in place when the
+ * sequence has the same shape, and re-instantiate when it does not — and
+ * replays it, with `stream` made to wait for it.  Calls made while
+ * capturing enqueue nothing until the launch.  Every call of this library
+ * used in a frame is capturable (kernels and device-to-device copies only;
+ * vpt_trace_gi may allocate its stream's scratch, which the relaxed capture
+ * mode used here allows).  vpt_graph_abort
+ * ends a capture without launching it (after an error in the frame). */
+typedef struct vpt_graph vpt_graph;
+typedef struct vpt_graph_stats {
+    uint64_t launches;
+    uint64_t instantiations;
+    uint64_t updates;
+    uint32_t nodes;          /* nodes of the last captured graph */
+    uint32_t pad;
+} vpt_graph_stats;
+VPT_API int vpt_graph_create(vpt_ctx* ctx, vpt_graph** out);
+VPT_API int vpt_graph_destroy(vpt_graph* g);
+VPT_API int vpt_graph_begin(vpt_graph* g, void* stream, void** capture_stream);
+VPT_API int vpt_graph_launch(vpt_graph* g, void* stream);
+VPT_API int vpt_graph_abort(vpt_graph* g);
+VPT_API int vpt_graph_stats_get(const vpt_graph* g, vpt_graph_stats* st);
 
 /* Ray queries (serving API): n rays, traversal cap max_iter. */
 VPT_API int vpt_trace_rays(vpt_world* w, const vpt_ray* d_rays, int n, int max_iter, vpt_hit* d_hits, void* stream);

@@ -1,8 +1,9 @@
 """The C++ host façades (include/vpt/World.hpp, Pipeline.hpp) drive the same
 passes as the Python host: the C++ example's denoised frame, denoised sun
 shadow and reflection hit distances equal the Python Pipeline's bit for bit,
-with the C++ pipeline tracing through the persistent + graph engine or one
-launch per pass (the Python pipeline: one launch per pass)."""
+with the C++ pipeline tracing through the persistent + graph engine, one
+launch per pass, or one launch per pass replayed as a HIP graph each frame
+(the Python pipeline: one launch per pass, eager)."""
 from __future__ import annotations
 
 import json
@@ -18,13 +19,13 @@ from voxelpathtracer_amd.world import generate_terrain
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("path", ["persistent", "plain"])
+@pytest.mark.parametrize("path", ["persistent", "plain", "graph"])
 def test_cpp_facade_matches_python_pipeline(ctx, tmp_path, path):
     import torch
-    W, H, frames = 320, 180, 3
+    W, H, frames = 320, 180, 4
     dump = tmp_path / "rgb.bin"
     print(f"running the C++ façade example ({path})", flush=True)
-    extra = ["--plain"] if path == "plain" else []
+    extra = {"plain": ["--plain"], "graph": ["--graph"]}.get(path, [])
     r = subprocess.run([str(DEMO), str(W), str(H), str(frames), *extra, "--dump", str(dump)], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr

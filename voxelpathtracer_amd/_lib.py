@@ -109,6 +109,11 @@ class ServerStats(C.Structure):
                 ("pending", C.c_uint32), ("active", C.c_uint32)]
 
 
+class GraphStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("instantiations", C.c_uint64), ("updates", C.c_uint64),
+                ("nodes", C.c_uint32), ("pad", C.c_uint32)]
+
+
 _vp = C.c_void_p
 _i = C.c_int
 _f = C.c_float
@@ -143,6 +148,12 @@ _PROTOS = {
                                   _vp]),
     "vpt_trace_frame": (_i, [_vp, C.POINTER(Camera), C.POINTER(GiParams), _i, _i, C.POINTER(GBuffer), _vp, _vp, _vp,
                              _vp, _vp]),
+    "vpt_graph_create": (_i, [_vp, C.POINTER(_vp)]),
+    "vpt_graph_destroy": (_i, [_vp]),
+    "vpt_graph_begin": (_i, [_vp, _vp, C.POINTER(_vp)]),
+    "vpt_graph_launch": (_i, [_vp, _vp]),
+    "vpt_graph_abort": (_i, [_vp]),
+    "vpt_graph_stats_get": (_i, [_vp, C.POINTER(GraphStats)]),
     "vpt_trace_rays": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "vpt_trace_gi_rays": (_i, [_vp, _vp, _i, _i, C.POINTER(_f), _i, _vp, _vp, _vp]),
     "vpt_world_set_sample_counter": (_i, [_vp, _vp]),

@@ -6,7 +6,10 @@
 // RGB (float32, H x W x 3), then its denoised shadow (uint8, H x W) and its
 // reflection hit distances (binary16 bits, H/4 x W/4).
 //
-//   vpt_frame_demo [W H FRAMES [--plain] [--dump FILE]]
+//   vpt_frame_demo [W H FRAMES [--plain | --graph] [--dump FILE]]
+//
+// --plain: one launch per pass; --graph: one launch per pass, every frame
+// replayed as a HIP graph; default: the persistent engine.
 #include <hip/hip_runtime_api.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -22,10 +25,14 @@ int main(int argc, char** argv)
     int W = argc > 1 ? atoi(argv[1]) : 640;
     int H = argc > 2 ? atoi(argv[2]) : 360;
     int frames = argc > 3 ? atoi(argv[3]) : 8;
-    bool persistent = true;
+    bool persistent = true, graph = false;
     const char* dump = nullptr;
     for (int i = 4; i < argc; i++) {
         if (!strcmp(argv[i], "--plain")) persistent = false;
+        if (!strcmp(argv[i], "--graph")) {
+            persistent = false;
+            graph = true;
+        }
         if (!strcmp(argv[i], "--dump") && i + 1 < argc) dump = argv[++i];
     }
     try {
@@ -36,7 +43,7 @@ int main(int argc, char** argv)
         world.Buffer();
         world.InitializeDistanceGenerator();
         world.GenerateDistanceField();
-        VoxelRT::Pipeline pipe(ctx, world, W, H, 50.0f, persistent);
+        VoxelRT::Pipeline pipe(ctx, world, W, H, 50.0f, persistent, graph);
         hipEvent_t e0, e1;
         (void)hipEventCreate(&e0);
         (void)hipEventCreate(&e1);
@@ -74,7 +81,8 @@ int main(int argc, char** argv)
         }
         printf("{\"frames\": %d, \"width\": %d, \"height\": %d, \"path\": \"%s\", \"ms_per_frame\": %.4f, "
                "\"denoised_mean\": %.6f, \"pick\": [%d, %d, %d, %d]}\n",
-               frames, W, H, persistent ? "persistent" : "plain", ms / frames, mean, pick[0], pick[1], pick[2],
+               frames, W, H, persistent ? "persistent" : (graph ? "graph" : "plain"), ms / frames, mean, pick[0],
+               pick[1], pick[2],
                pick[3]);
     } catch (const std::exception& e) {
         fprintf(stderr, "error: %s\n", e.what());

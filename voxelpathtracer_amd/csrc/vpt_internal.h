@@ -72,9 +72,11 @@ namespace vpt {
 // shadow-ray jobs (4 words each) and the job counter.  Calls on one stream
 // run in order, so they share it; calls on other streams get their own.
 // `done` is recorded after a call's last kernel and waited for (on the
-// device) before the next call's first, so an entry handed to another
-// stream (eviction, or a stream handle reused after destruction) is never
-// written while earlier work still reads it.
+// device) before the next call's first, so a stream handle reused after its
+// stream was destroyed never writes the scratch while earlier work still
+// reads it.  Buffers an entry outgrows, or an evicted entry's, are retired
+// (kept until the context is destroyed), never freed: a graph captured
+// earlier may still replay with them.
 struct GiScratch {
     void* stream = nullptr;
     void* done = nullptr;    // hipEvent_t
@@ -101,6 +103,7 @@ struct vpt_ctx {
     float albedo_host[128 * 3] = {};
     bool albedo_valid = false;
     std::vector<vpt::GiScratch> gi_scratch;
+    std::vector<void*> gi_retired;
     uint64_t gi_clock = 0;
     // vpt_trace_frame: the stream the shadow pass runs on beside GI, and the
     // fork / join events (created on first use, default priority)

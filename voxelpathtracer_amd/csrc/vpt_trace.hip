@@ -81,9 +81,21 @@ AlbedoTable take_albedo(vpt_ctx* ctx, const vpt_gi_params* prm)
     return t;
 }
 
-// The GI scratch of `stream`, grown to `npx` pixels, with `stream` made to
-// wait for the entry's previous use.  Called with ctx->mu held.
-static int gi_scratch_for(vpt_ctx* ctx, hipStream_t stream, size_t npx, GiScratch** out)
+static void gi_retire(vpt_ctx* ctx, GiScratch* sc)
+{
+    for (void* p : {(void*)sc->d_hits, (void*)sc->d_jobs, (void*)sc->d_njobs})
+        if (p) ctx->gi_retired.push_back(p);
+    sc->d_hits = sc->d_jobs = sc->d_njobs = nullptr;
+    sc->cap = 0;
+}
+
+// The GI scratch of `stream`, at least `npx` pixels, with `stream` made to
+// wait for the entry's previous use.  Called with ctx->mu held.  While
+// `stream` is being captured into a graph the ordering comes from the
+// stream the graph is replayed on instead of the event (a capture cannot
+// wait for an event recorded outside it), and growing allocates (a relaxed
+// capture allows it) without freeing.
+static int gi_scratch_for(vpt_ctx* ctx, hipStream_t stream, size_t npx, bool capturing, GiScratch** out)
 {
     GiScratch* sc = nullptr;
     for (GiScratch& e : ctx->gi_scratch)
@@ -96,6 +108,7 @@ static int gi_scratch_for(vpt_ctx* ctx, hipStream_t stream, size_t npx, GiScratc
             sc = &ctx->gi_scratch[0];
             for (GiScratch& e : ctx->gi_scratch)
                 if (e.last_use < sc->last_use) sc = &e;
+            gi_retire(ctx, sc);   // the evicted stream's graphs may still use them
         }
         sc->stream = (void*)stream;
     }
@@ -105,18 +118,21 @@ static int gi_scratch_for(vpt_ctx* ctx, hipStream_t stream, size_t npx, GiScratc
         sc->done = ev;
     }
     if (sc->cap < npx) {
-        // the old buffers may still be read by the entry's last use
-        HIP_TRY(hipEventSynchronize((hipEvent_t)sc->done));
-        (void)hipFree(sc->d_hits);
-        (void)hipFree(sc->d_jobs);
-        sc->d_hits = sc->d_jobs = nullptr;
-        sc->cap = 0;
-        HIP_TRY(hipMalloc(&sc->d_hits, npx * 2 * sizeof(uint32_t)));
-        HIP_TRY(hipMalloc(&sc->d_jobs, npx * 4 * sizeof(uint32_t)));
-        if (!sc->d_njobs) HIP_TRY(hipMalloc(&sc->d_njobs, sizeof(uint32_t)));
+        gi_retire(ctx, sc);
+        hipError_t e = hipMalloc(&sc->d_hits, npx * 2 * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMalloc(&sc->d_jobs, npx * 4 * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMalloc(&sc->d_njobs, sizeof(uint32_t));
+        if (e != hipSuccess) {
+            gi_retire(ctx, sc);
+            return set_error(capturing ? VPT_ESTATE : VPT_ENOMEM,
+                             "vpt_trace_gi: GI scratch of %zu pixels: %s%s", npx, hipGetErrorString(e),
+                             capturing ? " (this capture mode allows no allocation: run the pass once outside "
+                                         "capture first)"
+                                       : "");
+        }
         sc->cap = npx;
     }
-    HIP_TRY(hipStreamWaitEvent(stream, (hipEvent_t)sc->done, 0));
+    if (!capturing) HIP_TRY(hipStreamWaitEvent(stream, (hipEvent_t)sc->done, 0));
     sc->last_use = ++ctx->gi_clock;
     *out = sc;
     return VPT_OK;
@@ -854,13 +870,16 @@ extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const v
     const vpt::AlbedoTable albedo = vpt::take_albedo(ctx, prm);
     const size_t npx = (size_t)cam->width * cam->height;
     // the scratch stays this stream's until the launches below are enqueued
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(s, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
     std::lock_guard<std::mutex> lk(ctx->mu);
     vpt::GiScratch* sc = nullptr;
-    if (int rc = gi_scratch_for(ctx, s, npx, &sc)) return rc;
+    if (int rc = gi_scratch_for(ctx, s, npx, capturing, &sc)) return rc;
     if (int rc = gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<uint2*>(sc->d_hits),
                             reinterpret_cast<GiJob*>(sc->d_jobs), sc->d_njobs, s))
         return rc;
-    HIP_TRY(hipEventRecord((hipEvent_t)sc->done, s));
+    if (!capturing) HIP_TRY(hipEventRecord((hipEvent_t)sc->done, s));
     return VPT_OK;
 }
 

@@ -451,6 +451,7 @@ extern "C" VPT_API int vpt_ctx_destroy(vpt_ctx* ctx)
         (void)hipFree(sc.d_jobs);
         (void)hipFree(sc.d_njobs);
     }
+    for (void* p : ctx->gi_retired) (void)hipFree(p);
     if (ctx->side_stream) (void)hipStreamDestroy((hipStream_t)ctx->side_stream);
     if (ctx->ev_fork) (void)hipEventDestroy((hipEvent_t)ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy((hipEvent_t)ctx->ev_join);

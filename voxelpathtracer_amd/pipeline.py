@@ -105,6 +105,55 @@ class ShadowDenoiser:
             pass
 
 
+class Graph:
+    """vpt_graph: HIP-graph replay of one frame's calls.  Use as
+
+        with graph.capture(stream) as s:   # s: the stream to enqueue on
+            ...calls on s...
+
+    which captures the calls, instantiates (first frame) or updates the
+    graph in place, and replays it ordered after `stream`'s earlier work."""
+
+    def __init__(self, ctx: Context):
+        h = C.c_void_p()
+        call("vpt_graph_create", ctx.handle, C.byref(h))
+        self.handle = h
+
+    def capture(self, stream=None, join_stream=None):
+        """The replay starts after `stream`'s earlier work; `join_stream`
+        (default `stream`) waits for it to finish."""
+        import contextlib
+
+        @contextlib.contextmanager
+        def cm():
+            cap = C.c_void_p()
+            call("vpt_graph_begin", self.handle, _stream(stream), C.byref(cap))
+            try:
+                yield int(cap.value or 0)
+            except BaseException:
+                lib().vpt_graph_abort(self.handle)
+                raise
+            call("vpt_graph_launch", self.handle, _stream(stream if join_stream is None else join_stream))
+
+        return cm()
+
+    def stats(self) -> dict:
+        st = _lib.GraphStats()
+        call("vpt_graph_stats_get", self.handle, C.byref(st))
+        return {k: getattr(st, k) for k, _ in st._fields_ if k != "pad"}
+
+    def close(self):
+        if self.handle:
+            lib().vpt_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def viewproj(view: np.ndarray, proj: np.ndarray) -> np.ndarray:
     """proj * view for column-major float32 matrices, summed in k order in
     float32 (u_PrevProjection * u_PrevView, TemporalFilter.glsl:64)."""
@@ -128,7 +177,12 @@ class Pipeline:
     (1/4 resolution, 2 rough samples)."""
 
     def __init__(self, ctx: Context, world: World, width: int, height: int, sun_tick: float = 50.0,
-                 max_iter: int = 350, shadow_iter: int = 350):
+                 max_iter: int = 350, shadow_iter: int = 350, use_graph: bool = False):
+        """use_graph: replay each frame's two chains — the side chain
+        (shadow with transversal, shadow denoise, reflections) and the main
+        chain (GI, SVGF) — as HIP graphs launched on their own streams after
+        the primary pass, so that they still overlap; each is captured every
+        frame and its instantiated graph updated in place."""
         import torch
         self.ctx, self.world = ctx, world
         self.width, self.height = width, height
@@ -149,12 +203,16 @@ class Pipeline:
         self.frame_index = 0
         self.prev_view = None
         self.prev_proj = None
+        self.graphs = (Graph(ctx), Graph(ctx)) if use_graph else None   # side chain, main chain
+        self.fork = torch.cuda.Event()
+        self.join = torch.cuda.Event()
 
     def render(self, pos, yaw: float, pitch: float, stream=None):
         """One frame: primary; then the sun shadow with its transversal, the
         shadow denoise and the reflections on a side stream, beside GI and
         SVGF on `stream`.  Returns the camera."""
         import torch
+        main = torch.cuda.current_stream() if stream is None else torch.cuda.ExternalStream(int(stream))
         cam, view, proj = make_camera(pos, yaw, pitch, self.width, self.height)
         cur = self.gbuf[self.frame_index % 2]
         prev = self.gbuf[(self.frame_index + 1) % 2]
@@ -165,28 +223,42 @@ class Pipeline:
         if self.frame_index == 0:
             prev = cur
         prev_vp = viewproj(self.prev_view, self.prev_proj)
-        main = torch.cuda.current_stream() if stream is None else torch.cuda.ExternalStream(int(stream))
         self.world.trace_primary(cam, cur, self.max_iter, stream=main.cuda_stream)
+
         # the side passes read only the G-buffer and the shadow pass
-        fork = torch.cuda.Event()
-        fork.record(main)
-        self.side.wait_event(fork)
-        with torch.cuda.stream(self.side):
-            self.world.trace_shadow2(cam, cur, sun, self.shadow, self.transversal, self.shadow_iter)
-            self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal, self.shadow_denoised)
+        def side_chain(s):
+            self.world.trace_shadow2(cam, cur, sun, self.shadow, self.transversal, self.shadow_iter, stream=s)
+            self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal, self.shadow_denoised,
+                                       stream=s)
             if self.width >= 4 and self.height >= 4:
                 from .world import reflect_params
                 self.world.trace_reflect(cam, cur, sun, self.reflect_dist, self.reflect_hit,
-                                         **reflect_params(sun, self.frame_index))
-        join = torch.cuda.Event()
-        join.record(self.side)
-        self.world.trace_gi(cam, cur, prm, self.rgb, self.texels, stream=main.cuda_stream)
-        self.svgf.frame(cam, prev_vp, cur, prev, self.texels, self.denoised, time=self.frame_index / 60.0,
-                        stream=main.cuda_stream)
+                                         **reflect_params(sun, self.frame_index), stream=s)
+
+        def main_chain(s):
+            self.world.trace_gi(cam, cur, prm, self.rgb, self.texels, stream=s)
+            self.svgf.frame(cam, prev_vp, cur, prev, self.texels, self.denoised, time=self.frame_index / 60.0,
+                            stream=s)
+
+        fork, join = self.fork, self.join
+        if self.graphs is None:
+            fork.record(main)
+            self.side.wait_event(fork)
+            side_chain(self.side.cuda_stream)
+            join.record(self.side)
+            main_chain(main.cuda_stream)
+        else:
+            # both replays start after the primary pass on `main`; the side
+            # chain's completion is joined on the side stream
+            with self.graphs[0].capture(main.cuda_stream, join_stream=self.side.cuda_stream) as s:
+                side_chain(s)
+            join.record(self.side)
+            with self.graphs[1].capture(main.cuda_stream) as s:
+                main_chain(s)
         main.wait_event(join)
         self.prev_view, self.prev_proj = view, proj
         self.frame_index += 1
         return cam
 
 
-__all__ = ["SVGF", "Pipeline", "decode_texels", "svgf_params_default", "viewproj", "_lib"]
+__all__ = ["SVGF", "Pipeline", "Graph", "decode_texels", "svgf_params_default", "viewproj", "_lib"]
