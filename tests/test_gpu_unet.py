@@ -63,6 +63,36 @@ def test_forward_matches_cpu(net, n, h, w):
     assert _rel(out.cpu().numpy(), ref_out.numpy()) < 1e-2
 
 
+@pytest.mark.timeout(300)
+def test_forward_matches_cpu_1080p(ctx):
+    """The workload's resolution, 1920x1080: the persistent convolution
+    kernels loop over many 16x32 tiles per workgroup and the pooled levels
+    run at 960x540 and 480x270; every activation and the output equal the
+    CPU fp32 reference to the same tolerance as the small cases."""
+    torch = _torch()
+    n, h, w = 1, 1080, 1920
+    u = UNet(ctx, n, h, w)
+    rng = np.random.default_rng(17)
+    xb, xf = _inputs(rng, n, h, w)
+    u.init(13)
+    params = u.get_params()
+    out = torch.empty((n, h, w, 3), dtype=torch.float32, device="cuda")
+    print("unet forward 1x1080x1920", flush=True)
+    u.forward(torch.from_numpy(xb.view(np.int16)).cuda(), n, h, w, out)
+    torch.cuda.synchronize()
+    with torch.no_grad():
+        ref_out, acts = unet_ref.forward(unet_ref.unpack(params), torch.from_numpy(xf))
+    for i, name in enumerate(["x", "e0a", "e0b", "p0", "e1a", "e1b", "p1", "b0", "b1", "d1a", "d1b", "d0a", "d0b"]):
+        got = bf16_to_f32(u.activation(i, n, h, w))
+        ref = acts[name].numpy()
+        assert got.shape == ref.shape, name
+        err = _rel(got, ref)
+        assert err < 1e-2, (name, err)
+        assert np.abs(got - ref).max() <= 0.02 * np.abs(ref).max() + 1e-6, name
+    assert _rel(out.cpu().numpy(), ref_out.numpy()) < 1e-2
+    u.close()
+
+
 def test_gradients_match_cpu(net):
     torch = _torch()
     rng = np.random.default_rng(9)
