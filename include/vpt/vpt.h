@@ -280,6 +280,45 @@ VPT_API int vpt_reflect_params_default(const float light_dir[3], uint32_t frame,
 VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
                               const vpt_reflect_params* prm, uint16_t* d_dist, uint32_t* d_hit, void* stream);
 
+/* Hit shading of the reflection pass (ReflectionTraceFrag.glsl:582-593,
+ * 764-1035; run after the diffuse and shadow denoisers, Pipeline.cpp:
+ * 2854-3010).  d_diffuse: the frame's denoised diffuse (vpt_svgf_final),
+ * d_shadow: its denoised sun shadow (vpt_shadow_denoise_frame's d_out),
+ * both at the G-buffer's size. */
+typedef struct vpt_reflect_shading {
+    const vpt_svgf_texel* d_diffuse;
+    const uint8_t* d_shadow;
+    float viewproj[16];         /* u_Projection * u_View of the frame (column-major) */
+    float sun_radiance[3];      /* SAMPLED_COLOR_MIXED substitute (the hit's direct term uses 0.6x) */
+    float sky_radiance[3];      /* GetAtmosphere substitute for a reflected ray that escapes */
+    float albedo[128 * 3];      /* albedo per block id */
+} vpt_reflect_shading;
+
+/* sun_radiance = gi->light_color, sky_radiance = gi->sky_radiance, albedo =
+ * gi->albedo; the planes and viewproj are the caller's. */
+VPT_API int vpt_reflect_shading_from_gi(const vpt_gi_params* gi, const float viewproj[16],
+                                        const vpt_svgf_texel* d_diffuse, const uint8_t* d_shadow,
+                                        vpt_reflect_shading* out);
+
+/* vpt_trace_reflect plus the hit shading.  Each hit is reprojected to the
+ * screen with viewproj (ReprojectReflectionToScreenSpace, :582-593): when
+ * its uv is inside the 0.0035 border and the G-buffer there holds a
+ * position within 0.095 (squared) and the same normal, the hit takes its
+ * ambient from the denoised diffuse at uv (SHToIrradianceA, times
+ * pow(AO, 0.75) when that position is within 40 of the camera) and, for the
+ * shadow-casting samples, the denoised shadow at uv instead of a shadow
+ * ray; other hits take the ambient at the pixel.  A hit adds ambient *
+ * albedo + the diffuse directional light (sun_radiance * 0.6, F0 0.04), a
+ * miss adds sky_radiance.  d_color: RGBA binary16 per target pixel, the
+ * mean over spp clamped to [1e-7, 100] with alpha 1 (0 for sky pixels).
+ * d_hit adds bit 17 (the first hit was reprojected); its shadow bit is the
+ * last shadow value >= 0.5.  Mirror reflections (rough = 0) trace the ray
+ * and its shadow ray once and repeat the sample spp times, as the shader's
+ * loop does. */
+VPT_API int vpt_trace_reflect_shaded(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                     const vpt_reflect_params* prm, const vpt_reflect_shading* shading,
+                                     uint16_t* d_dist, uint32_t* d_hit, uint16_t* d_color, void* stream);
+
 /* DIFFUSE_TRACE (DiffuseRayTraceFrag.glsl:822-935), 1 spp, 1 bounce with a
  * shadow ray at the bounce hit.  d_rgb: 3 floats per pixel (clamped to
  * [0, 8]).  d_svgf_in (optional): SVGF input texels.  The hit records and
@@ -549,6 +588,11 @@ VPT_API int vpt_svgf_frame(vpt_svgf* s, const vpt_camera* cam, const float prev_
                            const vpt_gbuffer* cur, const vpt_gbuffer* prev, const vpt_svgf_texel* d_in, float time,
                            float* d_rgb_out, vpt_svgf_texel* d_texel_out, void* stream);
 
+/* The final a-trous plane of the last frame (what d_texel_out receives),
+ * without a copy: valid for work enqueued on the frame's stream before the
+ * next vpt_svgf_frame or vpt_svgf_destroy. */
+VPT_API int vpt_svgf_final(const vpt_svgf* s, const vpt_svgf_texel** out);
+
 /* Debug access to the planes after the last frame: which = 0 pre-spatial,
  * 1 temporal (history), 2 variance, 3 final a-trous output.  Copies
  * width*height texels to device memory d_dst. */
@@ -578,6 +622,43 @@ VPT_API int vpt_shadow_denoise_frame(vpt_shadow_denoise* d, const vpt_camera* ca
 /* The last frame's temporal planes: which = 0 colour (1 byte per pixel),
  * 1 frame count (binary16 bits). */
 VPT_API int vpt_shadow_denoise_read(vpt_shadow_denoise* d, int which, void* d_dst, void* stream);
+
+/* ---- colour composite -------------------------------------------------------
+ * The colour pass's shading of the G-buffer (ColorPassFrag.glsl:773-946,
+ * u_UseDFG) for flat dielectric materials: per pixel, the denoised diffuse
+ * decoded with the face normal (SHToIrridiance) times VXAO-faded AO, the
+ * Cook-Torrance sun and moon light mixed by SunVisibility and shadowed by
+ * the denoised shadow, and the reflection colour weighted by specular AO,
+ * horizon fade and the Karis environment BRDF.  Stand-ins for what the
+ * reference samples from textures outside the hot path: albedo per block
+ * id (the block albedo atlas), the face normal (normal maps), one roughness
+ * (PBR atlas, metalness 0), constant sun / moon / sky-pixel radiances (the
+ * atmosphere cubemap), full-resolution diffuse and shadow (the spatial
+ * upscale passes at scale 1), LINEAR sampling of the reflection colour for
+ * its upscale. */
+typedef struct vpt_composite_params {
+    float sun_dir[3];             /* u_SunDirection, normalised */
+    float moon_dir[3];            /* u_MoonDirection: normalize(-sun.x, -sun.y, sun.z), Pipeline.cpp:1663 */
+    float sun_radiance[3];        /* SAMPLED_SUN_COLOR substitute */
+    float moon_radiance[3];       /* SAMPLED_MOON_COLOR substitute */
+    float sky_pixel_radiance[3];  /* output of primary-miss pixels */
+    float roughness;              /* RoughnessAt of the flat material */
+    float albedo[128 * 3];        /* albedo per block id */
+} vpt_composite_params;
+
+/* sun_dir, albedo and sky_pixel_radiance from gi, sun_radiance =
+ * gi->light_color, moon_radiance = (0.1, 0.1, 1.0) (ATMOSPHERE_MOON_COLOR,
+ * ColorPassFrag.glsl:196), roughness 0.5 (the flat material's RoughnessAt
+ * before RoughnessBias). */
+VPT_API int vpt_composite_params_from_gi(const vpt_gi_params* gi, vpt_composite_params* out);
+
+/* d_rgb: 3 floats per pixel (W x H).  d_diffuse / d_shadow: W x H (as for
+ * vpt_reflect_shading); d_reflect: vpt_trace_reflect_shaded's d_color at
+ * (W / reflect_div) x (H / reflect_div).  Sky pixels (t <= 0) take
+ * sky_pixel_radiance. */
+VPT_API int vpt_composite(vpt_ctx* ctx, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_composite_params* prm,
+                          const vpt_svgf_texel* d_diffuse, const uint8_t* d_shadow, const uint16_t* d_reflect,
+                          int reflect_div, float* d_rgb, void* stream);
 
 /* ---- learned denoiser -------------------------------------------------------
  * A 3-level U-Net (32 / 64 / 128 channels, 3x3 convolutions, 2x2 average

@@ -26,6 +26,7 @@
  *     e ln2 with ln2 split as in vpt_expf.
  *   - vpt_powf: x^y = vpt_expf(y vpt_logf(x)) for x > 0 (GLSL pow is
  *     undefined for x < 0; 0^y = 0 for y > 0, x^0 = 1).
+ *   - vpt_exp2f: 2^x = vpt_expf(x ln2).
  *   - vpt_f32_to_f16 / vpt_f16_to_f32: IEEE binary16 conversion with
  *     round-to-nearest-even, the rounding of a GL RGBA16F render target.
  *   - vpt_unorm8_encode / vpt_unorm8_decode: GL unorm8 conversions
@@ -166,6 +167,9 @@ VPT_HD float vpt_powf(float x, float y)
     if (x == 1.0f) return 1.0f;
     return vpt_expf(y * vpt_logf(x));
 }
+
+/* 2^x (see the header comment). */
+VPT_HD float vpt_exp2f(float x) { return vpt_expf(x * 0.693147180559945309f); }
 
 /* x^n, n >= 0, by binary exponentiation (squares taken in a fixed order). */
 VPT_HD float vpt_powi(float x, int n)

@@ -133,6 +133,46 @@ def trace_reflect(grid, df, inv_view, inv_proj, gb_t, gb_face, light_dir, div: i
     return dict(dist=dist, hit=hit, samples=samples, rays=rays)
 
 
+def trace_reflect_shaded(grid, df, inv_view, inv_proj, gb_t, gb_face, light_dir, sh, cocg, ao, shadow, viewproj,
+                         sun_rad, sky_rad, albedo, div: int = 4, max_iter: int = 64, shadow_iter: int = 150,
+                         spp: int = 1, rough: int = 0, roughness: float = 0.425, frame: int = 0):
+    """The reflection pass with reprojection into the frame's denoised
+    diffuse and hit shading (ReflectionTraceFrag.glsl:582-593, 764-1035).
+    sh (H, W, 4), cocg (H, W, 2), ao (H, W): the denoised diffuse planes;
+    shadow (H, W): the denoised sun shadow in [0, 1]; viewproj: the frame's
+    u_Projection * u_View.  Adds colour: (Hr, Wr, 4) binary16 bits."""
+    nz, ny, nx = grid.shape
+    H, W = gb_t.shape
+    Hr, Wr = H // div, W // div
+    dist = np.empty((Hr, Wr), np.uint16)
+    hit = np.empty((Hr, Wr), np.uint32)
+    color = np.empty((Hr, Wr, 4), np.uint16)
+    samples = np.empty((Hr, Wr), np.int32)
+    rays = np.empty((Hr, Wr), np.int32)
+    lib().oracle_trace_reflect_shaded(_p(grid), _p(df), nx, ny, nz, _p(_f32(inv_view)), _p(_f32(inv_proj)), W, H,
+                                      _p(_f32(gb_t)), _p(np.ascontiguousarray(gb_face, np.uint8)), _p(_f32(light_dir)),
+                                      div, max_iter, shadow_iter, spp, rough, C.c_float(roughness), C.c_uint32(frame),
+                                      _p(_f32(sh)), _p(_f32(cocg)), _p(_f32(ao)), _p(_f32(shadow)), _p(_f32(viewproj)),
+                                      _p(_f32(sun_rad)), _p(_f32(sky_rad)), _p(_f32(albedo)), _p(dist), _p(hit),
+                                      _p(color), _p(samples), _p(rays))
+    return dict(dist=dist, hit=hit, color=color, samples=samples, rays=rays)
+
+
+def composite(inv_view, inv_proj, gb_t, gb_face, gb_block, sh, cocg, ao, shadow, refl, rdiv, sun_dir, moon_dir,
+              sun_rad, moon_rad, sky_pixel, albedo, roughness: float):
+    """The colour pass's composite (ColorPassFrag.glsl:773-946, DFG branch)
+    for flat dielectric materials: (H, W, 3) float32.  refl: (Hr, Wr, 4)
+    reflection colour values (floats of its binary16 texels)."""
+    H, W = gb_t.shape
+    out = np.empty((H, W, 3), np.float32)
+    lib().oracle_composite(_p(_f32(inv_view)), _p(_f32(inv_proj)), W, H, _p(_f32(gb_t)),
+                           _p(np.ascontiguousarray(gb_face, np.uint8)), _p(np.ascontiguousarray(gb_block, np.uint8)),
+                           _p(_f32(sh)), _p(_f32(cocg)), _p(_f32(ao)), _p(_f32(shadow)), _p(_f32(refl)), int(rdiv),
+                           _p(_f32(sun_dir)), _p(_f32(moon_dir)), _p(_f32(sun_rad)), _p(_f32(moon_rad)),
+                           _p(_f32(sky_pixel)), _p(_f32(albedo)), C.c_float(roughness), _p(out))
+    return out
+
+
 def shadow_temporal(inv_view, inv_proj, prev_viewproj, cur_t, cur_face, prev_t, mask, transversal, prev_color,
                     prev_frames):
     """ShadowTemporalFilter.main: (colour u8, frame count binary16 bits)."""

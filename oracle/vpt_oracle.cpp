@@ -675,6 +675,10 @@ enum Mutation {
     MUT_AT_VAR_PROP, MUT_AT_AO_STEP, MUT_AT_SKY, MUT_AT_LUMA_PHI,
     MUT_SF_SIZE, MUT_SF_SPACING, MUT_SF_SC24, MUT_SF_LUMA_SWITCH, MUT_SF_LM, MUT_SF_EXPO, MUT_SF_VE, MUT_SF_LE,
     MUT_SF_DEPTH, MUT_SF_PASS,
+    MUT_RS_Y, MUT_RS_COCG, MUT_RS_AO, MUT_RS_DIST, MUT_RS_BASE, MUT_RS_ALBEDO, MUT_RS_RAD, MUT_RS_FRESNEL,
+    MUT_RS_SHADOW_TRACE, MUT_RS_SKY, MUT_RS_MEAN,
+    MUT_CMP_D_PI, MUT_CMP_ALPHA, MUT_CMP_SUNVIS, MUT_CMP_SAT, MUT_CMP_SH_L1, MUT_CMP_COCG, MUT_CMP_AO_EXP,
+    MUT_CMP_SHADOW, MUT_CMP_SPEC_BORDER, MUT_CMP_KARIS, MUT_CMP_F0, MUT_CMP_SKY, MUT_CMP_KD,
     MUT_COUNT
 };
 
@@ -733,6 +737,30 @@ static const char* const kMutationNames[MUT_COUNT] = {
     "shadow filter: luminance error / 2 instead of / 3",
     "shadow filter: depth weight squared instead of cubed",
     "shadow filter: pass-through below sqrt(2) dropped",
+    "reflection shading: SHToIrradianceA Y = sqrt(pi) L0 instead of 2 sqrt(pi) L0",
+    "reflection shading: Co and Cg swapped in SHToIrradianceA",
+    "reflection shading: ambient times AO instead of AO^0.75",
+    "reflection shading: AO applied beyond distance 40 too",
+    "reflection shading: reprojected hits keep the ambient at the pixel",
+    "reflection shading: ambient not multiplied by the albedo",
+    "reflection shading: direct radiance without the 0.6 factor",
+    "reflection shading: Fresnel dropped from the diffuse term (kd = 1)",
+    "reflection shading: reprojected hits trace the shadow ray",
+    "reflection shading: misses add nothing instead of the sky",
+    "reflection shading: colour is the sum over samples, not the mean",
+    "composite: GGX D without the pi",
+    "composite: GGX alpha = r instead of r^2",
+    "composite: SunVisibility clamped to [0, 1]",
+    "composite: BasicSaturation weight 0.1 instead of 0.9",
+    "composite: SH L1 constant 0.511664 instead of 1.023326",
+    "composite: Co and Cg swapped in SHToIrridiance",
+    "composite: AO^1 instead of AO^2.2",
+    "composite: shadow ignored",
+    "composite: reflection sampled up to the screen edge",
+    "composite: Karis bias term 1.04 a004 sign flipped",
+    "composite: F0 0.02 instead of 0.04",
+    "composite: sky pixels 0 instead of the sky radiance",
+    "composite: diffuse not weighted by 1 - F",
 };
 
 static int g_mutation = MUT_NONE;
@@ -1682,10 +1710,54 @@ static inline V3 reflection_normal(V3 N, float r, uint32_t* state)
     return best;
 }
 
-/* REFLECTION_TRACE (ReflectionTraceFrag.glsl:764-1035) reduced to its
- * traversal core: flat materials (the normal-mapped G-buffer normal is the
- * face normal, the roughness one value), no jitter, screen-space
- * reprojection, LPV and player off.
+/* SHToIrradianceA (ReflectionTraceFrag.glsl:466-475): the direction-less
+ * decode of a diffuse SH / CoCg texel. */
+static inline V3 sh_to_irradiance_a(const float sh[4], const float cc[2])
+{
+    const float Y = fmaxf(0.0f, (mut(MUT_RS_Y) ? 1.7724525f : 3.544905f) * sh[3]);
+    const float k = (Y * 0.282095f) / (sh[3] + 1e-6f);
+    const int ic = mut(MUT_RS_COCG) ? 1 : 0;
+    const float co = cc[ic] * k, cg = cc[1 - ic] * k;
+    const float T = Y - cg * 0.5f;
+    const float G = cg + T;
+    const float B = T - co * 0.5f;
+    const float R = B + co;
+    return v3(fmaxf(R, 0.0f), fmaxf(G, 0.0f), fmaxf(B, 0.0f));
+}
+
+/* CalculateDirectionalLight of the reflection pass (ReflectionTraceFrag.glsl:
+ * 313-336) for a flat dielectric (metalness 0): F0 = mix(0.04, albedo, 0) =
+ * 0.04, kd = mix(1 - F, 0, 0) = 1 - F.  Its specular term is multiplied by
+ * radiance_s = radiance * 0.05 * 0 = 0; D and G are finite for every
+ * roughness in (0, 1] (denominators >= alpha^4 > 0 and k > 0), so that term
+ * is exactly +0 and is left out.  pow(x, 5.0) is vpt_powi(x, 5). */
+static inline V3 reflect_direct(V3 hp, V3 eye, V3 L, V3 rad, V3 alb, V3 N, float shadow)
+{
+    const float sh = fminf(shadow, 1.0f);
+    const V3 Lo = normalize3(v3(eye.x - hp.x, eye.y - hp.y, eye.z - hp.z));
+    const V3 Lh = normalize3(v3(L.x + Lo.x, L.y + Lo.y, L.z + Lo.z));
+    const float cos_li = fmaxf(0.0f, dot3(N, L));
+    const float F = 0.04f + (1.0f - 0.04f) * vpt_powi(1.0f - fmaxf(0.0f, dot3(Lh, Lo)), 5);
+    const float kd = mut(MUT_RS_FRESNEL) ? 1.0f : 1.0f - F;
+    const float vis = fclamp(1.0f - sh, 0.0f, 1.0f);
+    return v3(fmaxf(((kd * alb.x) * rad.x) * cos_li, 0.0f) * vis, fmaxf(((kd * alb.y) * rad.y) * cos_li, 0.0f) * vis,
+              fmaxf(((kd * alb.z) * rad.z) * cos_li, 0.0f) * vis);
+}
+
+/* Inputs of the shaded reflection pass (see oracle_trace_reflect_shaded). */
+struct ReflShade {
+    const float* sh;       /* W x H x 4: the frame's denoised diffuse SH (DiffuseDenoiseFBO 0) */
+    const float* cocg;     /* W x H x 2: its CoCg (DiffuseDenoiseFBO 1) */
+    const float* ao;       /* W x H: its AO channel (DiffuseDenoiseFBO 3, .x) */
+    const float* shadow;   /* W x H: the denoised sun shadow (ShadowFiltered) */
+    const float* vp;       /* u_Projection * u_View of the frame */
+    V3 sun_rad, sky_rad;   /* SAMPLED_COLOR_MIXED and GetAtmosphere substitutes */
+    const float* albedo;   /* 128 x RGB per block id */
+};
+
+/* REFLECTION_TRACE (ReflectionTraceFrag.glsl:764-1035) for flat materials
+ * (the normal-mapped G-buffer normal is the face normal, the roughness one
+ * value), no jitter, LPV and player off.
  *
  * The target is (W / div) x (H / div) (ReflectionTraceFBO, Pipeline.cpp:
  * 1141, 1744: the reference's resolution 0.25 is div = 4).  Each target
@@ -1710,21 +1782,51 @@ static inline V3 reflection_normal(V3 N, float r, uint32_t* state)
  * eye is the inverse view's translation (the reference's u_ViewerPosition,
  * the position its view matrix was built from).  hit_out (optional):
  * face | block << 8 of the first hit | shadow << 16 | hits << 24, 255
- * without a hit.  samples / rays (optional): DF samples and traced rays. */
-OAPI void oracle_trace_reflect(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+ * without a hit.  samples / rays (optional): DF samples and traced rays.
+ *
+ * With shading inputs (ReprojectReflectionToScreenSpace and the hit
+ * shading, :582-593, 843-1023, u_ReprojectToScreenSpace on) each hit is
+ * also reprojected with the frame's view-projection: uv = (VP HP).xy / w
+ * * 0.5 + 0.5; the G-buffer position there (t LINEAR, full resolution) and
+ * face (NEAREST) must lie within |dP|^2 < 0.095 with the hit's normal, and
+ * uv inside the 0.0035 screen border.  Reprojected hits take their ambient
+ * from the denoised diffuse there (SHToIrradianceA of LINEAR SH / CoCg,
+ * times pow(AO, 0.75) when the position there is geometry within 40 of the
+ * eye) and, for the shadow-casting samples, the denoised shadow there
+ * instead of a shadow ray; other hits take the ambient at the pixel itself
+ * (BaseIndirectDiffuse; the reference's LPV fallback is off).  A hit adds
+ * ambient * albedo + reflect_direct(sun radiance * 0.6) of its block, a
+ * miss the sky radiance; the colour is their mean over spp, clamped to
+ * [1e-7, 100], stored as RGBA binary16 (alpha 1; sky pixels 0).  Bit 17 of
+ * the hit record: the first hit was reprojected; the shadow bit: the last
+ * shadow value >= 0.5. */
+static void trace_reflect_impl(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
                                const float* inv_view, const float* inv_proj, int W, int H,
                                const float* gb_t, const uint8_t* gb_face, const float* light_dir, int div,
                                int max_iter, int shadow_iter, int spp, int rough, float roughness, uint32_t frame,
-                               uint16_t* dist_out, uint32_t* hit_out, int32_t* samples, int32_t* rays)
+                               const ReflShade* shade, uint16_t* dist_out, uint32_t* hit_out, uint16_t* color_out,
+                               int32_t* samples, int32_t* rays)
 {
     const Dims d = {nx, ny, nz};
     const int Wr = W / div, Hr = H / div;
     const Frame f = {inv_view, inv_proj, Wr, Hr};
+    const Frame ff = {inv_view, inv_proj, W, H};
     const Tex tt = {gb_t, W, H, 1};
     const V3 L = v3(light_dir[0], light_dir[1], light_dir[2]);
     const V3 eye = f.origin();
     const int shadow_rays = std::max(spp / 4, 1);
     const float rr = fclamp(roughness, 0.01f, 1.0f);
+    Tex tsh = {}, tcc = {}, tao = {}, tsd = {};
+    V3 rad = v3(0.0f, 0.0f, 0.0f);
+    if (shade) {
+        tsh = {shade->sh, W, H, 4};
+        tcc = {shade->cocg, W, H, 2};
+        tao = {shade->ao, W, H, 1};
+        tsd = {shade->shadow, W, H, 1};
+        const float f = mut(MUT_RS_RAD) ? 1.0f : 0.6f;
+        rad = v3(shade->sun_rad.x * f, shade->sun_rad.y * f, shade->sun_rad.z * f);
+    }
+    const float b = 0.0035f;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int ry = 0; ry < Hr; ry++) {
         for (int rx = 0; rx < Wr; rx++) {
@@ -1742,6 +1844,7 @@ OAPI void oracle_trace_reflect(const uint8_t* grid, const uint8_t* df, int nx, i
             float hd = -1.0f;
             uint32_t rec = 255u;
             int smp = 0, nr = 0;
+            float col[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             if (!(dist < 0.0f)) {
                 const V3 P = position_from(f, sx, sy, dist);
                 const V3 N = face_normal(face, 1.0f);
@@ -1751,6 +1854,16 @@ OAPI void oracle_trace_reflect(const uint8_t* grid, const uint8_t* df, int nx, i
                                                            (uint32_t)Wr * (uint32_t)Hr * frame) + 0x9E3779B9u);
                 float sum = 0.001f;
                 int hits = 0, shadow_itr = 0, shadow = 0;
+                float shadow_f = 0.0f;   /* ComputedShadow (:811) */
+                bool first_reproj = false;
+                V3 base_amb = v3(0.0f, 0.0f, 0.0f);
+                if (shade) {   /* BaseIndirectDiffuse (:804-806) at the target pixel */
+                    float bs[4], bc[2];
+                    for (int c = 0; c < 4; c++) bs[c] = bilinear(tsh, sx, sy, c);
+                    for (int c = 0; c < 2; c++) bc[c] = bilinear(tcc, sx, sy, c);
+                    base_amb = sh_to_irradiance_a(bs, bc);
+                }
+                float tot[3] = {0.0f, 0.0f, 0.0f};
                 TraceResult tr = {};
                 tr.t = -1.0f;
                 V3 R = N;
@@ -1764,38 +1877,305 @@ OAPI void oracle_trace_reflect(const uint8_t* grid, const uint8_t* df, int nx, i
                         nr++;
                     }
                     if (tr.t > 0.0f) {
+                        /* the traversal's normal vector (normal[MinIdx] =
+                         * -RaySign[MinIdx]), never zero on a hit: a step
+                         * along an axis whose direction component is 0
+                         * moves the other coordinates by d * inf, and the
+                         * ray leaves the volume */
+                        const V3 hn = face_normal(tr.face, 0.0f);
+                        const V3 hp = v3(o.x + R.x * tr.t, o.y + R.y * tr.t, o.z + R.z * tr.t);
+                        bool reproj = false;
+                        float ru = 0.0f, rv = 0.0f, t_at = 0.0f;
+                        V3 p_at = v3(0.0f, 0.0f, 0.0f);
+                        if (shade) {   /* ReprojectReflectionToScreenSpace (:582-593) */
+                            float p4[4] = {hp.x, hp.y, hp.z, 1.0f}, q[4];
+                            mat_vec4(shade->vp, p4, q);
+                            ru = (q[0] / q[3]) * 0.5f + 0.5f;
+                            rv = (q[1] / q[3]) * 0.5f + 0.5f;
+                            /* the test ANDs the screen border with the
+                             * position and normal checks: outside it (also
+                             * inf / NaN behind the eye) the fetches are not
+                             * needed */
+                            if (ru < 1.0f - b && ru > b && rv < 1.0f - b && rv > b) {
+                                t_at = bilinear(tt, ru, rv, 0);
+                                p_at = position_from(ff, ru, rv, t_at);
+                                const V3 n_at = face_normal(nearest_u8(gb_face, W, H, ru, rv), 1.0f);
+                                const V3 pd = v3(fabsf(p_at.x - hp.x), fabsf(p_at.y - hp.y), fabsf(p_at.z - hp.z));
+                                const float err = dot3(pd, pd);
+                                reproj = err < 0.095f && n_at.x == hn.x && n_at.y == hn.y && n_at.z == hn.z;
+                            }
+                            if (hits == 0) first_reproj = reproj;
+                        }
                         if (shadow_itr < shadow_rays) {
-                            if (rough || shadow_itr == 0) {
-                                /* the traversal's normal vector (normal[MinIdx] =
-                                 * -RaySign[MinIdx]), never zero on a hit: a step
-                                 * along an axis whose direction component is 0
-                                 * moves the other coordinates by d * inf, and the
-                                 * ray leaves the volume */
-                                const V3 hn = face_normal(tr.face, 0.0f);
-                                const V3 hp = v3(o.x + R.x * tr.t, o.y + R.y * tr.t, o.z + R.z * tr.t);
+                            if (shade && reproj && !mut(MUT_RS_SHADOW_TRACE)) {
+                                shadow_f = bilinear(tsd, ru, rv, 0);   /* :934-937 */
+                            } else if (rough || shadow_itr == 0) {
                                 const V3 so = v3(hp.x + hn.x * 0.055f, hp.y + hn.y * 0.055f, hp.z + hn.z * 0.055f);
                                 const TraceResult sr = voxel_traversal_df(grid, df, d, so, L, shadow_iter);
                                 smp += sr.samples;
                                 nr++;
                                 shadow = sr.t > 0.0f ? 1 : 0;
+                                shadow_f = (float)shadow;
                             }
                             shadow_itr++;
+                        }
+                        if (shade) {
+                            V3 amb = base_amb;
+                            if (reproj && !mut(MUT_RS_BASE)) {
+                                float rs[4], rc[2];
+                                for (int c = 0; c < 4; c++) rs[c] = bilinear(tsh, ru, rv, c);
+                                for (int c = 0; c < 2; c++) rc[c] = bilinear(tcc, ru, rv, c);
+                                amb = sh_to_irradiance_a(rs, rc);
+                                const float ao_at = bilinear(tao, ru, rv, 0);
+                                const float vx = mut(MUT_RS_AO) ? ao_at : vpt_powf(ao_at, 0.75f);
+                                if (t_at > 0.0f) {
+                                    const V3 dv = v3(p_at.x - eye.x, p_at.y - eye.y, p_at.z - eye.z);
+                                    if (sqrtf(dot3(dv, dv)) < 40.0f || mut(MUT_RS_DIST))
+                                        amb = v3(amb.x * vx, amb.y * vx, amb.z * vx);
+                                }
+                            }
+                            const int ref = std::min(std::max(tr.block, 0), 127);
+                            const V3 alb = v3(shade->albedo[3 * ref], shade->albedo[3 * ref + 1], shade->albedo[3 * ref + 2]);
+                            if (!mut(MUT_RS_ALBEDO))   /* (Ambient * 1 * clamp(AO, 0.1, 1)) * Albedo */
+                                amb = v3(amb.x * alb.x, amb.y * alb.y, amb.z * alb.z);
+                            const V3 dl = reflect_direct(hp, eye, L, rad, alb, hn, shadow_f);
+                            tot[0] = tot[0] + (amb.x + dl.x);
+                            tot[1] = tot[1] + (amb.y + dl.y);
+                            tot[2] = tot[2] + (amb.z + dl.z);
                         }
                         if (hits == 0) rec = (uint32_t)tr.face | ((uint32_t)tr.block << 8);
                         sum += tr.t;
                         hits++;
+                    } else if (shade && !mut(MUT_RS_SKY)) {
+                        tot[0] = tot[0] + shade->sky_rad.x;
+                        tot[1] = tot[1] + shade->sky_rad.y;
+                        tot[2] = tot[2] + shade->sky_rad.z;
                     }
                 }
+                if (shade) shadow = shadow_f >= 0.5f ? 1 : 0;
                 if (hits > 0) {
                     hd = sum / fmaxf((float)hits, 0.01f);
-                    rec |= ((uint32_t)shadow << 16) | ((uint32_t)hits << 24);
+                    rec |= ((uint32_t)shadow << 16) | ((uint32_t)hits << 24) | ((uint32_t)first_reproj << 17);
                 }
                 hd = fclamp(hd, -10.0f, 200.0f);
+                const float n = (float)spp;
+                for (int c = 0; c < 3; c++) col[c] = fclamp(mut(MUT_RS_MEAN) ? tot[c] : tot[c] / n, 0.0000001f, 100.0f);
+                col[3] = fclamp(n / n, 0.0000001f, 100.0f);
             }
             dist_out[i] = vpt_f32_to_f16(hd);
             if (hit_out) hit_out[i] = rec;
+            if (color_out)
+                for (int c = 0; c < 4; c++) color_out[4 * i + c] = vpt_f32_to_f16(col[c]);
             if (samples) samples[i] = smp;
             if (rays) rays[i] = nr;
+        }
+    }
+}
+
+OAPI void oracle_trace_reflect(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+                               const float* inv_view, const float* inv_proj, int W, int H,
+                               const float* gb_t, const uint8_t* gb_face, const float* light_dir, int div,
+                               int max_iter, int shadow_iter, int spp, int rough, float roughness, uint32_t frame,
+                               uint16_t* dist_out, uint32_t* hit_out, int32_t* samples, int32_t* rays)
+{
+    trace_reflect_impl(grid, df, nx, ny, nz, inv_view, inv_proj, W, H, gb_t, gb_face, light_dir, div, max_iter,
+                       shadow_iter, spp, rough, roughness, frame, nullptr, dist_out, hit_out, nullptr, samples, rays);
+}
+
+/* The reflection pass with its hit shading (see trace_reflect_impl). */
+OAPI void oracle_trace_reflect_shaded(const uint8_t* grid, const uint8_t* df, int nx, int ny, int nz,
+                                      const float* inv_view, const float* inv_proj, int W, int H,
+                                      const float* gb_t, const uint8_t* gb_face, const float* light_dir, int div,
+                                      int max_iter, int shadow_iter, int spp, int rough, float roughness,
+                                      uint32_t frame, const float* sh, const float* cocg, const float* ao,
+                                      const float* shadow, const float* viewproj, const float* sun_rad,
+                                      const float* sky_rad, const float* albedo, uint16_t* dist_out, uint32_t* hit_out,
+                                      uint16_t* color_out, int32_t* samples, int32_t* rays)
+{
+    ReflShade rs;
+    rs.sh = sh;
+    rs.cocg = cocg;
+    rs.ao = ao;
+    rs.shadow = shadow;
+    rs.vp = viewproj;
+    rs.sun_rad = v3(sun_rad[0], sun_rad[1], sun_rad[2]);
+    rs.sky_rad = v3(sky_rad[0], sky_rad[1], sky_rad[2]);
+    rs.albedo = albedo;
+    trace_reflect_impl(grid, df, nx, ny, nz, inv_view, inv_proj, W, H, gb_t, gb_face, light_dir, div, max_iter,
+                       shadow_iter, spp, rough, roughness, frame, &rs, dist_out, hit_out, color_out, samples, rays);
+}
+
+/* ======================================================================= */
+/* The colour pass's composite (ColorPassFrag.glsl main, :773-946, with    */
+/* u_UseDFG, Pipeline.cpp:3683) for flat dielectric materials.             */
+/* ======================================================================= */
+
+/* FresnelSchlickRoughness(Eye, N, F0, r) (ColorPassFrag.glsl:1205-1209),
+ * scalar F0. */
+static inline float fresnel_schlick_roughness(V3 eye, V3 n, float f0, float r)
+{
+    if (mut(MUT_CMP_F0)) f0 = 0.02f;
+    const float c = fclamp(dot3(eye, n), 0.00001f, 1.0f);
+    return f0 + (fmaxf(1.0f - r, f0) - f0) * vpt_powi(1.0f - c, 5);
+}
+
+/* CalculateDirectionalLight of the colour pass (ColorPassFrag.glsl:419-451),
+ * Cook-Torrance with its GGX specular term, metalness 0. */
+static inline V3 color_direct(V3 P, V3 eye, V3 Li, V3 rad, V3 alb, V3 N, float r, float shadow)
+{
+    const float PI = 3.14159265359f;
+    const float sh = fminf(shadow, 1.0f);
+    const V3 Lo = normalize3(v3(eye.x - P.x, eye.y - P.y, eye.z - P.z));
+    const float cos_lo = fmaxf(0.0f, dot3(N, Lo));
+    const V3 Lh = normalize3(v3(Li.x + Lo.x, Li.y + Lo.y, Li.z + Lo.z));
+    const float cos_li = fmaxf(0.0f, dot3(N, Li));
+    const float cos_lh = fmaxf(0.0f, dot3(N, Lh));
+    const float F = fresnel_schlick_roughness(Lo, N, 0.04f, r);
+    const float alpha = mut(MUT_CMP_ALPHA) ? r : r * r;
+    const float alpha2 = alpha * alpha;
+    const float denom = (cos_lh * cos_lh) * (alpha2 - 1.0f) + 1.0f;
+    const float D = alpha2 / (((mut(MUT_CMP_D_PI) ? 1.0f : PI) * denom) * denom);
+    const float rk = r + 1.0f;
+    const float k = (rk * rk) / 8.0f;
+    const float G = (cos_li / (cos_li * (1.0f - k) + k)) * (cos_lo / (cos_lo * (1.0f - k) + k));
+    const float kd = 1.0f - F;
+    const float spec = fclamp(((F * D) * G) / fmaxf(0.00001f, (4.0f * cos_li) * cos_lo), 0.0f, 2.0f);
+    const float vis = fclamp(1.0f - sh, 0.0f, 1.0f);
+    const float a[3] = {alb.x, alb.y, alb.z}, l[3] = {rad.x, rad.y, rad.z};
+    float o[3];
+    for (int c = 0; c < 3; c++)
+        o[c] = fclamp(((kd * a[c]) * l[c]) * cos_li + (spec * l[c]) * cos_li, 0.0f, 2.5f) * vis;
+    return v3(o[0], o[1], o[2]);
+}
+
+/* KarisEnvBRDFApprox (ColorPassFrag.glsl:385-392). */
+static inline void karis_env_brdf(float ndotv, float r, float* ax, float* ay)
+{
+    const float rx = r * -1.0f + 1.0f, ry = r * -0.0275f + 0.0425f, rz = r * -0.572f + 1.040f,
+                rw = r * 0.022f + -0.040f;
+    const float a004 = fminf(rx * rx, vpt_exp2f(-9.28f * ndotv)) * rx + ry;
+    *ax = -1.04f * a004 + rz;
+    *ay = (mut(MUT_CMP_KARIS) ? -1.04f : 1.04f) * a004 + rw;
+}
+
+/* Per pixel (uv at its centre): Dist = 1 / (1 / t) (SamplePositionAt reads
+ * the R32F 1/t plane, :1122-1126); Dist <= 0 is sky (:803, 1061-1066), the
+ * sky pixel radiance (GetAtmosphereAndClouds' substitute).  Otherwise, with
+ * the workload's substitutions (albedo per block id for the albedo G-buffer,
+ * the face normal for the normal map, PBR = (roughness, 0, -, 1) with no
+ * emission, full-resolution buffers for SpatiallyUpscaleBuffers, the
+ * reflection colour sampled LINEAR for the upscaled specular, no player):
+ *   Albedo = BasicSaturation(albedo, 1 - 0.1)                 (:812)
+ *   shadow = clamp(denoised shadow, 0, 1)                    (:837-840, 304-336)
+ *   Ed = SHToIrridiance(SH, CoCg, N) * clamp(pow(mix(AO, 1, fade), 2.2),
+ *        0.05, 1) inside the 0.00125 border, fade = |P.xz - eye.xz| >= 256
+ *                                                             (:848-876)
+ *   Direct = max(mix(CDL(sun), CDL(moon), SunVisibility), 1e-6) (:885-891)
+ *   Spec = max(0.0001, refl * clamp(pow(max(N.N, 1e-8), 2) 1.5, 0.1, 1)
+ *          inside the 0.0035 border) * SpecularAO^1.5 * Horizon^3
+ *   F = clamp(FresnelSchlickRoughness(Lo, N, 0.04, r), 0, 1)
+ *   out = Direct + ((1 - F) * Ed * Albedo * clamp(1, 0.2, 1.01) +
+ *         Spec * (F * Env.x + Env.y))                          (:914-946)
+ * The exponent exp2(-16 r - 1) of ComputeSpecularAO (:766-769) is one value
+ * per call, (float)exp2((double)(-16 r - 1)) on the host, as the library's
+ * launcher computes it.  refl: (W / rdiv) x (H / rdiv) x 4 binary16 values. */
+OAPI void oracle_composite(const float* inv_view, const float* inv_proj, int W, int H, const float* gb_t,
+                           const uint8_t* gb_face, const uint8_t* gb_block, const float* sh, const float* cocg,
+                           const float* ao, const float* shadow, const float* refl, int rdiv, const float* sun_dir,
+                           const float* moon_dir, const float* sun_rad, const float* moon_rad, const float* sky_pixel,
+                           const float* albedo, float roughness, float* out)
+{
+    const float spec_ao_exp = (float)exp2((double)(-16.0f * roughness - 1.0f));
+    const Frame f = {inv_view, inv_proj, W, H};
+    const V3 eye = f.origin();
+    const int Wr = std::max(W / rdiv, 1), Hr = std::max(H / rdiv, 1);
+    const Tex trf = {refl, Wr, Hr, 4};
+    const V3 Ls = v3(sun_dir[0], sun_dir[1], sun_dir[2]), Lm = v3(moon_dir[0], moon_dir[1], moon_dir[2]);
+    const V3 Rs = v3(sun_rad[0], sun_rad[1], sun_rad[2]), Rm = v3(moon_rad[0], moon_rad[1], moon_rad[2]);
+    const float r = roughness;
+    float sun_vis = fclamp(sun_dir[1] + 0.05f, 0.0f, 0.1f) * 12.0f;
+    sun_vis = 1.0f - sun_vis;
+    if (mut(MUT_CMP_SUNVIS)) sun_vis = fclamp(sun_vis, 0.0f, 1.0f);
+#pragma omp parallel for schedule(static)
+    for (int py = 0; py < H; py++) {
+        for (int px = 0; px < W; px++) {
+            const size_t i = (size_t)py * W + px;
+            const float u = texcoord(px, W), v = texcoord(py, H);
+            const float dist = 1.0f / (1.0f / gb_t[i]);
+            float* o = out + 3 * i;
+            if (!(dist > 0.0f)) {
+                const float z = mut(MUT_CMP_SKY) ? 0.0f : 1.0f;
+                o[0] = sky_pixel[0] * z;
+                o[1] = sky_pixel[1] * z;
+                o[2] = sky_pixel[2] * z;
+                continue;
+            }
+            const V3 P = position_from(f, u, v, dist);
+            const V3 N = face_normal(gb_face[i], 1.0f);
+            const int ref = gb_block[i] & 127;
+            const V3 a0 = v3(albedo[3 * ref], albedo[3 * ref + 1], albedo[3 * ref + 2]);
+            const float adj = mut(MUT_CMP_SAT) ? 0.1f : 1.0f - 0.1f;
+            const float lum = (a0.x * 0.2125f + a0.y * 0.7154f) + a0.z * 0.0721f;
+            const V3 A = v3(lum * (1.0f - adj) + a0.x * adj, lum * (1.0f - adj) + a0.y * adj,
+                            lum * (1.0f - adj) + a0.z * adj);
+            const float shd = mut(MUT_CMP_SHADOW)
+                                  ? 0.0f
+                                  : fclamp(fclamp(fclamp(shadow[i], 0.0f, 1.0f) + 0.0f, 0.0f, 1.0f), 0.0f, 1.0f);
+            /* SHToIrridiance(SHy, CoCg, N) (:356-368) */
+            const float* s4 = sh + 4 * i;
+            const float x = (s4[0] * N.x + s4[1] * N.y) + s4[2] * N.z;
+            float Y = 2.0f * ((mut(MUT_CMP_SH_L1) ? 0.511664f : 1.023326f) * x + 0.886226f * s4[3]);
+            Y = fmaxf(Y, 0.0f);
+            const float kk = (Y * 0.282095f) / (s4[3] + 1e-6f);
+            const int ic = mut(MUT_CMP_COCG) ? 1 : 0;
+            const float co = cocg[2 * i + ic] * kk, cg = cocg[2 * i + 1 - ic] * kk;
+            const float T = Y - cg * 0.5f;
+            const float G = cg + T;
+            const float B = T - co * 0.5f;
+            const float Rr = B + co;
+            V3 ed = v3(fmaxf(Rr, 0.0f), fmaxf(G, 0.0f), fmaxf(B, 0.0f));
+            const float vb = 0.00125f;
+            if (u > vb && u < 1.0f - vb && v > vb && v < 1.0f - vb) {
+                float aov = ao[i];
+                const float dx = P.x - eye.x, dz = P.z - eye.z;
+                const float fade = sqrtf(dx * dx + dz * dz) < 256.0f ? 0.0f : 1.0f;
+                aov = aov * (1.0f - fade) + 1.0f * fade;
+                if (!mut(MUT_CMP_AO_EXP)) aov = vpt_powf(aov, 2.2f);
+                const float m = fclamp(aov, 0.05f, 1.0f);
+                ed = v3(ed.x * m, ed.y * m, ed.z * m);
+            }
+            const V3 sd = color_direct(P, eye, Ls, Rs, A, N, r, shd);
+            const V3 md = color_direct(P, eye, Lm, Rm, A, N, r, shd);
+            V3 dl = v3(sd.x * (1.0f - sun_vis) + md.x * sun_vis, sd.y * (1.0f - sun_vis) + md.y * sun_vis,
+                       sd.z * (1.0f - sun_vis) + md.z * sun_vis);
+            dl = v3(fmaxf(dl.x, 0.000001f), fmaxf(dl.y, 0.000001f), fmaxf(dl.z, 0.000001f));
+            const V3 di = v3(ed.x * A.x, ed.y * A.y, ed.z * A.z);
+            V3 si = v3(0.0f, 0.0f, 0.0f);
+            const V3 I = normalize3(v3(P.x - eye.x, P.y - eye.y, P.z - eye.z));
+            const float kr = 2.0f * dot3(N, I);
+            const V3 R = v3(I.x - kr * N.x, I.y - kr * N.y, I.z - kr * N.z);
+            const float sb = 0.0035f;
+            if ((u > sb && u < 1.0f - sb && v > sb && v < 1.0f - sb) || mut(MUT_CMP_SPEC_BORDER)) {
+                const float att = fclamp(vpt_powi(fmaxf(dot3(N, N), 0.00000001f), 2) * 1.5f, 0.1f, 1.0f);
+                si = v3(bilinear(trf, u, v, 0) * att, bilinear(trf, u, v, 1) * att, bilinear(trf, u, v, 2) * att);
+            }
+            si = v3(fmaxf(0.0001f, si.x), fmaxf(0.0001f, si.y), fmaxf(0.0001f, si.z));
+            const V3 Lo = normalize3(v3(eye.x - P.x, eye.y - P.y, eye.z - P.z));
+            const float nov = fclamp(dot3(N, Lo), 0.0000000001f, 1.0f);
+            float sao = fclamp((vpt_powf(nov + 1.0f, spec_ao_exp) - 1.0f) + 1.0f, 0.0f, 1.0f);
+            sao = vpt_powf(sao, 1.5f);
+            float hz = fminf(1.0f + dot3(R, N), 1.0f);
+            hz = hz * (hz * hz);
+            si = v3((si.x * sao) * hz, (si.y * sao) * hz, (si.z * sao) * hz);
+            const float F = fclamp(fresnel_schlick_roughness(Lo, N, 0.04f, r), 0.0f, 1.0f);
+            const float kd = mut(MUT_CMP_KD) ? 1.0f : (1.0f - F) * (1.0f - 0.0f);
+            float ex, ey;
+            karis_env_brdf(fclamp(fmaxf(-dot3(I, N), 0.000001f), 0.0f, 1.0f), fclamp(r, 0.0f, 1.0f), &ex, &ey);
+            const float fs = F * ex + ey;
+            const float ao_d = fclamp(1.0f, 0.2f, 1.01f);
+            o[0] = dl.x + (kd * (di.x * ao_d) + si.x * fs);
+            o[1] = dl.y + (kd * (di.y * ao_d) + si.y * fs);
+            o[2] = dl.z + (kd * (di.z * ao_d) + si.z * fs);
         }
     }
 }

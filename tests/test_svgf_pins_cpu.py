@@ -689,7 +689,9 @@ def _fails(fn):
 
 
 def _mutations():
-    return list(range(1, oracle.mutation_count()))
+    """The planted mistakes of the passes pinned here (the others belong to
+    the reflection-shading and composite pins)."""
+    return [m for m in range(1, oracle.mutation_count()) if oracle.mutation_name(m).split(":")[0] in CHECKS]
 
 
 @pytest.mark.parametrize("mid", _mutations(), ids=[oracle.mutation_name(m) for m in _mutations()])

@@ -61,6 +61,17 @@ class ReflectParams(C.Structure):
                 ("spp", C.c_int32), ("rough", C.c_int32), ("roughness", C.c_float), ("frame", C.c_uint32)]
 
 
+class ReflectShading(C.Structure):
+    _fields_ = [("d_diffuse", C.c_void_p), ("d_shadow", C.c_void_p), ("viewproj", C.c_float * 16),
+                ("sun_radiance", C.c_float * 3), ("sky_radiance", C.c_float * 3), ("albedo", C.c_float * 384)]
+
+
+class CompositeParams(C.Structure):
+    _fields_ = [("sun_dir", C.c_float * 3), ("moon_dir", C.c_float * 3), ("sun_radiance", C.c_float * 3),
+                ("moon_radiance", C.c_float * 3), ("sky_pixel_radiance", C.c_float * 3), ("roughness", C.c_float),
+                ("albedo", C.c_float * 384)]
+
+
 class McBlock(C.Structure):
     _fields_ = [("name", C.c_char_p), ("id", C.c_uint8), ("data", C.c_uint8)]
 
@@ -143,6 +154,12 @@ _PROTOS = {
     "vpt_trace_shadow2": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(_f), _i, _vp, _vp, _vp]),
     "vpt_trace_reflect": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(ReflectParams), _vp, _vp, _vp]),
     "vpt_reflect_params_default": (_i, [C.POINTER(_f), C.c_uint32, C.POINTER(ReflectParams)]),
+    "vpt_reflect_shading_from_gi": (_i, [C.POINTER(GiParams), C.POINTER(_f), _vp, _vp, C.POINTER(ReflectShading)]),
+    "vpt_trace_reflect_shaded": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(ReflectParams),
+                                      C.POINTER(ReflectShading), _vp, _vp, _vp, _vp]),
+    "vpt_composite_params_from_gi": (_i, [C.POINTER(GiParams), C.POINTER(CompositeParams)]),
+    "vpt_composite": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(CompositeParams), _vp, _vp, _vp, _i,
+                           _vp, _vp]),
     "vpt_trace_gi": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(GiParams), _vp, _vp, _vp]),
     "vpt_trace_gi_noalloc": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(GiParams), _vp, _vp,
                                   _vp]),
@@ -203,6 +220,7 @@ _PROTOS = {
     "vpt_svgf_frame": (_i, [_vp, C.POINTER(Camera), C.POINTER(_f), C.POINTER(GBuffer), C.POINTER(GBuffer), _vp, _f,
                             _vp, _vp, _vp]),
     "vpt_svgf_read_plane": (_i, [_vp, _i, _vp, _vp]),
+    "vpt_svgf_final": (_i, [_vp, C.POINTER(_vp)]),
     "vpt_svgf_read_utility": (_i, [_vp, _vp, _vp]),
     "vpt_shadow_denoise_create": (_i, [_vp, _i, _i, _vp]),
     "vpt_shadow_denoise_destroy": (_i, [_vp]),

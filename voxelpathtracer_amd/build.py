@@ -47,7 +47,7 @@ DEVICE = [
 ]
 
 SOURCES = ["vpt_host.cpp", "vpt_mca.cpp", "vpt_world.hip", "vpt_trace.hip", "vpt_batch.hip", "vpt_svgf.hip", "vpt_server.hip",
-           "vpt_unet.hip", "vpt_graph.hip"]
+           "vpt_unet.hip", "vpt_graph.hip", "vpt_composite.hip"]
 
 
 def _compile(src: str) -> Path:

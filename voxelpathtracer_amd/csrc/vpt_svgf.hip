@@ -1119,6 +1119,13 @@ extern "C" VPT_API int vpt_svgf_read_utility(vpt_svgf* s, uint16_t* d_dst, void*
     return VPT_OK;
 }
 
+extern "C" VPT_API int vpt_svgf_final(const vpt_svgf* s, const vpt_svgf_texel** out)
+{
+    if (!s || !out) return set_error(VPT_EINVAL, "vpt_svgf_final: null argument");
+    *out = s->d_atrous[s->last_final];
+    return VPT_OK;
+}
+
 extern "C" VPT_API int vpt_svgf_read_plane(vpt_svgf* s, int which, vpt_svgf_texel* d_dst, void* stream)
 {
     if (!s || !d_dst) return set_error(VPT_EINVAL, "vpt_svgf_read_plane: null argument");

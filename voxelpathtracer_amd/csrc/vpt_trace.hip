@@ -616,6 +616,91 @@ __device__ inline int reflect_shadow(const WorldView& w, Vec3 o, Vec3 R, const v
     return sh.t > 0.0f ? 1 : 0;
 }
 
+// Hit shading of the reflection pass (ReflectionTraceFrag.glsl:582-593,
+// 843-1023): the frame's denoised diffuse and sun shadow, its
+// view-projection, the light and sky radiances and the albedo table.
+struct ReflShade {
+    const vpt_svgf_texel* diffuse;   // W x H final SVGF texels
+    const uint8_t* shadow;           // W x H denoised sun shadow (unorm8)
+    float vp[16];                    // u_Projection * u_View
+    float rad[3];                    // SAMPLED_COLOR_MIXED substitute * 0.6
+    float sky[3];                    // GetAtmosphere substitute
+    vpt::AlbedoTable albedo;
+};
+
+// texture(LINEAR, REPEAT) of the SVGF texel plane's SH (4) and CoCg (2) at
+// (s, t), fp32 weights; and of its AO channel; and of a unorm8 plane.
+__device__ inline void texel_linear_sh(const vpt_svgf_texel* __restrict__ p, int W, int H, float s, float t,
+                                      float sh[4], float cc[2], float* ao)
+{
+    const float u = s * (float)W - 0.5f, v = t * (float)H - 0.5f;
+    const float fu = floorf(u), fv = floorf(v);
+    const float a = u - fu, b = v - fv;
+    const int x0 = repeat_idx((int)fu, W), x1 = repeat_idx((int)fu + 1, W);
+    const size_t r0 = (size_t)repeat_idx((int)fv, H) * W, r1 = (size_t)repeat_idx((int)fv + 1, H) * W;
+    const uint4 q00 = *reinterpret_cast<const uint4*>(p + r0 + x0), q10 = *reinterpret_cast<const uint4*>(p + r0 + x1);
+    const uint4 q01 = *reinterpret_cast<const uint4*>(p + r1 + x0), q11 = *reinterpret_cast<const uint4*>(p + r1 + x1);
+    auto lin = [&](uint32_t w00, uint32_t w10, uint32_t w01, uint32_t w11) {
+        const float top = vpt::hw_h2f(w00) * (1.0f - a) + vpt::hw_h2f(w10) * a;
+        const float bot = vpt::hw_h2f(w01) * (1.0f - a) + vpt::hw_h2f(w11) * a;
+        return top * (1.0f - b) + bot * b;
+    };
+    sh[0] = lin(q00.x & 0xffffu, q10.x & 0xffffu, q01.x & 0xffffu, q11.x & 0xffffu);
+    sh[1] = lin(q00.x >> 16, q10.x >> 16, q01.x >> 16, q11.x >> 16);
+    sh[2] = lin(q00.y & 0xffffu, q10.y & 0xffffu, q01.y & 0xffffu, q11.y & 0xffffu);
+    sh[3] = lin(q00.y >> 16, q10.y >> 16, q01.y >> 16, q11.y >> 16);
+    cc[0] = lin(q00.z & 0xffffu, q10.z & 0xffffu, q01.z & 0xffffu, q11.z & 0xffffu);
+    cc[1] = lin(q00.z >> 16, q10.z >> 16, q01.z >> 16, q11.z >> 16);
+    if (ao) {
+        auto d8 = [](uint32_t w) { return vpt_unorm8_decode((uint8_t)((w >> 16) & 0xffu)); };
+        const float top = d8(q00.w) * (1.0f - a) + d8(q10.w) * a;
+        const float bot = d8(q01.w) * (1.0f - a) + d8(q11.w) * a;
+        *ao = top * (1.0f - b) + bot * b;
+    }
+}
+
+__device__ inline float u8_linear(const uint8_t* __restrict__ p, int W, int H, float s, float t)
+{
+    const float u = s * (float)W - 0.5f, v = t * (float)H - 0.5f;
+    const float fu = floorf(u), fv = floorf(v);
+    const float a = u - fu, b = v - fv;
+    const int x0 = repeat_idx((int)fu, W), x1 = repeat_idx((int)fu + 1, W);
+    const size_t r0 = (size_t)repeat_idx((int)fv, H) * W, r1 = (size_t)repeat_idx((int)fv + 1, H) * W;
+    const float top = vpt_unorm8_decode(p[r0 + x0]) * (1.0f - a) + vpt_unorm8_decode(p[r0 + x1]) * a;
+    const float bot = vpt_unorm8_decode(p[r1 + x0]) * (1.0f - a) + vpt_unorm8_decode(p[r1 + x1]) * a;
+    return top * (1.0f - b) + bot * b;
+}
+
+// SHToIrradianceA (ReflectionTraceFrag.glsl:466-475).
+__device__ inline Vec3 sh_to_irradiance_a(const float sh[4], const float cc[2])
+{
+    const float Y = fmaxf(0.0f, 3.544905f * sh[3]);
+    const float k = (Y * 0.282095f) / (sh[3] + 1e-6f);
+    const float co = cc[0] * k, cg = cc[1] * k;
+    const float T = Y - cg * 0.5f;
+    const float G = cg + T;
+    const float B = T - co * 0.5f;
+    const float R = B + co;
+    return vpt::mk3(fmaxf(R, 0.0f), fmaxf(G, 0.0f), fmaxf(B, 0.0f));
+}
+
+// CalculateDirectionalLight of the reflection pass (:313-336), flat
+// dielectric: F0 = 0.04, kd = 1 - F; its specular term is multiplied by a
+// radiance of exactly 0 with finite D and G, so it is +0 and left out.
+__device__ inline Vec3 reflect_direct(Vec3 hp, Vec3 eye, Vec3 L, const float rad[3], Vec3 alb, Vec3 N, float shadow)
+{
+    const float sh = fminf(shadow, 1.0f);
+    const Vec3 Lo = vpt::normalize3(vpt::mk3(eye.x - hp.x, eye.y - hp.y, eye.z - hp.z));
+    const Vec3 Lh = vpt::normalize3(vpt::mk3(L.x + Lo.x, L.y + Lo.y, L.z + Lo.z));
+    const float cos_li = fmaxf(0.0f, vpt::dot3(N, L));
+    const float F = 0.04f + (1.0f - 0.04f) * vpt_powi(1.0f - fmaxf(0.0f, vpt::dot3(Lh, Lo)), 5);
+    const float kd = 1.0f - F;
+    const float vis = vpt::fclamp(1.0f - sh, 0.0f, 1.0f);
+    return vpt::mk3(fmaxf(((kd * alb.x) * rad[0]) * cos_li, 0.0f) * vis,
+                    fmaxf(((kd * alb.y) * rad[1]) * cos_li, 0.0f) * vis,
+                    fmaxf(((kd * alb.z) * rad[2]) * cos_li, 0.0f) * vis);
+}
+
 // One lane per target pixel: spp reflected rays (max_iter steps each) and,
 // for the first max(spp / 4, 1) that hit, the sun shadow ray (shadow_iter
 // steps), all in the lane.  kRough: GGX-sampled rays; otherwise mirror
@@ -626,12 +711,22 @@ __device__ inline int reflect_shadow(const WorldView& w, Vec3 o, Vec3 R, const v
 //             (0.001 + sum of T) / hits, clamped to [-10, 200]);
 //   hit_out:  face | block << 8 of the first hit | shadow << 16 |
 //             hits << 24, 255 without a hit.
-template <bool kRough>
+// kShade adds the hit shading (see ReflShade): each hit is reprojected with
+// the frame's view-projection; reprojected hits (position within 0.095 and
+// the same normal at uv, uv inside the 0.0035 border) take their ambient
+// from the denoised diffuse there, times pow(AO, 0.75) near the eye, and,
+// for the shadow-casting samples, the denoised shadow there instead of a
+// shadow ray; other hits take the ambient at the pixel itself.  A hit adds
+// ambient * albedo + reflect_direct(), a miss the sky; color_out: their mean
+// over spp clamped to [1e-7, 100], RGBA binary16 (sky pixels 0); hit_out bit
+// 17: the first hit was reprojected; the shadow bit: the last shadow >= 0.5.
+template <bool kRough, bool kShade>
 __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, int gW, int gH,
                                                       const float* __restrict__ gb_t,
                                                       const uint8_t* __restrict__ gb_face, ReflectConst rc,
                                                       uint16_t* __restrict__ dist_out, uint32_t* __restrict__ hit_out,
-                                                      unsigned long long* __restrict__ counter)
+                                                      unsigned long long* __restrict__ counter, const ReflShade shd,
+                                                      uint2* __restrict__ color_out)
 {
     int px, py;
     tile_pixel(cam.W, cam.H, px, py);
@@ -650,15 +745,89 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
         }
         float hd = -1.0f;
         uint32_t rec = 255u;
+        float col[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         if (!(dist < 0.0f)) {   // sky: o_HitDistance -1
             const Vec3 P = vpt::gbuffer_position(cam, px, py, dist);
             const Vec3 N = vpt::face_normal(face, 1.0f);   // GetNormalFromID: ids above 5 -> (1, 1, 1)
-            const Vec3 I = vpt::normalize3(vpt::mk3(P.x - cam.iv[12], P.y - cam.iv[13], P.z - cam.iv[14]));
+            const Vec3 eye = vpt::mk3(cam.iv[12], cam.iv[13], cam.iv[14]);
+            const Vec3 I = vpt::normalize3(vpt::mk3(P.x - eye.x, P.y - eye.y, P.z - eye.z));
             const Vec3 o = vpt::mk3(P.x + N.x * 0.035f, P.y + N.y * 0.035f, P.z + N.z * 0.035f);
             const Vec3 L = vpt::mk3(rc.lx, rc.ly, rc.lz);
             const int shadow_rays = max(rc.spp / 4, 1);
             float sum = 0.001f;
             int hits = 0, shadow = 0;
+            float shadow_f = 0.0f;   // ComputedShadow
+            bool first_reproj = false;
+            float tot[3] = {0.0f, 0.0f, 0.0f};
+            Vec3 base_amb = vpt::mk3(0.0f, 0.0f, 0.0f);
+            const float sx = vpt::texcoord(px, cam.W), sy = vpt::texcoord(py, cam.H);
+            if (kShade) {   // BaseIndirectDiffuse at the target pixel
+                float bs[4], bc[2];
+                texel_linear_sh(shd.diffuse, gW, gH, sx, sy, bs, bc, nullptr);
+                base_amb = sh_to_irradiance_a(bs, bc);
+            }
+            // one hit of a sample: reprojection, shadow, shading (kShade)
+            auto on_hit = [&](const Vec3& R, const vpt::Hit& h, bool trace_shadow) {
+                const Vec3 hn = vpt::face_normal(h.face, 0.0f);
+                const Vec3 hp = vpt::mk3(o.x + R.x * h.t, o.y + R.y * h.t, o.z + R.z * h.t);
+                bool reproj = false;
+                float ru = 0.0f, rv = 0.0f, t_at = 0.0f;
+                Vec3 p_at = vpt::mk3(0.0f, 0.0f, 0.0f);
+                if (kShade) {
+                    const float q0 = vpt::mat_row(shd.vp, 0, hp.x, hp.y, hp.z, 1.0f);
+                    const float q1 = vpt::mat_row(shd.vp, 1, hp.x, hp.y, hp.z, 1.0f);
+                    const float q3 = vpt::mat_row(shd.vp, 3, hp.x, hp.y, hp.z, 1.0f);
+                    ru = (q0 / q3) * 0.5f + 0.5f;
+                    rv = (q1 / q3) * 0.5f + 0.5f;
+                    const float b = 0.0035f;
+                    if (ru < 1.0f - b && ru > b && rv < 1.0f - b && rv > b) {
+                        t_at = gl_linear(gb_t, gW, gH, ru, rv);
+                        const Vec3 dir = vpt::normalize3(vpt::ray_dir_at(cam.iv, cam.ip, ru, rv));
+                        p_at = vpt::mk3(eye.x + dir.x * t_at, eye.y + dir.y * t_at, eye.z + dir.z * t_at);
+                        const Vec3 n_at = vpt::face_normal(gl_nearest(gb_face, gW, gH, ru, rv), 1.0f);
+                        const float dx = fabsf(p_at.x - hp.x), dy = fabsf(p_at.y - hp.y), dz = fabsf(p_at.z - hp.z);
+                        const float err = (dx * dx + dy * dy) + dz * dz;
+                        reproj = err < 0.095f && n_at.x == hn.x && n_at.y == hn.y && n_at.z == hn.z;
+                    }
+                    if (hits == 0) first_reproj = reproj;
+                }
+                if (trace_shadow) {
+                    if (kShade && reproj) {
+                        shadow_f = u8_linear(shd.shadow, gW, gH, ru, rv);
+                    } else {
+                        shadow = reflect_shadow(w, o, R, h, L, rc.shadow_iter, samples, rays);
+                        shadow_f = (float)shadow;
+                    }
+                }
+                if (kShade) {
+                    Vec3 amb = base_amb;
+                    if (reproj) {
+                        float rs[4], rcc[2], ao;
+                        texel_linear_sh(shd.diffuse, gW, gH, ru, rv, rs, rcc, &ao);
+                        amb = sh_to_irradiance_a(rs, rcc);
+                        const float vx = vpt_powf(ao, 0.75f);
+                        if (t_at > 0.0f) {
+                            const float ex = p_at.x - eye.x, ey = p_at.y - eye.y, ez = p_at.z - eye.z;
+                            if (sqrtf((ex * ex + ey * ey) + ez * ez) < 40.0f)
+                                amb = vpt::mk3(amb.x * vx, amb.y * vx, amb.z * vx);
+                        }
+                    }
+                    const int ref = min(max(h.block, 0), 127);
+                    const Vec3 alb = vpt::mk3(shd.albedo.v[3 * ref], shd.albedo.v[3 * ref + 1], shd.albedo.v[3 * ref + 2]);
+                    amb = vpt::mk3(amb.x * alb.x, amb.y * alb.y, amb.z * alb.z);
+                    const Vec3 dl = reflect_direct(hp, eye, L, shd.rad, alb, hn, shadow_f);
+                    tot[0] = tot[0] + (amb.x + dl.x);
+                    tot[1] = tot[1] + (amb.y + dl.y);
+                    tot[2] = tot[2] + (amb.z + dl.z);
+                }
+            };
+            auto on_miss = [&]() {
+                if (kShade) {
+                    tot[0] = tot[0] + shd.sky[0];
+                    tot[1] = tot[1] + shd.sky[1];
+                    tot[2] = tot[2] + shd.sky[2];
+                }
+            };
             if (kRough) {
                 uint32_t state = vpt_pcg_hash(vpt_pcg_hash((uint32_t)px + (uint32_t)cam.W * (uint32_t)py +
                                                            (uint32_t)cam.W * (uint32_t)cam.H * rc.frame) +
@@ -672,13 +841,33 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
                     samples += h.samples;
                     rays++;
                     if (h.t > 0.0f) {
-                        if (shadow_itr < shadow_rays) {
-                            shadow = reflect_shadow(w, o, R, h, L, rc.shadow_iter, samples, rays);
-                            shadow_itr++;
-                        }
+                        on_hit(R, h, shadow_itr < shadow_rays);
+                        if (shadow_itr < shadow_rays) shadow_itr++;
                         if (hits == 0) rec = (uint32_t)(h.face & 0xff) | ((uint32_t)(h.block & 0xff) << 8);
                         sum = sum + h.t;
                         hits++;
+                    } else {
+                        on_miss();
+                    }
+                }
+            } else if (kShade) {
+                // the reference repeats the mirror sample spp times: the ray
+                // and, when not reprojected, its shadow ray are traced once
+                const float k = 2.0f * vpt::dot3(N, I);
+                const Vec3 R = vpt::mk3(I.x - k * N.x, I.y - k * N.y, I.z - k * N.z);   // reflect(I, N)
+                const vpt::Hit h = vpt::traverse(w, o, R, rc.max_iter);
+                samples += h.samples;
+                rays++;
+                int shadow_itr = 0;
+                for (int s = 0; s < rc.spp; s++) {
+                    if (h.t > 0.0f) {
+                        on_hit(R, h, shadow_itr == 0);
+                        if (shadow_itr < shadow_rays) shadow_itr++;
+                        if (hits == 0) rec = (uint32_t)(h.face & 0xff) | ((uint32_t)(h.block & 0xff) << 8);
+                        sum = sum + h.t;
+                        hits++;
+                    } else {
+                        on_miss();
                     }
                 }
             } else {
@@ -694,14 +883,23 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
                     hits = rc.spp;
                 }
             }
+            if (kShade) shadow = shadow_f >= 0.5f ? 1 : 0;
             if (hits > 0) {
                 hd = sum / fmaxf((float)hits, 0.01f);
-                rec |= ((uint32_t)shadow << 16) | ((uint32_t)hits << 24);
+                rec |= ((uint32_t)shadow << 16) | ((uint32_t)hits << 24) | ((uint32_t)first_reproj << 17);
             }
             hd = vpt::fclamp(hd, -10.0f, 200.0f);
+            if (kShade) {
+                const float n = (float)rc.spp;
+                for (int c = 0; c < 3; c++) col[c] = vpt::fclamp(tot[c] / n, 0.0000001f, 100.0f);
+                col[3] = vpt::fclamp(n / n, 0.0000001f, 100.0f);
+            }
         }
         dist_out[i] = (uint16_t)vpt::hw_f2h(hd);
         if (hit_out) hit_out[i] = rec;
+        if (kShade)
+            color_out[i] = make_uint2(vpt::hw_f2h(col[0]) | (vpt::hw_f2h(col[1]) << 16),
+                                      vpt::hw_f2h(col[2]) | (vpt::hw_f2h(col[3]) << 16));
     }
     count_samples(counter, samples, rays);
 }
@@ -756,21 +954,35 @@ extern "C" VPT_API int vpt_trace_shadow2(vpt_world* w, const vpt_camera* cam, co
     return VPT_OK;
 }
 
-extern "C" VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
-                                         const vpt_reflect_params* prm, uint16_t* d_dist, uint32_t* d_hit,
-                                         void* stream)
+static int reflect_launch(const char* fn, vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                          const vpt_reflect_params* prm, const vpt_reflect_shading* shading, uint16_t* d_dist,
+                          uint32_t* d_hit, uint16_t* d_color, void* stream)
 {
-    if (!w || !prm || !d_dist) return set_error(VPT_EINVAL, "vpt_trace_reflect: null argument");
-    if (int rc = vpt::check_cam(cam, "vpt_trace_reflect")) return rc;
-    if (int rc = vpt::check_gbuffer(gb, cam, "vpt_trace_reflect", false)) return rc;
-    if (int rc = vpt::check_iter(prm->max_iter, "vpt_trace_reflect")) return rc;
-    if (int rc = vpt::check_iter(prm->shadow_iter, "vpt_trace_reflect")) return rc;
+    if (!w || !prm || !d_dist) return set_error(VPT_EINVAL, "%s: null argument", fn);
+    if (int rc = vpt::check_cam(cam, fn)) return rc;
+    if (int rc = vpt::check_gbuffer(gb, cam, fn, false)) return rc;
+    if (int rc = vpt::check_iter(prm->max_iter, fn)) return rc;
+    if (int rc = vpt::check_iter(prm->shadow_iter, fn)) return rc;
     const int div = prm->div;
     if (div < 1 || div > 16 || cam->width / div < 1 || cam->height / div < 1)
-        return set_error(VPT_EINVAL, "vpt_trace_reflect: div must be 1..16 and leave at least one pixel");
-    if (prm->spp < 1 || prm->spp > 16) return set_error(VPT_EINVAL, "vpt_trace_reflect: spp %d not in 1..16", prm->spp);
-    if (prm->rough != 0 && prm->rough != 1) return set_error(VPT_EINVAL, "vpt_trace_reflect: rough must be 0 or 1");
-    if (!(prm->roughness == prm->roughness)) return set_error(VPT_EINVAL, "vpt_trace_reflect: roughness is NaN");
+        return set_error(VPT_EINVAL, "%s: div must be 1..16 and leave at least one pixel", fn);
+    if (prm->spp < 1 || prm->spp > 16) return set_error(VPT_EINVAL, "%s: spp %d not in 1..16", fn, prm->spp);
+    if (prm->rough != 0 && prm->rough != 1) return set_error(VPT_EINVAL, "%s: rough must be 0 or 1", fn);
+    if (!(prm->roughness == prm->roughness)) return set_error(VPT_EINVAL, "%s: roughness is NaN", fn);
+    ReflShade shd{};
+    if (shading) {
+        if (!shading->d_diffuse || !shading->d_shadow || !d_color)
+            return set_error(VPT_EINVAL, "%s: diffuse, shadow and colour planes are required", fn);
+        shd.diffuse = shading->d_diffuse;
+        shd.shadow = shading->d_shadow;
+        memcpy(shd.vp, shading->viewproj, sizeof(shd.vp));
+        for (int c = 0; c < 3; c++) {
+            shd.rad[c] = shading->sun_radiance[c] * 0.6f;
+            shd.sky[c] = shading->sky_radiance[c];
+        }
+        static_assert(sizeof(shd.albedo.v) == sizeof(shading->albedo), "albedo table size");
+        memcpy(shd.albedo.v, shading->albedo, sizeof(shd.albedo.v));
+    }
     DeviceGuard g(w->device);
     CamView cv = vpt::make_cam(cam);
     cv.W = cam->width / div;
@@ -778,15 +990,47 @@ extern "C" VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, co
     const ReflectConst rc = {prm->light_dir[0], prm->light_dir[1], prm->light_dir[2], div, prm->max_iter,
                              prm->shadow_iter, prm->spp, prm->rough, fminf(fmaxf(prm->roughness, 0.01f), 1.0f),
                              prm->frame};
-    if (rc.rough)
-        hipLaunchKernelGGL(reflect_kernel<true>, tile_grid(cv.W, cv.H), dim3(256), 0, (hipStream_t)stream,
-                           vpt::make_view(w), cv, cam->width, cam->height, gb->d_t, gb->d_face, rc, d_dist, d_hit,
-                           w->d_counter);
+    uint2* col = reinterpret_cast<uint2*>(d_color);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, tile_grid(cv.W, cv.H), dim3(256), 0, (hipStream_t)stream, vpt::make_view(w), cv,
+                           cam->width, cam->height, gb->d_t, gb->d_face, rc, d_dist, d_hit, w->d_counter, shd, col);
+    };
+    if (shading)
+        rc.rough ? go(reflect_kernel<true, true>) : go(reflect_kernel<false, true>);
     else
-        hipLaunchKernelGGL(reflect_kernel<false>, tile_grid(cv.W, cv.H), dim3(256), 0, (hipStream_t)stream,
-                           vpt::make_view(w), cv, cam->width, cam->height, gb->d_t, gb->d_face, rc, d_dist, d_hit,
-                           w->d_counter);
+        rc.rough ? go(reflect_kernel<true, false>) : go(reflect_kernel<false, false>);
     HIP_TRY(hipGetLastError());
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                         const vpt_reflect_params* prm, uint16_t* d_dist, uint32_t* d_hit,
+                                         void* stream)
+{
+    return reflect_launch("vpt_trace_reflect", w, cam, gb, prm, nullptr, d_dist, d_hit, nullptr, stream);
+}
+
+extern "C" VPT_API int vpt_trace_reflect_shaded(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                                const vpt_reflect_params* prm, const vpt_reflect_shading* shading,
+                                                uint16_t* d_dist, uint32_t* d_hit, uint16_t* d_color, void* stream)
+{
+    if (!shading) return set_error(VPT_EINVAL, "vpt_trace_reflect_shaded: null shading");
+    return reflect_launch("vpt_trace_reflect_shaded", w, cam, gb, prm, shading, d_dist, d_hit, d_color, stream);
+}
+
+extern "C" VPT_API int vpt_reflect_shading_from_gi(const vpt_gi_params* gi, const float viewproj[16],
+                                                   const vpt_svgf_texel* d_diffuse, const uint8_t* d_shadow,
+                                                   vpt_reflect_shading* out)
+{
+    if (!gi || !viewproj || !out) return set_error(VPT_EINVAL, "vpt_reflect_shading_from_gi: null argument");
+    vpt_reflect_shading r{};
+    r.d_diffuse = d_diffuse;
+    r.d_shadow = d_shadow;
+    memcpy(r.viewproj, viewproj, sizeof(r.viewproj));
+    memcpy(r.sun_radiance, gi->light_color, sizeof(r.sun_radiance));
+    memcpy(r.sky_radiance, gi->sky_radiance, sizeof(r.sky_radiance));
+    memcpy(r.albedo, gi->albedo, sizeof(r.albedo));
+    *out = r;
     return VPT_OK;
 }
 

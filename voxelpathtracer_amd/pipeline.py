@@ -56,6 +56,13 @@ class SVGF:
     def read_utility(self, dst, stream=None):
         call("vpt_svgf_read_utility", self.handle, _ptr(dst), _stream(stream))
 
+    def final(self) -> int:
+        """Device address of the last frame's final texels (vpt_svgf_final),
+        valid until the next frame is enqueued."""
+        p = C.c_void_p()
+        call("vpt_svgf_final", self.handle, C.byref(p))
+        return p.value or 0
+
     def close(self):
         if self.handle:
             lib().vpt_svgf_destroy(self.handle)

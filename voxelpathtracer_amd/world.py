@@ -171,6 +171,19 @@ class World:
         call("vpt_trace_reflect", self.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), _ptr(dist),
              _ptr(hit), _stream(stream))
 
+    def trace_reflect_shaded(self, cam: Camera, gbuf: "DeviceGBuffer", light_dir, shading, dist, color, hit=None,
+                             div: int = 4, max_iter: int = 64, shadow_iter: int = 150, spp: int = 1, rough: int = 0,
+                             roughness: float = 0.425, frame: int = 0, stream=None) -> None:
+        """trace_reflect plus the hit shading (vpt_trace_reflect_shaded):
+        hits reprojected into the frame's denoised diffuse and shadow,
+        shaded with the albedo table; `color` receives RGBA binary16 bits
+        per target pixel ((H // div, W // div, 4) int16/uint16 tensor).
+        shading: a ReflectShading (reflect_shading() builds one)."""
+        prm = _lib.ReflectParams((C.c_float * 3)(*map(float, light_dir)), div, max_iter, shadow_iter, spp, rough,
+                                 float(roughness), int(frame))
+        call("vpt_trace_reflect_shaded", self.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm),
+             C.byref(shading), _ptr(dist), _ptr(hit), _ptr(color), _stream(stream))
+
     def trace_frame(self, cam: Camera, gbuf: "DeviceGBuffer", prm, mask, rgb, texels=None, primary_iter: int = 350,
                     shadow_iter: int = 350, transversal=None, stream=None) -> None:
         """Primary, shadow and GI passes of one frame; the shadow pass runs
@@ -304,6 +317,41 @@ def mc_import(directory: str, dims, origin=(0.0, 0.0, 0.0), mc_to_block=None, na
     call("vpt_mc_import", str(directory).encode(), nx, ny, nz, org, lut.ctypes.data_as(C.c_void_p),
          C.cast(table, C.c_void_p), len(items), grid.ctypes.data_as(C.c_void_p), stats)
     return grid, list(stats)
+
+
+def _addr(x) -> int:
+    """Device address of a tensor or an integer pointer."""
+    return x if isinstance(x, int) else _ptr(x).value or 0
+
+
+def reflect_shading(gi: _lib.GiParams, viewproj, diffuse, shadow) -> _lib.ReflectShading:
+    """Hit shading of the reflection pass from the frame's GI constants
+    (vpt_reflect_shading_from_gi): diffuse: the denoised SVGF texels (a
+    tensor or SVGF.final()), shadow: the denoised shadow (uint8 tensor),
+    viewproj: the frame's projection * view."""
+    out = _lib.ReflectShading()
+    vp = (C.c_float * 16)(*map(float, np.asarray(viewproj, np.float32).ravel()))
+    call("vpt_reflect_shading_from_gi", C.byref(gi), vp, C.c_void_p(_addr(diffuse)), C.c_void_p(_addr(shadow)),
+         C.byref(out))
+    return out
+
+
+def composite_params(gi: _lib.GiParams) -> _lib.CompositeParams:
+    """The colour composite's constants from the frame's GI constants
+    (vpt_composite_params_from_gi)."""
+    out = _lib.CompositeParams()
+    call("vpt_composite_params_from_gi", C.byref(gi), C.byref(out))
+    return out
+
+
+def composite(ctx: Context, cam: Camera, gbuf: "DeviceGBuffer", prm: _lib.CompositeParams, diffuse, shadow, reflect,
+              reflect_div: int, rgb, stream=None) -> None:
+    """The colour pass's composite (vpt_composite): rgb (H, W, 3) float32
+    from the G-buffer, the denoised diffuse texels (tensor or
+    SVGF.final()), the denoised shadow and the shaded reflection colour at
+    (H // reflect_div, W // reflect_div)."""
+    call("vpt_composite", ctx.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), C.c_void_p(_addr(diffuse)),
+         _ptr(shadow), _ptr(reflect), int(reflect_div), _ptr(rgb), _stream(stream))
 
 
 def reflect_params(light_dir, frame: int = 0) -> dict:
