@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving benchmark")
     ap.add_argument("--serve-requests", type=int, default=3000)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-denoiser section")
+    ap.add_argument("--no-final-image", action="store_true", help="skip the final-image PSNR section")
     ap.add_argument("--unet-pairs", type=int, default=2048)
     ap.add_argument("--unet-steps", type=int, default=2000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -248,6 +249,7 @@ def main():
     nf = args.svgf_frames
     seq = []
     shadow_in = []   # per frame: shadow mask and transversal (vpt_trace_shadow2)
+    cur_vps = []     # per frame: its own projection * view
     prev_vm = None
     for f in range(nf):
         pos = (CAM_POS[0] + 0.1 * f, CAM_POS[1], CAM_POS[2])
@@ -262,6 +264,7 @@ def main():
         vp_f = viewproj(*(prev_vm if prev_vm else (v_f, p_f)))
         seq.append((c_f, vp_f, g_f, t_f))
         shadow_in.append((m_f, tv_f))
+        cur_vps.append(viewproj(v_f, p_f))
         prev_vm = (v_f, p_f)
     torch.cuda.synchronize()
     sv = SVGF(ctx, W, H)
@@ -269,13 +272,17 @@ def main():
 
     eval_frames = [f for f in (15, 31, 47, 63) if f < nf]
     svgf_out = {}
+    svgf_texels = {}   # eval frames: the final SVGF texels (the composite's diffuse)
 
     def svgf_sequence(keep=False):
         sv.reset()
         for f, (c_f, vp_f, g_f, t_f) in enumerate(seq):
             g_prev = seq[f - 1][2] if f else g_f
-            sv.frame(c_f, vp_f, g_f, g_prev, t_f, den, time=f / 60.0)
-            if keep and f in eval_frames:
+            keep_f = keep and f in eval_frames
+            if keep_f:
+                svgf_texels[f] = torch.empty((H, W, 8), dtype=torch.int16, device="cuda")
+            sv.frame(c_f, vp_f, g_f, g_prev, t_f, den, time=f / 60.0, texels_out=svgf_texels[f] if keep_f else None)
+            if keep_f:
                 svgf_out[f] = den.clone()
 
     svgf_sequence(keep=True)
@@ -293,14 +300,18 @@ def main():
     sd = ShadowDenoiser(ctx, W, H)
     sh_out = torch.empty((H, W), dtype=torch.uint8, device="cuda")
 
-    def shadow_sequence():
+    shadow_eval = {}   # eval frames: (denoised shadow, raw mask)
+
+    def shadow_sequence(keep=False):
         sd.reset()
         for f, (c_f, vp_f, g_f, _) in enumerate(seq):
             g_prev = seq[f - 1][2] if f else g_f
             m_f, tv_f = shadow_in[f]
             sd.frame(c_f, vp_f, g_f, g_prev, m_f, tv_f, sh_out)
+            if keep and f in eval_frames:
+                shadow_eval[f] = (sh_out.clone(), m_f.clone())
 
-    shadow_sequence()
+    shadow_sequence(keep=True)
     torch.cuda.synchronize()
     s0, s1 = ev(), ev()
     s0.record(stream)
@@ -374,7 +385,8 @@ def main():
     p1.record(stream)
     torch.cuda.synchronize()
     pipeline_graph_ms = p0.elapsed_time(p1) / 16
-    pipeline_graph_stats = {"side_chain": pipe.graphs[0].stats(), "main_chain": pipe.graphs[1].stats()}
+    pipeline_graph_stats = {"side_chain": pipe.graphs[0].stats(), "main_chain": pipe.graphs[1].stats(),
+                            "tail": pipe.graphs[2].stats()}
     del pipe
 
     result = {
@@ -414,9 +426,13 @@ def main():
                      "traffic": profiled_traffic(f"{dom} pass"), "algorithmic_bytes": alg[dom]},
     }
 
+    unet_out = {}
     if not args.no_learned:
-        result["learned"] = learned(ctx, world, seq, eval_frames, svgf_out, svgf_ms, args)
-    del seq
+        result["learned"] = learned(ctx, world, seq, eval_frames, svgf_out, svgf_ms, args, keep=unet_out)
+    if not args.no_final_image:
+        result["final_image"] = final_image(ctx, world, seq, cur_vps, eval_frames, svgf_texels, shadow_eval,
+                                            unet_out)
+    del seq, svgf_texels, shadow_eval, unet_out
     if not args.no_serve:
         result["serve"] = serve(args)
     if args.copy_probe:
@@ -520,13 +536,14 @@ def serve(args):
     return out
 
 
-def learned(ctx, world, seq, eval_frames, svgf_out, svgf_ms, args):
+def learned(ctx, world, seq, eval_frames, svgf_out, svgf_ms, args, keep=None):
     """Learned denoiser: trained on the GPU by its own kernels on pairs the
     tracer renders (1 spp -> 256 spp, 128x128 windows of a 1080p frustum
     over random seeded worlds and cameras), then run at 1920x1080 on the
     SVGF sequence.  PSNR (tonemapped x/(1+x)) against 256-spp renders of the
     same frames for the raw 1-spp irradiance, SVGF and the network; time per
-    frame of the network (assembly + forward) beside SVGF's."""
+    frame of the network (assembly + forward) beside SVGF's.  keep: filled
+    with the network's irradiance per evaluated frame."""
     import torch
     from voxelpathtracer_amd import gi_params
     from voxelpathtracer_amd.unet import UNet, irradiance_decode, make_dataset, texel_accumulate, tonemap_psnr, train
@@ -562,6 +579,8 @@ def learned(ctx, world, seq, eval_frames, svgf_out, svgf_ms, args):
         psnr["input_1spp"].append(tonemap_psnr(noisy.cpu().numpy(), r))
         psnr["svgf"].append(tonemap_psnr(svgf_out[f].cpu().numpy(), r))
         psnr["unet"].append(tonemap_psnr(out.cpu().numpy(), r))
+        if keep is not None:
+            keep[f] = out.clone()
     # time per frame at 1080p
     c_f, _, g_f, t_f = seq[-1]
     for _ in range(3):
@@ -588,6 +607,110 @@ def learned(ctx, world, seq, eval_frames, svgf_out, svgf_ms, args):
             "eval_frames": eval_frames, "psnr_db": {k: [round(x, 3) for x in v] for k, v in psnr.items()},
             "psnr_mean_db": mean, "unet_ms_per_frame": round(unet_ms, 4), "svgf_ms_per_frame": round(svgf_ms, 4),
             "unet_tflops": round(flops / (unet_ms * 1e-3) / 1e12, 2), "unet_gflop_per_frame": round(flops / 1e9, 2)}
+
+
+def _texels_from_irradiance(E, ao_texels):
+    """Texels whose SHToIrridiance at any normal is E (H, W, 3): L1 = 0,
+    L0 = Y / (2 * 0.886226) and CoCg over the decode's CoCg scale (YCoCg-R
+    of E), rounded to binary16; the AO byte from ao_texels."""
+    import torch
+    r, g, b = E.unbind(-1)
+    Y = r / 4 + g / 2 + b / 4
+    co, cg = r - b, g - (r + b) / 2
+    l0 = (Y / (2.0 * 0.886226)).half().float()
+    y_dec = 2.0 * (0.886226 * l0)
+    kk = (y_dec * 0.282095) / (l0 + 1e-6)
+    kk = torch.where(kk > 0, kk, torch.ones_like(kk))
+    out = torch.zeros(E.shape[:2] + (8,), dtype=torch.float16, device=E.device)
+    out[..., 3] = l0.half()
+    out[..., 4] = (co / kk).half()
+    out[..., 5] = (cg / kk).half()
+    t = out.view(torch.int16)
+    t[..., 6] = 0
+    t[..., 7] = ao_texels[..., 7] & 0xFF
+    return t
+
+
+def _accumulate_texels(world, cam, gb, frame, spp, W, H):
+    """Mean of `spp` GI texel planes (seeds 100000 + spp * frame + k): SH and
+    CoCg in binary16, AO byte rounded."""
+    import torch
+    from voxelpathtracer_amd import gi_params
+    from voxelpathtracer_amd.unet import texel_accumulate
+    rgb = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    tex = torch.empty((H, W, 8), dtype=torch.int16, device="cuda")
+    acc = torch.empty((H, W, 6), dtype=torch.float32, device="cuda")
+    ao = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+    for k in range(spp):
+        world.trace_gi(cam, gb, gi_params(50.0, 100000 + spp * frame + k), rgb, tex)
+        texel_accumulate(tex, W * H, acc, k == 0)
+        ao += (tex[..., 7] & 0xFF).float()
+    out = torch.zeros((H, W, 8), dtype=torch.float16, device="cuda")
+    out[..., :6] = (acc / spp).half()
+    t = out.view(torch.int16)
+    t[..., 6] = 0
+    t[..., 7] = torch.round(ao / spp).to(torch.int16)
+    return t
+
+
+def final_image(ctx, world, seq, cur_vps, eval_frames, svgf_texels, shadow_eval, unet_out, ref_spp=256):
+    """Final-image quality of the frame loop: the composite (vpt_composite)
+    of each evaluated 1080p frame of the SVGF sequence with the reflections
+    at the reference's settings (1/4 resolution, 2 rough samples) shaded
+    from the same planes, for (a) the raw 1-spp GI texels and the raw shadow
+    mask, (b) the SVGF chain's texels and the denoised shadow, (c) the
+    learned denoiser's irradiance (as texels) and the denoised shadow;
+    against the converged frame: 256-spp GI texels, the exact (hard) shadow
+    and the reflection colour averaged over 256 seeds.  PSNR of the
+    tonemapped x / (1 + x) image (peak 1) over the whole frame."""
+    import torch
+    from voxelpathtracer_amd import gi_params
+    from voxelpathtracer_amd.unet import tonemap_psnr
+    from voxelpathtracer_amd.world import composite, composite_params, reflect_params, reflect_shading
+    _, _, g0, _ = seq[0]
+    H, W = g0.height, g0.width
+    sun = list(gi_params(50.0, 0).sun_dir)
+    rp = reflect_params(sun, 0)
+    div = rp["div"]
+    dist = torch.empty((H // div, W // div), dtype=torch.int16, device="cuda")
+    rcol = torch.empty((H // div, W // div, 4), dtype=torch.int16, device="cuda")
+    racc = torch.empty((H // div, W // div, 4), dtype=torch.float32, device="cuda")
+    img = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+
+    def render(f, texels, shadow, seeds):
+        c_f, _, g_f, _ = seq[f]
+        prm = gi_params(50.0, f)
+        shading = reflect_shading(prm, cur_vps[f], texels, shadow)
+        for k, seed in enumerate(seeds):
+            world.trace_reflect_shaded(c_f, g_f, sun, shading, dist, rcol, **{**rp, "frame": seed})
+            if len(seeds) > 1:
+                v = rcol.view(torch.float16).float()
+                if k == 0:
+                    racc.copy_(v)
+                else:
+                    racc.add_(v)
+        if len(seeds) > 1:
+            rcol.view(torch.float16).copy_((racc / len(seeds)).half())
+        composite(ctx, c_f, g_f, composite_params(prm), texels, shadow, rcol, div, img)
+        torch.cuda.synchronize()
+        return img.cpu().numpy()
+
+    psnr = {"input_1spp": [], "svgf": []}
+    if unet_out:
+        psnr["unet"] = []
+    for f in eval_frames:
+        c_f, _, g_f, t_f = seq[f]
+        den_shadow, mask = shadow_eval[f]
+        ref_tex = _accumulate_texels(world, c_f, g_f, f, ref_spp, W, H)
+        ref = render(f, ref_tex, mask * 255, [1000 + ref_spp * f + k for k in range(ref_spp)])
+        psnr["input_1spp"].append(tonemap_psnr(render(f, t_f, mask * 255, [f]), ref))
+        psnr["svgf"].append(tonemap_psnr(render(f, svgf_texels[f], den_shadow, [f]), ref))
+        if unet_out:
+            u_tex = _texels_from_irradiance(unet_out[f], svgf_texels[f])
+            psnr["unet"].append(tonemap_psnr(render(f, u_tex, den_shadow, [f]), ref))
+    return {"eval_frames": eval_frames, "ref_spp": ref_spp, "image": [W, H],
+            "psnr_db": {k: [round(x, 3) for x in v] for k, v in psnr.items()},
+            "psnr_mean_db": {k: round(float(np.mean(v)), 3) for k, v in psnr.items()}}
 
 
 def cpu_baseline(grid, gb, mask, rgb, cam, prm, reps):

@@ -3,9 +3,11 @@
 // into a ping-pong G-buffer (InitialTraceFBO / InitialTraceFBOPrev,
 // :1970-1971), diffuse GI trace (:2171-2281), SVGF (:2284-2615), the sun
 // shadow trace with its transversal (:2790-2852), the sun-shadow denoise
-// (:2854-2948) and the reflection trace's core at the reference's settings
-// (:3000-3060).  Device buffers are plain hipMalloc allocations owned by the
-// pipeline.  Header-only; links against libvpt.so and the HIP runtime.
+// (:2854-2948), the reflection trace at the reference's settings with its hit
+// shading from the denoised planes (:3000-3060), and the colour pass's
+// composite (:3611-3700).  Device buffers are plain hipMalloc allocations
+// owned by the pipeline.  Header-only; links against libvpt.so and the HIP
+// runtime.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -22,14 +24,15 @@ public:
     // persistent: trace primary, sun shadow and GI with the persistent engine
     // (vpt_frame_trace) instead of one launch per pass (vpt_trace_primary,
     // vpt_trace_gi; the faster path).  Either way the shadow pass with its
-    // transversal, the shadow denoise and the reflections run on a side
-    // stream beside GI and SVGF.  use_graph (one launch per pass only): after
-    // the primary pass, the side chain and the main chain are each captured
-    // and replayed as a HIP graph (vpt_graph_*) every frame, the instantiated
-    // graphs updated in place.
+    // transversal and the shadow denoise run on a side stream beside GI and
+    // SVGF; the shaded reflections and the composite follow both.
+    // use_graph (one launch per pass only): after the primary pass, the side
+    // chain, the main chain and the tail are each captured and replayed as a
+    // HIP graph (vpt_graph_*) every frame, the instantiated graphs updated in
+    // place.
     Pipeline(Context& ctx, World& world, int width, int height, float sun_tick = 50.0f, bool persistent = false,
              bool use_graph = false)
-        : m_World(world), m_W(width), m_H(height), m_SunTick(sun_tick), m_Persistent(persistent)
+        : m_Ctx(ctx.Handle()), m_World(world), m_W(width), m_H(height), m_SunTick(sun_tick), m_Persistent(persistent)
     {
         if (use_graph && persistent)
             throw std::invalid_argument("Pipeline: the persistent engine replays its own graph; use_graph is for "
@@ -46,10 +49,13 @@ public:
         Alloc(&m_Mask, px);
         Alloc(&m_Transversal, px * sizeof(uint16_t));
         Alloc(&m_ShadowOut, px);
-        m_RW = width / 4;    // ReflectionTraceResolution 0.25
-        m_RH = height / 4;
+        m_RDiv = width >= 4 && height >= 4 ? 4 : 1;   // ReflectionTraceResolution 0.25
+        m_RW = width / m_RDiv;
+        m_RH = height / m_RDiv;
         Alloc(&m_ReflDist, (size_t)m_RW * m_RH * sizeof(uint16_t) + 16);
         Alloc(&m_ReflHit, (size_t)m_RW * m_RH * sizeof(uint32_t) + 16);
+        Alloc(&m_ReflColor, (size_t)m_RW * m_RH * 4 * sizeof(uint16_t) + 16);
+        Alloc(&m_Color, px * 3 * sizeof(float));
         Alloc(&m_RGB, px * 3 * sizeof(float));
         Alloc(&m_Denoised, px * 3 * sizeof(float));
         Alloc(&m_Texels, px * sizeof(vpt_svgf_texel));
@@ -62,10 +68,8 @@ public:
             hipEventCreateWithFlags(&m_Join, hipEventDisableTiming) != hipSuccess)
             throw std::runtime_error("Pipeline: side stream");
         if (persistent) vpt_check(vpt_frame_create(ctx.Handle(), width, height, &m_Frame));
-        if (use_graph) {
-            vpt_check(vpt_graph_create(ctx.Handle(), &m_Graphs[0]));
-            vpt_check(vpt_graph_create(ctx.Handle(), &m_Graphs[1]));
-        }
+        if (use_graph)
+            for (vpt_graph*& g : m_Graphs) vpt_check(vpt_graph_create(ctx.Handle(), &g));
     }
 
     ~Pipeline()
@@ -90,6 +94,8 @@ public:
         (void)hipFree(m_ShadowOut);
         (void)hipFree(m_ReflDist);
         (void)hipFree(m_ReflHit);
+        (void)hipFree(m_ReflColor);
+        (void)hipFree(m_Color);
         (void)hipFree(m_RGB);
         (void)hipFree(m_Denoised);
         (void)hipFree(m_Texels);
@@ -120,25 +126,36 @@ public:
                                       1, stream));
         else
             vpt_check(vpt_trace_primary(m_World.Handle(), &cam, 350, &cur, stream));
-        float prev_vp[16];
+        float prev_vp[16], cur_vp[16];
         ViewProj(m_PrevView, m_PrevProj, prev_vp);
-        // the side chain (shadow with its transversal, shadow denoise,
-        // reflections) reads only the G-buffer and the shadow pass and runs
-        // beside the main chain (GI, SVGF)
+        ViewProj(view, proj, cur_vp);
+        // the side chain (shadow with its transversal, shadow denoise) reads
+        // only the G-buffer and the shadow pass and runs beside the main
+        // chain (GI, SVGF); the tail (reflections shaded from both chains'
+        // outputs, composite) follows both
         auto side_chain = [&](void* ss) {
             vpt_check(vpt_trace_shadow2(m_World.Handle(), &cam, &cur, prm.sun_dir, 350, m_Mask, m_Transversal, ss));
             vpt_check(vpt_shadow_denoise_frame(m_ShadowDn, &cam, prev_vp, &cur, &prev, m_Mask, m_Transversal, 1.0f,
                                                m_ShadowOut, ss));
-            if (m_RW > 0 && m_RH > 0) {
-                vpt_reflect_params rp;
-                vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
-                vpt_check(vpt_trace_reflect(m_World.Handle(), &cam, &cur, &rp, m_ReflDist, m_ReflHit, ss));
-            }
         };
         auto main_chain = [&](void* ms) {
             if (!m_Persistent) vpt_check(vpt_trace_gi(m_World.Handle(), &cam, &cur, &prm, m_RGB, m_Texels, ms));
             vpt_check(vpt_svgf_frame(m_Svgf, &cam, prev_vp, &cur, &prev, m_Texels, (float)m_Frame_ / 60.0f,
                                      m_Denoised, nullptr, ms));
+        };
+        auto tail = [&](void* ts) {
+            const vpt_svgf_texel* diffuse = nullptr;
+            vpt_check(vpt_svgf_final(m_Svgf, &diffuse));
+            vpt_reflect_params rp;
+            vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
+            rp.div = m_RDiv;
+            vpt_reflect_shading rs;
+            vpt_check(vpt_reflect_shading_from_gi(&prm, cur_vp, diffuse, m_ShadowOut, &rs));
+            vpt_check(vpt_trace_reflect_shaded(m_World.Handle(), &cam, &cur, &rp, &rs, m_ReflDist, m_ReflHit,
+                                               m_ReflColor, ts));
+            vpt_composite_params cp;
+            vpt_check(vpt_composite_params_from_gi(&prm, &cp));
+            vpt_check(vpt_composite(m_Ctx, &cam, &cur, &cp, diffuse, m_ShadowOut, m_ReflColor, m_RDiv, m_Color, ts));
         };
         if (!m_Graphs[0]) {
             if (hipEventRecord(m_Fork, s) != hipSuccess || hipStreamWaitEvent(m_Side, m_Fork, 0) != hipSuccess)
@@ -146,15 +163,19 @@ public:
             side_chain(m_Side);
             if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
             main_chain(stream);
+            if (hipStreamWaitEvent(s, m_Join, 0) != hipSuccess) throw std::runtime_error("Pipeline: join");
+            tail(stream);
         } else {
             // each chain is captured and replayed as a graph of its own
-            // (the two replays start after the primary pass and overlap);
-            // the side chain's completion is joined on the side stream
+            // (the first two replays start after the primary pass and
+            // overlap; the side chain's completion is joined on the side
+            // stream); the tail's replay waits for both
             Replay(m_Graphs[0], stream, m_Side, side_chain);
             if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
             Replay(m_Graphs[1], stream, stream, main_chain);
+            if (hipStreamWaitEvent(s, m_Join, 0) != hipSuccess) throw std::runtime_error("Pipeline: join");
+            Replay(m_Graphs[2], stream, stream, tail);
         }
-        if (hipStreamWaitEvent(s, m_Join, 0) != hipSuccess) throw std::runtime_error("Pipeline: join");
         for (int i = 0; i < 16; i++) {
             m_PrevView[i] = view[i];
             m_PrevProj[i] = proj[i];
@@ -182,14 +203,18 @@ private:
 public:
     const vpt_gbuffer& GBuffer() const { return m_GBuf[(m_Frame_ + 1) % 2]; }
     const float* DenoisedRGB() const { return m_Denoised; }
+    // the composite: 3 floats per pixel (vpt_composite)
+    const float* Color() const { return m_Color; }
     const float* RawRGB() const { return m_RGB; }
     const uint8_t* ShadowMask() const { return m_Mask; }
     // the denoised sun shadow (unorm8 per pixel)
     const uint8_t* DenoisedShadow() const { return m_ShadowOut; }
-    // reflections at (width / 4) x (height / 4): o_HitDistance (binary16) and
-    // the hit record of vpt_trace_reflect
+    // reflections at (width / 4) x (height / 4) (full size below 4 pixels):
+    // o_HitDistance (binary16), the hit record and the RGBA binary16 colour
+    // of vpt_trace_reflect_shaded
     const uint16_t* ReflectionHitDistance() const { return m_ReflDist; }
     const uint32_t* ReflectionHits() const { return m_ReflHit; }
+    const uint16_t* ReflectionColor() const { return m_ReflColor; }
     int ReflectionWidth() const { return m_RW; }
     int ReflectionHeight() const { return m_RH; }
     int FrameIndex() const { return m_Frame_; }
@@ -212,6 +237,7 @@ private:
             }
     }
 
+    vpt_ctx* m_Ctx;
     World& m_World;
     int m_W, m_H;
     float m_SunTick;
@@ -223,16 +249,18 @@ private:
     uint8_t* m_ShadowOut = nullptr;
     uint16_t* m_ReflDist = nullptr;
     uint32_t* m_ReflHit = nullptr;
-    int m_RW = 0, m_RH = 0;
+    uint16_t* m_ReflColor = nullptr;
+    float* m_Color = nullptr;
+    int m_RDiv = 4, m_RW = 0, m_RH = 0;
     vpt_shadow_denoise* m_ShadowDn = nullptr;
-    hipStream_t m_Side = nullptr;   // shadow, shadow denoise, reflections (default priority)
+    hipStream_t m_Side = nullptr;   // shadow, shadow denoise (default priority)
     hipEvent_t m_Fork = nullptr, m_Join = nullptr;
     float* m_RGB = nullptr;
     float* m_Denoised = nullptr;
     vpt_svgf_texel* m_Texels = nullptr;
     vpt_svgf* m_Svgf = nullptr;
     vpt_frame* m_Frame = nullptr;
-    vpt_graph* m_Graphs[2] = {nullptr, nullptr};   // side chain, main chain
+    vpt_graph* m_Graphs[3] = {nullptr, nullptr, nullptr};   // side chain, main chain, tail
     float m_PrevView[16] = {}, m_PrevProj[16] = {};
 };
 

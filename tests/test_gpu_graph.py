@@ -43,17 +43,17 @@ def test_pipeline_graph_replays_equal_eager(ctx, world):
         busy.add_(1.0)
         torch.cuda.synchronize()
         for name in ("denoised", "rgb", "texels", "shadow", "transversal", "shadow_denoised", "reflect_dist",
-                     "reflect_hit"):
+                     "reflect_hit", "reflect_color", "color"):
             a, b = getattr(eager, name), getattr(graphed, name)
             assert torch.equal(a, b), f"frame {f}: {name} differs between graph replay and eager run"
         ga, gb_ = eager.gbuf[f % 2].numpy(), graphed.gbuf[f % 2].numpy()
         for k in ("t", "face", "block", "voxel"):
             np.testing.assert_array_equal(ga[k], gb_[k], err_msg=f"frame {f}: G-buffer {k}")
-    for chain, g in zip(("side", "main"), graphed.graphs):
+    for chain, g in zip(("side", "main", "tail"), graphed.graphs):
         st = g.stats()
         assert st["launches"] == frames, (chain, st)
         assert st["instantiations"] == 1 and st["updates"] == frames - 1, (chain, st)
-        assert st["nodes"] >= 3, (chain, st)
+        assert st["nodes"] >= 2, (chain, st)
     assert float(busy[0]) == frames
 
 

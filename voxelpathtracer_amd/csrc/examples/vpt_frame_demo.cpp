@@ -1,10 +1,12 @@
 // A C++ host driving the hot path through the VoxelRT::World /
 // VoxelRT::Pipeline façades (include/vpt/World.hpp, Pipeline.hpp): terrain
 // world, GPU distance field, N frames of primary + shadow + GI + SVGF +
-// shadow denoise + reflections along the SVGF workload's camera path.
-// Prints one JSON line; with --dump FILE writes the last frame's denoised
-// RGB (float32, H x W x 3), then its denoised shadow (uint8, H x W) and its
-// reflection hit distances (binary16 bits, H/4 x W/4).
+// shadow denoise + shaded reflections + composite along the SVGF workload's
+// camera path.  Prints one JSON line; with --dump FILE writes the last
+// frame's denoised RGB (float32, H x W x 3), then its denoised shadow (uint8,
+// H x W), its reflection hit distances (binary16 bits, H/4 x W/4), its
+// reflection colour (RGBA binary16 bits, H/4 x W/4 x 4) and its composite
+// (float32, H x W x 3).
 //
 //   vpt_frame_demo [W H FRAMES [--plain | --graph] [--dump FILE]]
 //
@@ -69,14 +71,21 @@ int main(int argc, char** argv)
         if (dump) {
             std::vector<uint8_t> shadow((size_t)W * H);
             std::vector<uint16_t> refl((size_t)pipe.ReflectionWidth() * pipe.ReflectionHeight());
+            std::vector<uint16_t> refl_color(refl.size() * 4);
+            std::vector<float> color((size_t)W * H * 3);
             (void)hipMemcpy(shadow.data(), pipe.DenoisedShadow(), shadow.size(), hipMemcpyDeviceToHost);
             (void)hipMemcpy(refl.data(), pipe.ReflectionHitDistance(), refl.size() * sizeof(uint16_t),
                             hipMemcpyDeviceToHost);
+            (void)hipMemcpy(refl_color.data(), pipe.ReflectionColor(), refl_color.size() * sizeof(uint16_t),
+                            hipMemcpyDeviceToHost);
+            (void)hipMemcpy(color.data(), pipe.Color(), color.size() * sizeof(float), hipMemcpyDeviceToHost);
             FILE* fp = fopen(dump, "wb");
             if (!fp) throw std::runtime_error("cannot open dump file");
             fwrite(rgb.data(), sizeof(float), rgb.size(), fp);
             fwrite(shadow.data(), 1, shadow.size(), fp);
             fwrite(refl.data(), sizeof(uint16_t), refl.size(), fp);
+            fwrite(refl_color.data(), sizeof(uint16_t), refl_color.size(), fp);
+            fwrite(color.data(), sizeof(float), color.size(), fp);
             fclose(fp);
         }
         printf("{\"frames\": %d, \"width\": %d, \"height\": %d, \"path\": \"%s\", \"ms_per_frame\": %.4f, "

@@ -178,18 +178,20 @@ def viewproj(view: np.ndarray, proj: np.ndarray) -> np.ndarray:
 
 class Pipeline:
     """Per-frame pass chain for one view of one world, in the reference's
-    order (MainPipeline::StartPipeline, Pipeline.cpp:1960-3060): primary,
-    sun shadow (with its transversal) beside diffuse GI, SVGF, sun-shadow
-    denoise, and the reflection trace's core at the reference's settings
-    (1/4 resolution, 2 rough samples)."""
+    order (MainPipeline::StartPipeline, Pipeline.cpp:1960-3700): primary,
+    sun shadow (with its transversal) and its denoise beside diffuse GI and
+    SVGF, then the reflection trace at the reference's settings (1/4
+    resolution, 2 rough samples) shaded from the denoised planes, and the
+    colour pass's composite into `color` (H, W, 3) float32."""
 
     def __init__(self, ctx: Context, world: World, width: int, height: int, sun_tick: float = 50.0,
                  max_iter: int = 350, shadow_iter: int = 350, use_graph: bool = False):
-        """use_graph: replay each frame's two chains — the side chain
-        (shadow with transversal, shadow denoise, reflections) and the main
-        chain (GI, SVGF) — as HIP graphs launched on their own streams after
-        the primary pass, so that they still overlap; each is captured every
-        frame and its instantiated graph updated in place."""
+        """use_graph: replay each frame's three chains — the side chain
+        (shadow with transversal, shadow denoise) and the main chain (GI,
+        SVGF), launched on their own streams after the primary pass so that
+        they still overlap, and the tail (shaded reflections, composite)
+        after both — as HIP graphs; each is captured every frame and its
+        instantiated graph updated in place."""
         import torch
         self.ctx, self.world = ctx, world
         self.width, self.height = width, height
@@ -200,24 +202,29 @@ class Pipeline:
         self.shadow = torch.empty((height, width), dtype=torch.uint8, device="cuda")
         self.transversal = torch.empty((height, width), dtype=torch.int16, device="cuda")
         self.shadow_denoised = torch.empty((height, width), dtype=torch.uint8, device="cuda")
-        self.reflect_dist = torch.empty((height // 4, width // 4), dtype=torch.int16, device="cuda")
-        self.reflect_hit = torch.empty((height // 4, width // 4), dtype=torch.int32, device="cuda")
+        self.reflect_div = 4 if width >= 4 and height >= 4 else 1   # ReflectionTraceResolution 0.25
+        hr, wr = height // self.reflect_div, width // self.reflect_div
+        self.reflect_dist = torch.empty((hr, wr), dtype=torch.int16, device="cuda")
+        self.reflect_hit = torch.empty((hr, wr), dtype=torch.int32, device="cuda")
+        self.reflect_color = torch.empty((hr, wr, 4), dtype=torch.int16, device="cuda")
+        self.color = torch.empty((height, width, 3), dtype=torch.float32, device="cuda")
         self.svgf = SVGF(ctx, width, height)
         self.shadow_denoiser = ShadowDenoiser(ctx, width, height)
-        self.side = torch.cuda.Stream()   # shadow, shadow denoise, reflections (default priority)
+        self.side = torch.cuda.Stream()   # shadow, shadow denoise (default priority)
         self.sun_tick = sun_tick
         self.max_iter, self.shadow_iter = max_iter, shadow_iter
         self.frame_index = 0
         self.prev_view = None
         self.prev_proj = None
-        self.graphs = (Graph(ctx), Graph(ctx)) if use_graph else None   # side chain, main chain
+        self.graphs = (Graph(ctx), Graph(ctx), Graph(ctx)) if use_graph else None   # side, main, tail
         self.fork = torch.cuda.Event()
         self.join = torch.cuda.Event()
 
     def render(self, pos, yaw: float, pitch: float, stream=None):
-        """One frame: primary; then the sun shadow with its transversal, the
-        shadow denoise and the reflections on a side stream, beside GI and
-        SVGF on `stream`.  Returns the camera."""
+        """One frame: primary; then the sun shadow with its transversal and
+        the shadow denoise on a side stream, beside GI and SVGF on `stream`;
+        then the shaded reflections and the composite.  Returns the
+        camera."""
         import torch
         main = torch.cuda.current_stream() if stream is None else torch.cuda.ExternalStream(int(stream))
         cam, view, proj = make_camera(pos, yaw, pitch, self.width, self.height)
@@ -230,6 +237,7 @@ class Pipeline:
         if self.frame_index == 0:
             prev = cur
         prev_vp = viewproj(self.prev_view, self.prev_proj)
+        cur_vp = viewproj(view, proj)
         self.world.trace_primary(cam, cur, self.max_iter, stream=main.cuda_stream)
 
         # the side passes read only the G-buffer and the shadow pass
@@ -237,15 +245,23 @@ class Pipeline:
             self.world.trace_shadow2(cam, cur, sun, self.shadow, self.transversal, self.shadow_iter, stream=s)
             self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal, self.shadow_denoised,
                                        stream=s)
-            if self.width >= 4 and self.height >= 4:
-                from .world import reflect_params
-                self.world.trace_reflect(cam, cur, sun, self.reflect_dist, self.reflect_hit,
-                                         **reflect_params(sun, self.frame_index), stream=s)
 
         def main_chain(s):
             self.world.trace_gi(cam, cur, prm, self.rgb, self.texels, stream=s)
             self.svgf.frame(cam, prev_vp, cur, prev, self.texels, self.denoised, time=self.frame_index / 60.0,
                             stream=s)
+
+        # reflections shaded from both chains' outputs, then the composite
+        def tail(s):
+            from .world import composite, composite_params, reflect_params, reflect_shading
+            diffuse = self.svgf.final()
+            rp = reflect_params(sun, self.frame_index)
+            rp["div"] = self.reflect_div
+            shading = reflect_shading(prm, cur_vp, diffuse, self.shadow_denoised)
+            self.world.trace_reflect_shaded(cam, cur, sun, shading, self.reflect_dist, self.reflect_color,
+                                            self.reflect_hit, **rp, stream=s)
+            composite(self.ctx, cam, cur, composite_params(prm), diffuse, self.shadow_denoised, self.reflect_color,
+                      self.reflect_div, self.color, stream=s)
 
         fork, join = self.fork, self.join
         if self.graphs is None:
@@ -254,15 +270,20 @@ class Pipeline:
             side_chain(self.side.cuda_stream)
             join.record(self.side)
             main_chain(main.cuda_stream)
+            main.wait_event(join)
+            tail(main.cuda_stream)
         else:
-            # both replays start after the primary pass on `main`; the side
-            # chain's completion is joined on the side stream
+            # the first two replays start after the primary pass on `main`;
+            # the side chain's completion is joined on the side stream; the
+            # tail's replay waits for both
             with self.graphs[0].capture(main.cuda_stream, join_stream=self.side.cuda_stream) as s:
                 side_chain(s)
             join.record(self.side)
             with self.graphs[1].capture(main.cuda_stream) as s:
                 main_chain(s)
-        main.wait_event(join)
+            main.wait_event(join)
+            with self.graphs[2].capture(main.cuda_stream) as s:
+                tail(s)
         self.prev_view, self.prev_proj = view, proj
         self.frame_index += 1
         return cam
