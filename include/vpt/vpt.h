@@ -26,10 +26,11 @@
  *     complete when the call returns.
  *   - One host thread per object at a time; concurrency comes from streams.
  *     Calls on different streams may share a context and its worlds: the
- *     scratch a call needs (GI hit records and shadow jobs) is kept per
+ *     scratch a call needs (the GI pass's shadow-ray jobs) is kept per
  *     stream, and its constants (camera, GI parameters, the albedo table)
  *     travel with its launches, so no call reads what another stream's call
  *     writes.  The caller orders its own buffers (G-buffers, outputs).
+ *     (vpt_trace_gi_noalloc is the exception: see there.)
  *   - Matrices are 4x4 float, column-major (GLSL / glm layout).
  *   - Images are row-major W x H; row 0 is the bottom row (v_TexCoords.y
  *     smallest), pixel centre uv = ((px + 0.5) / W, (py + 0.5) / H).
@@ -321,20 +322,23 @@ VPT_API int vpt_trace_reflect_shaded(vpt_world* w, const vpt_camera* cam, const 
 
 /* DIFFUSE_TRACE (DiffuseRayTraceFrag.glsl:822-935), 1 spp, 1 bounce with a
  * shadow ray at the bounce hit.  d_rgb: 3 floats per pixel (clamped to
- * [0, 8]).  d_svgf_in (optional): SVGF input texels.  The hit records and
- * shadow jobs between its three kernels live in scratch the context keeps
- * for `stream` (24 B per pixel, grown on demand; outgrown buffers are kept
+ * [0, 8]).  d_svgf_in (optional): SVGF input texels.  Two kernels: the
+ * bounce rays (regrouped by direction within each 16x16 block), which shade
+ * their pixels at once unless the hit faces the sun; those pixels become
+ * jobs on a compacted list for the second kernel, which traces their
+ * shadow rays and shades them.  The jobs live in scratch the context keeps
+ * for `stream` (16 B per pixel, grown on demand; outgrown buffers are kept
  * until the context is destroyed, since a captured graph may still use
- * them).  The albedo table in prm is passed with the launch
- * and also becomes the context's "last table" for vpt_unet_assemble. */
+ * them).  The albedo table in prm is passed with the launch and also
+ * becomes the context's "last table" for vpt_unet_assemble. */
 VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
                          float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream);
 
 /* No counterpart in the reference: vpt_trace_gi for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving
- * loops, harnesses that forbid hipMalloc).  Same arguments and results,
- * same kernels; the hit records and shadow jobs live in a fixed array of
- * 48 MiB + 256 B that each GPU the library runs on holds from context
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and
+ * kernels; the shadow jobs live in a fixed array of
+ * 32 MiB + 256 B that each GPU the library runs on holds from context
  * creation until the process ends, so images are limited to 2,097,152
  * pixels (1920 x 1080 fits; larger is VPT_EINVAL).  It calls no hipMalloc
  * or hipFree.  Every call on one GPU shares that array: calls that overlap

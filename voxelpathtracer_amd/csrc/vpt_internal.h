@@ -68,8 +68,8 @@ int gi_noalloc_init(int device);
 
 namespace vpt {
 
-// Scratch of the GI pass for one stream: per-pixel hit records (2 words),
-// shadow-ray jobs (4 words each) and the job counter.  Calls on one stream
+// Scratch of the GI pass for one stream: the shadow-ray jobs (4 words
+// each, at most one per pixel) and the job counter.  Calls on one stream
 // run in order, so they share it; calls on other streams get their own.
 // `done` is recorded after a call's last kernel and waited for (on the
 // device) before the next call's first, so a stream handle reused after its
@@ -80,7 +80,6 @@ namespace vpt {
 struct GiScratch {
     void* stream = nullptr;
     void* done = nullptr;    // hipEvent_t
-    uint32_t* d_hits = nullptr;
     uint32_t* d_jobs = nullptr;
     uint32_t* d_njobs = nullptr;
     size_t cap = 0;          // pixels

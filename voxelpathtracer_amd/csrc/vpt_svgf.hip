@@ -83,9 +83,8 @@ struct Tx {
 __device__ inline float h2f(uint32_t h) { return vpt::hw_h2f(h); }
 __device__ inline uint32_t f2h(float x) { return vpt::hw_f2h(x); }
 
-__device__ inline Tx load_tx(const vpt_svgf_texel* p, size_t i)
+__device__ inline Tx load_tx_word(uint4 v)
 {
-    uint4 v = *reinterpret_cast<const uint4*>(p + i);
     Tx t;
     t.sh[0] = h2f(v.x & 0xffffu);
     t.sh[1] = h2f(v.x >> 16);
@@ -98,6 +97,8 @@ __device__ inline Tx load_tx(const vpt_svgf_texel* p, size_t i)
     t.sky = vpt_unorm8_decode((uint8_t)(v.w >> 24));
     return t;
 }
+
+__device__ inline Tx load_tx(const vpt_svgf_texel* p, size_t i) { return load_tx_word(*reinterpret_cast<const uint4*>(p + i)); }
 
 __device__ inline void store_tx(vpt_svgf_texel* p, size_t i, const float sh[4], float co, float cg, float x, float ao,
                                 float sky)
@@ -396,6 +397,18 @@ __global__ void __launch_bounds__(256) temporal_kernel(CamView cam, Mat16 prev_v
 // index, SH, CoCg, luminance, moment) instead of once per tap.  A block in
 // which no pixel has a short history skips the staging.  Taps are visited
 // in the reference's order and combined with the same operations.
+//
+// Uniform blocks: when every staged on-screen cell holds the same bits
+// (depth, normal index, SH, CoCg, luminance, moment; depth and luminance
+// finite) -- the interior of the sky, whose pixels never accumulate
+// history and take the 9x9 estimate every frame -- every tap has depth and
+// luminance differences 0, so its weight is the normal weight of the cell's
+// class against itself (w = w2 = that weight, >= 1, above both floors) and
+// it adds the same increment to each accumulator.  The sums then depend only
+// on the number of on-screen taps, kx * ky (the screen test is separable),
+// and are read from a table of the sequential sums, k = 0..81, built with
+// the same float additions.  The moment does not enter the weights: when
+// only it differs between cells, the moment sum alone walks the taps.
 constexpr int kVarR = 4;
 constexpr int kVarSW = 32 + 2 * kVarR, kVarSH = 8 + 2 * kVarR, kVarCells = kVarSW * kVarSH;
 
@@ -412,6 +425,7 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
     __shared__ float s_t[kVarCells];
     __shared__ uint8_t s_n[kVarCells];
     __shared__ float s_nw[7 * 8];         // normal weight by (base, sample) normal index
+    __shared__ float s_tab[9][82];        // uniform blocks: tw, tmom, tsh[4], tco, tcg, tlum after k taps
     const uint32_t tiles_x = (uint32_t)(W + 31) / 32u;
     const uint32_t tile = vpt::xcd_remap(blockIdx.x, tiles_x * ((uint32_t)(H + 7) / 8u));
     const int x0 = (int)(tile % tiles_x) * 32, y0 = (int)(tile / tiles_x) * 8;
@@ -427,6 +441,7 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
         base_moment = h2f(bu.x >> 16);
     }
     const bool spatial = valid && frames < thresh;
+    bool uniform = false, uniform_m = false;
     if (__syncthreads_or(spatial)) {
         for (int c = (int)threadIdx.x; c < kVarCells; c += (int)blockDim.x) {
             const int gx = x0 - kVarR + c % kVarSW, gy = y0 - kVarR + c / kVarSW;
@@ -444,6 +459,41 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
             s_nw[a * 8 + b] = vpt_powi(fmaxf(vpt::dot3(na, nb), 0.0f), 16);
         }
         __syncthreads();
+        // uniform block: every staged on-screen cell equals the block's
+        // first pixel (always on screen) bit for bit
+        const int ref = kVarR * kVarSW + kVarR;
+        const float4 rs = s_sh[ref], ra = s_aux[ref];
+        const float rt = s_t[ref];
+        const uint8_t rn = s_n[ref];
+        bool eq = isfinite(rt) && isfinite(ra.z), eq_m = true;
+        for (int c = (int)threadIdx.x; c < kVarCells; c += (int)blockDim.x) {
+            const int gx = x0 - kVarR + c % kVarSW, gy = y0 - kVarR + c / kVarSW;
+            if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;
+            const float4 cs = s_sh[c], ca = s_aux[c];
+            eq = eq && __float_as_uint(s_t[c]) == __float_as_uint(rt) && s_n[c] == rn &&
+                 __float_as_uint(cs.x) == __float_as_uint(rs.x) && __float_as_uint(cs.y) == __float_as_uint(rs.y) &&
+                 __float_as_uint(cs.z) == __float_as_uint(rs.z) && __float_as_uint(cs.w) == __float_as_uint(rs.w) &&
+                 __float_as_uint(ca.x) == __float_as_uint(ra.x) && __float_as_uint(ca.y) == __float_as_uint(ra.y) &&
+                 __float_as_uint(ca.z) == __float_as_uint(ra.z);
+            eq_m = eq_m && __float_as_uint(ca.w) == __float_as_uint(ra.w);
+        }
+        uniform = __syncthreads_and(eq);
+        uniform_m = __syncthreads_and(eq_m);
+        if (uniform) {
+            if (threadIdx.x < 9) {
+                const float nw0 = s_nw[rn * 8 + rn];
+                const float src[9] = {1.0f, ra.w, rs.x, rs.y, rs.z, rs.w, ra.x, ra.y, ra.z};
+                const int a = (int)threadIdx.x;
+                const float inc = a == 0 ? nw0 : src[a] * nw0;
+                float acc = 0.0f;
+                s_tab[a][0] = acc;
+                for (int k = 1; k <= 81; k++) {
+                    acc = acc + inc;
+                    s_tab[a][k] = acc;
+                }
+            }
+            __syncthreads();
+        }
     }
     if (!valid) return;
     const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
@@ -459,6 +509,37 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
         float tw = 0.0f, tmom = 0.0f, tlum = 0.0f, tw2 = 0.0f;
         float tsh[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         float tco = 0.0f, tcg = 0.0f;
+        if (uniform) {
+            int kx = 0, ky = 0;
+            for (int x = -4; x <= 4; x++) {
+                const float sx = uvx + (float)x * tex_w;
+                kx += (sx > 0.0f && sx < 1.0f) ? 1 : 0;
+            }
+            for (int y = -4; y <= 4; y++) {
+                const float sy = uvy + (float)y * tex_h;
+                ky += (sy > 0.0f && sy < 1.0f) ? 1 : 0;
+            }
+            const int k = kx * ky;
+            tw = s_tab[0][k];
+            tw2 = tw;
+            if (uniform_m) {
+                tmom = s_tab[1][k];
+            } else {
+                const float nw0 = nw_row[s_n[kVarR * kVarSW + kVarR]];
+                for (int x = -4; x <= 4; x++) {
+                    for (int y = -4; y <= 4; y++) {
+                        float sx = uvx + (float)x * tex_w;
+                        float sy = uvy + (float)y * tex_h;
+                        if (!in_screen(sx, sy)) continue;
+                        tmom = tmom + s_aux[(ly + kVarR + y) * kVarSW + (lx + kVarR + x)].w * nw0;
+                    }
+                }
+            }
+            for (int c = 0; c < 4; c++) tsh[c] = s_tab[2 + c][k];
+            tco = s_tab[6][k];
+            tcg = s_tab[7][k];
+            tlum = s_tab[8][k];
+        } else
         for (int x = -4; x <= 4; x++) {
             for (int y = -4; y <= 4; y++) {
                 float sx = uvx + (float)x * tex_w;
@@ -543,32 +624,68 @@ __device__ inline float gradient_noise(int px, int py, float time)
 // ---- SpatialFilter.main (SVGF/SpatialFilter.glsl:175-355), one pass ------
 //
 // The pass reads texels, depth and face at virtual texel coordinates
-// (before GL REPEAT wrapping) through this source.  (A variant that staged
-// each tile's texel region in LDS gave identical results but ran 10-25%
-// slower: the pass is bound by its per-tap VALU work, not by the gathers.)
+// (before GL REPEAT wrapping) through this source, addressed with 32-bit
+// byte offsets from the planes' bases (vpt_svgf_create keeps a texel plane
+// below 4 GiB), so that the gathers take a scalar base and one 32-bit
+// offset register instead of 64-bit address arithmetic per tap.  (A variant
+// that staged each tile's texel region in LDS gave identical results but ran
+// 10-25% slower: the pass is bound by its per-tap VALU work, not by the
+// gathers.)
 struct GlobalSrc {
     const vpt_svgf_texel* in;
     const float* t;
     const uint8_t* face;
     int W, H;
-    __device__ size_t idx(int vx, int vy) const { return (size_t)wrap1(vy, H) * W + wrap1(vx, W); }
-    __device__ Tx tex(int vx, int vy) const { return load_tx(in, idx(vx, vy)); }
-    __device__ float depth(int vx, int vy) const { return t[idx(vx, vy)]; }
+    __device__ uint32_t idx(int vx, int vy) const { return (uint32_t)wrap1(vy, H) * (uint32_t)W + (uint32_t)wrap1(vx, W); }
+    __device__ Tx tex_at(uint32_t j) const
+    {
+        return load_tx_word(*reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(in) + j * 16u));
+    }
+    __device__ float depth_at(uint32_t j) const
+    {
+        return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(t) + j * 4u);
+    }
+    __device__ Tx tex(int vx, int vy) const { return tex_at(idx(vx, vy)); }
+    __device__ float depth(int vx, int vy) const { return depth_at(idx(vx, vy)); }
     __device__ int fc(int vx, int vy) const { return face[idx(vx, vy)]; }
-    __device__ float xch(int vx, int vy) const { return load_x(in, idx(vx, vy)); }
+    __device__ float xch(int vx, int vy) const
+    {
+        return h2f(*reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(in) + idx(vx, vy) * 16u + 12u) &
+                   0xffffu);
+    }
 };
+
+// Normal-weight class of a face byte: the six faces, or 6 for every other
+// value (GetNormalFromID's (1, 1, 1)).
+__device__ inline uint32_t face_class(int f) { return f <= 5 ? (uint32_t)f : 6u; }
+
+// clamp(pow(max(dot(n_a, n_b), 0), 32), 0.001, 1) of two face normals is 1
+// where the dot is positive (the same face; (1, 1, 1) against itself or
+// against +x, +y, +z) and 0.001 elsewhere (dots 0 and -1): bit a * 8 + b of
+// this mask.
+constexpr uint64_t kNw32Mask = (1ull << (0 * 8 + 0)) | (1ull << (1 * 8 + 1)) | (1ull << (2 * 8 + 2)) |
+                               (1ull << (3 * 8 + 3)) | (1ull << (4 * 8 + 4)) | (1ull << (5 * 8 + 5)) |
+                               (1ull << (0 * 8 + 6)) | (1ull << (2 * 8 + 6)) | (1ull << (5 * 8 + 6)) |
+                               (1ull << (6 * 8 + 0)) | (1ull << (6 * 8 + 2)) | (1ull << (6 * 8 + 5)) |
+                               (1ull << (6 * 8 + 6));
+
+__device__ inline float nw32_of(uint32_t base_class, uint32_t sample_class)
+{
+    return ((kNw32Mask >> (base_class * 8u + sample_class)) & 1ull) ? 1.0f : 0.001f;
+}
 
 // One pixel of the pass, reading through `src`.  kMode is the pass's AO /
 // sky filtering as a compile-time value (0: both, step <= 4; 1: sky only,
 // step <= 6; 2: neither), so that the passes that filter neither skip those
-// two channels' taps entirely.
+// two channels' taps entirely.  The 3 x 3 tap loops are unrolled, so the
+// a-trous weights and offsets are constants.
 template <class Src, int kMode>
 __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, int W, int H, int px, int py,
                                                                 const uint2* __restrict__ tutil, int step,
                                                                 float scale, float color_phi_bias, float time,
                                                                 vpt_svgf_texel* __restrict__ out)
 {
-    const size_t i = (size_t)py * W + px;
+    const uint32_t i = (uint32_t)py * (uint32_t)W + (uint32_t)px;
     const float atrous[3] = {1.0f, 2.0f / 3.0f, 1.0f / 6.0f};
     const float gauss[2] = {0.60283f, 0.198585f};
     const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
@@ -576,14 +693,16 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
     constexpr bool filter_sky = kMode <= 1;
     float uvx = texc(px, W), uvy = texc(py, H);
     int jit = (int)((gradient_noise(px, py, time) - 0.5f) * ((float)step * 0.8f));
-    float base_depth = src.depth(px, py);
-    Vec3 base_n = vpt::face_normal(src.fc(px, py), 1.0f);
-    Tx base = src.tex(px, py);
+    float base_depth = src.depth_at(i);
+    const uint32_t base_class = face_class(src.face[i]);
+    Tx base = src.tex_at(i);
     float base_lum = sh_to_y(base.sh[3]);
 
     // GaussianVariance (:94-129) over the previous pass's variance
     float base_var = 0.0f, vsum = 0.0f, ksum = 0.0f;
+#pragma unroll
     for (int x = -1; x <= 1; x++) {
+#pragma unroll
         for (int y = -1; y <= 1; y++) {
             float sx = uvx + (float)x * tex_w;
             float sy = uvy + (float)y * tex_h;
@@ -619,11 +738,14 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
     phi = phi / fmaxf(color_phi_bias, 0.1f);
 
     if (!(base_depth < 0.0f)) {
+        const float jh = (float)jit * 0.5f;
+#pragma unroll
         for (int x = -1; x <= 1; x++) {
+#pragma unroll
             for (int y = -1; y <= 1; y++) {
                 if (x == 0 && y == 0) continue;
-                float ox = ((float)x * (float)step) * scale + (float)jit * 0.5f;
-                float oy = ((float)y * (float)step) * scale + (float)jit * 0.5f;
+                float ox = ((float)x * (float)step) * scale + jh;
+                float oy = ((float)y * (float)step) * scale + jh;
                 float sx = uvx + ox * tex_w;
                 float sy = uvy + oy * tex_h;
                 if (!in_screen(sx, sy)) continue;
@@ -632,12 +754,14 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
                 const float fu = floorf(u), fv = floorf(v);
                 const float a = u - fu, b = v - fv;
                 const int x0 = (int)fu, y0 = (int)fv;
-                float sdepth = bil(src.depth(x0, y0), src.depth(x0 + 1, y0), src.depth(x0, y0 + 1),
-                                   src.depth(x0 + 1, y0 + 1), a, b);
+                const uint32_t c0 = (uint32_t)wrap1(x0, W), c1 = (uint32_t)wrap1(x0 + 1, W);
+                const uint32_t r0 = (uint32_t)wrap1(y0, H) * (uint32_t)W, r1 = (uint32_t)wrap1(y0 + 1, H) * (uint32_t)W;
+                const uint32_t j00 = r0 + c0, j10 = r0 + c1, j01 = r1 + c0, j11 = r1 + c1;
+                float sdepth = bil(src.depth_at(j00), src.depth_at(j10), src.depth_at(j01), src.depth_at(j11), a, b);
                 float ddiff = fabsf(sdepth - base_depth);
-                Vec3 sn = vpt::face_normal(src.fc((int)floorf(sx * (float)W), (int)floorf(sy * (float)H)), 1.0f);
-                const Tx q00 = src.tex(x0, y0), q10 = src.tex(x0 + 1, y0), q01 = src.tex(x0, y0 + 1),
-                         q11 = src.tex(x0 + 1, y0 + 1);
+                const float nw = nw32_of(base_class,
+                                         face_class(src.fc((int)floorf(sx * (float)W), (int)floorf(sy * (float)H))));
+                const Tx q00 = src.tex_at(j00), q10 = src.tex_at(j10), q01 = src.tex_at(j01), q11 = src.tex_at(j11);
                 Tx s;
                 for (int c = 0; c < 4; c++) s.sh[c] = bil(q00.sh[c], q10.sh[c], q01.sh[c], q11.sh[c], a, b);
                 s.co = bil(q00.co, q10.co, q01.co, q11.co, a, b);
@@ -646,8 +770,6 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
                 s.ao = bil(q00.ao, q10.ao, q01.ao, q11.ao, a, b);
                 s.sky = bil(q00.sky, q10.sky, q01.sky, q11.sky, a, b);
                 float slum = sh_to_y(s.sh[3]);
-                float nw = vpt_powi(fmaxf(vpt::dot3(base_n, sn), 0.0f), 32);
-                nw = fclamp(nw, 0.001f, 1.0f);
                 float lw = fabsf(slum - base_lum) / phi;
                 // exp(-d)^2 < 1e-4 for every float d >= 4.7, so the depth
                 // weight sits on its lower clamp there without the exp
@@ -893,6 +1015,19 @@ __global__ void __launch_bounds__(256) shadow_temporal_kernel(CamView cam, Mat16
 }
 
 // ShadowFilter.main / ShadowSpatial (ShadowFilter.glsl:69-165).
+//
+// A block filters a 32x8 tile.  Its neighbourhood -- the tile plus an
+// 11-pixel apron, which holds every bilinear footprint and nearest tap of
+// the 7x7 kernel at filter scales up to 1 (offsets up to 3 * 1.2 * 2.6 =
+// 9.36 pixels) -- is staged once in LDS: depth and the decoded colour per
+// cell, and the face's normal class.  A tap whose footprint leaves the
+// staged cells or wraps at the image edge (REPEAT) reads global memory;
+// both paths read the same values.  The normal weight pow(max(dot, 1e-9),
+// 32) of two face normals takes one of three values and comes from a 7x7
+// table built with the same expression.
+constexpr int kSfA = 11;
+constexpr int kSfW = 32 + 2 * kSfA, kSfH = 8 + 2 * kSfA, kSfCells = kSfW * kSfH;
+
 __global__ void __launch_bounds__(256) shadow_filter_kernel(int W, int H, const float* __restrict__ gb_t,
                                                             const uint8_t* __restrict__ gb_face,
                                                             const uint8_t* __restrict__ color,
@@ -900,16 +1035,36 @@ __global__ void __launch_bounds__(256) shadow_filter_kernel(int W, int H, const 
                                                             const uint16_t* __restrict__ frames, float scale,
                                                             uint8_t* __restrict__ out)
 {
-    int px, py;
-    if (!pixel_of(W, H, px, py)) return;
-    const size_t i = (size_t)py * W + px;
+    __shared__ float2 s_dc[kSfCells];   // depth, decoded colour
+    __shared__ uint8_t s_f[kSfCells];   // normal class (face, or 6)
+    __shared__ float s_nw[7 * 8];
+    const uint32_t tiles_x = (uint32_t)(W + 31) / 32u;
+    const uint32_t tile = vpt::xcd_remap(blockIdx.x, tiles_x * ((uint32_t)(H + 7) / 8u));
+    const int tx0 = (int)(tile % tiles_x) * 32, ty0 = (int)(tile / tiles_x) * 8;
+    const int gx0 = tx0 - kSfA, gy0 = ty0 - kSfA;   // image position of staged cell (0, 0)
+    for (int c = (int)threadIdx.x; c < kSfCells; c += (int)blockDim.x) {
+        const int gx = gx0 + c % kSfW, gy = gy0 + c / kSfW;
+        if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;   // never read from LDS
+        const uint32_t j = (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
+        s_dc[c] = make_float2(gb_t[j], dec8(color[j]));
+        s_f[c] = (uint8_t)face_class(gb_face[j]);
+    }
+    if (threadIdx.x < 49) {
+        const int a = (int)threadIdx.x / 7, b = (int)threadIdx.x % 7;
+        const Vec3 na = vpt::face_normal(a == 6 ? 255 : a, 1.0f), nb = vpt::face_normal(b == 6 ? 255 : b, 1.0f);
+        s_nw[a * 8 + b] = vpt_powi(fmaxf(vpt::dot3(na, nb), 0.000000001f), 32);
+    }
+    __syncthreads();
+    const int px = tx0 + (int)(threadIdx.x & 31), py = ty0 + (int)(threadIdx.x >> 5);
+    if (px >= W || py >= H) return;
+    const uint32_t i = (uint32_t)py * (uint32_t)W + (uint32_t)px;
     const float uvx = texc(px, W), uvy = texc(py, H);
     const float tex_w = 1.0f / (float)W, tex_h = 1.0f / (float)H;
     const float fr = h2f(frames[i]);
     const bool apply_luma = fr > 7.5f;
     const float dist = gb_t[i];
     const bool sky = dist < 0.0f;
-    const Vec3 cn = vpt::face_normal(gb_face[i], 1.0f);
+    const float* nw_row = s_nw + face_class(gb_face[i]) * 8;
     const float cs = dec8(color[i]);
     const float T = h2f(tv[i]) * 100.0f;
     const float cutoff = sqrtf(2.0f);
@@ -937,21 +1092,42 @@ __global__ void __launch_bounds__(256) shadow_filter_kernel(int W, int H, const 
         for (int y = -size; y <= size; y++) {
             const float sx = uvx + (((float)x * tex_w) * 1.2f * sc) * scale;
             const float sy = uvy + (((float)y * tex_h) * 1.2f * sc) * scale;
-            const Bil f = bil_setup_any(sx, sy, W, H);
-            const int nxp = wrapn((int)floorf(sx * (float)W), W), nyp = wrapn((int)floorf(sy * (float)H), H);
-            const Vec3 sn = vpt::face_normal(gb_face[(uint32_t)nyp * (uint32_t)W + (uint32_t)nxp], 1.0f);
-            const float sa = bil(dec8(color[f.i00]), dec8(color[f.i10]), dec8(color[f.i01]), dec8(color[f.i11]), f.a,
-                                 f.b);
+            // GL LINEAR footprint (as bil_setup_any) and NEAREST texel
+            const float u = sx * (float)W - 0.5f, v = sy * (float)H - 0.5f;
+            const float fu = floorf(u), fv = floorf(v);
+            const float fa = u - fu, fb = v - fv;
+            const int x0 = (int)fu, y0 = (int)fv;
+            const int nxi = (int)floorf(sx * (float)W), nyi = (int)floorf(sy * (float)H);
+            const int lx = x0 - gx0, ly = y0 - gy0, nlx = nxi - gx0, nly = nyi - gy0;
+            const bool staged = x0 >= 0 && x0 + 1 < W && y0 >= 0 && y0 + 1 < H && lx >= 0 && lx + 1 < kSfW &&
+                                ly >= 0 && ly + 1 < kSfH && nxi >= 0 && nxi < W && nyi >= 0 && nyi < H &&
+                                nlx >= 0 && nlx < kSfW && nly >= 0 && nly < kSfH;
+            float d00, d10, d01, d11, c00, c10, c01, c11, nw;
+            if (staged) {
+                const int c = ly * kSfW + lx;
+                const float2 q00 = s_dc[c], q10 = s_dc[c + 1], q01 = s_dc[c + kSfW], q11 = s_dc[c + kSfW + 1];
+                d00 = q00.x; d10 = q10.x; d01 = q01.x; d11 = q11.x;
+                c00 = q00.y; c10 = q10.y; c01 = q01.y; c11 = q11.y;
+                nw = nw_row[s_f[nly * kSfW + nlx]];
+            } else {
+                const uint32_t wx0 = (uint32_t)wrapn(x0, W), wx1 = (uint32_t)wrapn(x0 + 1, W);
+                const uint32_t r0 = (uint32_t)wrapn(y0, H) * (uint32_t)W, r1 = (uint32_t)wrapn(y0 + 1, H) * (uint32_t)W;
+                d00 = gb_t[r0 + wx0]; d10 = gb_t[r0 + wx1]; d01 = gb_t[r1 + wx0]; d11 = gb_t[r1 + wx1];
+                c00 = dec8(color[r0 + wx0]); c10 = dec8(color[r0 + wx1]);
+                c01 = dec8(color[r1 + wx0]); c11 = dec8(color[r1 + wx1]);
+                const uint32_t ni = (uint32_t)wrapn(nyi, H) * (uint32_t)W + (uint32_t)wrapn(nxi, W);
+                nw = nw_row[face_class(gb_face[ni])];
+            }
+            const float sa = bil(c00, c10, c01, c11, fa, fb);
             // The weight is ((clamp(pow) * dw) * nw) with pow <= 1 (vpt_expf of a
             // non-positive argument, checked for every float), so it sits on
             // its 1e-9 clamp once dw * nw < 1e-9: nw is exactly 0 or 1 here
             // (axis normals, or (1,1,1) against an axis normal), and
             // exp(-d)^3 < 1e-9 for every float d >= 21.  Those taps skip the
             // exp and the pow.
-            const float nw = vpt_powi(fmaxf(vpt::dot3(cn, sn), 0.000000001f), 32);
             float w = 0.000000001f;
             if (nw != 0.0f) {
-                const float sd = bil(gb_t[f.i00], gb_t[f.i10], gb_t[f.i01], gb_t[f.i11], f.a, f.b);
+                const float sd = bil(d00, d10, d01, d11, fa, fb);
                 const float dd = fabsf(dist - sd);
                 if (dd < 21.0f) {
                     const float dw = vpt_powi(vpt_expf(-dd), 3);
@@ -995,8 +1171,10 @@ CamView make_cam(const vpt_camera* c)
 extern "C" VPT_API int vpt_svgf_create(vpt_ctx* ctx, int width, int height, const vpt_svgf_params* prm, vpt_svgf** out)
 {
     if (!ctx || !out) return set_error(VPT_EINVAL, "vpt_svgf_create: null argument");
-    if (width <= 1 || height <= 1 || width > 16384 || height > 16384)
-        return set_error(VPT_EINVAL, "vpt_svgf_create: bad size %dx%d", width, height);
+    if (width <= 1 || height <= 1 || width > 16384 || height > 16384 ||
+        (uint64_t)width * (uint64_t)height * sizeof(vpt_svgf_texel) > 0xFFFFFFFFull)
+        return set_error(VPT_EINVAL, "vpt_svgf_create: bad size %dx%d (at most 16384 per side, a texel plane "
+                                     "below 4 GiB)", width, height);
     DeviceGuard g(ctx->device);
     vpt_svgf* s = new vpt_svgf();
     s->device = ctx->device;

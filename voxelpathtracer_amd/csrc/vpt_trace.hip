@@ -83,9 +83,9 @@ AlbedoTable take_albedo(vpt_ctx* ctx, const vpt_gi_params* prm)
 
 static void gi_retire(vpt_ctx* ctx, GiScratch* sc)
 {
-    for (void* p : {(void*)sc->d_hits, (void*)sc->d_jobs, (void*)sc->d_njobs})
+    for (void* p : {(void*)sc->d_jobs, (void*)sc->d_njobs})
         if (p) ctx->gi_retired.push_back(p);
-    sc->d_hits = sc->d_jobs = sc->d_njobs = nullptr;
+    sc->d_jobs = sc->d_njobs = nullptr;
     sc->cap = 0;
 }
 
@@ -119,8 +119,7 @@ static int gi_scratch_for(vpt_ctx* ctx, hipStream_t stream, size_t npx, bool cap
     }
     if (sc->cap < npx) {
         gi_retire(ctx, sc);
-        hipError_t e = hipMalloc(&sc->d_hits, npx * 2 * sizeof(uint32_t));
-        if (e == hipSuccess) e = hipMalloc(&sc->d_jobs, npx * 4 * sizeof(uint32_t));
+        hipError_t e = hipMalloc(&sc->d_jobs, npx * 4 * sizeof(uint32_t));
         if (e == hipSuccess) e = hipMalloc(&sc->d_njobs, sizeof(uint32_t));
         if (e != hipSuccess) {
             gi_retire(ctx, sc);
@@ -312,7 +311,7 @@ __global__ void __launch_bounds__(256) shadow2_kernel(WorldView w, CamView cam, 
 
 // DIFFUSE_TRACE main + CalculateDiffuse (DiffuseRayTraceFrag.glsl:535-664,
 // 822-935), 1 spp, 1 bounce; Lambert sun term, PCG randoms, per-id albedo,
-// constant sky.  Three kernels, so that the sun-shadow rays at the bounce
+// constant sky.  Two kernels, so that the sun-shadow rays at the bounce
 // hits (up to 128 steps) do not hold idle the lanes whose bounce missed or
 // faced away:
 //   gi_bounce_kernel   one lane per bounce ray: the block's 16x16 pixels
@@ -320,34 +319,33 @@ __global__ void __launch_bounds__(256) shadow2_kernel(WorldView w, CamView cam, 
 //                      the ray's y direction (16 bins, through LDS), so that
 //                      a wavefront traces rays of similar length (rays
 //                      leaving upward escape in a few large steps, grazing
-//                      ones walk the terrain); stores the hit record and,
-//                      when the hit faces the sun, appends the shadow ray to
-//                      a compacted job list (one atomic per wavefront);
-//   gi_shadow_kernel   one lane per job: the shadow ray, which sets the
-//                      record's shadowed bit;
-//   gi_shade_kernel    one lane per pixel: shading from the record.
+//                      ones walk the terrain).  A pixel whose hit faces the
+//                      sun becomes a job on a compacted list (one atomic per
+//                      wavefront); the tracing lane shades every other pixel
+//                      (and each lane its own sky pixel) at once;
+//   gi_shadow_kernel   one lane per job: the pixel's ray again (from the
+//                      G-buffer), the shadow ray, then the pixel's shading.
 // Every pixel goes through the same functions as a single-kernel pass, so
-// the outputs are identical; only the lane that traces a pixel changes.
+// the outputs are identical; only the lane that traces and shades a pixel
+// changes.
 constexpr int kGiBins = 16;
 
+// A pixel waiting for its shadow ray: index, bounce hit distance, face |
+// block << 8.
 struct GiJob {
-    float ox, oy, oz;
     uint32_t pix;
+    float T;
+    uint32_t fb;
+    uint32_t pad;
 };
-
-// Hit record: x = T (float bits), y = face | block << 8 | hit << 16 | shadowed << 17.
-__device__ inline uint2 gi_record(float T, int face, int block, bool hit)
-{
-    return make_uint2(__float_as_uint(T),
-                      (uint32_t)(face & 0xff) | ((uint32_t)(block & 0xff) << 8) | ((uint32_t)hit << 16));
-}
 
 __global__ void gi_jobs_clear_kernel(uint32_t* __restrict__ n) { *n = 0u; }
 
 __global__ void __launch_bounds__(256) gi_bounce_kernel(WorldView w, CamView cam, const float* __restrict__ gb_t,
                                                         const uint8_t* __restrict__ gb_face, GiConst g,
-                                                        uint2* __restrict__ hits, GiJob* __restrict__ jobs,
-                                                        uint32_t* __restrict__ njobs,
+                                                        const vpt::AlbedoTable albedo, float* __restrict__ rgb,
+                                                        vpt_svgf_texel* __restrict__ texels,
+                                                        GiJob* __restrict__ jobs, uint32_t* __restrict__ njobs,
                                                         unsigned long long* __restrict__ counter)
 {
     __shared__ float s_ray[6][256];
@@ -357,20 +355,24 @@ __global__ void __launch_bounds__(256) gi_bounce_kernel(WorldView w, CamView cam
     tile_pixel(cam.W, cam.H, px, py);
     int samples = 0, rays = 0;
     bool job = false;
-    GiJob jb = {0.0f, 0.0f, 0.0f, 0u};
+    GiJob jb = {0u, 0.0f, 0u, 0u};
     if (threadIdx.x <= kGiBins) s_bin[threadIdx.x] = 0;
     __syncthreads();
-    // set up this pixel's ray and rank it within its y-direction bin
+    // set up this pixel's ray and rank it within its y-direction bin; sky
+    // pixels are shaded from the G-buffer alone, by their own lane
     Vec3 ro = vpt::mk3(0.0f, 0.0f, 0.0f), rd = ro;
     int bin = kGiBins;
     uint32_t rpix = 0xffffffffu;
     if (px < cam.W && py < cam.H) {
         const size_t i0 = (size_t)py * cam.W + px;
         const float dist = gb_t[i0];
-        if (!(dist < 0.0f)) {   // sky pixels are shaded from the G-buffer alone
-            vpt::gi_ray(cam, px, py, dist, vpt::face_normal(gb_face[i0], 0.5f), g.frame, ro, rd);
+        const Vec3 N = vpt::face_normal(gb_face[i0], 0.5f);
+        if (!(dist < 0.0f)) {
+            vpt::gi_ray(cam, px, py, dist, N, g.frame, ro, rd);
             bin = min(kGiBins - 1, max(0, (int)((rd.y + 1.0f) * (0.5f * kGiBins))));
             rpix = (uint32_t)i0;
+        } else {
+            vpt::gi_write_sky_pixel(g, N, rgb + 3 * i0, texels ? texels + i0 : nullptr);
         }
     }
     const int rank = atomicAdd(&s_bin[bin], 1);
@@ -402,15 +404,17 @@ __global__ void __launch_bounds__(256) gi_bounce_kernel(WorldView w, CamView cam
         samples = h.samples;
         rays = 1;
         const bool hit = h.t > 0.0f && h.block > 0;
-        hits[pix] = gi_record(h.t, h.face, h.block, hit);
+        const int face = h.face & 0xff, block = h.block & 0xff;
         Vec3 so;
         float ndotl;
         if (hit && vpt::gi_shadow_ray(o, dir, h.t, h.face, g, so, ndotl)) {
             job = true;
-            jb.ox = so.x;
-            jb.oy = so.y;
-            jb.oz = so.z;
             jb.pix = pix;
+            jb.T = h.t;
+            jb.fb = (uint32_t)face | ((uint32_t)block << 8);
+        } else {
+            vpt::gi_write_hit_pixel(g, albedo.v, dir, h.t, hit, face, block, 0.0f, rgb + 3 * (size_t)pix,
+                                    texels ? texels + pix : nullptr);
         }
     }
     // compacted append, one atomic per wavefront (every lane reaches it)
@@ -425,8 +429,11 @@ __global__ void __launch_bounds__(256) gi_bounce_kernel(WorldView w, CamView cam
     count_samples(counter, samples, rays);
 }
 
-__global__ void __launch_bounds__(256) gi_shadow_kernel(WorldView w, GiConst g, const GiJob* __restrict__ jobs,
-                                                        const uint32_t* __restrict__ njobs, uint2* __restrict__ hits,
+__global__ void __launch_bounds__(256) gi_shadow_kernel(WorldView w, CamView cam, const float* __restrict__ gb_t,
+                                                        const uint8_t* __restrict__ gb_face, GiConst g,
+                                                        const vpt::AlbedoTable albedo, const GiJob* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ njobs, float* __restrict__ rgb,
+                                                        vpt_svgf_texel* __restrict__ texels,
                                                         unsigned long long* __restrict__ counter)
 {
     const uint32_t n = *njobs;   // at most one job per pixel: n <= the grid's lanes
@@ -434,36 +441,21 @@ __global__ void __launch_bounds__(256) gi_shadow_kernel(WorldView w, GiConst g, 
     int samples = 0, rays = 0;
     if (k < n) {
         const GiJob jb = jobs[k];
-        const vpt::Hit sh =
-            vpt::traverse(w, vpt::mk3(jb.ox, jb.oy, jb.oz), vpt::mk3(g.sun[0], g.sun[1], g.sun[2]), g.shadow_iter);
+        const int px = (int)(jb.pix % (uint32_t)cam.W), py = (int)(jb.pix / (uint32_t)cam.W);
+        const float dist = gb_t[jb.pix];
+        Vec3 o, dir;
+        vpt::gi_ray(cam, px, py, dist, vpt::face_normal(gb_face[jb.pix], 0.5f), g.frame, o, dir);
+        const int face = (int)(jb.fb & 0xffu), block = (int)((jb.fb >> 8) & 0xffu);
+        Vec3 so;
+        float ndotl;
+        vpt::gi_shadow_ray(o, dir, jb.T, face, g, so, ndotl);   // true: the bounce kernel made the job
+        const vpt::Hit sh = vpt::traverse(w, so, vpt::mk3(g.sun[0], g.sun[1], g.sun[2]), g.shadow_iter);
         samples = sh.samples;
         rays = 1;
-        if (sh.t > 0.0f) hits[jb.pix].y |= 1u << 17;   // one job per pixel: no other writer
+        vpt::gi_write_hit_pixel(g, albedo.v, dir, jb.T, true, face, block, sh.t > 0.0f ? 1.0f : 0.0f,
+                                rgb + 3 * (size_t)jb.pix, texels ? texels + jb.pix : nullptr);
     }
     count_samples(counter, samples, rays);
-}
-
-__global__ void __launch_bounds__(256) gi_shade_kernel(CamView cam, const float* __restrict__ gb_t,
-                                                       const uint8_t* __restrict__ gb_face, GiConst g,
-                                                       const uint2* __restrict__ hits, const vpt::AlbedoTable albedo,
-                                                       float* __restrict__ rgb, vpt_svgf_texel* __restrict__ texels)
-{
-    int px, py;
-    tile_pixel(cam.W, cam.H, px, py);
-    if (px >= cam.W || py >= cam.H) return;
-    const size_t i = (size_t)py * cam.W + px;
-    const float dist = gb_t[i];
-    const Vec3 N = vpt::face_normal(gb_face[i], 0.5f);
-    vpt_svgf_texel* tx = texels ? texels + i : nullptr;
-    if (dist < 0.0f) {
-        vpt::gi_write_sky_pixel(g, N, rgb + 3 * i, tx);
-        return;
-    }
-    Vec3 o, dir;
-    vpt::gi_ray(cam, px, py, dist, N, g.frame, o, dir);
-    const uint2 rec = hits[i];
-    vpt::gi_write_hit_pixel(g, albedo.v, dir, __uint_as_float(rec.x), (rec.y >> 16) & 1u, (int)(rec.y & 0xffu),
-                            (int)((rec.y >> 8) & 0xffu), ((rec.y >> 17) & 1u) ? 1.0f : 0.0f, rgb + 3 * i, tx);
 }
 
 // Ray queries: one lane per ray.
@@ -1078,24 +1070,22 @@ extern "C" VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, cons
     return rc_shadow ? rc_shadow : rc_gi;
 }
 
-// The GI pass's three kernels over the given scratch (the job counter is
-// cleared by a kernel, so the sequence can be captured in a graph).
+// The GI pass's kernels over the given scratch (the job counter is cleared
+// by a kernel, so the sequence can be captured in a graph).
 static int gi_enqueue(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
-                      const vpt::AlbedoTable& albedo, float* d_rgb, vpt_svgf_texel* d_svgf_in, uint2* hits,
-                      GiJob* jobs, uint32_t* njobs, hipStream_t s)
+                      const vpt::AlbedoTable& albedo, float* d_rgb, vpt_svgf_texel* d_svgf_in, GiJob* jobs,
+                      uint32_t* njobs, hipStream_t s)
 {
     const size_t npx = (size_t)cam->width * cam->height;
     const GiConst gc = vpt::make_gi(prm);
+    const CamView cv = vpt::make_cam(cam);
     hipLaunchKernelGGL(gi_jobs_clear_kernel, dim3(1), dim3(1), 0, s, njobs);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(gi_bounce_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w),
-                       vpt::make_cam(cam), gb->d_t, gb->d_face, gc, hits, jobs, njobs, w->d_counter);
+    hipLaunchKernelGGL(gi_bounce_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w), cv,
+                       gb->d_t, gb->d_face, gc, albedo, d_rgb, d_svgf_in, jobs, njobs, w->d_counter);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(gi_shadow_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, vpt::make_view(w), gc,
-                       jobs, njobs, hits, w->d_counter);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(gi_shade_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_cam(cam),
-                       gb->d_t, gb->d_face, gc, hits, albedo, d_rgb, d_svgf_in);
+    hipLaunchKernelGGL(gi_shadow_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, vpt::make_view(w), cv,
+                       gb->d_t, gb->d_face, gc, albedo, jobs, njobs, d_rgb, d_svgf_in, w->d_counter);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }
@@ -1120,8 +1110,8 @@ extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const v
     std::lock_guard<std::mutex> lk(ctx->mu);
     vpt::GiScratch* sc = nullptr;
     if (int rc = gi_scratch_for(ctx, s, npx, capturing, &sc)) return rc;
-    if (int rc = gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<uint2*>(sc->d_hits),
-                            reinterpret_cast<GiJob*>(sc->d_jobs), sc->d_njobs, s))
+    if (int rc = gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<GiJob*>(sc->d_jobs),
+                            sc->d_njobs, s))
         return rc;
     if (!capturing) HIP_TRY(hipEventRecord((hipEvent_t)sc->done, s));
     return VPT_OK;
@@ -1130,17 +1120,16 @@ extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const v
 // ---------------------------------------------------------------------------
 // Allocation-free GI pass (vpt_trace_gi_noalloc): the same kernels over a
 // fixed scratch array instead of the context's per-stream scratch.
-// Layout: the job counter in the first 256 bytes, then the hit records
-// (8 B per pixel), then the shadow jobs (16 B per pixel), for images of at
-// most kNoAllocGiMaxPixels pixels: 48 MiB + 256 B.  Each GPU the library
-// runs on holds its own copy until the process ends.
+// Layout: the job counter in the first 256 bytes, then the shadow jobs
+// (16 B per pixel), for images of at most kNoAllocGiMaxPixels pixels:
+// 32 MiB + 256 B.  Each GPU the library runs on holds its own copy until the
+// process ends.
 // ---------------------------------------------------------------------------
 
 namespace {
 
 constexpr size_t kNoAllocGiMaxPixels = (size_t)1 << 21;   // 2,097,152: 1920 x 1080 fits
-constexpr size_t kNoAllocGiHitsOff = 256;
-constexpr size_t kNoAllocGiJobsOff = kNoAllocGiHitsOff + kNoAllocGiMaxPixels * sizeof(uint2);
+constexpr size_t kNoAllocGiJobsOff = 256;
 constexpr size_t kNoAllocGiBytes = kNoAllocGiJobsOff + kNoAllocGiMaxPixels * sizeof(GiJob);
 constexpr int kNoAllocMaxDevices = 64;
 
@@ -1202,9 +1191,8 @@ extern "C" VPT_API int vpt_trace_gi_noalloc(vpt_world* w, const vpt_camera* cam,
     }
     uint8_t* b = static_cast<uint8_t*>(base);
     const vpt::AlbedoTable albedo = vpt::take_albedo(w->ctx, prm);
-    return gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<uint2*>(b + kNoAllocGiHitsOff),
-                      reinterpret_cast<GiJob*>(b + kNoAllocGiJobsOff), reinterpret_cast<uint32_t*>(b),
-                      (hipStream_t)stream);
+    return gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<GiJob*>(b + kNoAllocGiJobsOff),
+                      reinterpret_cast<uint32_t*>(b), (hipStream_t)stream);
 }
 
 extern "C" VPT_API int vpt_trace_rays(vpt_world* w, const vpt_ray* d_rays, int n, int max_iter, vpt_hit* d_hits,

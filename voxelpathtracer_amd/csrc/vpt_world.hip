@@ -447,7 +447,6 @@ extern "C" VPT_API int vpt_ctx_destroy(vpt_ctx* ctx)
     DeviceGuard g(ctx->device);
     for (vpt::GiScratch& sc : ctx->gi_scratch) {
         if (sc.done) (void)hipEventDestroy((hipEvent_t)sc.done);
-        (void)hipFree(sc.d_hits);
         (void)hipFree(sc.d_jobs);
         (void)hipFree(sc.d_njobs);
     }
