@@ -134,6 +134,78 @@ __global__ void set_blocks_args_kernel(uint8_t* __restrict__ lin, uint8_t* __res
 
 // ---- DF passes ---------------------------------------------------------------
 
+// Packed relaxation of four byte columns at once: a column chunk's bytes
+// 0 / 2 and 1 / 3 as two pairs of 16-bit lanes (values stay at most 255, so
+// prev + 1 never carries into the next lane).
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+struct Cols4 {
+    u16x2 e, o;   // bytes 0, 2 and 1, 3
+};
+
+__device__ inline Cols4 cols4_of(uint32_t d)
+{
+    return Cols4{__builtin_bit_cast(u16x2, d & 0x00ff00ffu), __builtin_bit_cast(u16x2, (d >> 8) & 0x00ff00ffu)};
+}
+
+__device__ inline uint32_t cols4_pack(Cols4 c)
+{
+    return __builtin_bit_cast(uint32_t, c.e) | (__builtin_bit_cast(uint32_t, c.o) << 8);
+}
+
+// min(d, c + k) per column
+__device__ inline Cols4 cols4_relax(Cols4 d, Cols4 c, unsigned short k)
+{
+    const u16x2 kk = {k, k};
+    return Cols4{__builtin_elementwise_min(d.e, c.e + kk), __builtin_elementwise_min(d.o, c.o + kk)};
+}
+
+// One segment's share of a column relaxation d[i] = min(d[i], d[i -+ 1] + 1)
+// over n cells of LDS dwords p[0], p[stride], ... (4 columns per dword),
+// forward (dir = +1) or backward (dir = -1): first the segment's own chain,
+// its end published in ends[seg]; then, after the block barrier, the value
+// entering the segment from the segments before it (F = min(end, cin +
+// len - 1), cin = F + 1, folded from the far end) and min(cell, cin + offset)
+// over the segment -- what the sequential sweep gives, since its result is
+// min over the cells behind of (d + distance).  Every thread of the block
+// calls it (it holds barriers).
+__device__ inline void relax_segments(uint32_t* p, int stride, int len, int seg, int nseg, int dir, bool active,
+                                      uint32_t (*ends)[2], int q, int qn)
+{
+    const int first = dir > 0 ? seg * len : (seg + 1) * len - 1;
+    if (active) {
+        Cols4 prev = cols4_of(p[first * stride]);
+        for (int k = 1; k < len; k++) {
+            const int i = first + dir * k;
+            prev = cols4_relax(cols4_of(p[i * stride]), prev, 1);
+            p[i * stride] = cols4_pack(prev);
+        }
+        ends[seg * qn + q][0] = __builtin_bit_cast(uint32_t, prev.e);
+        ends[seg * qn + q][1] = __builtin_bit_cast(uint32_t, prev.o);
+    }
+    __syncthreads();
+    if (active) {
+        const unsigned short big = 0x4000;
+        Cols4 cin{u16x2{big, big}, u16x2{big, big}};
+        bool any = false;
+        for (int s2 = dir > 0 ? 0 : nseg - 1; s2 != seg; s2 += dir) {
+            const Cols4 e{__builtin_bit_cast(u16x2, ends[s2 * qn + q][0]), __builtin_bit_cast(u16x2, ends[s2 * qn + q][1])};
+            cin = cols4_relax(e, cin, (unsigned short)(len - 1));
+            const u16x2 one = {1, 1};
+            cin.e = cin.e + one;
+            cin.o = cin.o + one;
+            any = true;
+        }
+        if (any)
+            for (int k = 0; k < len; k++) {
+                const int i = first + dir * k;
+                p[i * stride] = cols4_pack(cols4_relax(cols4_of(p[i * stride]), cin, (unsigned short)k));
+            }
+    }
+    __syncthreads();
+}
+
+
 constexpr int kDfCap = 254;
 
 // X then Y pass over one z-slab held in LDS (slabs of at most kSlabMax
@@ -141,33 +213,34 @@ constexpr int kDfCap = 254;
 //
 // X pass: the reference's forward/backward sweep along a row equals
 // min(cap, distance to the nearest solid cell of the row), which one
-// wavefront computes per row with two scans: each lane owns 4 consecutive
-// bytes, a wave prefix-max gives the last solid position at or before every
-// byte and a wave suffix-min the next solid position at or after it.
+// wavefront computes per row with two prefix-max scans (DPP): each lane owns
+// 4 consecutive bytes; the first scan gives the last solid position at or
+// before every byte, whose distance (at most 255) replaces the byte in
+// place -- 0 exactly at solid cells; the second runs over the mirrored row
+// (position nx - 1 - x) and so gives the next solid position at or after
+// every byte, and the row takes min(cap, forward, backward).
 // Y pass: one lane per column, forward and backward relaxation with the
 // running value in a register (LDS reads independent of it).
 constexpr int kSlabMax = 128 * 1024;
 constexpr int kNone = 1 << 20;
 
+// Inclusive prefix max over the wavefront's 64 lanes with DPP moves: within
+// each row of 16 lanes (row_shr 1, 2, 4, 8; a lane without a source keeps
+// -kNone), then the last lane of rows 0 / 2 into rows 1 / 3 (row_bcast 15),
+// then lane 31 into rows 2 and 3 (row_bcast 31).
 __device__ inline int wave_incl_max(int v)
 {
-    const int lane = threadIdx.x & 63;
-    for (int d = 1; d < 64; d <<= 1) {
-        int t = __shfl_up(v, d);
-        if (lane >= d) v = max(v, t);
-    }
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x111, 0xf, 0xf, false));   // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x112, 0xf, 0xf, false));   // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x114, 0xf, 0xf, false));   // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x118, 0xf, 0xf, false));   // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x142, 0xa, 0xf, false));   // row_bcast:15
+    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x143, 0xc, 0xf, false));   // row_bcast:31
     return v;
 }
 
-__device__ inline int wave_incl_min_rev(int v)
-{
-    const int lane = threadIdx.x & 63;
-    for (int d = 1; d < 64; d <<= 1) {
-        int t = __shfl_down(v, d);
-        if (lane + d < 64) v = min(v, t);
-    }
-    return v;
-}
+// The previous lane's value (wave_shr:1); lane 0 gets -kNone.
+__device__ inline int wave_prev(int v) { return __builtin_amdgcn_update_dpp(-kNone, v, 0x138, 0xf, 0xf, false); }
 
 __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restrict__ grid, uint8_t* __restrict__ out,
                                                          int nx, int ny, int nz, const int32_t* __restrict__ gate)
@@ -188,13 +261,32 @@ __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restri
     // ---- X pass, one wavefront per row ----------------------------------
     for (int y = wave; y < ny; y += waves) {
         uint8_t* row = slab + y * nx;
-        int fwd[4][4];
-        int carry = -kNone;   // last solid position left of the chunk
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            if (c >= chunks) break;
+        // a row without a solid cell becomes cap everywhere, a row of solid
+        // cells 0 everywhere (the scans below give exactly that)
+        bool any_solid = false, any_air = false;
+        for (int c = 0; c < chunks; c++) {
             const int x0 = c * 256 + lane * 4;
-            uint32_t v = x0 < nx ? *reinterpret_cast<const uint32_t*>(row + x0) : 0u;
+            if (x0 < nx) {
+                const uint32_t v = *reinterpret_cast<const uint32_t*>(row + x0);
+                any_solid = any_solid || v != 0u;
+                any_air = any_air || (((v - 0x01010101u) & ~v & 0x80808080u) != 0u);   // a zero byte
+            }
+        }
+        const bool mixed = __ballot(any_solid) != 0ull && __ballot(any_air) != 0ull;
+        if (!mixed) {
+            const uint32_t fill = __ballot(any_solid) != 0ull ? 0u : (uint32_t)cap * 0x01010101u;
+            for (int c = 0; c < chunks; c++) {
+                const int x0 = c * 256 + lane * 4;
+                if (x0 < nx) *reinterpret_cast<uint32_t*>(row + x0) = fill;
+            }
+            continue;
+        }
+        // forward: distance to the last solid at or before x (255: none),
+        // written in place
+        int carry = -kNone;   // last solid position left of the chunk
+        for (int c = 0; c < chunks; c++) {
+            const int x0 = c * 256 + lane * 4;
+            const uint32_t v = x0 < nx ? *reinterpret_cast<const uint32_t*>(row + x0) : 0u;
             int last_local = -kNone;
             int lastpos[4];
 #pragma unroll
@@ -202,78 +294,78 @@ __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restri
                 if ((v >> (8 * k)) & 0xffu) last_local = x0 + k;
                 lastpos[k] = last_local;
             }
-            int incl = wave_incl_max(last_local);
-            int excl = __shfl_up(incl, 1);
-            if (lane == 0) excl = -kNone;
-            excl = max(excl, carry);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                int l = max(lastpos[k], excl);
-                fwd[c][k] = l < 0 ? kNone : x0 + k - l;
-            }
-            carry = max(carry, __shfl(incl, 63));
-        }
-        int carry_r = kNone;  // next solid position right of the chunk
-#pragma unroll
-        for (int c = 3; c >= 0; c--) {
-            if (c >= chunks) continue;
-            const int x0 = c * 256 + lane * 4;
-            uint32_t v = x0 < nx ? *reinterpret_cast<const uint32_t*>(row + x0) : 0u;
-            int next_local = kNone;
-            int nextpos[4];
-#pragma unroll
-            for (int k = 3; k >= 0; k--) {
-                if ((v >> (8 * k)) & 0xffu) next_local = x0 + k;
-                nextpos[k] = next_local;
-            }
-            int incl = wave_incl_min_rev(next_local);
-            int excl = __shfl_down(incl, 1);
-            if (lane == 63) excl = kNone;
-            excl = min(excl, carry_r);
+            const int incl = wave_incl_max(last_local);
+            const int excl = max(wave_prev(incl), carry);
             uint32_t outw = 0;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                int nx_pos = min(nextpos[k], excl);
-                int bwd = nx_pos >= kNone ? kNone : nx_pos - (x0 + k);
-                int d = min(cap, min(fwd[c][k], bwd));
+                const int l = max(lastpos[k], excl);
+                const int d = l < 0 ? 255 : min(255, x0 + k - l);
                 outw |= (uint32_t)d << (8 * k);
             }
             if (x0 < nx) *reinterpret_cast<uint32_t*>(row + x0) = outw;
-            carry_r = min(carry_r, __shfl(incl, 0));
+            carry = max(carry, __builtin_amdgcn_readlane(incl, 63));
+        }
+        // backward, over mirrored positions q = nx - 1 - x: a lane's bytes
+        // q0 .. q0 + 3 are the dword at x = nx - 4 - q0, reversed; a solid
+        // is a forward distance of 0
+        carry = -kNone;
+        for (int c = 0; c < chunks; c++) {
+            const int q0 = c * 256 + lane * 4;
+            const int xb = nx - 4 - q0;
+            const uint32_t v = q0 < nx ? *reinterpret_cast<const uint32_t*>(row + xb) : 0xffffffffu;
+            int last_local = -kNone;
+            int lastpos[4], fw[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                fw[k] = (int)((v >> (8 * (3 - k))) & 0xffu);
+                if (fw[k] == 0) last_local = q0 + k;
+                lastpos[k] = last_local;
+            }
+            const int incl = wave_incl_max(last_local);
+            const int excl = max(wave_prev(incl), carry);
+            uint32_t outw = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int l = max(lastpos[k], excl);
+                const int bwd = l < 0 ? kNone : q0 + k - l;
+                const int d = min(cap, min(fw[k], bwd));
+                outw |= (uint32_t)d << (8 * (3 - k));
+            }
+            if (q0 < nx) *reinterpret_cast<uint32_t*>(row + xb) = outw;
+            carry = max(carry, __builtin_amdgcn_readlane(incl, 63));
         }
     }
     __syncthreads();
-    // ---- Y pass (ManhattanDistanceY.comp:103-118), one lane per column ---
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-        int prev = slab[x];
-#pragma unroll 8
-        for (int y = 1; y < ny; y++) {
-            int v = min((int)slab[y * nx + x], prev + 1);
-            slab[y * nx + x] = (uint8_t)v;
-            prev = v;
-        }
-        int next = prev;
-#pragma unroll 8
-        for (int y = ny - 2; y >= 0; y--) {
-            int v = min((int)slab[y * nx + x], next + 1);
-            slab[y * nx + x] = (uint8_t)v;
-            next = v;
-        }
+    // ---- Y pass (ManhattanDistanceY.comp:103-118): four adjacent columns per
+    // lane (one LDS dword per row), the rows split into segments
+    // (relax_segments); every thread takes part in its barriers
+    {
+        __shared__ uint32_t s_end_y[512][2];
+        const int qn = nx / 4;
+        int nseg = 8;
+        while (nseg > 1 && (qn * nseg > (int)blockDim.x || ny % nseg != 0)) nseg >>= 1;
+        const bool active = (int)threadIdx.x < qn * nseg;
+        const int q = active ? (int)threadIdx.x % qn : 0, sg = active ? (int)threadIdx.x / qn : 0;
+        uint32_t* col = reinterpret_cast<uint32_t*>(slab) + q;
+        relax_segments(col, qn, ny / nseg, sg, nseg, +1, active, s_end_y, q, qn);
+        relax_segments(col, qn, ny / nseg, sg, nseg, -1, active, s_end_y, q, qn);
     }
-    __syncthreads();
     for (int i = threadIdx.x * 16; i < bytes; i += blockDim.x * 16)
         *reinterpret_cast<uint4*>(out + base + i) = *reinterpret_cast<const uint4*>(slab + i);
 }
 
 // Z pass over a 32 x 8 x nz column tile held in LDS (tiles of at most
-// kZTileMax bytes; larger ones use df_z_kernel): one lane per (x, y)
-// column, forward and backward relaxation along z (ManhattanDistanceZ.comp:
-// 151-165) with the running value in a register, then the tile -- four
-// whole 8x8 bricks wide and one brick high -- is written to the bricked DF
-// with 16-byte stores.
+// kZTileMax bytes; larger ones use df_z_kernel): the forward and backward
+// relaxations along z (ManhattanDistanceZ.comp:151-165) of the tile's 256
+// columns, four adjacent columns per lane (one LDS dword per z) and each
+// column's z range split into kZSeg segments (relax_segments); then the
+// tile -- four whole 8x8 bricks wide and one brick high -- is written to
+// the bricked DF with 16-byte stores.
 constexpr int kZTileMax = 128 * 1024;
+constexpr int kZSeg = 8;
 
-__global__ void __launch_bounds__(256) df_z_tile_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ df_brick,
+__global__ void __launch_bounds__(64 * kZSeg) df_z_tile_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ df_brick,
                                                         int nx, int ny, int nz, uint32_t row_stride,
                                                         uint32_t slab_stride, const int32_t* __restrict__ gate)
 {
@@ -295,23 +387,12 @@ __global__ void __launch_bounds__(256) df_z_tile_kernel(const uint8_t* __restric
     }
     __syncthreads();
     {
-        const int c = threadIdx.x;   // column: y = c / 32, x = c % 32
-        int prev = tile[c];
-#pragma unroll 8
-        for (int z = 1; z < nz; z++) {
-            int v = min((int)tile[z * 256 + c], prev + 1);
-            tile[z * 256 + c] = (uint8_t)v;
-            prev = v;
-        }
-        int next = prev;
-#pragma unroll 8
-        for (int z = nz - 2; z >= 0; z--) {
-            int v = min((int)tile[z * 256 + c], next + 1);
-            tile[z * 256 + c] = (uint8_t)v;
-            next = v;
-        }
+        __shared__ uint32_t s_end[kZSeg * 64][2];
+        const int q = (int)(threadIdx.x & 63u), sg = (int)(threadIdx.x >> 6);   // dword of a z row, segment
+        uint32_t* col = reinterpret_cast<uint32_t*>(tile) + q;
+        relax_segments(col, 64, nz / kZSeg, sg, kZSeg, +1, true, s_end, q, 64);
+        relax_segments(col, 64, nz / kZSeg, sg, kZSeg, -1, true, s_end, q, 64);
     }
-    __syncthreads();
     // write whole bricks: brick (bx, bz) of this tile, 512 bytes each, one
     // 16-byte group per lane: bytes m..m+15 of the Morton order
     const int nbz = nz / 8;
@@ -650,10 +731,11 @@ int df_build_launch(vpt_world* w, hipStream_t s, const int32_t* gate)
     }
     uint32_t rs = (uint32_t)w->nbx * vpt::kBrickBytes, ss = rs * (uint32_t)w->nby;
     const size_t ztile = (size_t)nz * 256;
-    if (nx % 32 == 0 && ztile <= (size_t)kZTileMax) {
+    if (nx % 32 == 0 && nz % kZSeg == 0 && ztile <= (size_t)kZTileMax) {
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_z_tile_kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kZTileMax));
-        hipLaunchKernelGGL(df_z_tile_kernel, dim3((unsigned)((nx / 32) * (ny / 8))), dim3(256), ztile, s, w->d_tmp,
+        hipLaunchKernelGGL(df_z_tile_kernel, dim3((unsigned)((nx / 32) * (ny / 8))), dim3(64 * kZSeg), ztile, s,
+                           w->d_tmp,
                            w->d_df, nx, ny, nz, rs, ss, gate);
     } else {
         size_t cols = (size_t)nx * ny;
