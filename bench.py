@@ -33,7 +33,7 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 # the kernels each frame pass launches (the roofline covers the whole pass)
 PASS_KERNELS = {"primary": "primary_kernel", "shadow": "shadow_kernel",
-                "gi": "gi pass: gi_bounce_kernel + gi_shadow_kernel + gi_shade_kernel"}
+                "gi": "gi pass: gi_bounce_kernel + gi_shadow_kernel"}
 NX, NY, NZ = 384, 128, 384
 CAM_POS = (192.5, 100.5, 192.5)
 CAM_YAW, CAM_PITCH = 45.0, -20.0
@@ -388,6 +388,7 @@ def main():
     pipeline_graph_stats = {"side_chain": pipe.graphs[0].stats(), "main_chain": pipe.graphs[1].stats(),
                             "tail": pipe.graphs[2].stats()}
     del pipe
+    pipeline_cpp = cpp_pipeline(W, H)
 
     result = {
         "metric": "Grays/s DDA+GI @1080p (bit-exact hits); SVGF ms/frame; served p50/p99 latency",
@@ -421,6 +422,7 @@ def main():
         "reflect": reflect,
         "pipeline_ms_per_frame": round(pipeline_ms, 4),
         "pipeline_graph": {"ms_per_frame": round(pipeline_graph_ms, 4), "graphs": pipeline_graph_stats},
+        "pipeline_cpp": pipeline_cpp,
         "roofline": {"bound": "hbm", "kernel": PASS_KERNELS[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": profiled_traffic(f"{dom} pass"), "algorithmic_bytes": alg[dom]},
@@ -445,6 +447,24 @@ def main():
         torch.cuda.synchronize()
         result["cpu_baseline"] = cpu_baseline(grid, gb, mask, rgb, cam, prm, args.cpu_reps)
     print(json.dumps(result), flush=True)
+
+
+def cpp_pipeline(W: int, H: int, frames: int = 16, warmup: int = 4) -> dict:
+    """The same frame loop through the C++ host (VoxelRT::Pipeline, the
+    in-tree example binary): one launch per pass, and each frame's three
+    chains replayed as HIP graphs; ms per frame over `frames` frames after
+    `warmup` (GPU events in the binary)."""
+    import subprocess
+    from voxelpathtracer_amd.build import DEMO
+    out = {}
+    for mode in ("plain", "graph"):
+        r = subprocess.run([str(DEMO), str(W), str(H), str(frames), f"--{mode}", "--warmup", str(warmup)],
+                           capture_output=True, text=True, timeout=120)
+        if r.returncode != 0:
+            raise RuntimeError(f"vpt_frame_demo --{mode} failed: {r.stderr[-500:]}")
+        out[f"{mode}_ms_per_frame"] = json.loads(r.stdout.strip().splitlines()[-1])["ms_per_frame"]
+    out.update(frames=frames, warmup=warmup)
+    return out
 
 
 def df_update_timing(world, grid, torch, edits: int = 24):

@@ -24,8 +24,10 @@ public:
     // persistent: trace primary, sun shadow and GI with the persistent engine
     // (vpt_frame_trace) instead of one launch per pass (vpt_trace_primary,
     // vpt_trace_gi; the faster path).  Either way the shadow pass with its
-    // transversal and the shadow denoise run on a side stream beside GI and
-    // SVGF; the shaded reflections and the composite follow both.
+    // transversal, the shadow denoise and the reflection rays
+    // (vpt_trace_reflect_deferred) run on a side stream beside GI and SVGF;
+    // the reflection shading (vpt_shade_reflect) and the composite follow
+    // both.
     // use_graph (one launch per pass only): after the primary pass, the side
     // chain, the main chain and the tail are each captured and replayed as a
     // HIP graph (vpt_graph_*) every frame, the instantiated graphs updated in
@@ -55,6 +57,12 @@ public:
         Alloc(&m_ReflDist, (size_t)m_RW * m_RH * sizeof(uint16_t) + 16);
         Alloc(&m_ReflHit, (size_t)m_RW * m_RH * sizeof(uint32_t) + 16);
         Alloc(&m_ReflColor, (size_t)m_RW * m_RH * 4 * sizeof(uint16_t) + 16);
+        {
+            vpt_reflect_params rp;
+            const float up[3] = {0.0f, 1.0f, 0.0f};
+            vpt_check(vpt_reflect_params_default(up, 0, &rp));
+            Alloc(&m_ReflRecords, (size_t)rp.spp * m_RW * m_RH * VPT_REFLECT_RECORD_BYTES);
+        }
         Alloc(&m_Color, px * 3 * sizeof(float));
         Alloc(&m_RGB, px * 3 * sizeof(float));
         Alloc(&m_Denoised, px * 3 * sizeof(float));
@@ -95,6 +103,7 @@ public:
         (void)hipFree(m_ReflDist);
         (void)hipFree(m_ReflHit);
         (void)hipFree(m_ReflColor);
+        (void)hipFree(m_ReflRecords);
         (void)hipFree(m_Color);
         (void)hipFree(m_RGB);
         (void)hipFree(m_Denoised);
@@ -129,14 +138,19 @@ public:
         float prev_vp[16], cur_vp[16];
         ViewProj(m_PrevView, m_PrevProj, prev_vp);
         ViewProj(view, proj, cur_vp);
-        // the side chain (shadow with its transversal, shadow denoise) reads
-        // only the G-buffer and the shadow pass and runs beside the main
-        // chain (GI, SVGF); the tail (reflections shaded from both chains'
-        // outputs, composite) follows both
+        // the side chain (shadow with its transversal, shadow denoise,
+        // reflection rays) reads only the G-buffer and the shadow pass and
+        // runs beside the main chain (GI, SVGF); the tail (reflection hits
+        // shaded from both chains' outputs, composite) follows both
+        vpt_reflect_params rp;
+        vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
+        rp.div = m_RDiv;
         auto side_chain = [&](void* ss) {
             vpt_check(vpt_trace_shadow2(m_World.Handle(), &cam, &cur, prm.sun_dir, 350, m_Mask, m_Transversal, ss));
             vpt_check(vpt_shadow_denoise_frame(m_ShadowDn, &cam, prev_vp, &cur, &prev, m_Mask, m_Transversal, 1.0f,
                                                m_ShadowOut, ss));
+            vpt_check(vpt_trace_reflect_deferred(m_World.Handle(), &cam, &cur, &rp, cur_vp, m_ReflDist, m_ReflHit,
+                                                 m_ReflRecords, ss));
         };
         auto main_chain = [&](void* ms) {
             if (!m_Persistent) vpt_check(vpt_trace_gi(m_World.Handle(), &cam, &cur, &prm, m_RGB, m_Texels, ms));
@@ -146,33 +160,30 @@ public:
         auto tail = [&](void* ts) {
             const vpt_svgf_texel* diffuse = nullptr;
             vpt_check(vpt_svgf_final(m_Svgf, &diffuse));
-            vpt_reflect_params rp;
-            vpt_check(vpt_reflect_params_default(prm.sun_dir, (uint32_t)m_Frame_, &rp));
-            rp.div = m_RDiv;
             vpt_reflect_shading rs;
             vpt_check(vpt_reflect_shading_from_gi(&prm, cur_vp, diffuse, m_ShadowOut, &rs));
-            vpt_check(vpt_trace_reflect_shaded(m_World.Handle(), &cam, &cur, &rp, &rs, m_ReflDist, m_ReflHit,
-                                               m_ReflColor, ts));
+            vpt_check(vpt_shade_reflect(m_Ctx, &cam, &cur, &rp, &rs, m_ReflRecords, m_ReflHit, m_ReflColor, ts));
             vpt_composite_params cp;
             vpt_check(vpt_composite_params_from_gi(&prm, &cp));
             vpt_check(vpt_composite(m_Ctx, &cam, &cur, &cp, diffuse, m_ShadowOut, m_ReflColor, m_RDiv, m_Color, ts));
         };
+        // the main chain (the longer one) is enqueued first; the side chain
+        // starts from the fork event after the primary pass
+        if (hipEventRecord(m_Fork, s) != hipSuccess || hipStreamWaitEvent(m_Side, m_Fork, 0) != hipSuccess)
+            throw std::runtime_error("Pipeline: fork");
         if (!m_Graphs[0]) {
-            if (hipEventRecord(m_Fork, s) != hipSuccess || hipStreamWaitEvent(m_Side, m_Fork, 0) != hipSuccess)
-                throw std::runtime_error("Pipeline: fork");
+            main_chain(stream);
             side_chain(m_Side);
             if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
-            main_chain(stream);
             if (hipStreamWaitEvent(s, m_Join, 0) != hipSuccess) throw std::runtime_error("Pipeline: join");
             tail(stream);
         } else {
-            // each chain is captured and replayed as a graph of its own
-            // (the first two replays start after the primary pass and
-            // overlap; the side chain's completion is joined on the side
-            // stream); the tail's replay waits for both
-            Replay(m_Graphs[0], stream, m_Side, side_chain);
-            if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
+            // each chain is captured and replayed as a graph of its own,
+            // its completion joined on the stream it started from (the first
+            // two overlap); the tail's replay waits for both
             Replay(m_Graphs[1], stream, stream, main_chain);
+            Replay(m_Graphs[0], m_Side, m_Side, side_chain);
+            if (hipEventRecord(m_Join, m_Side) != hipSuccess) throw std::runtime_error("Pipeline: join");
             if (hipStreamWaitEvent(s, m_Join, 0) != hipSuccess) throw std::runtime_error("Pipeline: join");
             Replay(m_Graphs[2], stream, stream, tail);
         }
@@ -250,10 +261,11 @@ private:
     uint16_t* m_ReflDist = nullptr;
     uint32_t* m_ReflHit = nullptr;
     uint16_t* m_ReflColor = nullptr;
+    void* m_ReflRecords = nullptr;   // the deferred reflection pass's per-sample records
     float* m_Color = nullptr;
     int m_RDiv = 4, m_RW = 0, m_RH = 0;
     vpt_shadow_denoise* m_ShadowDn = nullptr;
-    hipStream_t m_Side = nullptr;   // shadow, shadow denoise (default priority)
+    hipStream_t m_Side = nullptr;   // shadow, shadow denoise, reflection rays (default priority)
     hipEvent_t m_Fork = nullptr, m_Join = nullptr;
     float* m_RGB = nullptr;
     float* m_Denoised = nullptr;

@@ -320,6 +320,27 @@ VPT_API int vpt_trace_reflect_shaded(vpt_world* w, const vpt_camera* cam, const 
                                      const vpt_reflect_params* prm, const vpt_reflect_shading* shading,
                                      uint16_t* d_dist, uint32_t* d_hit, uint16_t* d_color, void* stream);
 
+/* Bytes of one sample's record in the deferred form (spp records per target
+ * pixel, (W / div) x (H / div) target pixels). */
+#define VPT_REFLECT_RECORD_BYTES 16
+
+/* vpt_trace_reflect_shaded in two calls, so that the rays, which need only
+ * the G-buffer, can be traced while the denoisers run.
+ * vpt_trace_reflect_deferred traces the reflected rays and the shadow rays
+ * of the samples whose hits do not reproject with viewproj (which must equal
+ * the shading's viewproj given to vpt_shade_reflect), writes d_dist and
+ * d_hit (with the shadow bit 0) and each sample's hit to d_records;
+ * vpt_shade_reflect then shades the recorded hits from the denoised planes,
+ * writes d_color and sets d_hit's shadow bit (d_hit may be NULL in both).
+ * Together they give vpt_trace_reflect_shaded's outputs for the same
+ * arguments, bit for bit. */
+VPT_API int vpt_trace_reflect_deferred(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                       const vpt_reflect_params* prm, const float viewproj[16], uint16_t* d_dist,
+                                       uint32_t* d_hit, void* d_records, void* stream);
+VPT_API int vpt_shade_reflect(vpt_ctx* ctx, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_reflect_params* prm,
+                              const vpt_reflect_shading* shading, const void* d_records, uint32_t* d_hit,
+                              uint16_t* d_color, void* stream);
+
 /* DIFFUSE_TRACE (DiffuseRayTraceFrag.glsl:822-935), 1 spp, 1 bounce with a
  * shadow ray at the bounce hit.  d_rgb: 3 floats per pixel (clamped to
  * [0, 8]).  d_svgf_in (optional): SVGF input texels.  Two kernels: the

@@ -187,3 +187,34 @@ def test_composite_argument_errors(ctx, scene):
         d = torch.empty((H // 4, W // 4), dtype=torch.int16, device="cuda")
         w.trace_reflect_shaded(cam, gb, (0.0, 1.0, 0.0), shading, d, refl, div=4)
     assert reflect_params((0.0, 1.0, 0.0))["spp"] == 2
+
+
+@pytest.mark.parametrize("W,H,div,spp,rough", [(1920, 1080, 4, 2, 1), (1920, 1080, 4, 2, 0), (640, 360, 1, 3, 1),
+                                               (333, 187, 3, 5, 1), (640, 360, 2, 4, 0)])
+def test_reflect_deferred_equals_shaded(ctx, scene, W, H, div, spp, rough):
+    """vpt_trace_reflect_deferred + vpt_shade_reflect equal one
+    vpt_trace_reflect_shaded call bit for bit (distances, hit records,
+    colours, ray and sample counts)."""
+    import torch
+    from voxelpathtracer_amd.world import RECORD_BYTES, shade_reflect
+    g, df, w = scene
+    print(f"reflect deferred {W}x{H} div {div} spp {spp} rough {rough}", flush=True)
+    fr = _frame(ctx, w, W, H, frames=2)
+    dist, hit, color, ctr = _reflect(w, fr, W, H, div, spp, rough)
+    Wr, Hr = W // div, H // div
+    d2 = torch.full_like(dist, 0x7E00)
+    h2 = torch.full_like(hit, 7)
+    c2 = torch.full_like(color, 0x7E00)
+    recs = torch.full((spp * Hr * Wr * RECORD_BYTES // 4,), -1, dtype=torch.int32, device="cuda")
+    ctr2 = torch.zeros(2, dtype=torch.int64, device="cuda")
+    prm = fr["prm"]
+    kw = dict(div=div, max_iter=64, shadow_iter=150, spp=spp, rough=rough, roughness=0.425, frame=fr["frame"])
+    w.set_sample_counter(ctr2)
+    w.trace_reflect_deferred(fr["cam"], fr["gb"], list(prm.sun_dir), fr["vp"], d2, recs, h2, **kw)
+    torch.cuda.synchronize()
+    w.set_sample_counter(None)
+    shading = reflect_shading(prm, fr["vp"], fr["final_ptr"], fr["shadow"])
+    shade_reflect(ctx, fr["cam"], fr["gb"], list(prm.sun_dir), shading, recs, c2, h2, **kw)
+    torch.cuda.synchronize()
+    assert torch.equal(d2, dist) and torch.equal(h2, hit) and torch.equal(c2, color)
+    assert torch.equal(ctr2, ctr)

@@ -157,6 +157,10 @@ _PROTOS = {
     "vpt_reflect_shading_from_gi": (_i, [C.POINTER(GiParams), C.POINTER(_f), _vp, _vp, C.POINTER(ReflectShading)]),
     "vpt_trace_reflect_shaded": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(ReflectParams),
                                       C.POINTER(ReflectShading), _vp, _vp, _vp, _vp]),
+    "vpt_trace_reflect_deferred": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(ReflectParams),
+                                        C.POINTER(_f), _vp, _vp, _vp, _vp]),
+    "vpt_shade_reflect": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(ReflectParams),
+                               C.POINTER(ReflectShading), _vp, _vp, _vp, _vp]),
     "vpt_composite_params_from_gi": (_i, [C.POINTER(GiParams), C.POINTER(CompositeParams)]),
     "vpt_composite": (_i, [_vp, C.POINTER(Camera), C.POINTER(GBuffer), C.POINTER(CompositeParams), _vp, _vp, _vp, _i,
                            _vp, _vp]),

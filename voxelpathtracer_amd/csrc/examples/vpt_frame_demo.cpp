@@ -8,10 +8,11 @@
 // reflection colour (RGBA binary16 bits, H/4 x W/4 x 4) and its composite
 // (float32, H x W x 3).
 //
-//   vpt_frame_demo [W H FRAMES [--plain | --graph] [--dump FILE]]
+//   vpt_frame_demo [W H FRAMES [--plain | --graph] [--warmup N] [--dump FILE]]
 //
 // --plain: one launch per pass; --graph: one launch per pass, every frame
-// replayed as a HIP graph; default: the persistent engine.
+// replayed as a HIP graph; default: the persistent engine.  --warmup N:
+// N frames of the same camera path before the FRAMES timed ones.
 #include <hip/hip_runtime_api.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -29,7 +30,9 @@ int main(int argc, char** argv)
     int frames = argc > 3 ? atoi(argv[3]) : 8;
     bool persistent = true, graph = false;
     const char* dump = nullptr;
+    int warmup = 0;
     for (int i = 4; i < argc; i++) {
+        if (!strcmp(argv[i], "--warmup") && i + 1 < argc) warmup = atoi(argv[++i]);
         if (!strcmp(argv[i], "--plain")) persistent = false;
         if (!strcmp(argv[i], "--graph")) {
             persistent = false;
@@ -49,9 +52,13 @@ int main(int argc, char** argv)
         hipEvent_t e0, e1;
         (void)hipEventCreate(&e0);
         (void)hipEventCreate(&e1);
+        for (int f = 0; f < warmup; f++) {
+            const float pos[3] = {192.5f + 0.1f * (float)f, 100.5f, 192.5f};
+            pipe.RenderFrame(pos, 45.0f + 0.2f * (float)f, -20.0f);
+        }
         (void)hipDeviceSynchronize();
         (void)hipEventRecord(e0, nullptr);
-        for (int f = 0; f < frames; f++) {
+        for (int f = warmup; f < warmup + frames; f++) {
             const float pos[3] = {192.5f + 0.1f * (float)f, 100.5f, 192.5f};
             pipe.RenderFrame(pos, 45.0f + 0.2f * (float)f, -20.0f);
         }

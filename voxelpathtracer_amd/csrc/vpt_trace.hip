@@ -693,48 +693,130 @@ __device__ inline Vec3 reflect_direct(Vec3 hp, Vec3 eye, Vec3 L, const float rad
                     fmaxf(((kd * alb.z) * rad[2]) * cos_li, 0.0f) * vis);
 }
 
+// Reprojection of a reflection hit onto the frame (ReprojectReflectionToScreenSpace,
+// ReflectionTraceFrag.glsl:582-593): uv of the hit through the view-projection;
+// inside the 0.0035 border, the G-buffer position there (t LINEAR, the ray
+// at uv) and its face (NEAREST) must match the hit (squared distance
+// < 0.095, same normal).
+struct Reproj {
+    bool ok;
+    float ru, rv, t_at;
+    Vec3 p_at;
+};
+
+__device__ inline Reproj reproject(const float vp[16], const CamView& cam, int gW, int gH,
+                                   const float* __restrict__ gb_t, const uint8_t* __restrict__ gb_face, Vec3 eye,
+                                   Vec3 hp, Vec3 hn)
+{
+    Reproj r;
+    r.ok = false;
+    r.t_at = 0.0f;
+    r.p_at = vpt::mk3(0.0f, 0.0f, 0.0f);
+    const float q0 = vpt::mat_row(vp, 0, hp.x, hp.y, hp.z, 1.0f);
+    const float q1 = vpt::mat_row(vp, 1, hp.x, hp.y, hp.z, 1.0f);
+    const float q3 = vpt::mat_row(vp, 3, hp.x, hp.y, hp.z, 1.0f);
+    r.ru = (q0 / q3) * 0.5f + 0.5f;
+    r.rv = (q1 / q3) * 0.5f + 0.5f;
+    const float b = 0.0035f;
+    if (r.ru < 1.0f - b && r.ru > b && r.rv < 1.0f - b && r.rv > b) {
+        r.t_at = gl_linear(gb_t, gW, gH, r.ru, r.rv);
+        const Vec3 dir = vpt::normalize3(vpt::ray_dir_at(cam.iv, cam.ip, r.ru, r.rv));
+        r.p_at = vpt::mk3(eye.x + dir.x * r.t_at, eye.y + dir.y * r.t_at, eye.z + dir.z * r.t_at);
+        const Vec3 n_at = vpt::face_normal(gl_nearest(gb_face, gW, gH, r.ru, r.rv), 1.0f);
+        const float dx = fabsf(r.p_at.x - hp.x), dy = fabsf(r.p_at.y - hp.y), dz = fabsf(r.p_at.z - hp.z);
+        const float err = (dx * dx + dy * dy) + dz * dz;
+        r.ok = err < 0.095f && n_at.x == hn.x && n_at.y == hn.y && n_at.z == hn.z;
+    }
+    return r;
+}
+
+// The colour one reflection hit adds: the ambient (the denoised diffuse at
+// the reprojected uv, times pow(AO, 0.75) within 40 of the camera, else the
+// ambient at the pixel) times the block's albedo, plus reflect_direct().
+__device__ inline void shade_hit(const ReflShade& shd, int gW, int gH, const Reproj& rp, Vec3 base_amb, Vec3 eye,
+                                 Vec3 L, Vec3 hp, Vec3 hn, int block, float shadow_f, float tot[3])
+{
+    Vec3 amb = base_amb;
+    if (rp.ok) {
+        float rs[4], rcc[2], ao;
+        texel_linear_sh(shd.diffuse, gW, gH, rp.ru, rp.rv, rs, rcc, &ao);
+        amb = sh_to_irradiance_a(rs, rcc);
+        const float vx = vpt_powf(ao, 0.75f);
+        if (rp.t_at > 0.0f) {
+            const float ex = rp.p_at.x - eye.x, ey = rp.p_at.y - eye.y, ez = rp.p_at.z - eye.z;
+            if (sqrtf((ex * ex + ey * ey) + ez * ez) < 40.0f) amb = vpt::mk3(amb.x * vx, amb.y * vx, amb.z * vx);
+        }
+    }
+    const int ref = min(max(block, 0), 127);
+    const Vec3 alb = vpt::mk3(shd.albedo.v[3 * ref], shd.albedo.v[3 * ref + 1], shd.albedo.v[3 * ref + 2]);
+    amb = vpt::mk3(amb.x * alb.x, amb.y * alb.y, amb.z * alb.z);
+    const Vec3 dl = reflect_direct(hp, eye, L, shd.rad, alb, hn, shadow_f);
+    tot[0] = tot[0] + (amb.x + dl.x);
+    tot[1] = tot[1] + (amb.y + dl.y);
+    tot[2] = tot[2] + (amb.z + dl.z);
+}
+
+// A sample's record in the deferred form: the hit position, and face |
+// block << 8 | hit << 16 | shadow ray traced << 17 | its result << 18.
+__device__ inline uint4 reflect_record(Vec3 hp, int face, int block, bool hit, bool traced, int shadow)
+{
+    return make_uint4(__float_as_uint(hp.x), __float_as_uint(hp.y), __float_as_uint(hp.z),
+                      (uint32_t)(face & 0xff) | ((uint32_t)(block & 0xff) << 8) | ((uint32_t)hit << 16) |
+                          ((uint32_t)traced << 17) | ((uint32_t)shadow << 18));
+}
+
+// G-buffer distance and face at a target pixel (its centre sampled LINEAR /
+// NEAREST when the target is smaller).
+__device__ inline void reflect_gbuffer(const CamView& cam, int gW, int gH, const float* __restrict__ gb_t,
+                                       const uint8_t* __restrict__ gb_face, int div, int px, int py, float& dist,
+                                       int& face)
+{
+    if (div == 1) {
+        const size_t i = (size_t)py * cam.W + px;
+        dist = gb_t[i];
+        face = gb_face[i];
+    } else {
+        const float sx = vpt::texcoord(px, cam.W), sy = vpt::texcoord(py, cam.H);
+        dist = gl_linear(gb_t, gW, gH, sx, sy);
+        face = gl_nearest(gb_face, gW, gH, sx, sy);
+    }
+}
+
 // One lane per target pixel: spp reflected rays (max_iter steps each) and,
 // for the first max(spp / 4, 1) that hit, the sun shadow ray (shadow_iter
-// steps), all in the lane.  kRough: GGX-sampled rays; otherwise mirror
-// rays, of which the one ray is traced once and its result summed spp
-// times in the same order.  Mirror reflections of neighbouring pixels stay
-// coherent, so the lanes of a wavefront walk similar paths.
-//   dist_out: binary16 of the hit distance (sky / no hit -1, else
-//             (0.001 + sum of T) / hits, clamped to [-10, 200]);
+// steps), all in the lane.
+//   dist_out: binary16 o_HitDistance, -1 for sky and misses.
 //   hit_out:  face | block << 8 of the first hit | shadow << 16 |
 //             hits << 24, 255 without a hit.
-// kShade adds the hit shading (see ReflShade): each hit is reprojected with
-// the frame's view-projection; reprojected hits (position within 0.095 and
-// the same normal at uv, uv inside the 0.0035 border) take their ambient
-// from the denoised diffuse there, times pow(AO, 0.75) near the eye, and,
-// for the shadow-casting samples, the denoised shadow there instead of a
-// shadow ray; other hits take the ambient at the pixel itself.  A hit adds
-// ambient * albedo + reflect_direct(), a miss the sky; color_out: their mean
-// over spp clamped to [1e-7, 100], RGBA binary16 (sky pixels 0); hit_out bit
-// 17: the first hit was reprojected; the shadow bit: the last shadow >= 0.5.
-template <bool kRough, bool kShade>
+// kMode 1 adds the hit shading (see ReflShade): each hit is reprojected
+// with the frame's view-projection (reproject()); reprojected hits take
+// their ambient from the denoised diffuse there and, for the
+// shadow-casting samples, the denoised shadow there instead of a shadow
+// ray; other hits take the ambient at the pixel itself (shade_hit()).
+// color_out: the mean over spp clamped to [1e-7, 100], RGBA binary16 (sky
+// pixels 0); hit_out bit 17: the first hit was reprojected; the shadow bit:
+// the last shadow >= 0.5.  kMode 2 is kMode 1's traversal half: the same
+// rays, reprojection tests and shadow rays, with each sample's hit written
+// to `records` (reflect_record, sample-major) and the shadow bit left for
+// reflect_shade_kernel, which does the shading from them.
+template <bool kRough, int kMode>
 __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, int gW, int gH,
                                                       const float* __restrict__ gb_t,
                                                       const uint8_t* __restrict__ gb_face, ReflectConst rc,
                                                       uint16_t* __restrict__ dist_out, uint32_t* __restrict__ hit_out,
                                                       unsigned long long* __restrict__ counter, const ReflShade shd,
-                                                      uint2* __restrict__ color_out)
+                                                      uint2* __restrict__ color_out, uint4* __restrict__ records)
 {
+    constexpr bool kShade = kMode == 1, kReproj = kMode >= 1;
     int px, py;
     tile_pixel(cam.W, cam.H, px, py);
     int samples = 0, rays = 0;
     if (px < cam.W && py < cam.H) {
         const size_t i = (size_t)py * cam.W + px;
+        const size_t npx = (size_t)cam.W * cam.H;
         float dist;
         int face;
-        if (rc.div == 1) {
-            dist = gb_t[i];
-            face = gb_face[i];
-        } else {
-            const float sx = vpt::texcoord(px, cam.W), sy = vpt::texcoord(py, cam.H);
-            dist = gl_linear(gb_t, gW, gH, sx, sy);
-            face = gl_nearest(gb_face, gW, gH, sx, sy);
-        }
+        reflect_gbuffer(cam, gW, gH, gb_t, gb_face, rc.div, px, py, dist, face);
         float hd = -1.0f;
         uint32_t rec = 255u;
         float col[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -752,73 +834,42 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
             bool first_reproj = false;
             float tot[3] = {0.0f, 0.0f, 0.0f};
             Vec3 base_amb = vpt::mk3(0.0f, 0.0f, 0.0f);
-            const float sx = vpt::texcoord(px, cam.W), sy = vpt::texcoord(py, cam.H);
             if (kShade) {   // BaseIndirectDiffuse at the target pixel
                 float bs[4], bc[2];
-                texel_linear_sh(shd.diffuse, gW, gH, sx, sy, bs, bc, nullptr);
+                texel_linear_sh(shd.diffuse, gW, gH, vpt::texcoord(px, cam.W), vpt::texcoord(py, cam.H), bs, bc,
+                                nullptr);
                 base_amb = sh_to_irradiance_a(bs, bc);
             }
-            // one hit of a sample: reprojection, shadow, shading (kShade)
-            auto on_hit = [&](const Vec3& R, const vpt::Hit& h, bool trace_shadow) {
+            // one hit of sample s: reprojection, shadow, shading or record
+            auto on_hit = [&](int s, const Vec3& R, const vpt::Hit& h, bool trace_shadow) {
                 const Vec3 hn = vpt::face_normal(h.face, 0.0f);
                 const Vec3 hp = vpt::mk3(o.x + R.x * h.t, o.y + R.y * h.t, o.z + R.z * h.t);
-                bool reproj = false;
-                float ru = 0.0f, rv = 0.0f, t_at = 0.0f;
-                Vec3 p_at = vpt::mk3(0.0f, 0.0f, 0.0f);
-                if (kShade) {
-                    const float q0 = vpt::mat_row(shd.vp, 0, hp.x, hp.y, hp.z, 1.0f);
-                    const float q1 = vpt::mat_row(shd.vp, 1, hp.x, hp.y, hp.z, 1.0f);
-                    const float q3 = vpt::mat_row(shd.vp, 3, hp.x, hp.y, hp.z, 1.0f);
-                    ru = (q0 / q3) * 0.5f + 0.5f;
-                    rv = (q1 / q3) * 0.5f + 0.5f;
-                    const float b = 0.0035f;
-                    if (ru < 1.0f - b && ru > b && rv < 1.0f - b && rv > b) {
-                        t_at = gl_linear(gb_t, gW, gH, ru, rv);
-                        const Vec3 dir = vpt::normalize3(vpt::ray_dir_at(cam.iv, cam.ip, ru, rv));
-                        p_at = vpt::mk3(eye.x + dir.x * t_at, eye.y + dir.y * t_at, eye.z + dir.z * t_at);
-                        const Vec3 n_at = vpt::face_normal(gl_nearest(gb_face, gW, gH, ru, rv), 1.0f);
-                        const float dx = fabsf(p_at.x - hp.x), dy = fabsf(p_at.y - hp.y), dz = fabsf(p_at.z - hp.z);
-                        const float err = (dx * dx + dy * dy) + dz * dz;
-                        reproj = err < 0.095f && n_at.x == hn.x && n_at.y == hn.y && n_at.z == hn.z;
-                    }
-                    if (hits == 0) first_reproj = reproj;
+                Reproj rp;
+                rp.ok = false;
+                if (kReproj) {
+                    rp = reproject(shd.vp, cam, gW, gH, gb_t, gb_face, eye, hp, hn);
+                    if (hits == 0) first_reproj = rp.ok;
                 }
+                bool traced = false;
                 if (trace_shadow) {
-                    if (kShade && reproj) {
-                        shadow_f = u8_linear(shd.shadow, gW, gH, ru, rv);
+                    if (kReproj && rp.ok) {
+                        if (kShade) shadow_f = u8_linear(shd.shadow, gW, gH, rp.ru, rp.rv);
                     } else {
                         shadow = reflect_shadow(w, o, R, h, L, rc.shadow_iter, samples, rays);
                         shadow_f = (float)shadow;
+                        traced = true;
                     }
                 }
-                if (kShade) {
-                    Vec3 amb = base_amb;
-                    if (reproj) {
-                        float rs[4], rcc[2], ao;
-                        texel_linear_sh(shd.diffuse, gW, gH, ru, rv, rs, rcc, &ao);
-                        amb = sh_to_irradiance_a(rs, rcc);
-                        const float vx = vpt_powf(ao, 0.75f);
-                        if (t_at > 0.0f) {
-                            const float ex = p_at.x - eye.x, ey = p_at.y - eye.y, ez = p_at.z - eye.z;
-                            if (sqrtf((ex * ex + ey * ey) + ez * ez) < 40.0f)
-                                amb = vpt::mk3(amb.x * vx, amb.y * vx, amb.z * vx);
-                        }
-                    }
-                    const int ref = min(max(h.block, 0), 127);
-                    const Vec3 alb = vpt::mk3(shd.albedo.v[3 * ref], shd.albedo.v[3 * ref + 1], shd.albedo.v[3 * ref + 2]);
-                    amb = vpt::mk3(amb.x * alb.x, amb.y * alb.y, amb.z * alb.z);
-                    const Vec3 dl = reflect_direct(hp, eye, L, shd.rad, alb, hn, shadow_f);
-                    tot[0] = tot[0] + (amb.x + dl.x);
-                    tot[1] = tot[1] + (amb.y + dl.y);
-                    tot[2] = tot[2] + (amb.z + dl.z);
-                }
+                if (kShade) shade_hit(shd, gW, gH, rp, base_amb, eye, L, hp, hn, h.block, shadow_f, tot);
+                if (kMode == 2) records[(size_t)s * npx + i] = reflect_record(hp, h.face, h.block, true, traced, shadow);
             };
-            auto on_miss = [&]() {
+            auto on_miss = [&](int s) {
                 if (kShade) {
                     tot[0] = tot[0] + shd.sky[0];
                     tot[1] = tot[1] + shd.sky[1];
                     tot[2] = tot[2] + shd.sky[2];
                 }
+                if (kMode == 2) records[(size_t)s * npx + i] = reflect_record(vpt::mk3(0.0f, 0.0f, 0.0f), 255, 0, false, false, 0);
             };
             if (kRough) {
                 uint32_t state = vpt_pcg_hash(vpt_pcg_hash((uint32_t)px + (uint32_t)cam.W * (uint32_t)py +
@@ -833,18 +884,19 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
                     samples += h.samples;
                     rays++;
                     if (h.t > 0.0f) {
-                        on_hit(R, h, shadow_itr < shadow_rays);
+                        on_hit(s, R, h, shadow_itr < shadow_rays);
                         if (shadow_itr < shadow_rays) shadow_itr++;
                         if (hits == 0) rec = (uint32_t)(h.face & 0xff) | ((uint32_t)(h.block & 0xff) << 8);
                         sum = sum + h.t;
                         hits++;
                     } else {
-                        on_miss();
+                        on_miss(s);
                     }
                 }
-            } else if (kShade) {
+            } else if (kReproj) {
                 // the reference repeats the mirror sample spp times: the ray
                 // and, when not reprojected, its shadow ray are traced once
+                // (the deferred form records it once, as sample 0)
                 const float k = 2.0f * vpt::dot3(N, I);
                 const Vec3 R = vpt::mk3(I.x - k * N.x, I.y - k * N.y, I.z - k * N.z);   // reflect(I, N)
                 const vpt::Hit h = vpt::traverse(w, o, R, rc.max_iter);
@@ -853,13 +905,13 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
                 int shadow_itr = 0;
                 for (int s = 0; s < rc.spp; s++) {
                     if (h.t > 0.0f) {
-                        on_hit(R, h, shadow_itr == 0);
+                        if (kMode == 1 || s == 0) on_hit(s, R, h, shadow_itr == 0);
                         if (shadow_itr < shadow_rays) shadow_itr++;
                         if (hits == 0) rec = (uint32_t)(h.face & 0xff) | ((uint32_t)(h.block & 0xff) << 8);
                         sum = sum + h.t;
                         hits++;
                     } else {
-                        on_miss();
+                        if (kMode == 1 || s == 0) on_miss(s);
                     }
                 }
             } else {
@@ -876,6 +928,7 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
                 }
             }
             if (kShade) shadow = shadow_f >= 0.5f ? 1 : 0;
+            if (kMode == 2) shadow = 0;   // set by reflect_shade_kernel
             if (hits > 0) {
                 hd = sum / fmaxf((float)hits, 0.01f);
                 rec |= ((uint32_t)shadow << 16) | ((uint32_t)hits << 24) | ((uint32_t)first_reproj << 17);
@@ -894,6 +947,67 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
                                       vpt::hw_f2h(col[2]) | (vpt::hw_f2h(col[3]) << 16));
     }
     count_samples(counter, samples, rays);
+}
+
+// The shading half of the deferred form: per target pixel, the samples of
+// reflect_kernel<., 2>'s records in order -- the same reprojection (from
+// the recorded hit position), the same choice of which samples take a
+// shadow and from where (the denoised plane when reprojected, else the
+// recorded shadow ray), the same shade_hit() -- then the colour and the hit
+// record's shadow bit, exactly as reflect_kernel<., 1> computes them.
+template <bool kRough>
+__global__ void __launch_bounds__(256) reflect_shade_kernel(CamView cam, int gW, int gH,
+                                                            const float* __restrict__ gb_t,
+                                                            const uint8_t* __restrict__ gb_face, ReflectConst rc,
+                                                            const ReflShade shd, const uint4* __restrict__ records,
+                                                            uint32_t* __restrict__ hit_out,
+                                                            uint2* __restrict__ color_out)
+{
+    int px, py;
+    tile_pixel(cam.W, cam.H, px, py);
+    if (px >= cam.W || py >= cam.H) return;
+    const size_t i = (size_t)py * cam.W + px;
+    const size_t npx = (size_t)cam.W * cam.H;
+    float dist;
+    int face;
+    reflect_gbuffer(cam, gW, gH, gb_t, gb_face, rc.div, px, py, dist, face);
+    float col[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (!(dist < 0.0f)) {
+        const Vec3 eye = vpt::mk3(cam.iv[12], cam.iv[13], cam.iv[14]);
+        const Vec3 L = vpt::mk3(rc.lx, rc.ly, rc.lz);
+        const int shadow_rays = max(rc.spp / 4, 1);
+        float bs[4], bc[2];
+        texel_linear_sh(shd.diffuse, gW, gH, vpt::texcoord(px, cam.W), vpt::texcoord(py, cam.H), bs, bc, nullptr);
+        const Vec3 base_amb = sh_to_irradiance_a(bs, bc);
+        float shadow_f = 0.0f;
+        float tot[3] = {0.0f, 0.0f, 0.0f};
+        int hits = 0, shadow_itr = 0;
+        for (int s = 0; s < rc.spp; s++) {
+            const uint4 r = records[(size_t)(kRough ? s : 0) * npx + i];
+            if ((r.w >> 16) & 1u) {
+                const int hface = (int)(r.w & 0xffu), hblock = (int)((r.w >> 8) & 0xffu);
+                const Vec3 hp = vpt::mk3(__uint_as_float(r.x), __uint_as_float(r.y), __uint_as_float(r.z));
+                const Vec3 hn = vpt::face_normal(hface, 0.0f);
+                const Reproj rp = reproject(shd.vp, cam, gW, gH, gb_t, gb_face, eye, hp, hn);
+                const bool trace_shadow = kRough ? shadow_itr < shadow_rays : shadow_itr == 0;
+                if (trace_shadow)
+                    shadow_f = rp.ok ? u8_linear(shd.shadow, gW, gH, rp.ru, rp.rv) : (float)((r.w >> 18) & 1u);
+                if (shadow_itr < shadow_rays) shadow_itr++;
+                shade_hit(shd, gW, gH, rp, base_amb, eye, L, hp, hn, hblock, shadow_f, tot);
+                hits++;
+            } else {
+                tot[0] = tot[0] + shd.sky[0];
+                tot[1] = tot[1] + shd.sky[1];
+                tot[2] = tot[2] + shd.sky[2];
+            }
+        }
+        if (hits > 0 && hit_out) hit_out[i] = (hit_out[i] & ~(1u << 16)) | ((shadow_f >= 0.5f ? 1u : 0u) << 16);
+        const float n = (float)rc.spp;
+        for (int c = 0; c < 3; c++) col[c] = vpt::fclamp(tot[c] / n, 0.0000001f, 100.0f);
+        col[3] = vpt::fclamp(n / n, 0.0000001f, 100.0f);
+    }
+    color_out[i] = make_uint2(vpt::hw_f2h(col[0]) | (vpt::hw_f2h(col[1]) << 16),
+                              vpt::hw_f2h(col[2]) | (vpt::hw_f2h(col[3]) << 16));
 }
 
 dim3 tile_grid(int W, int H) { return dim3((unsigned)(((W + 15) / 16) * ((H + 15) / 16))); }
@@ -946,11 +1060,8 @@ extern "C" VPT_API int vpt_trace_shadow2(vpt_world* w, const vpt_camera* cam, co
     return VPT_OK;
 }
 
-static int reflect_launch(const char* fn, vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
-                          const vpt_reflect_params* prm, const vpt_reflect_shading* shading, uint16_t* d_dist,
-                          uint32_t* d_hit, uint16_t* d_color, void* stream)
+static int reflect_check(const char* fn, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_reflect_params* prm)
 {
-    if (!w || !prm || !d_dist) return set_error(VPT_EINVAL, "%s: null argument", fn);
     if (int rc = vpt::check_cam(cam, fn)) return rc;
     if (int rc = vpt::check_gbuffer(gb, cam, fn, false)) return rc;
     if (int rc = vpt::check_iter(prm->max_iter, fn)) return rc;
@@ -961,36 +1072,67 @@ static int reflect_launch(const char* fn, vpt_world* w, const vpt_camera* cam, c
     if (prm->spp < 1 || prm->spp > 16) return set_error(VPT_EINVAL, "%s: spp %d not in 1..16", fn, prm->spp);
     if (prm->rough != 0 && prm->rough != 1) return set_error(VPT_EINVAL, "%s: rough must be 0 or 1", fn);
     if (!(prm->roughness == prm->roughness)) return set_error(VPT_EINVAL, "%s: roughness is NaN", fn);
+    return VPT_OK;
+}
+
+static ReflectConst reflect_const(const vpt_reflect_params* prm)
+{
+    return {prm->light_dir[0], prm->light_dir[1], prm->light_dir[2], prm->div, prm->max_iter,
+            prm->shadow_iter, prm->spp, prm->rough, fminf(fmaxf(prm->roughness, 0.01f), 1.0f),
+            prm->frame};
+}
+
+static ReflShade reflect_shade_of(const vpt_reflect_shading* shading)
+{
     ReflShade shd{};
-    if (shading) {
+    shd.diffuse = shading->d_diffuse;
+    shd.shadow = shading->d_shadow;
+    memcpy(shd.vp, shading->viewproj, sizeof(shd.vp));
+    for (int c = 0; c < 3; c++) {
+        shd.rad[c] = shading->sun_radiance[c] * 0.6f;
+        shd.sky[c] = shading->sky_radiance[c];
+    }
+    static_assert(sizeof(shd.albedo.v) == sizeof(shading->albedo), "albedo table size");
+    memcpy(shd.albedo.v, shading->albedo, sizeof(shd.albedo.v));
+    return shd;
+}
+
+// mode 0: plain, 1: with the hit shading, 2: the deferred traversal half
+// (viewproj from `vp`, records to d_records)
+static int reflect_launch(const char* fn, vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                          const vpt_reflect_params* prm, int mode, const vpt_reflect_shading* shading,
+                          const float* vp, uint16_t* d_dist, uint32_t* d_hit, uint16_t* d_color, void* d_records,
+                          void* stream)
+{
+    if (!w || !prm || !d_dist) return set_error(VPT_EINVAL, "%s: null argument", fn);
+    if (int rc = reflect_check(fn, cam, gb, prm)) return rc;
+    ReflShade shd{};
+    if (mode == 1) {
         if (!shading->d_diffuse || !shading->d_shadow || !d_color)
             return set_error(VPT_EINVAL, "%s: diffuse, shadow and colour planes are required", fn);
-        shd.diffuse = shading->d_diffuse;
-        shd.shadow = shading->d_shadow;
-        memcpy(shd.vp, shading->viewproj, sizeof(shd.vp));
-        for (int c = 0; c < 3; c++) {
-            shd.rad[c] = shading->sun_radiance[c] * 0.6f;
-            shd.sky[c] = shading->sky_radiance[c];
-        }
-        static_assert(sizeof(shd.albedo.v) == sizeof(shading->albedo), "albedo table size");
-        memcpy(shd.albedo.v, shading->albedo, sizeof(shd.albedo.v));
+        shd = reflect_shade_of(shading);
+    } else if (mode == 2) {
+        if (!vp || !d_records) return set_error(VPT_EINVAL, "%s: viewproj and records are required", fn);
+        memcpy(shd.vp, vp, sizeof(shd.vp));
     }
     DeviceGuard g(w->device);
     CamView cv = vpt::make_cam(cam);
-    cv.W = cam->width / div;
-    cv.H = cam->height / div;
-    const ReflectConst rc = {prm->light_dir[0], prm->light_dir[1], prm->light_dir[2], div, prm->max_iter,
-                             prm->shadow_iter, prm->spp, prm->rough, fminf(fmaxf(prm->roughness, 0.01f), 1.0f),
-                             prm->frame};
+    cv.W = cam->width / prm->div;
+    cv.H = cam->height / prm->div;
+    const ReflectConst rc = reflect_const(prm);
     uint2* col = reinterpret_cast<uint2*>(d_color);
+    uint4* recs = reinterpret_cast<uint4*>(d_records);
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, tile_grid(cv.W, cv.H), dim3(256), 0, (hipStream_t)stream, vpt::make_view(w), cv,
-                           cam->width, cam->height, gb->d_t, gb->d_face, rc, d_dist, d_hit, w->d_counter, shd, col);
+                           cam->width, cam->height, gb->d_t, gb->d_face, rc, d_dist, d_hit, w->d_counter, shd, col,
+                           recs);
     };
-    if (shading)
-        rc.rough ? go(reflect_kernel<true, true>) : go(reflect_kernel<false, true>);
+    if (mode == 1)
+        rc.rough ? go(reflect_kernel<true, 1>) : go(reflect_kernel<false, 1>);
+    else if (mode == 2)
+        rc.rough ? go(reflect_kernel<true, 2>) : go(reflect_kernel<false, 2>);
     else
-        rc.rough ? go(reflect_kernel<true, false>) : go(reflect_kernel<false, false>);
+        rc.rough ? go(reflect_kernel<true, 0>) : go(reflect_kernel<false, 0>);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }
@@ -999,7 +1141,8 @@ extern "C" VPT_API int vpt_trace_reflect(vpt_world* w, const vpt_camera* cam, co
                                          const vpt_reflect_params* prm, uint16_t* d_dist, uint32_t* d_hit,
                                          void* stream)
 {
-    return reflect_launch("vpt_trace_reflect", w, cam, gb, prm, nullptr, d_dist, d_hit, nullptr, stream);
+    return reflect_launch("vpt_trace_reflect", w, cam, gb, prm, 0, nullptr, nullptr, d_dist, d_hit, nullptr, nullptr,
+                          stream);
 }
 
 extern "C" VPT_API int vpt_trace_reflect_shaded(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
@@ -1007,7 +1150,41 @@ extern "C" VPT_API int vpt_trace_reflect_shaded(vpt_world* w, const vpt_camera* 
                                                 uint16_t* d_dist, uint32_t* d_hit, uint16_t* d_color, void* stream)
 {
     if (!shading) return set_error(VPT_EINVAL, "vpt_trace_reflect_shaded: null shading");
-    return reflect_launch("vpt_trace_reflect_shaded", w, cam, gb, prm, shading, d_dist, d_hit, d_color, stream);
+    return reflect_launch("vpt_trace_reflect_shaded", w, cam, gb, prm, 1, shading, nullptr, d_dist, d_hit, d_color,
+                          nullptr, stream);
+}
+
+extern "C" VPT_API int vpt_trace_reflect_deferred(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                                  const vpt_reflect_params* prm, const float viewproj[16],
+                                                  uint16_t* d_dist, uint32_t* d_hit, void* d_records, void* stream)
+{
+    return reflect_launch("vpt_trace_reflect_deferred", w, cam, gb, prm, 2, nullptr, viewproj, d_dist, d_hit, nullptr,
+                          d_records, stream);
+}
+
+extern "C" VPT_API int vpt_shade_reflect(vpt_ctx* ctx, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                         const vpt_reflect_params* prm, const vpt_reflect_shading* shading,
+                                         const void* d_records, uint32_t* d_hit, uint16_t* d_color, void* stream)
+{
+    const char* fn = "vpt_shade_reflect";
+    if (!ctx || !prm || !shading || !d_records || !d_color) return set_error(VPT_EINVAL, "%s: null argument", fn);
+    if (int rc = reflect_check(fn, cam, gb, prm)) return rc;
+    if (!shading->d_diffuse || !shading->d_shadow)
+        return set_error(VPT_EINVAL, "%s: diffuse and shadow planes are required", fn);
+    DeviceGuard g(ctx->device);
+    CamView cv = vpt::make_cam(cam);
+    cv.W = cam->width / prm->div;
+    cv.H = cam->height / prm->div;
+    const ReflectConst rc = reflect_const(prm);
+    const ReflShade shd = reflect_shade_of(shading);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, tile_grid(cv.W, cv.H), dim3(256), 0, (hipStream_t)stream, cv, cam->width,
+                           cam->height, gb->d_t, gb->d_face, rc, shd, reinterpret_cast<const uint4*>(d_records), d_hit,
+                           reinterpret_cast<uint2*>(d_color));
+    };
+    rc.rough ? go(reflect_shade_kernel<true>) : go(reflect_shade_kernel<false>);
+    HIP_TRY(hipGetLastError());
+    return VPT_OK;
 }
 
 extern "C" VPT_API int vpt_reflect_shading_from_gi(const vpt_gi_params* gi, const float viewproj[16],

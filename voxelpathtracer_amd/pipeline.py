@@ -163,35 +163,33 @@ class Graph:
 
 def viewproj(view: np.ndarray, proj: np.ndarray) -> np.ndarray:
     """proj * view for column-major float32 matrices, summed in k order in
-    float32 (u_PrevProjection * u_PrevView, TemporalFilter.glsl:64)."""
+    float32 from 0 (u_PrevProjection * u_PrevView, TemporalFilter.glsl:64)."""
     V = np.asarray(view, np.float32).reshape(4, 4)   # V[col][row]
     P = np.asarray(proj, np.float32).reshape(4, 4)
-    out = np.zeros((4, 4), np.float32)
-    for c in range(4):
-        for r in range(4):
-            acc = np.float32(0.0)
-            for k in range(4):
-                acc = np.float32(acc + np.float32(P[k, r] * V[c, k]))
-            out[c, r] = acc
-    return out.ravel()
+    prod = V[:, :, None] * P[None, :, :]             # prod[c, k, r] = P[k][r] * V[c][k], rounded to float32
+    out = np.float32(0.0) + prod[:, 0, :]
+    for k in range(1, 4):
+        out = out + prod[:, k, :]
+    return out.astype(np.float32).ravel()
 
 
 class Pipeline:
     """Per-frame pass chain for one view of one world, in the reference's
     order (MainPipeline::StartPipeline, Pipeline.cpp:1960-3700): primary,
-    sun shadow (with its transversal) and its denoise beside diffuse GI and
-    SVGF, then the reflection trace at the reference's settings (1/4
-    resolution, 2 rough samples) shaded from the denoised planes, and the
-    colour pass's composite into `color` (H, W, 3) float32."""
+    sun shadow (with its transversal), its denoise and the reflection rays
+    at the reference's settings (1/4 resolution, 2 rough samples) beside
+    diffuse GI and SVGF, then the reflection hits shaded from the denoised
+    planes and the colour pass's composite into `color` (H, W, 3)
+    float32."""
 
     def __init__(self, ctx: Context, world: World, width: int, height: int, sun_tick: float = 50.0,
                  max_iter: int = 350, shadow_iter: int = 350, use_graph: bool = False):
         """use_graph: replay each frame's three chains — the side chain
-        (shadow with transversal, shadow denoise) and the main chain (GI,
-        SVGF), launched on their own streams after the primary pass so that
-        they still overlap, and the tail (shaded reflections, composite)
-        after both — as HIP graphs; each is captured every frame and its
-        instantiated graph updated in place."""
+        (shadow with transversal, shadow denoise, reflection rays) and the
+        main chain (GI, SVGF), launched on their own streams after the
+        primary pass so that they still overlap, and the tail (reflection
+        shading, composite) after both — as HIP graphs; each is captured
+        every frame and its instantiated graph updated in place."""
         import torch
         self.ctx, self.world = ctx, world
         self.width, self.height = width, height
@@ -207,10 +205,14 @@ class Pipeline:
         self.reflect_dist = torch.empty((hr, wr), dtype=torch.int16, device="cuda")
         self.reflect_hit = torch.empty((hr, wr), dtype=torch.int32, device="cuda")
         self.reflect_color = torch.empty((hr, wr, 4), dtype=torch.int16, device="cuda")
+        from .world import RECORD_BYTES, reflect_params
+        self.reflect_spp = reflect_params((0.0, 1.0, 0.0))["spp"]
+        self.reflect_records = torch.empty((self.reflect_spp * hr * wr * RECORD_BYTES // 4,), dtype=torch.int32,
+                                           device="cuda")
         self.color = torch.empty((height, width, 3), dtype=torch.float32, device="cuda")
         self.svgf = SVGF(ctx, width, height)
         self.shadow_denoiser = ShadowDenoiser(ctx, width, height)
-        self.side = torch.cuda.Stream()   # shadow, shadow denoise (default priority)
+        self.side = torch.cuda.Stream()   # shadow, shadow denoise, reflection rays (default priority)
         self.sun_tick = sun_tick
         self.max_iter, self.shadow_iter = max_iter, shadow_iter
         self.frame_index = 0
@@ -221,10 +223,10 @@ class Pipeline:
         self.join = torch.cuda.Event()
 
     def render(self, pos, yaw: float, pitch: float, stream=None):
-        """One frame: primary; then the sun shadow with its transversal and
-        the shadow denoise on a side stream, beside GI and SVGF on `stream`;
-        then the shaded reflections and the composite.  Returns the
-        camera."""
+        """One frame: primary; then the sun shadow with its transversal, the
+        shadow denoise and the reflection rays on a side stream, beside GI
+        and SVGF on `stream`; then the reflection shading and the composite.
+        Returns the camera."""
         import torch
         main = torch.cuda.current_stream() if stream is None else torch.cuda.ExternalStream(int(stream))
         cam, view, proj = make_camera(pos, yaw, pitch, self.width, self.height)
@@ -240,47 +242,51 @@ class Pipeline:
         cur_vp = viewproj(view, proj)
         self.world.trace_primary(cam, cur, self.max_iter, stream=main.cuda_stream)
 
+        from .world import composite, composite_params, reflect_params, reflect_shading, shade_reflect
+        rp = reflect_params(sun, self.frame_index)
+        rp["div"] = self.reflect_div
+
         # the side passes read only the G-buffer and the shadow pass
         def side_chain(s):
             self.world.trace_shadow2(cam, cur, sun, self.shadow, self.transversal, self.shadow_iter, stream=s)
             self.shadow_denoiser.frame(cam, prev_vp, cur, prev, self.shadow, self.transversal, self.shadow_denoised,
                                        stream=s)
+            self.world.trace_reflect_deferred(cam, cur, sun, cur_vp, self.reflect_dist, self.reflect_records,
+                                              self.reflect_hit, **rp, stream=s)
 
         def main_chain(s):
             self.world.trace_gi(cam, cur, prm, self.rgb, self.texels, stream=s)
             self.svgf.frame(cam, prev_vp, cur, prev, self.texels, self.denoised, time=self.frame_index / 60.0,
                             stream=s)
 
-        # reflections shaded from both chains' outputs, then the composite
+        # reflection hits shaded from both chains' outputs, then the composite
         def tail(s):
-            from .world import composite, composite_params, reflect_params, reflect_shading
             diffuse = self.svgf.final()
-            rp = reflect_params(sun, self.frame_index)
-            rp["div"] = self.reflect_div
             shading = reflect_shading(prm, cur_vp, diffuse, self.shadow_denoised)
-            self.world.trace_reflect_shaded(cam, cur, sun, shading, self.reflect_dist, self.reflect_color,
-                                            self.reflect_hit, **rp, stream=s)
+            shade_reflect(self.ctx, cam, cur, sun, shading, self.reflect_records, self.reflect_color,
+                          self.reflect_hit, **rp, stream=s)
             composite(self.ctx, cam, cur, composite_params(prm), diffuse, self.shadow_denoised, self.reflect_color,
                       self.reflect_div, self.color, stream=s)
 
+        # the main chain (the longer one) is enqueued first; the side chain
+        # starts from the fork event after the primary pass
         fork, join = self.fork, self.join
+        fork.record(main)
+        self.side.wait_event(fork)
         if self.graphs is None:
-            fork.record(main)
-            self.side.wait_event(fork)
+            main_chain(main.cuda_stream)
             side_chain(self.side.cuda_stream)
             join.record(self.side)
-            main_chain(main.cuda_stream)
             main.wait_event(join)
             tail(main.cuda_stream)
         else:
-            # the first two replays start after the primary pass on `main`;
-            # the side chain's completion is joined on the side stream; the
-            # tail's replay waits for both
-            with self.graphs[0].capture(main.cuda_stream, join_stream=self.side.cuda_stream) as s:
-                side_chain(s)
-            join.record(self.side)
+            # each replay's completion is joined on the stream it started
+            # from; the tail's replay waits for both chains
             with self.graphs[1].capture(main.cuda_stream) as s:
                 main_chain(s)
+            with self.graphs[0].capture(self.side.cuda_stream) as s:
+                side_chain(s)
+            join.record(self.side)
             main.wait_event(join)
             with self.graphs[2].capture(main.cuda_stream) as s:
                 tail(s)

@@ -184,6 +184,19 @@ class World:
         call("vpt_trace_reflect_shaded", self.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm),
              C.byref(shading), _ptr(dist), _ptr(hit), _ptr(color), _stream(stream))
 
+    def trace_reflect_deferred(self, cam: Camera, gbuf: "DeviceGBuffer", light_dir, viewproj, dist, records, hit=None,
+                               div: int = 4, max_iter: int = 64, shadow_iter: int = 150, spp: int = 1, rough: int = 0,
+                               roughness: float = 0.425, frame: int = 0, stream=None) -> None:
+        """The traversal half of trace_reflect_shaded (vpt_trace_reflect_deferred):
+        `records` receives spp records of RECORD_BYTES per target pixel
+        (an int32 tensor of spp * (H // div) * (W // div) * 4 elements);
+        shade_reflect() completes the pass."""
+        prm = _lib.ReflectParams((C.c_float * 3)(*map(float, light_dir)), div, max_iter, shadow_iter, spp, rough,
+                                 float(roughness), int(frame))
+        vp = (C.c_float * 16)(*map(float, np.asarray(viewproj, np.float32).ravel()))
+        call("vpt_trace_reflect_deferred", self.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), vp,
+             _ptr(dist), _ptr(hit), _ptr(records), _stream(stream))
+
     def trace_frame(self, cam: Camera, gbuf: "DeviceGBuffer", prm, mask, rgb, texels=None, primary_iter: int = 350,
                     shadow_iter: int = 350, transversal=None, stream=None) -> None:
         """Primary, shadow and GI passes of one frame; the shadow pass runs
@@ -352,6 +365,21 @@ def composite(ctx: Context, cam: Camera, gbuf: "DeviceGBuffer", prm: _lib.Compos
     (H // reflect_div, W // reflect_div)."""
     call("vpt_composite", ctx.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), C.c_void_p(_addr(diffuse)),
          _ptr(shadow), _ptr(reflect), int(reflect_div), _ptr(rgb), _stream(stream))
+
+
+RECORD_BYTES = 16   # VPT_REFLECT_RECORD_BYTES
+
+
+def shade_reflect(ctx: Context, cam: Camera, gbuf: "DeviceGBuffer", light_dir, shading, records, color, hit=None,
+                  div: int = 4, max_iter: int = 64, shadow_iter: int = 150, spp: int = 1, rough: int = 0,
+                  roughness: float = 0.425, frame: int = 0, stream=None) -> None:
+    """The shading half of the deferred reflection pass (vpt_shade_reflect):
+    the same parameters as the World.trace_reflect_deferred call whose
+    records it shades."""
+    prm = _lib.ReflectParams((C.c_float * 3)(*map(float, light_dir)), div, max_iter, shadow_iter, spp, rough,
+                             float(roughness), int(frame))
+    call("vpt_shade_reflect", ctx.handle, C.byref(cam), C.byref(gbuf.struct()), C.byref(prm), C.byref(shading),
+         _ptr(records), _ptr(hit), _ptr(color), _stream(stream))
 
 
 def reflect_params(light_dir, frame: int = 0) -> dict:
