@@ -310,9 +310,11 @@ __global__ void __launch_bounds__(256) persistent_kernel(WorldView w0, const Wor
         // world from the kernel arguments)
         const WorldView& w = (MULTI || SRC == 2) ? wv : w0;
         uint32_t off = 0;
-        const bool ld = s.active && vpt::trav_prepare(w, s.r, off);
+        vpt::Cell cell{};
+        const bool ld = s.active && vpt::trav_prepare(w, s.r, off, cell);
         const int m = ld ? (int)w.df[off] : 0;
-        if (s.active && !(ld && vpt::trav_advance(s.r, m))) finish_slot<SRC>(s, w, n, out, fc, n_samples, n_rays);
+        if (s.active && !(ld && vpt::trav_advance(s.r, m, cell)))
+            finish_slot<SRC>(s, w, n, out, fc, n_samples, n_rays);
     }
     wave_count(counter, n_samples, n_rays);
 }

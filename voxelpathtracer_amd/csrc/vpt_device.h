@@ -133,10 +133,20 @@ __device__ inline int face_from_axis(int axis, int s)
 // The DF step E of the reference for a DF byte: floor(M * 0.57735026918f),
 // except M == 1 -> 1.  The unorm8 round trip (M/255)*255 is exact for every
 // byte, so M is the byte itself (the host tests check this against
-// vpt_df_step for all 256 values).
+// vpt_df_step for all 256 values).  E is 0 only for M = 0 and 1 only for
+// M = 1, 2, 3 (3 * 0.57735026918f = 1.73, 4 * 0.57735026918f = 2.31), which
+// is how trav_advance branches on the byte; the jump length E - 1 is then
+// df_jump.
 __device__ inline int df_step(int m)
 {
     return m == 1 ? 1 : (int)((float)m * 0.57735026918f);
+}
+
+// (float)(E - 1) for a byte M >= 4: truncf is floor for the positive
+// product, and E - 1 < 2^24 subtracts exactly.
+__device__ inline float df_jump(int m)
+{
+    return truncf((float)m * 0.57735026918f) - 1.0f;
 }
 
 struct Hit {
@@ -163,25 +173,36 @@ __device__ inline uint32_t pack_cell(int x, int y, int z)
 //                 false = the loop has ended,
 //   trav_finish   the code after the loop.
 // The per-ray state is the reference's: origin, direction, the reciprocal
-// direction, initial origin, MinIdx, Intersection.
+// direction, initial origin, MinIdx, Intersection.  `samples` counts the
+// loop's turns (one DF sample each); the loop ends after max_iter of them.
 struct Trav {
     float ox, oy, oz;
     float dx, dy, dz;
     float ix, iy, iz;
     float o0x, o0y, o0z;
-    int left;       // iterations still allowed
+    int max_iter;
     int min_idx;
     int inter;
     int samples;
+};
+
+// The cell of the position, found by trav_prepare and used by trav_advance
+// in the same loop turn.
+struct Cell {
+    uint32_t x, y, z;
 };
 
 __device__ inline int sgn(float v) { return v > 0.0f ? 1 : (v < 0.0f ? -1 : 0); }
 
 __device__ __attribute__((always_inline)) inline void trav_init(Trav& r, Vec3 o, Vec3 d, int max_iter)
 {
-    r.ox = o.x;
-    r.oy = o.y;
-    r.oz = o.z;
+    // o + 0 turns a -0 component into +0 and leaves every other value as it
+    // is.  The two behave alike in all the loop's arithmetic (both floor to a
+    // zero inside the volume), and no later step can produce -0 (a sum is -0
+    // only when both terms are), so trav_prepare's bounds test never sees it.
+    r.ox = o.x + 0.0f;
+    r.oy = o.y + 0.0f;
+    r.oz = o.z + 0.0f;
     r.dx = d.x;
     r.dy = d.y;
     r.dz = d.z;
@@ -191,64 +212,74 @@ __device__ __attribute__((always_inline)) inline void trav_init(Trav& r, Vec3 o,
     r.o0x = o.x;
     r.o0y = o.y;
     r.o0z = o.z;
-    r.left = max_iter;
+    r.max_iter = max_iter;
     r.min_idx = 0;
     r.inter = 0;
     r.samples = 0;
 }
 
-__device__ __attribute__((always_inline)) inline bool trav_prepare(const WorldView& w, Trav& r, uint32_t& off)
+// Bit pattern of (float)dim, dim = max + 1 (0 for the empty view, max = -1).
+__device__ inline uint32_t trav_bound(float max) { return __float_as_uint(max + 1.0f); }
+
+__device__ __attribute__((always_inline)) inline bool trav_prepare(const WorldView& w, Trav& r, uint32_t& off,
+                                                                   Cell& c)
 {
-    if (r.left <= 0) return false;
-    float fx = floorf(r.ox), fy = floorf(r.oy), fz = floorf(r.oz);
-    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax)) {
+    if (r.samples >= r.max_iter) return false;
+    // IsInVolume(floor(o)), one unsigned compare per axis: floor(o) lies in
+    // [0, dim - 1] exactly when o lies in [+0, dim); non-negative floats
+    // order as their bit patterns, and negative values and NaN have patterns
+    // above every bound.
+    if (!(__float_as_uint(r.ox) < trav_bound(w.xmax) && __float_as_uint(r.oy) < trav_bound(w.ymax) &&
+          __float_as_uint(r.oz) < trav_bound(w.zmax))) {
         r.inter = 0;
         return false;
     }
-    off = brick_offset_fast((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, w.row_stride, w.slab_stride);
+    // truncation is floor for o >= 0
+    c.x = (uint32_t)r.ox;
+    c.y = (uint32_t)r.oy;
+    c.z = (uint32_t)r.oz;
+    off = brick_offset_fast(c.x, c.y, c.z, w.row_stride, w.slab_stride);
     return true;
 }
 
-__device__ __attribute__((always_inline)) inline bool trav_advance(Trav& r, int m)
+__device__ __attribute__((always_inline)) inline bool trav_advance(Trav& r, int m, const Cell& c)
 {
     r.samples++;
-    int e = df_step(m);
-    if (e == 0) return false;
-    if (e == 1) {
-        // the one-cell step with selects on the step axis (no divergent
-        // branches: fewer exec-mask moves in the loop)
+    if (m == 0) return false;   // E == 0
+    if (m < 4) {
+        // E == 1: the one-cell step, with selects on the step axis (no
+        // divergent branches: fewer exec-mask moves in the loop)
         const int sx = sgn(r.dx), sy = sgn(r.dy), sz = sgn(r.dz);
         const int hx = (1 + sx) >> 1, hy = (1 + sy) >> 1, hz = (1 + sz) >> 1;
-        const int gx = (int)r.ox, gy = (int)r.oy, gz = (int)r.oz;
-        float wx = r.ox - (float)gx, wy = r.oy - (float)gy, wz = r.oz - (float)gz;
+        // float(int(o)) of the reference is the cell's corner
+        const float gx = (float)c.x, gy = (float)c.y, gz = (float)c.z;
+        const float wx = r.ox - gx, wy = r.oy - gy, wz = r.oz - gz;
         const float fxx = ((float)hx - wx) * r.ix;
         const float fyy = ((float)hy - wy) * r.iy;
         const float fzz = ((float)hz - wz) * r.iz;
         const int mi = (fxx < fyy && sx != 0) ? ((fxx < fzz || sz == 0) ? 0 : 2) : ((fyy < fzz || sz == 0) ? 1 : 2);
         const bool m0 = mi == 0, m1 = mi == 1, m2 = mi == 2;
         const float fm = m0 ? fxx : (m1 ? fyy : fzz);
-        wx = m0 ? (float)(1 - hx) : wx + r.dx * fm;
-        wy = m1 ? (float)(1 - hy) : wy + r.dy * fm;
-        wz = m2 ? (float)(1 - hz) : wz + r.dz * fm;
-        const float nx = (float)(gx + (m0 ? sx : 0)) + wx;
-        const float ny = (float)(gy + (m1 ? sy : 0)) + wy;
-        const float nz = (float)(gz + (m2 ? sz : 0)) + wz;
-        const float ax = m0 ? nx + (float)sx * 0.0001f : nx;
-        const float ay = m1 ? ny + (float)sy * 0.0001f : ny;
-        const float az = m2 ? nz + (float)sz * 0.0001f : nz;
+        // The reference forms float(int(o) + s) + w, then adds s * 0.0001 on
+        // the step axis.  On that axis w is 1 - h, and the integers add
+        // exactly: (g + s) + (1 - h) = g + (s + 1 - h), also for the s = 0
+        // that the y axis can be picked with.  On the others s is 0 and w
+        // moves by d * fm.
+        const float ax = m0 ? (gx + (float)(sx + 1 - hx)) + (float)sx * 0.0001f : gx + (wx + r.dx * fm);
+        const float ay = m1 ? (gy + (float)(sy + 1 - hy)) + (float)sy * 0.0001f : gy + (wy + r.dy * fm);
+        const float az = m2 ? (gz + (float)(sz + 1 - hz)) + (float)sz * 0.0001f : gz + (wz + r.dz * fm);
         r.ox = ax;
         r.oy = ay;
         r.oz = az;
         r.min_idx = mi;
         r.inter = 1;
     } else {
-        float k = (float)(e - 1);
+        const float k = df_jump(m);
         r.ox = r.ox + k * r.dx;
         r.oy = r.oy + k * r.dy;
         r.oz = r.oz + k * r.dz;
     }
-    r.left--;
-    return r.left > 0;
+    return r.samples < r.max_iter;
 }
 
 __device__ __attribute__((always_inline)) inline Hit trav_finish(const WorldView& w, const Trav& r)
@@ -287,8 +318,9 @@ __device__ inline Hit traverse(const WorldView& w, Vec3 o, Vec3 d, int max_iter)
     Trav r;
     trav_init(r, o, d, max_iter);
     uint32_t off;
-    while (trav_prepare(w, r, off)) {
-        if (!trav_advance(r, w.df[off])) break;
+    Cell c;
+    while (trav_prepare(w, r, off, c)) {
+        if (!trav_advance(r, w.df[off], c)) break;
     }
     return trav_finish(w, r);
 }
