@@ -194,6 +194,15 @@ struct Cell {
 
 __device__ inline int sgn(float v) { return v > 0.0f ? 1 : (v < 0.0f ? -1 : 0); }
 
+// c ? x : y with both values computed in every lane first (the empty asm
+// only pins them to registers and emits nothing), so that the choice stays
+// one v_cndmask_b32 instead of two exec-masked arms.
+__device__ __attribute__((always_inline)) inline float pick(bool c, float x, float y)
+{
+    asm volatile("" : "+v"(x), "+v"(y));
+    return c ? x : y;
+}
+
 __device__ __attribute__((always_inline)) inline void trav_init(Trav& r, Vec3 o, Vec3 d, int max_iter)
 {
     // o + 0 turns a -0 component into +0 and leaves every other value as it
@@ -221,10 +230,11 @@ __device__ __attribute__((always_inline)) inline void trav_init(Trav& r, Vec3 o,
 // Bit pattern of (float)dim, dim = max + 1 (0 for the empty view, max = -1).
 __device__ inline uint32_t trav_bound(float max) { return __float_as_uint(max + 1.0f); }
 
-__device__ __attribute__((always_inline)) inline bool trav_prepare(const WorldView& w, Trav& r, uint32_t& off,
-                                                                   Cell& c)
+// trav_prepare without the iteration cap, for loops that test the cap
+// where trav_advance returns.
+__device__ __attribute__((always_inline)) inline bool trav_locate(const WorldView& w, Trav& r, uint32_t& off,
+                                                                  Cell& c)
 {
-    if (r.samples >= r.max_iter) return false;
     // IsInVolume(floor(o)), one unsigned compare per axis: floor(o) lies in
     // [0, dim - 1] exactly when o lies in [+0, dim); non-negative floats
     // order as their bit patterns, and negative values and NaN have patterns
@@ -240,6 +250,12 @@ __device__ __attribute__((always_inline)) inline bool trav_prepare(const WorldVi
     c.z = (uint32_t)r.oz;
     off = brick_offset_fast(c.x, c.y, c.z, w.row_stride, w.slab_stride);
     return true;
+}
+
+__device__ __attribute__((always_inline)) inline bool trav_prepare(const WorldView& w, Trav& r, uint32_t& off,
+                                                                   Cell& c)
+{
+    return r.samples < r.max_iter && trav_locate(w, r, off, c);
 }
 
 __device__ __attribute__((always_inline)) inline bool trav_advance(Trav& r, int m, const Cell& c)
@@ -265,9 +281,9 @@ __device__ __attribute__((always_inline)) inline bool trav_advance(Trav& r, int 
         // exactly: (g + s) + (1 - h) = g + (s + 1 - h), also for the s = 0
         // that the y axis can be picked with.  On the others s is 0 and w
         // moves by d * fm.
-        const float ax = m0 ? (gx + (float)(sx + 1 - hx)) + (float)sx * 0.0001f : gx + (wx + r.dx * fm);
-        const float ay = m1 ? (gy + (float)(sy + 1 - hy)) + (float)sy * 0.0001f : gy + (wy + r.dy * fm);
-        const float az = m2 ? (gz + (float)(sz + 1 - hz)) + (float)sz * 0.0001f : gz + (wz + r.dz * fm);
+        const float ax = pick(m0, (gx + (float)(sx + 1 - hx)) + (float)sx * 0.0001f, gx + (wx + r.dx * fm));
+        const float ay = pick(m1, (gy + (float)(sy + 1 - hy)) + (float)sy * 0.0001f, gy + (wy + r.dy * fm));
+        const float az = pick(m2, (gz + (float)(sz + 1 - hz)) + (float)sz * 0.0001f, gz + (wz + r.dz * fm));
         r.ox = ax;
         r.oy = ay;
         r.oz = az;
@@ -319,8 +335,11 @@ __device__ inline Hit traverse(const WorldView& w, Vec3 o, Vec3 d, int max_iter)
     trav_init(r, o, d, max_iter);
     uint32_t off;
     Cell c;
-    while (trav_prepare(w, r, off, c)) {
-        if (!trav_advance(r, w.df[off], c)) break;
+    // the cap is tested once here and then where trav_advance returns
+    if (max_iter > 0) {
+        while (trav_locate(w, r, off, c)) {
+            if (!trav_advance(r, w.df[off], c)) break;
+        }
     }
     return trav_finish(w, r);
 }
