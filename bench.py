@@ -33,7 +33,7 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 # the kernels each frame pass launches (the roofline covers the whole pass)
 PASS_KERNELS = {"primary": "primary_kernel", "shadow": "shadow_kernel",
-                "gi": "gi pass: gi_bounce_kernel + gi_shadow_kernel"}
+                "gi": "gi pass: gi_bounce_kernel + gi_cont_kernel + gi_shadow_kernel"}
 NX, NY, NZ = 384, 128, 384
 CAM_POS = (192.5, 100.5, 192.5)
 CAM_YAW, CAM_PITCH = 45.0, -20.0

@@ -343,14 +343,14 @@ VPT_API int vpt_shade_reflect(vpt_ctx* ctx, const vpt_camera* cam, const vpt_gbu
 
 /* DIFFUSE_TRACE (DiffuseRayTraceFrag.glsl:822-935), 1 spp, 1 bounce with a
  * shadow ray at the bounce hit.  d_rgb: 3 floats per pixel (clamped to
- * [0, 8]).  d_svgf_in (optional): SVGF input texels.  Two kernels: the
- * bounce rays (regrouped by direction within each 16x16 block), which shade
- * their pixels at once unless the hit faces the sun; those pixels become
- * jobs on a compacted list for the second kernel, which traces their
- * shadow rays and shades them.  The jobs live in scratch the context keeps
- * for `stream` (16 B per pixel, grown on demand; outgrown buffers are kept
- * until the context is destroyed, since a captured graph may still use
- * them).  The albedo table in prm is passed with the launch and also
+ * [0, 8]).  d_svgf_in (optional): SVGF input texels.  Three kernels: the
+ * bounce rays (regrouped by direction within each 16x16 block) for at most
+ * 24 steps, shading their pixels at once unless the hit faces the sun; the
+ * rays still running, resumed densely from compacted queues; the shadow
+ * rays of the pixels whose hit faces the sun, also from compacted queues.
+ * The queues live in scratch the context keeps for `stream` (32 B per
+ * pixel + 4 KiB, grown on demand; outgrown buffers are kept until the
+ * context is destroyed, since a captured graph may still use them).  The albedo table in prm is passed with the launch and also
  * becomes the context's "last table" for vpt_unet_assemble. */
 VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
                          float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream);
@@ -358,8 +358,8 @@ VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer*
 /* No counterpart in the reference: vpt_trace_gi for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  Same arguments, results and
- * kernels; the shadow jobs live in a fixed array of
- * 32 MiB + 256 B that each GPU the library runs on holds from context
+ * kernels; the queues live in a fixed array of
+ * 64 MiB + 4 KiB that each GPU the library runs on holds from context
  * creation until the process ends, so images are limited to 2,097,152
  * pixels (1920 x 1080 fits; larger is VPT_EINVAL).  It calls no hipMalloc
  * or hipFree.  Every call on one GPU shares that array: calls that overlap

@@ -68,9 +68,11 @@ int gi_noalloc_init(int device);
 
 namespace vpt {
 
-// Scratch of the GI pass for one stream: the shadow-ray jobs (4 words
-// each, at most one per pixel) and the job counter.  Calls on one stream
-// run in order, so they share it; calls on other streams get their own.
+// Scratch of the GI pass for one stream, one allocation: the queue
+// counters (first 256 bytes), then the bounce continuations and the
+// shadow-ray jobs (16 bytes each, at most one of each per pixel).  Calls on
+// one stream run in order, so they share it; calls on other streams get
+// their own.
 // `done` is recorded after a call's last kernel and waited for (on the
 // device) before the next call's first, so a stream handle reused after its
 // stream was destroyed never writes the scratch while earlier work still
@@ -80,8 +82,7 @@ namespace vpt {
 struct GiScratch {
     void* stream = nullptr;
     void* done = nullptr;    // hipEvent_t
-    uint32_t* d_jobs = nullptr;
-    uint32_t* d_njobs = nullptr;
+    uint8_t* d_buf = nullptr;
     size_t cap = 0;          // pixels
     uint64_t last_use = 0;
 };

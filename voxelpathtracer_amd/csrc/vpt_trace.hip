@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "vpt_device.h"
 #include "vpt_gi.h"
 #include "vpt_internal.h"
@@ -26,6 +28,14 @@ using vpt::WorldView;
     } while (0)
 
 namespace vpt {
+
+// Scratch of one GI pass: the 16 queue counters (continuations, then
+// shadow jobs, one per XCD), each on its own 256-byte line, then the
+// 16-byte continuation records and the 16-byte shadow jobs, one of each per
+// pixel.
+constexpr size_t kGiCounterStride = 64;   // words
+constexpr size_t kGiCountersBytes = 16 * kGiCounterStride * sizeof(uint32_t);
+inline size_t gi_scratch_bytes(size_t npx) { return kGiCountersBytes + 2 * 16 * npx; }
 
 WorldView make_view(const vpt_world* w)
 {
@@ -83,9 +93,8 @@ AlbedoTable take_albedo(vpt_ctx* ctx, const vpt_gi_params* prm)
 
 static void gi_retire(vpt_ctx* ctx, GiScratch* sc)
 {
-    for (void* p : {(void*)sc->d_jobs, (void*)sc->d_njobs})
-        if (p) ctx->gi_retired.push_back(p);
-    sc->d_jobs = sc->d_njobs = nullptr;
+    if (sc->d_buf) ctx->gi_retired.push_back(sc->d_buf);
+    sc->d_buf = nullptr;
     sc->cap = 0;
 }
 
@@ -119,8 +128,7 @@ static int gi_scratch_for(vpt_ctx* ctx, hipStream_t stream, size_t npx, bool cap
     }
     if (sc->cap < npx) {
         gi_retire(ctx, sc);
-        hipError_t e = hipMalloc(&sc->d_jobs, npx * 4 * sizeof(uint32_t));
-        if (e == hipSuccess) e = hipMalloc(&sc->d_njobs, sizeof(uint32_t));
+        hipError_t e = hipMalloc(&sc->d_buf, gi_scratch_bytes(npx));
         if (e != hipSuccess) {
             gi_retire(ctx, sc);
             return set_error(capturing ? VPT_ESTATE : VPT_ENOMEM,
@@ -311,24 +319,39 @@ __global__ void __launch_bounds__(256) shadow2_kernel(WorldView w, CamView cam, 
 
 // DIFFUSE_TRACE main + CalculateDiffuse (DiffuseRayTraceFrag.glsl:535-664,
 // 822-935), 1 spp, 1 bounce; Lambert sun term, PCG randoms, per-id albedo,
-// constant sky.  Two kernels, so that the sun-shadow rays at the bounce
-// hits (up to 128 steps) do not hold idle the lanes whose bounce missed or
-// faced away:
+// constant sky.  The pass is split so that a wavefront's lanes stay busy:
+// the bounce rays' lengths vary widely (a sixth hit the terrain in two
+// steps, a twentieth run to the 48-step cap), and the sun-shadow rays at
+// the bounce hits (up to 128 steps) are cast by only a quarter of the
+// pixels.
 //   gi_bounce_kernel   one lane per bounce ray: the block's 16x16 pixels
 //                      set up their rays, which are regrouped onto lanes by
 //                      the ray's y direction (16 bins, through LDS), so that
 //                      a wavefront traces rays of similar length (rays
 //                      leaving upward escape in a few large steps, grazing
-//                      ones walk the terrain).  A pixel whose hit faces the
-//                      sun becomes a job on a compacted list (one atomic per
-//                      wavefront); the tracing lane shades every other pixel
-//                      (and each lane its own sky pixel) at once;
-//   gi_shadow_kernel   one lane per job: the pixel's ray again (from the
-//                      G-buffer), the shadow ray, then the pixel's shading.
-// Every pixel goes through the same functions as a single-kernel pass, so
-// the outputs are identical; only the lane that traces and shades a pixel
-// changes.
+//                      ones walk the terrain).  Each ray runs at most
+//                      kGiPhase1 steps here; a ray still running then is
+//                      saved as a continuation.  A finished ray's pixel is
+//                      shaded at once, or, when its hit faces the sun,
+//                      becomes a shadow job (sky pixels are shaded by their
+//                      own lanes);
+//   gi_cont_kernel     one lane per continuation: the pixel's ray again
+//                      (from the G-buffer), resumed from the saved state to
+//                      its end, then shaded or made a job;
+//   gi_shadow_kernel   one lane per job: the pixel's ray again, the shadow
+//                      ray, then the pixel's shading.
+// Continuations and jobs are appended with one atomic per wavefront to
+// queues kept per XCD (GiQueues), whose counters sit on separate cache
+// lines: atomics from every wavefront of the chip on one line serialise,
+// and cost the bounce kernel more than its tracing.  The next kernel takes
+// the queues' records in order as one dense list, so every XCD gets an
+// equal share however the hits fall on the screen.
+// Every pixel goes through the same functions as a single-kernel pass, and
+// a resumed traversal continues from its exact state, so the outputs are
+// identical; only the lane that traces and shades a pixel changes.
 constexpr int kGiBins = 16;
+constexpr int kGiPhase1 = 24;   // bounce steps before a ray is continued
+using vpt::kGiCounterStride;
 
 // A pixel waiting for its shadow ray: index, bounce hit distance, face |
 // block << 8.
@@ -339,13 +362,150 @@ struct GiJob {
     uint32_t pad;
 };
 
-__global__ void gi_jobs_clear_kernel(uint32_t* __restrict__ n) { *n = 0u; }
+// A bounce ray after kGiPhase1 steps: its position, and the pixel index |
+// MinIdx << 28 | Intersection << 30 (pixel indices are below 2^28).  The
+// rest of the traversal state is the ray itself, recomputed from the
+// pixel, and the sample count, kGiPhase1.
+struct GiCont {
+    float ox, oy, oz;
+    uint32_t meta;
+};
+
+// The per-XCD queues of a W x H pass.  The hardware deals workgroups to the
+// 8 XCDs round-robin, so bounce block b runs on XCD b % 8; queue k holds
+// what those blocks produce, in slots [base[k], base[k] + cap[k]) of the
+// continuation and of the job array, and cap[k] is the number of pixels of
+// their tiles, so a queue cannot overflow.
+struct GiQueues {
+    uint32_t base[8];
+    uint32_t cap[8];
+};
+
+// a[q] for the block-uniform q, by selects: indexing the by-value kernel
+// argument directly would copy it to scratch memory.
+__device__ inline uint32_t pick8(const uint32_t (&a)[8], uint32_t q)
+{
+    uint32_t v = a[0];
+#pragma unroll
+    for (uint32_t k = 1; k < 8; k++) v = q == k ? a[k] : v;
+    return v;
+}
+
+// Slot of the k-th record of the queues taken in order (queue q holds
+// min(counts[q], cap[q]) records from base[q]), and its queue; false past
+// the last one.  The counts are the same for every lane (scalar loads).
+__device__ inline bool queue_slot(uint32_t k, const uint32_t* __restrict__ counts, const GiQueues& qs, uint32_t& slot,
+                                  uint32_t& queue)
+{
+    uint32_t start = 0;
+    bool found = false;
+#pragma unroll
+    for (uint32_t q = 0; q < 8; q++) {
+        const uint32_t n = min(counts[q * kGiCounterStride], qs.cap[q]);
+        const bool in = k >= start && k - start < n;
+        slot = in ? qs.base[q] + (k - start) : slot;
+        queue = in ? q : queue;
+        found = found || in;
+        start += n;
+    }
+    return found;
+}
+
+// Pixels of the 16x16 tiles [a, b) of a W x H image (tiles row-major).
+size_t tile_range_pixels(uint32_t a, uint32_t b, int W, int H)
+{
+    const uint32_t tx = (uint32_t)(W + 15) / 16u;
+    auto width_upto = [&](uint32_t x1) -> size_t {   // columns of tiles 0 .. x1 - 1 of a row
+        return x1 == 0 ? 0 : (size_t)16 * (x1 - 1) + (size_t)std::min(16, W - 16 * (int)(x1 - 1));
+    };
+    size_t n = 0;
+    for (uint32_t t = a; t < b;) {
+        const uint32_t y = t / tx, x0 = t % tx, x1 = std::min(tx, x0 + (b - t));
+        n += (width_upto(x1) - width_upto(x0)) * (size_t)std::min(16, H - 16 * (int)y);
+        t += x1 - x0;
+    }
+    return n;
+}
+
+// The queues for gi_bounce_kernel's grid (tile_grid, tiles dealt by
+// xcd_remap: XCD k takes tiles [k * per, (k + 1) * per), per = tiles / 8,
+// and the tiles from 8 * per on go one each to XCDs 0, 1, ...).  Returns
+// the total capacity, W * H.
+uint32_t gi_queues(int W, int H, GiQueues& q)
+{
+    const uint32_t tiles = ((uint32_t)(W + 15) / 16u) * ((uint32_t)(H + 15) / 16u);
+    const uint32_t per = tiles / 8u;
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < 8; k++) {
+        size_t cap = tile_range_pixels(k * per, (k + 1) * per, W, H);
+        if (8u * per + k < tiles) cap += tile_range_pixels(8u * per + k, 8u * per + k + 1, W, H);
+        q.base[k] = base;
+        q.cap[k] = (uint32_t)cap;
+        base += (uint32_t)cap;
+    }
+    return base;
+}
+
+__global__ void gi_queues_clear_kernel(uint32_t* __restrict__ counts)
+{
+    if (threadIdx.x < 16) counts[threadIdx.x * kGiCounterStride] = 0u;
+}
+
+// A queue record (GiJob, GiCont) as one 16-byte register quad.
+typedef uint32_t Rec16 __attribute__((ext_vector_type(4)));
+
+// Appends the record `rec` of every lane with `take` to the queue (count,
+// slots from base, at most cap), one atomic per wavefront.  Every lane of
+// the wavefront must reach this call.
+__device__ inline void queue_append(bool take, Rec16 rec, Rec16* __restrict__ arr, uint32_t* __restrict__ count,
+                                    uint32_t base, uint32_t cap)
+{
+    const uint64_t m = __ballot(take);
+    if (!m) return;
+    const int lane = (int)(threadIdx.x & 63), first = __ffsll((long long)m) - 1;
+    uint32_t b = 0;
+    if (lane == first) b = atomicAdd(count, (uint32_t)__popcll(m));
+    b = __shfl(b, first);
+    const uint32_t idx = b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (take && idx < cap) arr[base + idx] = rec;
+}
+
+// queue_append with each lane's own queue (qi, 0..7): one atomic per
+// wavefront and queue present.  Every lane of the wavefront must reach it.
+__device__ inline void queue_append_to(bool take, uint32_t qi, Rec16 rec, Rec16* __restrict__ arr,
+                                       uint32_t* __restrict__ counts, const GiQueues& qs)
+{
+#pragma unroll
+    for (uint32_t q = 0; q < 8; q++) {
+        const bool t = take && qi == q;
+        if (__ballot(t) == 0ull) continue;
+        queue_append(t, rec, arr, counts + q * kGiCounterStride, qs.base[q], qs.cap[q]);
+    }
+}
+
+// The bounce's result for its pixel: a hit facing the sun becomes a shadow
+// job (returned with job = true), every other pixel is shaded now.
+__device__ inline Rec16 gi_bounce_result(const GiConst& g, const float* albedo, Vec3 o, Vec3 dir, const vpt::Hit& h,
+                                         uint32_t pix, float* __restrict__ rgb, vpt_svgf_texel* __restrict__ texels,
+                                         bool& job)
+{
+    const bool hit = h.t > 0.0f && h.block > 0;
+    const int face = h.face & 0xff, block = h.block & 0xff;
+    Vec3 so;
+    float ndotl;
+    job = hit && vpt::gi_shadow_ray(o, dir, h.t, h.face, g, so, ndotl);
+    if (!job)
+        vpt::gi_write_hit_pixel(g, albedo, dir, h.t, hit, face, block, 0.0f, rgb + 3 * (size_t)pix,
+                                texels ? texels + pix : nullptr);
+    return Rec16{pix, __float_as_uint(h.t), (uint32_t)face | ((uint32_t)block << 8), 0u};
+}
 
 __global__ void __launch_bounds__(256) gi_bounce_kernel(WorldView w, CamView cam, const float* __restrict__ gb_t,
                                                         const uint8_t* __restrict__ gb_face, GiConst g,
                                                         const vpt::AlbedoTable albedo, float* __restrict__ rgb,
-                                                        vpt_svgf_texel* __restrict__ texels,
-                                                        GiJob* __restrict__ jobs, uint32_t* __restrict__ njobs,
+                                                        vpt_svgf_texel* __restrict__ texels, GiQueues qs,
+                                                        GiCont* __restrict__ conts, GiJob* __restrict__ jobs,
+                                                        uint32_t* __restrict__ counts,
                                                         unsigned long long* __restrict__ counter)
 {
     __shared__ float s_ray[6][256];
@@ -354,8 +514,8 @@ __global__ void __launch_bounds__(256) gi_bounce_kernel(WorldView w, CamView cam
     int px, py;
     tile_pixel(cam.W, cam.H, px, py);
     int samples = 0, rays = 0;
-    bool job = false;
-    GiJob jb = {0u, 0.0f, 0u, 0u};
+    bool job = false, cont = false;
+    Rec16 jb = {0u, 0u, 0u, 0u}, cr = jb;   // a GiJob, a GiCont
     if (threadIdx.x <= kGiBins) s_bin[threadIdx.x] = 0;
     __syncthreads();
     // set up this pixel's ray and rank it within its y-direction bin; sky
@@ -395,52 +555,96 @@ __global__ void __launch_bounds__(256) gi_bounce_kernel(WorldView w, CamView cam
     s_ray[5][slot] = rd.z;
     s_pix[slot] = rpix;
     __syncthreads();
-    // trace the ray in this lane's slot
+    // trace the ray in this lane's slot, at most kGiPhase1 steps
     const uint32_t pix = s_pix[threadIdx.x];
     if (pix != 0xffffffffu) {
         const Vec3 o = vpt::mk3(s_ray[0][threadIdx.x], s_ray[1][threadIdx.x], s_ray[2][threadIdx.x]);
         const Vec3 dir = vpt::mk3(s_ray[3][threadIdx.x], s_ray[4][threadIdx.x], s_ray[5][threadIdx.x]);
-        const vpt::Hit h = vpt::traverse(w, o, dir, g.gi_iter);
-        samples = h.samples;
+        vpt::Trav r;
+        vpt::trav_init(r, o, dir, min(g.gi_iter, kGiPhase1));
+        bool capped = false;   // stopped by the phase's step cap, not by a hit or the volume's end
+        if (r.max_iter > 0) {
+            uint32_t off;
+            vpt::Cell c;
+            while (vpt::trav_locate(w, r, off, c)) {
+                const int m = w.df[off];
+                if (!vpt::trav_advance(r, m, c)) {
+                    capped = m != 0;
+                    break;
+                }
+            }
+        }
         rays = 1;
-        const bool hit = h.t > 0.0f && h.block > 0;
-        const int face = h.face & 0xff, block = h.block & 0xff;
-        Vec3 so;
-        float ndotl;
-        if (hit && vpt::gi_shadow_ray(o, dir, h.t, h.face, g, so, ndotl)) {
-            job = true;
-            jb.pix = pix;
-            jb.T = h.t;
-            jb.fb = (uint32_t)face | ((uint32_t)block << 8);
+        if (capped && g.gi_iter > kGiPhase1) {
+            cont = true;
+            samples = r.samples;
+            cr = Rec16{__float_as_uint(r.ox), __float_as_uint(r.oy), __float_as_uint(r.oz),
+                       pix | ((uint32_t)r.min_idx << 28) | ((uint32_t)r.inter << 30)};
         } else {
-            vpt::gi_write_hit_pixel(g, albedo.v, dir, h.t, hit, face, block, 0.0f, rgb + 3 * (size_t)pix,
-                                    texels ? texels + pix : nullptr);
+            const vpt::Hit h = vpt::trav_finish(w, r);
+            samples = h.samples;
+            jb = gi_bounce_result(g, albedo.v, o, dir, h, pix, rgb, texels, job);
         }
     }
-    // compacted append, one atomic per wavefront (every lane reaches it)
-    const uint64_t m = __ballot(job);
-    if (m) {
-        const int lane = (int)(threadIdx.x & 63), first = __ffsll((long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == first) base = atomicAdd(njobs, (uint32_t)__popcll(m));
-        base = __shfl(base, first);
-        if (job) jobs[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = jb;
-    }
+    const uint32_t q = blockIdx.x & 7u, qbase = pick8(qs.base, q), qcap = pick8(qs.cap, q);
+    queue_append(cont, cr, reinterpret_cast<Rec16*>(conts), counts + q * kGiCounterStride, qbase, qcap);
+    queue_append(job, jb, reinterpret_cast<Rec16*>(jobs), counts + (8 + q) * kGiCounterStride, qbase, qcap);
     count_samples(counter, samples, rays);
+}
+
+__global__ void __launch_bounds__(256) gi_cont_kernel(WorldView w, CamView cam, const float* __restrict__ gb_t,
+                                                      const uint8_t* __restrict__ gb_face, GiConst g,
+                                                      const vpt::AlbedoTable albedo, float* __restrict__ rgb,
+                                                      vpt_svgf_texel* __restrict__ texels, GiQueues qs,
+                                                      const GiCont* __restrict__ conts, GiJob* __restrict__ jobs,
+                                                      uint32_t* __restrict__ counts,
+                                                      unsigned long long* __restrict__ counter)
+{
+    uint32_t slot = 0, queue = 0;
+    int samples = 0;
+    bool job = false;
+    Rec16 jb = {0u, 0u, 0u, 0u};
+    if (queue_slot(blockIdx.x * 256u + threadIdx.x, counts, qs, slot, queue)) {
+        const Rec16 cr = reinterpret_cast<const Rec16*>(conts)[slot];   // a GiCont
+        const uint32_t meta = cr.w, pix = meta & 0x0fffffffu;
+        const int px = (int)(pix % (uint32_t)cam.W), py = (int)(pix / (uint32_t)cam.W);
+        Vec3 o, dir;
+        vpt::gi_ray(cam, px, py, gb_t[pix], vpt::face_normal(gb_face[pix], 0.5f), g.frame, o, dir);
+        vpt::Trav r;
+        vpt::trav_init(r, o, dir, g.gi_iter);
+        r.ox = __uint_as_float(cr.x);
+        r.oy = __uint_as_float(cr.y);
+        r.oz = __uint_as_float(cr.z);
+        r.samples = kGiPhase1;   // < gi_iter: only such rays are continued
+        r.min_idx = (int)((meta >> 28) & 3u);
+        r.inter = (int)((meta >> 30) & 1u);
+        uint32_t off;
+        vpt::Cell c;
+        while (vpt::trav_locate(w, r, off, c)) {
+            if (!vpt::trav_advance(r, w.df[off], c)) break;
+        }
+        const vpt::Hit h = vpt::trav_finish(w, r);
+        samples = h.samples - kGiPhase1;   // the first kGiPhase1 were counted by the bounce kernel
+        jb = gi_bounce_result(g, albedo.v, o, dir, h, pix, rgb, texels, job);
+    }
+    // a pixel's job goes to the queue its continuation came from, which
+    // holds a slot for every pixel of that queue's tiles
+    queue_append_to(job, queue, jb, reinterpret_cast<Rec16*>(jobs), counts + 8 * kGiCounterStride, qs);
+    count_samples(counter, samples, 0);
 }
 
 __global__ void __launch_bounds__(256) gi_shadow_kernel(WorldView w, CamView cam, const float* __restrict__ gb_t,
                                                         const uint8_t* __restrict__ gb_face, GiConst g,
-                                                        const vpt::AlbedoTable albedo, const GiJob* __restrict__ jobs,
-                                                        const uint32_t* __restrict__ njobs, float* __restrict__ rgb,
+                                                        const vpt::AlbedoTable albedo, GiQueues qs,
+                                                        const GiJob* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ counts, float* __restrict__ rgb,
                                                         vpt_svgf_texel* __restrict__ texels,
                                                         unsigned long long* __restrict__ counter)
 {
-    const uint32_t n = *njobs;   // at most one job per pixel: n <= the grid's lanes
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t slot = 0, queue = 0;
     int samples = 0, rays = 0;
-    if (k < n) {
-        const GiJob jb = jobs[k];
+    if (queue_slot(blockIdx.x * 256u + threadIdx.x, counts + 8 * kGiCounterStride, qs, slot, queue)) {
+        const GiJob jb = jobs[slot];
         const int px = (int)(jb.pix % (uint32_t)cam.W), py = (int)(jb.pix / (uint32_t)cam.W);
         const float dist = gb_t[jb.pix];
         Vec3 o, dir;
@@ -448,7 +652,7 @@ __global__ void __launch_bounds__(256) gi_shadow_kernel(WorldView w, CamView cam
         const int face = (int)(jb.fb & 0xffu), block = (int)((jb.fb >> 8) & 0xffu);
         Vec3 so;
         float ndotl;
-        vpt::gi_shadow_ray(o, dir, jb.T, face, g, so, ndotl);   // true: the bounce kernel made the job
+        vpt::gi_shadow_ray(o, dir, jb.T, face, g, so, ndotl);   // true: the job was made for it
         const vpt::Hit sh = vpt::traverse(w, so, vpt::mk3(g.sun[0], g.sun[1], g.sun[2]), g.shadow_iter);
         samples = sh.samples;
         rays = 1;
@@ -1249,20 +1453,31 @@ extern "C" VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, cons
 
 // The GI pass's kernels over the given scratch (the job counter is cleared
 // by a kernel, so the sequence can be captured in a graph).
+// The GI pass's kernels over the given scratch: 16 queue counters, then
+// continuation and job arrays of at least one record per pixel each.  The
+// counters are cleared by a kernel, so the sequence can be captured in a
+// graph.
 static int gi_enqueue(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
-                      const vpt::AlbedoTable& albedo, float* d_rgb, vpt_svgf_texel* d_svgf_in, GiJob* jobs,
-                      uint32_t* njobs, hipStream_t s)
+                      const vpt::AlbedoTable& albedo, float* d_rgb, vpt_svgf_texel* d_svgf_in, uint32_t* counts,
+                      GiCont* conts, GiJob* jobs, hipStream_t s)
 {
-    const size_t npx = (size_t)cam->width * cam->height;
     const GiConst gc = vpt::make_gi(prm);
     const CamView cv = vpt::make_cam(cam);
-    hipLaunchKernelGGL(gi_jobs_clear_kernel, dim3(1), dim3(1), 0, s, njobs);
+    GiQueues qs;
+    const uint32_t total = gi_queues(cam->width, cam->height, qs);   // = W * H
+    const dim3 qgrid((total + 255u) / 256u);
+    hipLaunchKernelGGL(gi_queues_clear_kernel, dim3(1), dim3(64), 0, s, counts);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(gi_bounce_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w), cv,
-                       gb->d_t, gb->d_face, gc, albedo, d_rgb, d_svgf_in, jobs, njobs, w->d_counter);
+                       gb->d_t, gb->d_face, gc, albedo, d_rgb, d_svgf_in, qs, conts, jobs, counts, w->d_counter);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(gi_shadow_kernel, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, vpt::make_view(w), cv,
-                       gb->d_t, gb->d_face, gc, albedo, jobs, njobs, d_rgb, d_svgf_in, w->d_counter);
+    if (prm->gi_iter > kGiPhase1) {
+        hipLaunchKernelGGL(gi_cont_kernel, qgrid, dim3(256), 0, s, vpt::make_view(w), cv, gb->d_t, gb->d_face, gc,
+                           albedo, d_rgb, d_svgf_in, qs, conts, jobs, counts, w->d_counter);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(gi_shadow_kernel, qgrid, dim3(256), 0, s, vpt::make_view(w), cv, gb->d_t, gb->d_face, gc,
+                       albedo, qs, jobs, counts, d_rgb, d_svgf_in, w->d_counter);
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }
@@ -1287,8 +1502,9 @@ extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const v
     std::lock_guard<std::mutex> lk(ctx->mu);
     vpt::GiScratch* sc = nullptr;
     if (int rc = gi_scratch_for(ctx, s, npx, capturing, &sc)) return rc;
-    if (int rc = gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<GiJob*>(sc->d_jobs),
-                            sc->d_njobs, s))
+    if (int rc = gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<uint32_t*>(sc->d_buf),
+                            reinterpret_cast<GiCont*>(sc->d_buf + vpt::kGiCountersBytes),
+                            reinterpret_cast<GiJob*>(sc->d_buf + vpt::kGiCountersBytes + sizeof(GiCont) * sc->cap), s))
         return rc;
     if (!capturing) HIP_TRY(hipEventRecord((hipEvent_t)sc->done, s));
     return VPT_OK;
@@ -1297,17 +1513,19 @@ extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const v
 // ---------------------------------------------------------------------------
 // Allocation-free GI pass (vpt_trace_gi_noalloc): the same kernels over a
 // fixed scratch array instead of the context's per-stream scratch.
-// Layout: the job counter in the first 256 bytes, then the shadow jobs
-// (16 B per pixel), for images of at most kNoAllocGiMaxPixels pixels:
-// 32 MiB + 256 B.  Each GPU the library runs on holds its own copy until the
-// process ends.
+// Layout: the queue counters in the first 4 KiB, then the
+// continuations and the shadow jobs (16 B per pixel each), for images of at
+// most kNoAllocGiMaxPixels pixels: 64 MiB + 4 KiB.  Each GPU the library
+// runs on holds its own copy until the process ends.
 // ---------------------------------------------------------------------------
 
 namespace {
 
 constexpr size_t kNoAllocGiMaxPixels = (size_t)1 << 21;   // 2,097,152: 1920 x 1080 fits
-constexpr size_t kNoAllocGiJobsOff = 256;
+constexpr size_t kNoAllocGiContsOff = vpt::kGiCountersBytes;
+constexpr size_t kNoAllocGiJobsOff = kNoAllocGiContsOff + kNoAllocGiMaxPixels * sizeof(GiCont);
 constexpr size_t kNoAllocGiBytes = kNoAllocGiJobsOff + kNoAllocGiMaxPixels * sizeof(GiJob);
+static_assert(kNoAllocGiBytes == ((size_t)64 << 20) + 4096, "64 MiB + 4 KiB");
 constexpr int kNoAllocMaxDevices = 64;
 
 static __device__ __align__(256) uint8_t scratch[kNoAllocGiBytes];
@@ -1368,8 +1586,9 @@ extern "C" VPT_API int vpt_trace_gi_noalloc(vpt_world* w, const vpt_camera* cam,
     }
     uint8_t* b = static_cast<uint8_t*>(base);
     const vpt::AlbedoTable albedo = vpt::take_albedo(w->ctx, prm);
-    return gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<GiJob*>(b + kNoAllocGiJobsOff),
-                      reinterpret_cast<uint32_t*>(b), (hipStream_t)stream);
+    return gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<uint32_t*>(b),
+                      reinterpret_cast<GiCont*>(b + kNoAllocGiContsOff), reinterpret_cast<GiJob*>(b + kNoAllocGiJobsOff),
+                      (hipStream_t)stream);
 }
 
 extern "C" VPT_API int vpt_trace_rays(vpt_world* w, const vpt_ray* d_rays, int n, int max_iter, vpt_hit* d_hits,

@@ -528,8 +528,7 @@ extern "C" VPT_API int vpt_ctx_destroy(vpt_ctx* ctx)
     DeviceGuard g(ctx->device);
     for (vpt::GiScratch& sc : ctx->gi_scratch) {
         if (sc.done) (void)hipEventDestroy((hipEvent_t)sc.done);
-        (void)hipFree(sc.d_jobs);
-        (void)hipFree(sc.d_njobs);
+        (void)hipFree(sc.d_buf);
     }
     for (void* p : ctx->gi_retired) (void)hipFree(p);
     if (ctx->side_stream) (void)hipStreamDestroy((hipStream_t)ctx->side_stream);
