@@ -355,50 +355,55 @@ __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restri
         *reinterpret_cast<uint4*>(out + base + i) = *reinterpret_cast<const uint4*>(slab + i);
 }
 
-// Z pass over a 32 x 8 x nz column tile held in LDS (tiles of at most
-// kZTileMax bytes; larger ones use df_z_kernel): the forward and backward
-// relaxations along z (ManhattanDistanceZ.comp:151-165) of the tile's 256
-// columns, four adjacent columns per lane (one LDS dword per z) and each
-// column's z range split into kZSeg segments (relax_segments); then the
-// tile -- four whole 8x8 bricks wide and one brick high -- is written to
-// the bricked DF with 16-byte stores.
+// Z pass over a TW x 8 x nz column tile held in LDS (TW = 16 or 32; tiles
+// of at most kZTileMax bytes; larger ones use df_z_kernel): the forward and
+// backward relaxations along z (ManhattanDistanceZ.comp:151-165) of the
+// tile's 8 * TW columns, four adjacent columns per lane (one LDS dword per
+// z) and each column's z range split into 512 / (2 * TW) segments
+// (relax_segments); then the tile -- TW / 8 whole 8x8 bricks wide and one
+// brick high -- is written to the bricked DF with 16-byte stores.
 constexpr int kZTileMax = 128 * 1024;
-constexpr int kZSeg = 8;
 
-__global__ void __launch_bounds__(64 * kZSeg) df_z_tile_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ df_brick,
+template <int TW>
+__global__ void __launch_bounds__(512) df_z_tile_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ df_brick,
                                                         int nx, int ny, int nz, uint32_t row_stride,
                                                         uint32_t slab_stride, const int32_t* __restrict__ gate)
 {
+    static_assert(TW == 16 || TW == 32, "tile width: two or four bricks");
+    constexpr int kRow = 8 * TW;               // bytes of one z layer of the tile
+    constexpr int kQ = kRow / 4;               // dwords of a layer: the block's columns, four per dword
+    constexpr int kSeg = 512 / kQ;             // z segments per column
+    constexpr int kVec = TW / 16;              // 16-byte groups per row
     if (gate && !*gate) return;
-    extern __shared__ uint8_t tile[];   // [z][y 0..7][x 0..31]
-    const int tiles_x = nx / 32;
-    // consecutive tiles on one XCD: the four 32-byte-wide tiles that share a
-    // 128-byte line of a row meet in the same L2
+    extern __shared__ uint8_t tile[];   // [z][y 0..7][x 0..TW-1]
+    const int tiles_x = nx / TW;
+    // consecutive tiles on one XCD: the tiles that share a 128-byte line of
+    // a row meet in the same L2
     const uint32_t t = vpt::xcd_remap(blockIdx.x, gridDim.x);
-    const int x0 = (int)(t % (uint32_t)tiles_x) * 32, y0 = (int)(t / (uint32_t)tiles_x) * 8;
+    const int x0 = (int)(t % (uint32_t)tiles_x) * TW, y0 = (int)(t / (uint32_t)tiles_x) * 8;
     const size_t plane = (size_t)nx * ny;
-    // stage: 8 rows of 32 bytes per z, two 16-byte loads per row; unrolled
-    // so that a wavefront has several loads in flight per wait
+    // stage: 8 rows of TW bytes per z, 16-byte loads; unrolled so that a
+    // wavefront has several loads in flight per wait
 #pragma unroll 8
-    for (int i = threadIdx.x; i < nz * 16; i += blockDim.x) {
-        const int z = i >> 4, r = (i >> 1) & 7, h = i & 1;
-        *reinterpret_cast<uint4*>(tile + z * 256 + r * 32 + h * 16) =
+    for (int i = threadIdx.x; i < nz * 8 * kVec; i += blockDim.x) {
+        const int z = i / (8 * kVec), r = (i / kVec) & 7, h = i % kVec;
+        *reinterpret_cast<uint4*>(tile + z * kRow + r * TW + h * 16) =
             *reinterpret_cast<const uint4*>(in + (size_t)z * plane + (size_t)(y0 + r) * nx + x0 + h * 16);
     }
     __syncthreads();
     {
-        __shared__ uint32_t s_end[kZSeg * 64][2];
-        const int q = (int)(threadIdx.x & 63u), sg = (int)(threadIdx.x >> 6);   // dword of a z row, segment
+        __shared__ uint32_t s_end[512][2];
+        const int q = (int)(threadIdx.x % kQ), sg = (int)(threadIdx.x / kQ);   // dword of a z layer, segment
         uint32_t* col = reinterpret_cast<uint32_t*>(tile) + q;
-        relax_segments(col, 64, nz / kZSeg, sg, kZSeg, +1, true, s_end, q, 64);
-        relax_segments(col, 64, nz / kZSeg, sg, kZSeg, -1, true, s_end, q, 64);
+        relax_segments(col, kQ, nz / kSeg, sg, kSeg, +1, true, s_end, q, kQ);
+        relax_segments(col, kQ, nz / kSeg, sg, kSeg, -1, true, s_end, q, kQ);
     }
     // write whole bricks: brick (bx, bz) of this tile, 512 bytes each, one
     // 16-byte group per lane: bytes m..m+15 of the Morton order
     const int nbz = nz / 8;
-    for (int i = threadIdx.x; i < 4 * nbz * 32; i += blockDim.x) {
+    for (int i = threadIdx.x; i < (TW / 8) * nbz * 32; i += blockDim.x) {
         const int g = i & 31, brick = i >> 5;       // 32 groups of 16 bytes per brick
-        const int bxl = brick & 3, bz = brick >> 2;
+        const int bxl = brick % (TW / 8), bz = brick / (TW / 8);
         uint32_t words[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
         for (int k = 0; k < 16; k++) {
@@ -406,7 +411,7 @@ __global__ void __launch_bounds__(64 * kZSeg) df_z_tile_kernel(const uint8_t* __
             const uint32_t lx = (m & 1u) | ((m >> 2) & 2u) | ((m >> 4) & 4u);
             const uint32_t ly = ((m >> 1) & 1u) | ((m >> 3) & 2u) | ((m >> 5) & 4u);
             const uint32_t lz = ((m >> 2) & 1u) | ((m >> 4) & 2u) | ((m >> 6) & 4u);
-            const uint32_t b = tile[(bz * 8 + lz) * 256 + ly * 32 + bxl * 8 + lx];
+            const uint32_t b = tile[(bz * 8 + lz) * kRow + ly * TW + bxl * 8 + lx];
             words[k >> 2] |= b << (8 * (k & 3));
         }
         const size_t dst = (size_t)((x0 >> 3) + bxl) * vpt::kBrickBytes + (size_t)(y0 >> 3) * row_stride +
@@ -729,13 +734,19 @@ int df_build_launch(vpt_world* w, hipStream_t s, const int32_t* gate)
         HIP_TRY(hipGetLastError());
     }
     uint32_t rs = (uint32_t)w->nbx * vpt::kBrickBytes, ss = rs * (uint32_t)w->nby;
-    const size_t ztile = (size_t)nz * 256;
-    if (nx % 32 == 0 && nz % kZSeg == 0 && ztile <= (size_t)kZTileMax) {
-        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_z_tile_kernel),
+    // 16-byte-wide tiles (two bricks) give 3 blocks per CU in LDS at nz =
+    // 384 and twice the blocks of 32-wide ones; both need nz to split into
+    // their segments (16 / 8) and the tile to fit in LDS
+    if (nx % 16 == 0 && nz % 16 == 0 && (size_t)nz * 128 <= (size_t)kZTileMax) {
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_z_tile_kernel<16>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kZTileMax));
-        hipLaunchKernelGGL(df_z_tile_kernel, dim3((unsigned)((nx / 32) * (ny / 8))), dim3(64 * kZSeg), ztile, s,
-                           w->d_tmp,
-                           w->d_df, nx, ny, nz, rs, ss, gate);
+        hipLaunchKernelGGL(df_z_tile_kernel<16>, dim3((unsigned)((nx / 16) * (ny / 8))), dim3(512), (size_t)nz * 128,
+                           s, w->d_tmp, w->d_df, nx, ny, nz, rs, ss, gate);
+    } else if (nx % 32 == 0 && nz % 8 == 0 && (size_t)nz * 256 <= (size_t)kZTileMax) {
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_z_tile_kernel<32>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kZTileMax));
+        hipLaunchKernelGGL(df_z_tile_kernel<32>, dim3((unsigned)((nx / 32) * (ny / 8))), dim3(512), (size_t)nz * 256,
+                           s, w->d_tmp, w->d_df, nx, ny, nz, rs, ss, gate);
     } else {
         size_t cols = (size_t)nx * ny;
         hipLaunchKernelGGL(df_z_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, w->d_tmp, w->d_df, nx,
