@@ -178,6 +178,50 @@ def test_gi_parity(terrain_world, gbuffer_1080):
     np.testing.assert_array_equal(d["ao"], planes["ao"])
 
 
+@pytest.mark.parametrize("W,H,gi_iter", [
+    (17, 9, 48),        # one 16x16 tile row, fewer tiles than XCDs
+    (40, 40, 48),       # 9 tiles: one per XCD plus a leftover
+    (129, 3, 48),       # ragged last tile, 1-pixel-high rows
+    (333, 97, 48),
+    (640, 360, 0),      # no bounce step at all
+    (640, 360, 1),
+    (640, 360, 24),     # exactly the bounce kernel's step budget: no continuation
+    (640, 360, 25),     # continued rays take one more step
+    (640, 360, 200),
+])
+def test_gi_parity_sizes_and_caps(terrain_world, W, H, gi_iter):
+    """The GI pass (bounce kernel, continuations after 24 steps, shadow
+    jobs; per-XCD queues sized from the tile layout) equals the oracle at
+    sizes with fewer tiles than XCDs, ragged tiles and iteration caps on
+    both sides of the bounce kernel's 24 steps, and counts the oracle's DF
+    samples and rays."""
+    g, df, w = terrain_world
+    torch = _torch()
+    print(f"gi parity {W}x{H} gi_iter {gi_iter}", flush=True)
+    cam, view, proj, iv, ip = camera(W, H)
+    gb = DeviceGBuffer(W, H)
+    w.trace_primary(cam, gb, 350)
+    prm = gi_params(frame=5)
+    prm.gi_iter = gi_iter
+    rgb = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    tex = torch.empty((H, W, 8), dtype=torch.int16, device="cuda")
+    counter = torch.zeros(2, dtype=torch.int64, device="cuda")
+    w.set_sample_counter(counter)
+    w.trace_gi(cam, gb, prm, rgb, tex)
+    w.set_sample_counter(None)
+    torch.cuda.synchronize()
+    host = gb.numpy()
+    ref = oracle.trace_gi(g, df, iv, ip, host["t"], host["face"], oracle.gi_params_from(prm))
+    np.testing.assert_array_equal(rgb.cpu().numpy(), ref["rgb"])
+    d = decode_texels(tex.cpu().numpy().view(np.uint16))
+    planes = gi_planes_from_oracle(ref)
+    np.testing.assert_array_equal(d["sh"], planes["sh"])
+    np.testing.assert_array_equal(d["ao"], planes["ao"])
+    c = counter.cpu().numpy()
+    assert int(c[0]) == int(ref["samples"].sum()) and int(c[1]) == int(ref["rays"].sum()), (c, ref["samples"].sum(),
+                                                                                         ref["rays"].sum())
+
+
 def test_trace_frame_parity(terrain_world):
     """vpt_trace_frame (shadow pass beside GI on the context's side stream)
     equals the oracle's three passes, frame after frame, with the DF samples
