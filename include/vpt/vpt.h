@@ -370,8 +370,9 @@ VPT_API int vpt_trace_gi_noalloc(vpt_world* w, const vpt_camera* cam, const vpt_
 
 /* The three trace passes of one frame, one launch each (INITIAL_TRACE,
  * SHADOW_TRACE, DIFFUSE_TRACE, Pipeline.cpp:1963-2852): the primary pass on
- * `stream` fills gb, then the shadow pass runs on a stream of the context
- * beside the GI pass on `stream`, and `stream` waits for both.  Outputs
+ * `stream` fills gb, then the GI pass runs on `stream` and, from the end of
+ * its bounce kernel on, the shadow pass on a stream of the context beside
+ * it, and `stream` waits for both.  Outputs
  * equal vpt_trace_primary + vpt_trace_shadow + vpt_trace_gi, or, with
  * d_transversal (optional), vpt_trace_shadow2's two outputs in place of
  * vpt_trace_shadow's; the overlap fills the tail of each pass with the

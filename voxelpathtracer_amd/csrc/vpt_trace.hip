@@ -1407,6 +1407,84 @@ extern "C" VPT_API int vpt_reflect_shading_from_gi(const vpt_gi_params* gi, cons
     return VPT_OK;
 }
 
+// The GI pass's kernels over the given scratch (the job counter is cleared
+// by a kernel, so the sequence can be captured in a graph).
+// The GI pass's kernels over the given scratch: 16 queue counters, then
+// continuation and job arrays of at least one record per pixel each.  The
+// counters are cleared by a kernel, so the sequence can be captured in a
+// graph.
+// Where vpt_trace_frame starts the shadow pass beside the GI pass: after
+// the bounce kernel, so that it fills the chip beside the continuation and
+// shadow-job kernels, whose few long rays leave most CUs idle, rather than
+// competing with the bounce kernel, which fills the chip by itself.
+// 0: before the GI pass, 1: after the bounce kernel, 2: after the
+// continuation kernel.
+constexpr int kFrameForkStage = 1;
+
+static int gi_enqueue(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
+                      const vpt::AlbedoTable& albedo, float* d_rgb, vpt_svgf_texel* d_svgf_in, uint32_t* counts,
+                      GiCont* conts, GiJob* jobs, hipStream_t s, hipEvent_t fork = nullptr)
+{
+    const GiConst gc = vpt::make_gi(prm);
+    const CamView cv = vpt::make_cam(cam);
+    GiQueues qs;
+    const uint32_t total = gi_queues(cam->width, cam->height, qs);   // = W * H
+    const dim3 qgrid((total + 255u) / 256u);
+    if (fork && kFrameForkStage == 0) HIP_TRY(hipEventRecord(fork, s));
+    hipLaunchKernelGGL(gi_queues_clear_kernel, dim3(1), dim3(64), 0, s, counts);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(gi_bounce_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w), cv,
+                       gb->d_t, gb->d_face, gc, albedo, d_rgb, d_svgf_in, qs, conts, jobs, counts, w->d_counter);
+    HIP_TRY(hipGetLastError());
+    if (fork && kFrameForkStage == 1) HIP_TRY(hipEventRecord(fork, s));
+    if (prm->gi_iter > kGiPhase1) {
+        hipLaunchKernelGGL(gi_cont_kernel, qgrid, dim3(256), 0, s, vpt::make_view(w), cv, gb->d_t, gb->d_face, gc,
+                           albedo, d_rgb, d_svgf_in, qs, conts, jobs, counts, w->d_counter);
+        HIP_TRY(hipGetLastError());
+    }
+    if (fork && kFrameForkStage == 2) HIP_TRY(hipEventRecord(fork, s));
+    hipLaunchKernelGGL(gi_shadow_kernel, qgrid, dim3(256), 0, s, vpt::make_view(w), cv, gb->d_t, gb->d_face, gc,
+                       albedo, qs, jobs, counts, d_rgb, d_svgf_in, w->d_counter);
+    HIP_TRY(hipGetLastError());
+    return VPT_OK;
+}
+
+// The GI pass on `s` over the context's scratch for `s` (arguments already
+// checked); `fork`, when given, is recorded on `s` at kFrameForkStage.
+static int gi_pass(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm, float* d_rgb,
+                   vpt_svgf_texel* d_svgf_in, hipStream_t s, hipEvent_t fork)
+{
+    vpt_ctx* ctx = w->ctx;
+    const vpt::AlbedoTable albedo = vpt::take_albedo(ctx, prm);
+    const size_t npx = (size_t)cam->width * cam->height;
+    // the scratch stays this stream's until the launches below are enqueued
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(s, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    vpt::GiScratch* sc = nullptr;
+    if (int rc = gi_scratch_for(ctx, s, npx, capturing, &sc)) return rc;
+    if (int rc = gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<uint32_t*>(sc->d_buf),
+                            reinterpret_cast<GiCont*>(sc->d_buf + vpt::kGiCountersBytes),
+                            reinterpret_cast<GiJob*>(sc->d_buf + vpt::kGiCountersBytes + sizeof(GiCont) * sc->cap), s,
+                            fork))
+        return rc;
+    if (!capturing) HIP_TRY(hipEventRecord((hipEvent_t)sc->done, s));
+    return VPT_OK;
+}
+
+extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
+                                    const vpt_gi_params* prm, float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream)
+{
+    if (!w || !prm || !d_rgb) return set_error(VPT_EINVAL, "vpt_trace_gi: null argument");
+    if (int rc = vpt::check_cam(cam, "vpt_trace_gi")) return rc;
+    if (int rc = vpt::check_gbuffer(gb, cam, "vpt_trace_gi", false)) return rc;
+    if (int rc = vpt::check_iter(prm->gi_iter, "vpt_trace_gi")) return rc;
+    if (int rc = vpt::check_iter(prm->shadow_iter, "vpt_trace_gi")) return rc;
+    DeviceGuard g(w->device);
+    return gi_pass(w, cam, gb, prm, d_rgb, d_svgf_in, (hipStream_t)stream, nullptr);
+}
+
 extern "C" VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, const vpt_gi_params* prm,
                                        int primary_iter, int shadow_iter, const vpt_gbuffer* gb, uint8_t* d_mask,
                                        uint16_t* d_transversal, float* d_rgb, vpt_svgf_texel* d_svgf_in,
@@ -1438,77 +1516,20 @@ extern "C" VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, cons
     }
     hipStream_t s = (hipStream_t)stream, side = (hipStream_t)ctx->side_stream;
     if (int rc = vpt_trace_primary(w, cam, primary_iter, gb, stream)) return rc;
-    // fork: the shadow pass reads the G-buffer beside GI (both only read
-    // it); join before anything later on `stream`
-    HIP_TRY(hipEventRecord((hipEvent_t)ctx->ev_fork, s));
+    // the GI pass on `stream` records the fork event partway (after its
+    // bounce kernel); the shadow pass, which like GI only reads the
+    // G-buffer, starts there on the side stream; join before anything later
+    // on `stream`
+    if (int rc = gi_pass(w, cam, gb, prm, d_rgb, d_svgf_in, s, (hipEvent_t)ctx->ev_fork)) return rc;
     HIP_TRY(hipStreamWaitEvent(side, (hipEvent_t)ctx->ev_fork, 0));
     const int rc_shadow = d_transversal
                               ? vpt_trace_shadow2(w, cam, gb, prm->sun_dir, shadow_iter, d_mask, d_transversal, side)
                               : vpt_trace_shadow(w, cam, gb, prm->sun_dir, shadow_iter, d_mask, side);
     HIP_TRY(hipEventRecord((hipEvent_t)ctx->ev_join, side));
-    const int rc_gi = vpt_trace_gi(w, cam, gb, prm, d_rgb, d_svgf_in, stream);
     HIP_TRY(hipStreamWaitEvent(s, (hipEvent_t)ctx->ev_join, 0));
-    return rc_shadow ? rc_shadow : rc_gi;
+    return rc_shadow;
 }
 
-// The GI pass's kernels over the given scratch (the job counter is cleared
-// by a kernel, so the sequence can be captured in a graph).
-// The GI pass's kernels over the given scratch: 16 queue counters, then
-// continuation and job arrays of at least one record per pixel each.  The
-// counters are cleared by a kernel, so the sequence can be captured in a
-// graph.
-static int gi_enqueue(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
-                      const vpt::AlbedoTable& albedo, float* d_rgb, vpt_svgf_texel* d_svgf_in, uint32_t* counts,
-                      GiCont* conts, GiJob* jobs, hipStream_t s)
-{
-    const GiConst gc = vpt::make_gi(prm);
-    const CamView cv = vpt::make_cam(cam);
-    GiQueues qs;
-    const uint32_t total = gi_queues(cam->width, cam->height, qs);   // = W * H
-    const dim3 qgrid((total + 255u) / 256u);
-    hipLaunchKernelGGL(gi_queues_clear_kernel, dim3(1), dim3(64), 0, s, counts);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(gi_bounce_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w), cv,
-                       gb->d_t, gb->d_face, gc, albedo, d_rgb, d_svgf_in, qs, conts, jobs, counts, w->d_counter);
-    HIP_TRY(hipGetLastError());
-    if (prm->gi_iter > kGiPhase1) {
-        hipLaunchKernelGGL(gi_cont_kernel, qgrid, dim3(256), 0, s, vpt::make_view(w), cv, gb->d_t, gb->d_face, gc,
-                           albedo, d_rgb, d_svgf_in, qs, conts, jobs, counts, w->d_counter);
-        HIP_TRY(hipGetLastError());
-    }
-    hipLaunchKernelGGL(gi_shadow_kernel, qgrid, dim3(256), 0, s, vpt::make_view(w), cv, gb->d_t, gb->d_face, gc,
-                       albedo, qs, jobs, counts, d_rgb, d_svgf_in, w->d_counter);
-    HIP_TRY(hipGetLastError());
-    return VPT_OK;
-}
-
-extern "C" VPT_API int vpt_trace_gi(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
-                                    const vpt_gi_params* prm, float* d_rgb, vpt_svgf_texel* d_svgf_in, void* stream)
-{
-    if (!w || !prm || !d_rgb) return set_error(VPT_EINVAL, "vpt_trace_gi: null argument");
-    if (int rc = vpt::check_cam(cam, "vpt_trace_gi")) return rc;
-    if (int rc = vpt::check_gbuffer(gb, cam, "vpt_trace_gi", false)) return rc;
-    if (int rc = vpt::check_iter(prm->gi_iter, "vpt_trace_gi")) return rc;
-    if (int rc = vpt::check_iter(prm->shadow_iter, "vpt_trace_gi")) return rc;
-    DeviceGuard g(w->device);
-    hipStream_t s = (hipStream_t)stream;
-    vpt_ctx* ctx = w->ctx;
-    const vpt::AlbedoTable albedo = vpt::take_albedo(ctx, prm);
-    const size_t npx = (size_t)cam->width * cam->height;
-    // the scratch stays this stream's until the launches below are enqueued
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    HIP_TRY(hipStreamIsCapturing(s, &cs));
-    const bool capturing = cs != hipStreamCaptureStatusNone;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    vpt::GiScratch* sc = nullptr;
-    if (int rc = gi_scratch_for(ctx, s, npx, capturing, &sc)) return rc;
-    if (int rc = gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<uint32_t*>(sc->d_buf),
-                            reinterpret_cast<GiCont*>(sc->d_buf + vpt::kGiCountersBytes),
-                            reinterpret_cast<GiJob*>(sc->d_buf + vpt::kGiCountersBytes + sizeof(GiCont) * sc->cap), s))
-        return rc;
-    if (!capturing) HIP_TRY(hipEventRecord((hipEvent_t)sc->done, s));
-    return VPT_OK;
-}
 
 // ---------------------------------------------------------------------------
 // Allocation-free GI pass (vpt_trace_gi_noalloc): the same kernels over a
