@@ -220,8 +220,12 @@ __device__ inline void count_samples(unsigned long long* counter, int samples, i
 __global__ void __launch_bounds__(256) primary_kernel(WorldView w, CamView cam, int max_iter, float* __restrict__ out_t,
                                                       uint8_t* __restrict__ out_face, uint8_t* __restrict__ out_block,
                                                       uint32_t* __restrict__ out_voxel,
-                                                      unsigned long long* __restrict__ counter)
+                                                      unsigned long long* __restrict__ counter,
+                                                      uint32_t* __restrict__ gi_counts)
 {
+    // vpt_trace_frame: the GI pass that follows on this stream finds its
+    // queue counters cleared (saves it a launch)
+    if (gi_counts && blockIdx.x == 0 && threadIdx.x < 16) gi_counts[threadIdx.x * vpt::kGiCounterStride] = 0u;
     int px, py;
     tile_pixel(cam.W, cam.H, px, py);
     int samples = 0, rays = 0;
@@ -1218,6 +1222,16 @@ dim3 tile_grid(int W, int H) { return dim3((unsigned)(((W + 15) / 16) * ((H + 15
 
 }  // namespace
 
+static int primary_launch(vpt_world* w, const vpt_camera* cam, int max_iter, const vpt_gbuffer* out, hipStream_t s,
+                          uint32_t* gi_counts)
+{
+    hipLaunchKernelGGL(primary_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w),
+                       vpt::make_cam(cam), max_iter, out->d_t, out->d_face, out->d_block, out->d_voxel, w->d_counter,
+                       gi_counts);
+    HIP_TRY(hipGetLastError());
+    return VPT_OK;
+}
+
 extern "C" VPT_API int vpt_trace_primary(vpt_world* w, const vpt_camera* cam, int max_iter, const vpt_gbuffer* out,
                                          void* stream)
 {
@@ -1226,11 +1240,7 @@ extern "C" VPT_API int vpt_trace_primary(vpt_world* w, const vpt_camera* cam, in
     if (int rc = vpt::check_gbuffer(out, cam, "vpt_trace_primary", true)) return rc;
     if (int rc = vpt::check_iter(max_iter, "vpt_trace_primary")) return rc;
     DeviceGuard g(w->device);
-    hipLaunchKernelGGL(primary_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, (hipStream_t)stream,
-                       vpt::make_view(w), vpt::make_cam(cam), max_iter, out->d_t, out->d_face, out->d_block,
-                       out->d_voxel, w->d_counter);
-    HIP_TRY(hipGetLastError());
-    return VPT_OK;
+    return primary_launch(w, cam, max_iter, out, (hipStream_t)stream, nullptr);
 }
 
 extern "C" VPT_API int vpt_trace_shadow(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb,
@@ -1423,7 +1433,7 @@ constexpr int kFrameForkStage = 1;
 
 static int gi_enqueue(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm,
                       const vpt::AlbedoTable& albedo, float* d_rgb, vpt_svgf_texel* d_svgf_in, uint32_t* counts,
-                      GiCont* conts, GiJob* jobs, hipStream_t s, hipEvent_t fork = nullptr)
+                      GiCont* conts, GiJob* jobs, hipStream_t s, hipEvent_t fork = nullptr, bool cleared = false)
 {
     const GiConst gc = vpt::make_gi(prm);
     const CamView cv = vpt::make_cam(cam);
@@ -1431,8 +1441,10 @@ static int gi_enqueue(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb
     const uint32_t total = gi_queues(cam->width, cam->height, qs);   // = W * H
     const dim3 qgrid((total + 255u) / 256u);
     if (fork && kFrameForkStage == 0) HIP_TRY(hipEventRecord(fork, s));
-    hipLaunchKernelGGL(gi_queues_clear_kernel, dim3(1), dim3(64), 0, s, counts);
-    HIP_TRY(hipGetLastError());
+    if (!cleared) {
+        hipLaunchKernelGGL(gi_queues_clear_kernel, dim3(1), dim3(64), 0, s, counts);
+        HIP_TRY(hipGetLastError());
+    }
     hipLaunchKernelGGL(gi_bounce_kernel, tile_grid(cam->width, cam->height), dim3(256), 0, s, vpt::make_view(w), cv,
                        gb->d_t, gb->d_face, gc, albedo, d_rgb, d_svgf_in, qs, conts, jobs, counts, w->d_counter);
     HIP_TRY(hipGetLastError());
@@ -1451,8 +1463,15 @@ static int gi_enqueue(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb
 
 // The GI pass on `s` over the context's scratch for `s` (arguments already
 // checked); `fork`, when given, is recorded on `s` at kFrameForkStage.
+// With `primary` (vpt_trace_frame), the primary pass is launched on `s`
+// first, once the scratch is this call's, and clears the queue counters.
+struct PrimaryFirst {
+    const vpt_gbuffer* gb;
+    int max_iter;
+};
+
 static int gi_pass(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, const vpt_gi_params* prm, float* d_rgb,
-                   vpt_svgf_texel* d_svgf_in, hipStream_t s, hipEvent_t fork)
+                   vpt_svgf_texel* d_svgf_in, hipStream_t s, hipEvent_t fork, const PrimaryFirst* primary = nullptr)
 {
     vpt_ctx* ctx = w->ctx;
     const vpt::AlbedoTable albedo = vpt::take_albedo(ctx, prm);
@@ -1464,10 +1483,13 @@ static int gi_pass(vpt_world* w, const vpt_camera* cam, const vpt_gbuffer* gb, c
     std::lock_guard<std::mutex> lk(ctx->mu);
     vpt::GiScratch* sc = nullptr;
     if (int rc = gi_scratch_for(ctx, s, npx, capturing, &sc)) return rc;
-    if (int rc = gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, reinterpret_cast<uint32_t*>(sc->d_buf),
+    uint32_t* counts = reinterpret_cast<uint32_t*>(sc->d_buf);
+    if (primary)
+        if (int rc = primary_launch(w, cam, primary->max_iter, primary->gb, s, counts)) return rc;
+    if (int rc = gi_enqueue(w, cam, gb, prm, albedo, d_rgb, d_svgf_in, counts,
                             reinterpret_cast<GiCont*>(sc->d_buf + vpt::kGiCountersBytes),
                             reinterpret_cast<GiJob*>(sc->d_buf + vpt::kGiCountersBytes + sizeof(GiCont) * sc->cap), s,
-                            fork))
+                            fork, primary != nullptr))
         return rc;
     if (!capturing) HIP_TRY(hipEventRecord((hipEvent_t)sc->done, s));
     return VPT_OK;
@@ -1515,12 +1537,12 @@ extern "C" VPT_API int vpt_trace_frame(vpt_world* w, const vpt_camera* cam, cons
         ctx->ev_join = e1;
     }
     hipStream_t s = (hipStream_t)stream, side = (hipStream_t)ctx->side_stream;
-    if (int rc = vpt_trace_primary(w, cam, primary_iter, gb, stream)) return rc;
-    // the GI pass on `stream` records the fork event partway (after its
-    // bounce kernel); the shadow pass, which like GI only reads the
-    // G-buffer, starts there on the side stream; join before anything later
-    // on `stream`
-    if (int rc = gi_pass(w, cam, gb, prm, d_rgb, d_svgf_in, s, (hipEvent_t)ctx->ev_fork)) return rc;
+    // the primary pass, then the GI pass on `stream`, which records the fork
+    // event partway (after its bounce kernel); the shadow pass, which like
+    // GI only reads the G-buffer, starts there on the side stream; join
+    // before anything later on `stream`
+    const PrimaryFirst primary = {gb, primary_iter};
+    if (int rc = gi_pass(w, cam, gb, prm, d_rgb, d_svgf_in, s, (hipEvent_t)ctx->ev_fork, &primary)) return rc;
     HIP_TRY(hipStreamWaitEvent(side, (hipEvent_t)ctx->ev_fork, 0));
     const int rc_shadow = d_transversal
                               ? vpt_trace_shadow2(w, cam, gb, prm->sun_dir, shadow_iter, d_mask, d_transversal, side)
