@@ -242,7 +242,7 @@ __device__ inline int wave_incl_max(int v)
 // The previous lane's value (wave_shr:1); lane 0 gets -kNone.
 __device__ inline int wave_prev(int v) { return __builtin_amdgcn_update_dpp(-kNone, v, 0x138, 0xf, 0xf, false); }
 
-__global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restrict__ grid, uint8_t* __restrict__ out,
+__global__ void __launch_bounds__(1024) df_xy_slab_kernel(const uint8_t* __restrict__ grid, uint8_t* __restrict__ out,
                                                          int nx, int ny, int nz, const int32_t* __restrict__ gate)
 {
     if (gate && !*gate) return;
@@ -341,7 +341,7 @@ __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restri
     // lane (one LDS dword per row), the rows split into segments
     // (relax_segments); every thread takes part in its barriers
     {
-        __shared__ uint32_t s_end_y[512][2];
+        __shared__ uint32_t s_end_y[1024][2];
         const int qn = nx / 4;
         int nseg = 8;
         while (nseg > 1 && (qn * nseg > (int)blockDim.x || ny % nseg != 0)) nseg >>= 1;
@@ -363,16 +363,17 @@ __global__ void __launch_bounds__(512) df_xy_slab_kernel(const uint8_t* __restri
 // (relax_segments); then the tile -- TW / 8 whole 8x8 bricks wide and one
 // brick high -- is written to the bricked DF with 16-byte stores.
 constexpr int kZTileMax = 128 * 1024;
+constexpr int kZThreads = 512;   // threads of a 16-wide tile
 
-template <int TW>
-__global__ void __launch_bounds__(512) df_z_tile_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ df_brick,
+template <int TW, int NT>
+__global__ void __launch_bounds__(NT) df_z_tile_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ df_brick,
                                                         int nx, int ny, int nz, uint32_t row_stride,
                                                         uint32_t slab_stride, const int32_t* __restrict__ gate)
 {
     static_assert(TW == 16 || TW == 32, "tile width: two or four bricks");
     constexpr int kRow = 8 * TW;               // bytes of one z layer of the tile
     constexpr int kQ = kRow / 4;               // dwords of a layer: the block's columns, four per dword
-    constexpr int kSeg = 512 / kQ;             // z segments per column
+    constexpr int kSeg = NT / kQ;              // z segments per column
     constexpr int kVec = TW / 16;              // 16-byte groups per row
     if (gate && !*gate) return;
     extern __shared__ uint8_t tile[];   // [z][y 0..7][x 0..TW-1]
@@ -392,7 +393,7 @@ __global__ void __launch_bounds__(512) df_z_tile_kernel(const uint8_t* __restric
     }
     __syncthreads();
     {
-        __shared__ uint32_t s_end[512][2];
+        __shared__ uint32_t s_end[NT][2];
         const int q = (int)(threadIdx.x % kQ), sg = (int)(threadIdx.x / kQ);   // dword of a z layer, segment
         uint32_t* col = reinterpret_cast<uint32_t*>(tile) + q;
         relax_segments(col, kQ, nz / kSeg, sg, kSeg, +1, true, s_end, q, kQ);
@@ -721,7 +722,8 @@ int df_build_launch(vpt_world* w, hipStream_t s, const int32_t* gate)
     if (slab <= (size_t)kSlabMax) {
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_xy_slab_kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kSlabMax));
-        hipLaunchKernelGGL(df_xy_slab_kernel, dim3(nz), dim3(512), slab, s, w->d_grid_lin, w->d_tmp, nx, ny, nz, gate);
+        hipLaunchKernelGGL(df_xy_slab_kernel, dim3(nz), dim3(1024), slab, s, w->d_grid_lin, w->d_tmp, nx, ny, nz,
+                           gate);
         HIP_TRY(hipGetLastError());
     } else {
         size_t rows = (size_t)ny * nz;
@@ -737,16 +739,16 @@ int df_build_launch(vpt_world* w, hipStream_t s, const int32_t* gate)
     // 16-byte-wide tiles (two bricks) give 3 blocks per CU in LDS at nz =
     // 384 and twice the blocks of 32-wide ones; both need nz to split into
     // their segments (16 / 8) and the tile to fit in LDS
-    if (nx % 16 == 0 && nz % 16 == 0 && (size_t)nz * 128 <= (size_t)kZTileMax) {
-        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_z_tile_kernel<16>),
+    if (nx % 16 == 0 && nz % (kZThreads / 32) == 0 && (size_t)nz * 128 <= (size_t)kZTileMax) {
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_z_tile_kernel<16, kZThreads>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kZTileMax));
-        hipLaunchKernelGGL(df_z_tile_kernel<16>, dim3((unsigned)((nx / 16) * (ny / 8))), dim3(512), (size_t)nz * 128,
-                           s, w->d_tmp, w->d_df, nx, ny, nz, rs, ss, gate);
+        hipLaunchKernelGGL((df_z_tile_kernel<16, kZThreads>), dim3((unsigned)((nx / 16) * (ny / 8))), dim3(kZThreads),
+                           (size_t)nz * 128, s, w->d_tmp, w->d_df, nx, ny, nz, rs, ss, gate);
     } else if (nx % 32 == 0 && nz % 8 == 0 && (size_t)nz * 256 <= (size_t)kZTileMax) {
-        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_z_tile_kernel<32>),
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_z_tile_kernel<32, 512>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kZTileMax));
-        hipLaunchKernelGGL(df_z_tile_kernel<32>, dim3((unsigned)((nx / 32) * (ny / 8))), dim3(512), (size_t)nz * 256,
-                           s, w->d_tmp, w->d_df, nx, ny, nz, rs, ss, gate);
+        hipLaunchKernelGGL((df_z_tile_kernel<32, 512>), dim3((unsigned)((nx / 32) * (ny / 8))), dim3(512),
+                           (size_t)nz * 256, s, w->d_tmp, w->d_df, nx, ny, nz, rs, ss, gate);
     } else {
         size_t cols = (size_t)nx * ny;
         hipLaunchKernelGGL(df_z_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, w->d_tmp, w->d_df, nx,
