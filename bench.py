@@ -6,9 +6,9 @@ Workload (one step = one frame of rays, inputs resident in HBM):
   distance field built on the GPU; at 1920x1080, 1 spp: one primary ray per
   pixel, one sun-shadow ray per lit pixel, one cosine-weighted diffuse GI
   ray per hit pixel plus a sun-shadow ray at the GI hit, run as the library
-  runs a frame (vpt_trace_frame: the shadow pass beside GI on a second
-  stream; the passes one after another are reported too, with each pass's
-  time).  Grays/s counts every ray whose traversal loop is entered.  SVGF ms/frame (pre-spatial,
+  runs a frame (vpt_trace_frame: the shadow pass on a second stream beside
+  the GI pass, from the end of its bounce kernel; the passes one after
+  another are reported too, with each pass's time).  Grays/s counts every ray whose traversal loop is entered.  SVGF ms/frame (pre-spatial,
   temporal, variance, 5 a-trous, decode at 1920x1080) and DF-build ms are
   reported beside it, with the wall time of an incremental DF update after
   a one-block edit, the sun-shadow denoiser's ms/frame over the same
