@@ -46,6 +46,15 @@ def _world(ctx, grid):
     ((48, 8, 16), 0.05, 4),
     ((64, 32, 64), 0.3, 5),
     ((16, 8, 8), 0.1, 6),          # smallest allowed dims
+    # rows of one to four 256-cell chunks of the slab X pass, partial last
+    # chunks, and more rows than one batch of a wavefront (with columns
+    # longer than the register-held Y segments)
+    ((80, 16, 16), 0.02, 7),
+    ((400, 24, 16), 0.01, 8),
+    ((640, 32, 16), 0.01, 11),
+    ((1008, 16, 8), 0.005, 9),
+    ((1024, 8, 16), 0.3, 10),
+    ((128, 512, 8), 0.01, 12),
 ])
 def test_df_random_worlds(ctx, shape, density, seed):
     rng = np.random.default_rng(seed)

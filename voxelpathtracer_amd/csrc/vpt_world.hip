@@ -9,16 +9,19 @@
 // is exact for 0..254), so the result is identical byte for byte.
 //
 //   df_xy_slab_kernel : one workgroup per z-slab when the slab (nx * ny
-//                       bytes) fits in 64 KiB of LDS: the slab is staged
-//                       with 16-byte loads, one lane per row runs the X pass,
-//                       one lane per column the Y pass, and the slab is
-//                       written back with 16-byte stores.
+//                       bytes) fits in 128 KiB of LDS: the slab is staged
+//                       with 16-byte loads, one wavefront per row runs the X
+//                       pass from the row's solid bitmap, four columns per
+//                       lane the Y pass in segments, and the slab is written
+//                       back with 16-byte stores.
 //   df_x_kernel / df_y_kernel : the same passes in global memory for larger
 //                       slabs (lanes along x for the Y pass, so each step is
 //                       a coalesced row access).
+//   df_z_tile_kernel  : the Z pass over a column tile staged in LDS; it
+//                       writes whole bricks of the bricked DF.
 //   df_z_kernel       : one lane per (x, y) column, lanes along x; the
 //                       backward sweep writes the final byte straight into
-//                       the bricked DF.
+//                       the bricked DF (tiles too large for LDS).
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -160,15 +163,45 @@ __device__ inline Cols4 cols4_relax(Cols4 d, Cols4 c, unsigned short k)
     return Cols4{__builtin_elementwise_min(d.e, c.e + kk), __builtin_elementwise_min(d.o, c.o + kk)};
 }
 
+// The value entering segment `seg` of a forward (dir = +1) or backward
+// (dir = -1) relaxation, folded from the published ends of the segments
+// behind it in sweep order: F = min(end, cin + len - 1), cin = F + 1.  The
+// ends are read together (at most kMaxSeg segments), then folded.
+template <int kMaxSeg>
+__device__ inline Cols4 fold_carry(const uint32_t (*ends)[2], int seg, int nseg, int dir, int len, int q, int qn)
+{
+    const int behind = dir > 0 ? seg : nseg - 1 - seg;
+    uint32_t e0[kMaxSeg], e1[kMaxSeg];
+#pragma unroll
+    for (int k = 0; k < kMaxSeg; k++)
+        if (k < behind) {
+            const int s2 = dir > 0 ? k : nseg - 1 - k;
+            e0[k] = ends[s2 * qn + q][0];
+            e1[k] = ends[s2 * qn + q][1];
+        }
+    const unsigned short big = 0x4000;
+    const u16x2 one = {1, 1};
+    Cols4 cin{u16x2{big, big}, u16x2{big, big}};
+#pragma unroll
+    for (int k = 0; k < kMaxSeg; k++)
+        if (k < behind) {
+            cin = cols4_relax(Cols4{__builtin_bit_cast(u16x2, e0[k]), __builtin_bit_cast(u16x2, e1[k])}, cin,
+                              (unsigned short)(len - 1));
+            cin.e = cin.e + one;
+            cin.o = cin.o + one;
+        }
+    return cin;
+}
+
 // One segment's share of a column relaxation d[i] = min(d[i], d[i -+ 1] + 1)
 // over n cells of LDS dwords p[0], p[stride], ... (4 columns per dword),
 // forward (dir = +1) or backward (dir = -1): first the segment's own chain,
 // its end published in ends[seg]; then, after the block barrier, the value
-// entering the segment from the segments before it (F = min(end, cin +
-// len - 1), cin = F + 1, folded from the far end) and min(cell, cin + offset)
-// over the segment -- what the sequential sweep gives, since its result is
-// min over the cells behind of (d + distance).  Every thread of the block
-// calls it (it holds barriers).
+// entering the segment from the segments before it (fold_carry) and
+// min(cell, cin + offset) over the segment -- what the sequential sweep
+// gives, since its result is min over the cells behind of (d + distance).
+// Every thread of the block calls it (it holds barriers); nseg <= kMaxSeg.
+template <int kMaxSeg>
 __device__ inline void relax_segments(uint32_t* p, int stride, int len, int seg, int nseg, int dir, bool active,
                                       uint32_t (*ends)[2], int q, int qn)
 {
@@ -184,23 +217,69 @@ __device__ inline void relax_segments(uint32_t* p, int stride, int len, int seg,
         ends[seg * qn + q][1] = __builtin_bit_cast(uint32_t, prev.o);
     }
     __syncthreads();
-    if (active) {
-        const unsigned short big = 0x4000;
-        Cols4 cin{u16x2{big, big}, u16x2{big, big}};
-        bool any = false;
-        for (int s2 = dir > 0 ? 0 : nseg - 1; s2 != seg; s2 += dir) {
-            const Cols4 e{__builtin_bit_cast(u16x2, ends[s2 * qn + q][0]), __builtin_bit_cast(u16x2, ends[s2 * qn + q][1])};
-            cin = cols4_relax(e, cin, (unsigned short)(len - 1));
-            const u16x2 one = {1, 1};
-            cin.e = cin.e + one;
-            cin.o = cin.o + one;
-            any = true;
+    if (active && (dir > 0 ? seg > 0 : seg < nseg - 1)) {
+        const Cols4 cin = fold_carry<kMaxSeg>(ends, seg, nseg, dir, len, q, qn);
+        for (int k = 0; k < len; k++) {
+            const int i = first + dir * k;
+            p[i * stride] = cols4_pack(cols4_relax(cols4_of(p[i * stride]), cin, (unsigned short)k));
         }
-        if (any)
-            for (int k = 0; k < len; k++) {
-                const int i = first + dir * k;
-                p[i * stride] = cols4_pack(cols4_relax(cols4_of(p[i * stride]), cin, (unsigned short)k));
+    }
+    __syncthreads();
+}
+
+// Both relaxations (forward, then backward) of one segment of at most kMax
+// cells held in registers (len <= kMax; the caller checks), with the same
+// carries between segments as relax_segments: the segment's cells are read
+// once, relaxed in registers, and written once.  The backward chain runs
+// before the forward carry is applied: over the cells at or after k, the
+// carry's term cin + j + (j - k) is least at j = k, so min(cell, cin + k)
+// may be taken after the backward chain as well as before it; the segment's
+// backward end includes it.  Every thread of the block calls it (it holds
+// barriers); nseg <= kMaxSeg.
+template <int kMax, int kMaxSeg>
+__device__ inline void relax_segments_reg(uint32_t* p, int stride, int len, int seg, int nseg, bool active,
+                                          uint32_t (*ends)[2], int q, int qn)
+{
+    Cols4 r[kMax];
+    const int first = seg * len;
+    const unsigned short big = 0x4000;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < kMax; k++)
+            if (k < len) r[k] = cols4_of(p[(first + k) * stride]);
+        Cols4 prev = r[0];
+#pragma unroll
+        for (int k = 1; k < kMax; k++)
+            if (k < len) {
+                r[k] = cols4_relax(r[k], r[k - 1], 1);
+                prev = r[k];
             }
+        ends[seg * qn + q][0] = __builtin_bit_cast(uint32_t, prev.e);
+        ends[seg * qn + q][1] = __builtin_bit_cast(uint32_t, prev.o);
+    }
+    __syncthreads();
+    Cols4 cin_f{u16x2{big, big}, u16x2{big, big}};
+    if (active) {
+        if (seg > 0) cin_f = fold_carry<kMaxSeg>(ends, seg, nseg, +1, len, q, qn);
+#pragma unroll
+        for (int k = kMax - 2; k >= 0; k--)
+            if (k < len - 1) r[k] = cols4_relax(r[k], r[k + 1], 1);
+    }
+    __syncthreads();   // every forward end read
+    if (active) {
+        const Cols4 e0 = cols4_relax(r[0], cin_f, 0);
+        ends[seg * qn + q][0] = __builtin_bit_cast(uint32_t, e0.e);
+        ends[seg * qn + q][1] = __builtin_bit_cast(uint32_t, e0.o);
+    }
+    __syncthreads();
+    if (active) {
+        Cols4 cin_b{u16x2{big, big}, u16x2{big, big}};
+        if (seg < nseg - 1) cin_b = fold_carry<kMaxSeg>(ends, seg, nseg, -1, len, q, qn);
+#pragma unroll
+        for (int k = 0; k < kMax; k++)
+            if (k < len)
+                p[(first + k) * stride] = cols4_pack(cols4_relax(cols4_relax(r[k], cin_f, (unsigned short)k), cin_b,
+                                                                 (unsigned short)(len - 1 - k)));
     }
     __syncthreads();
 }
@@ -213,37 +292,56 @@ constexpr int kDfCap = 254;
 //
 // X pass: the reference's forward/backward sweep along a row equals
 // min(cap, distance to the nearest solid cell of the row), which one
-// wavefront computes per row with two prefix-max scans (DPP): each lane owns
-// 4 consecutive bytes; the first scan gives the last solid position at or
-// before every byte, whose distance (at most 255) replaces the byte in
-// place -- 0 exactly at solid cells; the second runs over the mirrored row
-// (position nx - 1 - x) and so gives the next solid position at or after
-// every byte, and the row takes min(cap, forward, backward).
-// Y pass: one lane per column, forward and backward relaxation with the
-// running value in a register (LDS reads independent of it).
+// wavefront computes per row in registers.  A lane holds the dword of cells
+// 256 c + 4 l .. + 3 of each 256-cell chunk c; the ballots of "byte k is
+// solid" (k = 0..3) give, per chunk, which lanes hold a solid cell and the
+// highest / lowest solid byte of each.  The nearest solid cell before the
+// lane's dword is then the highest solid lane below it (its highest solid
+// byte), or else the last solid cell of the chunks before; the nearest after
+// it likewise; the four bytes take those or their own solid bytes.  The
+// dwords of several rows are read together, so a wavefront waits on LDS
+// once per batch of rows.  A row with no solid cell is cap everywhere, a row
+// of solid cells 0 everywhere.
+// Y pass: four adjacent columns per lane, the rows split into segments
+// (relax_segments).
 constexpr int kSlabMax = 128 * 1024;
 constexpr int kNone = 1 << 20;
 
-// Inclusive prefix max over the wavefront's 64 lanes with DPP moves: within
-// each row of 16 lanes (row_shr 1, 2, 4, 8; a lane without a source keeps
-// -kNone), then the last lane of rows 0 / 2 into rows 1 / 3 (row_bcast 15),
-// then lane 31 into rows 2 and 3 (row_bcast 31).
-__device__ inline int wave_incl_max(int v)
+// Ballots of "byte k of the lane's dword is solid" over one 256-cell chunk.
+struct ChunkBits {
+    uint64_t b[4];
+    __device__ uint64_t any() const { return b[0] | b[1] | b[2] | b[3]; }
+    // per lane: its highest solid byte is odd / at least 2
+    __device__ uint64_t hi_odd() const { return b[3] | (b[1] & ~b[2]); }
+    __device__ uint64_t hi_ge2() const { return b[2] | b[3]; }
+    // per lane: its lowest solid byte is odd / at least 2
+    __device__ uint64_t lo_odd() const { return (b[1] & ~b[0]) | (b[3] & ~(b[0] | b[1] | b[2])); }
+    __device__ uint64_t lo_ge2() const { return ~(b[0] | b[1]); }
+};
+
+__device__ inline ChunkBits chunk_bits(uint32_t v, bool valid)
 {
-    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x111, 0xf, 0xf, false));   // row_shr:1
-    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x112, 0xf, 0xf, false));   // row_shr:2
-    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x114, 0xf, 0xf, false));   // row_shr:4
-    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x118, 0xf, 0xf, false));   // row_shr:8
-    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x142, 0xa, 0xf, false));   // row_bcast:15
-    v = max(v, __builtin_amdgcn_update_dpp(-kNone, v, 0x143, 0xc, 0xf, false));   // row_bcast:31
-    return v;
+    ChunkBits cb;
+#pragma unroll
+    for (int k = 0; k < 4; k++) cb.b[k] = __ballot(valid && ((v >> (8 * k)) & 0xffu) != 0u);
+    return cb;
 }
 
-// The previous lane's value (wave_shr:1); lane 0 gets -kNone.
-__device__ inline int wave_prev(int v) { return __builtin_amdgcn_update_dpp(-kNone, v, 0x138, 0xf, 0xf, false); }
+// Cell of the highest / lowest solid byte of lane `l` of chunk c (l holds one).
+__device__ inline int chunk_last_cell(const ChunkBits& cb, int c, uint32_t l)
+{
+    return 256 * c + 4 * (int)l + 2 * (int)((cb.hi_ge2() >> l) & 1ull) + (int)((cb.hi_odd() >> l) & 1ull);
+}
+__device__ inline int chunk_first_cell(const ChunkBits& cb, int c, uint32_t l)
+{
+    return 256 * c + 4 * (int)l + 2 * (int)((cb.lo_ge2() >> l) & 1ull) + (int)((cb.lo_odd() >> l) & 1ull);
+}
 
-__global__ void __launch_bounds__(1024) df_xy_slab_kernel(const uint8_t* __restrict__ grid, uint8_t* __restrict__ out,
-                                                         int nx, int ny, int nz, const int32_t* __restrict__ gate)
+// kChunks: 256-cell chunks of a row (nx <= 256 * kChunks).  Two workgroups
+// per CU: at most 64 VGPRs.
+template <int kChunks>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) df_xy_slab_kernel(
+    const uint8_t* __restrict__ grid, uint8_t* __restrict__ out, int nx, int ny, int nz, const int32_t* __restrict__ gate)
 {
     if (gate && !*gate) return;
     extern __shared__ uint8_t slab[];
@@ -257,83 +355,81 @@ __global__ void __launch_bounds__(1024) df_xy_slab_kernel(const uint8_t* __restr
     __syncthreads();
     const int cap = min(kDfCap, nx + ny + nz);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-    const int chunks = (nx + 255) / 256;   // 256 bytes (64 lanes x 4) per chunk, nx <= 1024
-    // ---- X pass, one wavefront per row ----------------------------------
-    for (int y = wave; y < ny; y += waves) {
-        uint8_t* row = slab + y * nx;
-        // a row without a solid cell becomes cap everywhere, a row of solid
-        // cells 0 everywhere (the scans below give exactly that)
-        bool any_solid = false, any_air = false;
-        for (int c = 0; c < chunks; c++) {
-            const int x0 = c * 256 + lane * 4;
-            if (x0 < nx) {
-                const uint32_t v = *reinterpret_cast<const uint32_t*>(row + x0);
-                any_solid = any_solid || v != 0u;
-                any_air = any_air || (((v - 0x01010101u) & ~v & 0x80808080u) != 0u);   // a zero byte
-            }
-        }
-        const bool mixed = __ballot(any_solid) != 0ull && __ballot(any_air) != 0ull;
-        if (!mixed) {
-            const uint32_t fill = __ballot(any_solid) != 0ull ? 0u : (uint32_t)cap * 0x01010101u;
-            for (int c = 0; c < chunks; c++) {
-                const int x0 = c * 256 + lane * 4;
-                if (x0 < nx) *reinterpret_cast<uint32_t*>(row + x0) = fill;
-            }
-            continue;
-        }
-        // forward: distance to the last solid at or before x (255: none),
-        // written in place
-        int carry = -kNone;   // last solid position left of the chunk
-        for (int c = 0; c < chunks; c++) {
-            const int x0 = c * 256 + lane * 4;
-            const uint32_t v = x0 < nx ? *reinterpret_cast<const uint32_t*>(row + x0) : 0u;
-            int last_local = -kNone;
-            int lastpos[4];
+    // ---- X pass, one wavefront per row, rows in batches -----------------
+    constexpr int kBatch = 16 / kChunks;                        // rows whose dwords are read together
+    const uint64_t below = (1ull << lane) - 1ull;               // lanes under this one
+    const uint64_t above = lane == 63 ? 0ull : ~0ull << (lane + 1);
+    for (int y0 = wave; y0 < ny; y0 += waves * kBatch) {
+        uint32_t v[kBatch][kChunks];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if ((v >> (8 * k)) & 0xffu) last_local = x0 + k;
-                lastpos[k] = last_local;
-            }
-            const int incl = wave_incl_max(last_local);
-            const int excl = max(wave_prev(incl), carry);
-            uint32_t outw = 0;
+        for (int r = 0; r < kBatch; r++)
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int l = max(lastpos[k], excl);
-                const int d = l < 0 ? 255 : min(255, x0 + k - l);
-                outw |= (uint32_t)d << (8 * k);
+            for (int c = 0; c < kChunks; c++) {
+                const int y = y0 + r * waves, x0 = 256 * c + 4 * lane;
+                v[r][c] = y < ny && x0 < nx ? *reinterpret_cast<const uint32_t*>(slab + y * nx + x0) : 0u;
             }
-            if (x0 < nx) *reinterpret_cast<uint32_t*>(row + x0) = outw;
-            carry = max(carry, __builtin_amdgcn_readlane(incl, 63));
-        }
-        // backward, over mirrored positions q = nx - 1 - x: a lane's bytes
-        // q0 .. q0 + 3 are the dword at x = nx - 4 - q0, reversed; a solid
-        // is a forward distance of 0
-        carry = -kNone;
-        for (int c = 0; c < chunks; c++) {
-            const int q0 = c * 256 + lane * 4;
-            const int xb = nx - 4 - q0;
-            const uint32_t v = q0 < nx ? *reinterpret_cast<const uint32_t*>(row + xb) : 0xffffffffu;
-            int last_local = -kNone;
-            int lastpos[4], fw[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                fw[k] = (int)((v >> (8 * (3 - k))) & 0xffu);
-                if (fw[k] == 0) last_local = q0 + k;
-                lastpos[k] = last_local;
-            }
-            const int incl = wave_incl_max(last_local);
-            const int excl = max(wave_prev(incl), carry);
-            uint32_t outw = 0;
+        for (int r = 0; r < kBatch; r++) {
+            const int y = y0 + r * waves;
+            if (y >= ny) break;
+            uint8_t* row = slab + y * nx;
+            bool any_solid = false, any_air = false;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int l = max(lastpos[k], excl);
-                const int bwd = l < 0 ? kNone : q0 + k - l;
-                const int d = min(cap, min(fw[k], bwd));
-                outw |= (uint32_t)d << (8 * (3 - k));
+            for (int c = 0; c < kChunks; c++) {
+                if (256 * c + 4 * lane < nx) {
+                    const uint32_t w = v[r][c];
+                    any_solid = any_solid || w != 0u;
+                    any_air = any_air || (((w - 0x01010101u) & ~w & 0x80808080u) != 0u);   // a zero byte
+                }
             }
-            if (q0 < nx) *reinterpret_cast<uint32_t*>(row + xb) = outw;
-            carry = max(carry, __builtin_amdgcn_readlane(incl, 63));
+            const bool solid_row = __ballot(any_solid) != 0ull;
+            if (!solid_row || __ballot(any_air) == 0ull) {
+                const uint32_t fill = solid_row ? 0u : (uint32_t)cap * 0x01010101u;
+#pragma unroll
+                for (int c = 0; c < kChunks; c++)
+                    if (256 * c + 4 * lane < nx) *reinterpret_cast<uint32_t*>(row + 256 * c + 4 * lane) = fill;
+                continue;
+            }
+            // last / first solid cell of each chunk (scalars)
+            int last_of[kChunks], first_of[kChunks];
+#pragma unroll
+            for (int c = 0; c < kChunks; c++) {
+                const ChunkBits cb = chunk_bits(v[r][c], 256 * c + 4 * lane < nx);
+                const uint64_t a = cb.any();
+                last_of[c] = a ? chunk_last_cell(cb, c, 63u - (uint32_t)__clzll((long long)a)) : -kNone;
+                first_of[c] = a ? chunk_first_cell(cb, c, (uint32_t)__ffsll((long long)a) - 1u) : kNone;
+            }
+#pragma unroll
+            for (int c = 0; c < kChunks; c++) {
+                const int x0 = 256 * c + 4 * lane;
+                const uint32_t w = v[r][c];
+                const ChunkBits cb = chunk_bits(w, x0 < nx);
+                int prev_cell = -kNone, next_cell = kNone;   // nearest solid cells outside the lane's dword
+#pragma unroll
+                for (int c2 = 0; c2 < kChunks; c2++) {
+                    if (c2 < c) prev_cell = max(prev_cell, last_of[c2]);
+                    if (c2 > c) next_cell = min(next_cell, first_of[c2]);
+                }
+                const uint64_t mb = cb.any() & below, ma = cb.any() & above;
+                if (mb) prev_cell = chunk_last_cell(cb, c, 63u - (uint32_t)__clzll((long long)mb));
+                if (ma) next_cell = chunk_first_cell(cb, c, (uint32_t)__ffsll((long long)ma) - 1u);
+                int before[4], after[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    prev_cell = ((w >> (8 * k)) & 0xffu) ? x0 + k : prev_cell;
+                    before[k] = prev_cell;
+                }
+#pragma unroll
+                for (int k = 3; k >= 0; k--) {
+                    next_cell = ((w >> (8 * k)) & 0xffu) ? x0 + k : next_cell;
+                    after[k] = next_cell;
+                }
+                uint32_t o = 0u;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    o |= (uint32_t)min(cap, min(x0 + k - before[k], after[k] - x0 - k)) << (8 * k);
+                if (x0 < nx) *reinterpret_cast<uint32_t*>(row + x0) = o;
+            }
         }
     }
     __syncthreads();
@@ -344,12 +440,19 @@ __global__ void __launch_bounds__(1024) df_xy_slab_kernel(const uint8_t* __restr
         __shared__ uint32_t s_end_y[1024][2];
         const int qn = nx / 4;
         int nseg = 8;
-        while (nseg > 1 && (qn * nseg > (int)blockDim.x || ny % nseg != 0)) nseg >>= 1;
+        while (nseg > 1 && (qn * nseg > 1024 || ny % nseg != 0)) nseg >>= 1;
+        // uniform values in scalar registers, so the segment loops branch on them
+        nseg = __builtin_amdgcn_readfirstlane(nseg);
+        const int len = __builtin_amdgcn_readfirstlane(ny / nseg);
         const bool active = (int)threadIdx.x < qn * nseg;
         const int q = active ? (int)threadIdx.x % qn : 0, sg = active ? (int)threadIdx.x / qn : 0;
         uint32_t* col = reinterpret_cast<uint32_t*>(slab) + q;
-        relax_segments(col, qn, ny / nseg, sg, nseg, +1, active, s_end_y, q, qn);
-        relax_segments(col, qn, ny / nseg, sg, nseg, -1, active, s_end_y, q, qn);
+        if (len <= 16) {
+            relax_segments_reg<16, 8>(col, qn, len, sg, nseg, active, s_end_y, q, qn);
+        } else {
+            relax_segments<8>(col, qn, len, sg, nseg, +1, active, s_end_y, q, qn);
+            relax_segments<8>(col, qn, len, sg, nseg, -1, active, s_end_y, q, qn);
+        }
     }
     for (int i = threadIdx.x * 16; i < bytes; i += blockDim.x * 16)
         *reinterpret_cast<uint4*>(out + base + i) = *reinterpret_cast<const uint4*>(slab + i);
@@ -396,8 +499,8 @@ __global__ void __launch_bounds__(NT) df_z_tile_kernel(const uint8_t* __restrict
         __shared__ uint32_t s_end[NT][2];
         const int q = (int)(threadIdx.x % kQ), sg = (int)(threadIdx.x / kQ);   // dword of a z layer, segment
         uint32_t* col = reinterpret_cast<uint32_t*>(tile) + q;
-        relax_segments(col, kQ, nz / kSeg, sg, kSeg, +1, true, s_end, q, kQ);
-        relax_segments(col, kQ, nz / kSeg, sg, kSeg, -1, true, s_end, q, kQ);
+        relax_segments<kSeg>(col, kQ, nz / kSeg, sg, kSeg, +1, true, s_end, q, kQ);
+        relax_segments<kSeg>(col, kQ, nz / kSeg, sg, kSeg, -1, true, s_end, q, kQ);
     }
     // write whole bricks: brick (bx, bz) of this tile, 512 bytes each, one
     // 16-byte group per lane: bytes m..m+15 of the Morton order
@@ -720,10 +823,14 @@ int df_build_launch(vpt_world* w, hipStream_t s, const int32_t* gate)
     const int nx = w->nx, ny = w->ny, nz = w->nz;
     const size_t slab = (size_t)nx * ny;
     if (slab <= (size_t)kSlabMax) {
-        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&df_xy_slab_kernel),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, kSlabMax));
-        hipLaunchKernelGGL(df_xy_slab_kernel, dim3(nz), dim3(1024), slab, s, w->d_grid_lin, w->d_tmp, nx, ny, nz,
-                           gate);
+        const int chunks = (nx + 255) / 256;
+        auto kern = chunks == 1   ? df_xy_slab_kernel<1>
+                    : chunks == 2 ? df_xy_slab_kernel<2>
+                    : chunks == 3 ? df_xy_slab_kernel<3>
+                                  : df_xy_slab_kernel<4>;
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    kSlabMax));
+        hipLaunchKernelGGL(kern, dim3(nz), dim3(1024), slab, s, w->d_grid_lin, w->d_tmp, nx, ny, nz, gate);
         HIP_TRY(hipGetLastError());
     } else {
         size_t rows = (size_t)ny * nz;
