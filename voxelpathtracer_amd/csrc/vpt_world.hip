@@ -166,30 +166,30 @@ __device__ inline Cols4 cols4_relax(Cols4 d, Cols4 c, unsigned short k)
 // The value entering segment `seg` of a forward (dir = +1) or backward
 // (dir = -1) relaxation, folded from the published ends of the segments
 // behind it in sweep order: F = min(end, cin + len - 1), cin = F + 1.  The
-// ends are read together (at most kMaxSeg segments), then folded.
+// ends of all kMaxSeg slots are read together (slots past the segments read
+// a valid entry that is not used), then folded.
 template <int kMaxSeg>
-__device__ inline Cols4 fold_carry(const uint32_t (*ends)[2], int seg, int nseg, int dir, int len, int q, int qn)
+__device__ __attribute__((always_inline)) inline Cols4 fold_carry(const uint32_t (*ends)[2], int seg, int nseg, int dir, int len, int q, int qn)
 {
     const int behind = dir > 0 ? seg : nseg - 1 - seg;
     uint32_t e0[kMaxSeg], e1[kMaxSeg];
 #pragma unroll
-    for (int k = 0; k < kMaxSeg; k++)
-        if (k < behind) {
-            const int s2 = dir > 0 ? k : nseg - 1 - k;
-            e0[k] = ends[s2 * qn + q][0];
-            e1[k] = ends[s2 * qn + q][1];
-        }
+    for (int k = 0; k < kMaxSeg; k++) {
+        const int s2 = max(0, dir > 0 ? k : nseg - 1 - k);
+        e0[k] = ends[s2 * qn + q][0];
+        e1[k] = ends[s2 * qn + q][1];
+    }
     const unsigned short big = 0x4000;
     const u16x2 one = {1, 1};
     Cols4 cin{u16x2{big, big}, u16x2{big, big}};
 #pragma unroll
-    for (int k = 0; k < kMaxSeg; k++)
-        if (k < behind) {
-            cin = cols4_relax(Cols4{__builtin_bit_cast(u16x2, e0[k]), __builtin_bit_cast(u16x2, e1[k])}, cin,
+    for (int k = 0; k < kMaxSeg; k++) {
+        Cols4 f = cols4_relax(Cols4{__builtin_bit_cast(u16x2, e0[k]), __builtin_bit_cast(u16x2, e1[k])}, cin,
                               (unsigned short)(len - 1));
-            cin.e = cin.e + one;
-            cin.o = cin.o + one;
-        }
+        f.e = f.e + one;
+        f.o = f.o + one;
+        if (k < behind) cin = f;
+    }
     return cin;
 }
 
@@ -202,7 +202,7 @@ __device__ inline Cols4 fold_carry(const uint32_t (*ends)[2], int seg, int nseg,
 // gives, since its result is min over the cells behind of (d + distance).
 // Every thread of the block calls it (it holds barriers); nseg <= kMaxSeg.
 template <int kMaxSeg>
-__device__ inline void relax_segments(uint32_t* p, int stride, int len, int seg, int nseg, int dir, bool active,
+__device__ __attribute__((always_inline)) inline void relax_segments(uint32_t* p, int stride, int len, int seg, int nseg, int dir, bool active,
                                       uint32_t (*ends)[2], int q, int qn)
 {
     const int first = dir > 0 ? seg * len : (seg + 1) * len - 1;
@@ -227,43 +227,35 @@ __device__ inline void relax_segments(uint32_t* p, int stride, int len, int seg,
     __syncthreads();
 }
 
-// Both relaxations (forward, then backward) of one segment of at most kMax
-// cells held in registers (len <= kMax; the caller checks), with the same
-// carries between segments as relax_segments: the segment's cells are read
-// once, relaxed in registers, and written once.  The backward chain runs
-// before the forward carry is applied: over the cells at or after k, the
-// carry's term cin + j + (j - k) is least at j = k, so min(cell, cin + k)
-// may be taken after the backward chain as well as before it; the segment's
-// backward end includes it.  Every thread of the block calls it (it holds
-// barriers); nseg <= kMaxSeg.
-template <int kMax, int kMaxSeg>
-__device__ inline void relax_segments_reg(uint32_t* p, int stride, int len, int seg, int nseg, bool active,
+// Both relaxations (forward, then backward) of one segment of exactly kLen
+// cells held in registers, with the same carries between segments as
+// relax_segments: the segment's cells are read once, relaxed in registers,
+// and written once.  The backward chain runs before the forward carry is
+// applied: over the cells at or after k, the carry's term cin + j + (j - k)
+// is least at j = k, so min(cell, cin + k) may be taken after the backward
+// chain as well as before it; the segment's backward end includes it.
+// Every thread of the block calls it (it holds barriers); nseg <= kMaxSeg.
+template <int kLen, int kMaxSeg>
+__device__ __attribute__((always_inline)) inline void relax_segments_reg(uint32_t* p, int stride, int seg, int nseg, bool active,
                                           uint32_t (*ends)[2], int q, int qn)
 {
-    Cols4 r[kMax];
-    const int first = seg * len;
+    Cols4 r[kLen];
+    const int first = seg * kLen;
     const unsigned short big = 0x4000;
     if (active) {
 #pragma unroll
-        for (int k = 0; k < kMax; k++)
-            if (k < len) r[k] = cols4_of(p[(first + k) * stride]);
-        Cols4 prev = r[0];
+        for (int k = 0; k < kLen; k++) r[k] = cols4_of(p[(first + k) * stride]);
 #pragma unroll
-        for (int k = 1; k < kMax; k++)
-            if (k < len) {
-                r[k] = cols4_relax(r[k], r[k - 1], 1);
-                prev = r[k];
-            }
-        ends[seg * qn + q][0] = __builtin_bit_cast(uint32_t, prev.e);
-        ends[seg * qn + q][1] = __builtin_bit_cast(uint32_t, prev.o);
+        for (int k = 1; k < kLen; k++) r[k] = cols4_relax(r[k], r[k - 1], 1);
+        ends[seg * qn + q][0] = __builtin_bit_cast(uint32_t, r[kLen - 1].e);
+        ends[seg * qn + q][1] = __builtin_bit_cast(uint32_t, r[kLen - 1].o);
     }
     __syncthreads();
     Cols4 cin_f{u16x2{big, big}, u16x2{big, big}};
     if (active) {
-        if (seg > 0) cin_f = fold_carry<kMaxSeg>(ends, seg, nseg, +1, len, q, qn);
+        cin_f = fold_carry<kMaxSeg>(ends, seg, nseg, +1, kLen, q, qn);
 #pragma unroll
-        for (int k = kMax - 2; k >= 0; k--)
-            if (k < len - 1) r[k] = cols4_relax(r[k], r[k + 1], 1);
+        for (int k = kLen - 2; k >= 0; k--) r[k] = cols4_relax(r[k], r[k + 1], 1);
     }
     __syncthreads();   // every forward end read
     if (active) {
@@ -273,13 +265,11 @@ __device__ inline void relax_segments_reg(uint32_t* p, int stride, int len, int 
     }
     __syncthreads();
     if (active) {
-        Cols4 cin_b{u16x2{big, big}, u16x2{big, big}};
-        if (seg < nseg - 1) cin_b = fold_carry<kMaxSeg>(ends, seg, nseg, -1, len, q, qn);
+        const Cols4 cin_b = fold_carry<kMaxSeg>(ends, seg, nseg, -1, kLen, q, qn);
 #pragma unroll
-        for (int k = 0; k < kMax; k++)
-            if (k < len)
-                p[(first + k) * stride] = cols4_pack(cols4_relax(cols4_relax(r[k], cin_f, (unsigned short)k), cin_b,
-                                                                 (unsigned short)(len - 1 - k)));
+        for (int k = 0; k < kLen; k++)
+            p[(first + k) * stride] = cols4_pack(
+                cols4_relax(cols4_relax(r[k], cin_f, (unsigned short)k), cin_b, (unsigned short)(kLen - 1 - k)));
     }
     __syncthreads();
 }
@@ -305,6 +295,7 @@ constexpr int kDfCap = 254;
 // Y pass: four adjacent columns per lane, the rows split into segments
 // (relax_segments).
 constexpr int kSlabMax = 128 * 1024;
+constexpr int kSlabThreads = 1024;   // threads of a slab workgroup (the launch uses this)
 constexpr int kNone = 1 << 20;
 
 // Ballots of "byte k of the lane's dword is solid" over one 256-cell chunk.
@@ -340,7 +331,7 @@ __device__ inline int chunk_first_cell(const ChunkBits& cb, int c, uint32_t l)
 // kChunks: 256-cell chunks of a row (nx <= 256 * kChunks).  Two workgroups
 // per CU: at most 64 VGPRs.
 template <int kChunks>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) df_xy_slab_kernel(
+__global__ void __launch_bounds__(kSlabThreads) __attribute__((amdgpu_waves_per_eu(8))) df_xy_slab_kernel(
     const uint8_t* __restrict__ grid, uint8_t* __restrict__ out, int nx, int ny, int nz, const int32_t* __restrict__ gate)
 {
     if (gate && !*gate) return;
@@ -350,11 +341,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
     const int bytes = nx * ny;
     // unrolled so that a wavefront has several loads in flight per wait
 #pragma unroll 4
-    for (int i = threadIdx.x * 16; i < bytes; i += blockDim.x * 16)
+    for (int i = threadIdx.x * 16; i < bytes; i += kSlabThreads * 16)
         *reinterpret_cast<uint4*>(slab + i) = *reinterpret_cast<const uint4*>(grid + base + i);
     __syncthreads();
     const int cap = min(kDfCap, nx + ny + nz);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = kSlabThreads / 64;
     // ---- X pass, one wavefront per row, rows in batches -----------------
     constexpr int kBatch = 16 / kChunks;                        // rows whose dwords are read together
     const uint64_t below = (1ull << lane) - 1ull;               // lanes under this one
@@ -437,24 +428,28 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
     // lane (one LDS dword per row), the rows split into segments
     // (relax_segments); every thread takes part in its barriers
     {
-        __shared__ uint32_t s_end_y[1024][2];
+        __shared__ uint32_t s_end_y[kSlabThreads][2];
         const int qn = nx / 4;
         int nseg = 8;
-        while (nseg > 1 && (qn * nseg > 1024 || ny % nseg != 0)) nseg >>= 1;
+        if (ny % 16 == 0 && ny / 16 <= 8 && qn * (ny / 16) <= kSlabThreads) {
+            nseg = ny / 16;   // 16-row segments, relaxed in registers
+        } else {
+            while (nseg > 1 && (qn * nseg > kSlabThreads || ny % nseg != 0)) nseg >>= 1;
+        }
         // uniform values in scalar registers, so the segment loops branch on them
         nseg = __builtin_amdgcn_readfirstlane(nseg);
         const int len = __builtin_amdgcn_readfirstlane(ny / nseg);
         const bool active = (int)threadIdx.x < qn * nseg;
         const int q = active ? (int)threadIdx.x % qn : 0, sg = active ? (int)threadIdx.x / qn : 0;
         uint32_t* col = reinterpret_cast<uint32_t*>(slab) + q;
-        if (len <= 16) {
-            relax_segments_reg<16, 8>(col, qn, len, sg, nseg, active, s_end_y, q, qn);
+        if (len == 16) {
+            relax_segments_reg<16, 8>(col, qn, sg, nseg, active, s_end_y, q, qn);
         } else {
             relax_segments<8>(col, qn, len, sg, nseg, +1, active, s_end_y, q, qn);
             relax_segments<8>(col, qn, len, sg, nseg, -1, active, s_end_y, q, qn);
         }
     }
-    for (int i = threadIdx.x * 16; i < bytes; i += blockDim.x * 16)
+    for (int i = threadIdx.x * 16; i < bytes; i += kSlabThreads * 16)
         *reinterpret_cast<uint4*>(out + base + i) = *reinterpret_cast<const uint4*>(slab + i);
 }
 
@@ -469,7 +464,7 @@ constexpr int kZTileMax = 128 * 1024;
 constexpr int kZThreads = 512;   // threads of a 16-wide tile
 
 template <int TW, int NT>
-__global__ void __launch_bounds__(NT) df_z_tile_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ df_brick,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) df_z_tile_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ df_brick,
                                                         int nx, int ny, int nz, uint32_t row_stride,
                                                         uint32_t slab_stride, const int32_t* __restrict__ gate)
 {
@@ -489,7 +484,7 @@ __global__ void __launch_bounds__(NT) df_z_tile_kernel(const uint8_t* __restrict
     // stage: 8 rows of TW bytes per z, 16-byte loads; unrolled so that a
     // wavefront has several loads in flight per wait
 #pragma unroll 8
-    for (int i = threadIdx.x; i < nz * 8 * kVec; i += blockDim.x) {
+    for (int i = threadIdx.x; i < nz * 8 * kVec; i += NT) {
         const int z = i / (8 * kVec), r = (i / kVec) & 7, h = i % kVec;
         *reinterpret_cast<uint4*>(tile + z * kRow + r * TW + h * 16) =
             *reinterpret_cast<const uint4*>(in + (size_t)z * plane + (size_t)(y0 + r) * nx + x0 + h * 16);
@@ -499,13 +494,17 @@ __global__ void __launch_bounds__(NT) df_z_tile_kernel(const uint8_t* __restrict
         __shared__ uint32_t s_end[NT][2];
         const int q = (int)(threadIdx.x % kQ), sg = (int)(threadIdx.x / kQ);   // dword of a z layer, segment
         uint32_t* col = reinterpret_cast<uint32_t*>(tile) + q;
-        relax_segments<kSeg>(col, kQ, nz / kSeg, sg, kSeg, +1, true, s_end, q, kQ);
-        relax_segments<kSeg>(col, kQ, nz / kSeg, sg, kSeg, -1, true, s_end, q, kQ);
+        if (nz / kSeg == 24) {
+            relax_segments_reg<24, kSeg>(col, kQ, sg, kSeg, true, s_end, q, kQ);
+        } else {
+            relax_segments<kSeg>(col, kQ, nz / kSeg, sg, kSeg, +1, true, s_end, q, kQ);
+            relax_segments<kSeg>(col, kQ, nz / kSeg, sg, kSeg, -1, true, s_end, q, kQ);
+        }
     }
     // write whole bricks: brick (bx, bz) of this tile, 512 bytes each, one
     // 16-byte group per lane: bytes m..m+15 of the Morton order
     const int nbz = nz / 8;
-    for (int i = threadIdx.x; i < (TW / 8) * nbz * 32; i += blockDim.x) {
+    for (int i = threadIdx.x; i < (TW / 8) * nbz * 32; i += NT) {
         const int g = i & 31, brick = i >> 5;       // 32 groups of 16 bytes per brick
         const int bxl = brick % (TW / 8), bz = brick / (TW / 8);
         uint32_t words[4] = {0u, 0u, 0u, 0u};
@@ -830,7 +829,7 @@ int df_build_launch(vpt_world* w, hipStream_t s, const int32_t* gate)
                                   : df_xy_slab_kernel<4>;
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     kSlabMax));
-        hipLaunchKernelGGL(kern, dim3(nz), dim3(1024), slab, s, w->d_grid_lin, w->d_tmp, nx, ny, nz, gate);
+        hipLaunchKernelGGL(kern, dim3(nz), dim3(kSlabThreads), slab, s, w->d_grid_lin, w->d_tmp, nx, ny, nz, gate);
         HIP_TRY(hipGetLastError());
     } else {
         size_t rows = (size_t)ny * nz;
