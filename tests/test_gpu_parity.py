@@ -55,6 +55,7 @@ def _world(ctx, grid):
     ((1008, 16, 8), 0.005, 9),
     ((1024, 8, 16), 0.3, 10),
     ((128, 512, 8), 0.01, 12),
+    ((64, 16, 256), 0.02, 13),     # 16-layer Z segments
 ])
 def test_df_random_worlds(ctx, shape, density, seed):
     rng = np.random.default_rng(seed)

@@ -494,8 +494,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) df
         __shared__ uint32_t s_end[NT][2];
         const int q = (int)(threadIdx.x % kQ), sg = (int)(threadIdx.x / kQ);   // dword of a z layer, segment
         uint32_t* col = reinterpret_cast<uint32_t*>(tile) + q;
+        // segments of 8, 16 or 24 layers are relaxed in registers
         if (nz / kSeg == 24) {
             relax_segments_reg<24, kSeg>(col, kQ, sg, kSeg, true, s_end, q, kQ);
+        } else if (nz / kSeg == 16) {
+            relax_segments_reg<16, kSeg>(col, kQ, sg, kSeg, true, s_end, q, kQ);
+        } else if (nz / kSeg == 8) {
+            relax_segments_reg<8, kSeg>(col, kQ, sg, kSeg, true, s_end, q, kQ);
         } else {
             relax_segments<kSeg>(col, kQ, nz / kSeg, sg, kSeg, +1, true, s_end, q, kQ);
             relax_segments<kSeg>(col, kQ, nz / kSeg, sg, kSeg, -1, true, s_end, q, kQ);
