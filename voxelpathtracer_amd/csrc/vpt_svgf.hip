@@ -698,20 +698,26 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
     Tx base = src.tex_at(i);
     float base_lum = sh_to_y(base.sh[3]);
 
-    // GaussianVariance (:94-129) over the previous pass's variance
+    // GaussianVariance (:94-129) over the previous pass's variance.  The
+    // eight neighbours are read first (their wrapped indices are always in
+    // the image), so the reads are in flight together; the sums then take
+    // the in-screen ones, in the same order.
+    float nv[9];
+#pragma unroll
+    for (int x = -1; x <= 1; x++)
+#pragma unroll
+        for (int y = -1; y <= 1; y++) nv[(x + 1) * 3 + y + 1] = (x == 0 && y == 0) ? base.x : src.xch(px + x, py + y);
     float base_var = 0.0f, vsum = 0.0f, ksum = 0.0f;
 #pragma unroll
     for (int x = -1; x <= 1; x++) {
 #pragma unroll
         for (int y = -1; y <= 1; y++) {
-            float sx = uvx + (float)x * tex_w;
-            float sy = uvy + (float)y * tex_h;
-            if (!in_screen(sx, sy)) continue;
-            float kv = gauss[abs(x)] * gauss[abs(y)];
-            float v = (x == 0 && y == 0) ? base.x : src.xch(px + x, py + y);
-            if (x == 0 && y == 0) base_var = v;
-            vsum = vsum + v * kv;
-            ksum = ksum + kv;
+            const bool in = in_screen(uvx + (float)x * tex_w, uvy + (float)y * tex_h);
+            const float kv = gauss[abs(x)] * gauss[abs(y)];
+            const float v = nv[(x + 1) * 3 + y + 1];
+            if (x == 0 && y == 0) base_var = in ? v : base_var;
+            vsum = in ? vsum + v * kv : vsum;
+            ksum = in ? ksum + kv : ksum;
         }
     }
     float var_est = vsum / fmaxf(ksum, 0.01f);
