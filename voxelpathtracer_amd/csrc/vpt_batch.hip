@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(256) persistent_kernel(WorldView w0, const Wor
         uint32_t off = 0;
         vpt::Cell cell{};
         const bool ld = s.active && vpt::trav_prepare(w, s.r, off, cell);
-        const int m = ld ? (int)w.df[off] : 0;
+        const int m = ld ? (int)vpt::plane_byte(w.df, off) : 0;
         if (s.active && !(ld && vpt::trav_advance(s.r, m, cell)))
             finish_slot<SRC>(s, w, n, out, fc, n_samples, n_rays);
     }

@@ -28,6 +28,16 @@ struct WorldView {
     float xmax, ymax, zmax; // dims - 1 as float (IsInVolume bounds)
 };
 
+// Byte `off` of a world plane (grid or DF).  The planes are device
+// allocations, so the read is a global one also where the view was loaded
+// from memory (a served world) rather than passed as a kernel argument,
+// which would otherwise make it a flat access.
+__device__ inline uint8_t plane_byte(const uint8_t* p, uint32_t off)
+{
+    typedef const __attribute__((address_space(1))) uint8_t GlobalByte;
+    return ((GlobalByte*)p)[off];   // address-space cast
+}
+
 // Byte offset of cell (x, y, z) in a bricked plane: the three axes
 // contribute independent terms.
 __host__ __device__ inline uint32_t brick_offset(uint32_t x, uint32_t y, uint32_t z, uint32_t row_stride,
@@ -310,7 +320,7 @@ __device__ __attribute__((always_inline)) inline Hit trav_finish(const WorldView
         float fx = floorf(r.ox), fy = floorf(r.oy), fz = floorf(r.oz);
         if (fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax) {
             int cx = (int)fx, cy = (int)fy, cz = (int)fz;
-            int b = w.grid[brick_offset_fast(cx, cy, cz, w.row_stride, w.slab_stride)];
+            int b = plane_byte(w.grid, brick_offset_fast(cx, cy, cz, w.row_stride, w.slab_stride));
             if (b > 0) {
                 float dx = r.ox - r.o0x, dy = r.oy - r.o0y, dz = r.oz - r.o0z;
                 h.t = sqrtf((dx * dx + dy * dy) + dz * dz);
@@ -338,7 +348,7 @@ __device__ inline Hit traverse(const WorldView& w, Vec3 o, Vec3 d, int max_iter)
     // the cap is tested once here and then where trav_advance returns
     if (max_iter > 0) {
         while (trav_locate(w, r, off, c)) {
-            if (!trav_advance(r, w.df[off], c)) break;
+            if (!trav_advance(r, plane_byte(w.df, off), c)) break;
         }
     }
     return trav_finish(w, r);
@@ -349,8 +359,8 @@ __device__ inline int voxel_at(const WorldView& w, Vec3 p)
 {
     float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
     if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx <= w.xmax && fy <= w.ymax && fz <= w.zmax)) return 0;
-    return w.grid[brick_offset_fast((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, w.row_stride,
-                                    w.slab_stride)];
+    return plane_byte(w.grid, brick_offset_fast((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz,
+                                                w.row_stride, w.slab_stride));
 }
 
 // Hardware binary16 conversions (v_cvt_f32_f16 is exact; v_cvt_f16_f32

@@ -571,7 +571,7 @@ __global__ void __launch_bounds__(256) gi_bounce_kernel(WorldView w, CamView cam
             uint32_t off;
             vpt::Cell c;
             while (vpt::trav_locate(w, r, off, c)) {
-                const int m = w.df[off];
+                const int m = vpt::plane_byte(w.df, off);
                 if (!vpt::trav_advance(r, m, c)) {
                     capped = m != 0;
                     break;
@@ -625,7 +625,7 @@ __global__ void __launch_bounds__(256) gi_cont_kernel(WorldView w, CamView cam, 
         uint32_t off;
         vpt::Cell c;
         while (vpt::trav_locate(w, r, off, c)) {
-            if (!vpt::trav_advance(r, w.df[off], c)) break;
+            if (!vpt::trav_advance(r, vpt::plane_byte(w.df, off), c)) break;
         }
         const vpt::Hit h = vpt::trav_finish(w, r);
         samples = h.samples - kGiPhase1;   // the first kGiPhase1 were counted by the bounce kernel
