@@ -655,6 +655,29 @@ struct GlobalSrc {
     }
 };
 
+// The same source with the AO / sky bytes of a texel decoded through a
+// 256-entry table of vpt_unorm8_decode in LDS (one LDS read in place of the
+// three VALU operations of the exact decode; the table holds the same bits).
+struct GlobalSrcLut : GlobalSrc {
+    const float* unorm8;   // LDS
+    __device__ Tx tex_at(uint32_t j) const
+    {
+        const uint4 v = *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(in) + j * 16u);
+        Tx t;
+        t.sh[0] = h2f(v.x & 0xffffu);
+        t.sh[1] = h2f(v.x >> 16);
+        t.sh[2] = h2f(v.y & 0xffffu);
+        t.sh[3] = h2f(v.y >> 16);
+        t.co = h2f(v.z & 0xffffu);
+        t.cg = h2f(v.z >> 16);
+        t.x = h2f(v.w & 0xffffu);
+        typedef const __attribute__((address_space(3))) float LdsFloat;
+        t.ao = ((LdsFloat*)unorm8)[(v.w >> 16) & 0xffu];   // address-space cast: the table is in LDS
+        t.sky = ((LdsFloat*)unorm8)[v.w >> 24];
+        return t;
+    }
+};
+
 // Normal-weight class of a face byte: the six faces, or 6 for every other
 // value (GetNormalFromID's (1, 1, 1)).
 __device__ inline uint32_t face_class(int f) { return f <= 5 ? (uint32_t)f : 6u; }
@@ -823,9 +846,21 @@ __global__ void __launch_bounds__(256) atrous_kernel(int W, int H, const float* 
                                                      vpt_svgf_texel* __restrict__ out)
 {
     int px, py;
-    if (!pixel_of(W, H, px, py)) return;
-    const GlobalSrc src{in, gb_t, gb_face, W, H};
-    atrous_px<GlobalSrc, kMode>(src, W, H, px, py, tutil, step, scale, color_phi_bias, time, out);
+    if constexpr (kMode == 2) {
+        if (!pixel_of(W, H, px, py)) return;
+        const GlobalSrc src{in, gb_t, gb_face, W, H};
+        atrous_px<GlobalSrc, kMode>(src, W, H, px, py, tutil, step, scale, color_phi_bias, time, out);
+    } else {
+        // the passes that filter AO or sky decode their bytes through a table
+        __shared__ float s_unorm8[256];
+        s_unorm8[threadIdx.x] = vpt_unorm8_decode((uint8_t)threadIdx.x);   // 256 threads
+        __syncthreads();
+        if (!pixel_of(W, H, px, py)) return;
+        GlobalSrcLut src;
+        static_cast<GlobalSrc&>(src) = GlobalSrc{in, gb_t, gb_face, W, H};
+        src.unorm8 = s_unorm8;
+        atrous_px<GlobalSrcLut, kMode>(src, W, H, px, py, tutil, step, scale, color_phi_bias, time, out);
+    }
 }
 
 // ---- SHToIrridiance(shY, CoCg, v) (ColorPassFrag.glsl:356-368) ------------
