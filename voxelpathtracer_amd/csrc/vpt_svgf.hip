@@ -100,6 +100,32 @@ __device__ inline Tx load_tx_word(uint4 v)
 
 __device__ inline Tx load_tx(const vpt_svgf_texel* p, size_t i) { return load_tx_word(*reinterpret_cast<const uint4*>(p + i)); }
 
+// The same decode with the AO / sky bytes read from a 256-entry table of
+// vpt_unorm8_decode in LDS (one LDS read in place of the three VALU
+// operations of the exact decode; the table holds the same bits).
+__device__ inline Tx load_tx_word_lut(uint4 v, const float* unorm8)
+{
+    Tx t;
+    t.sh[0] = h2f(v.x & 0xffffu);
+    t.sh[1] = h2f(v.x >> 16);
+    t.sh[2] = h2f(v.y & 0xffffu);
+    t.sh[3] = h2f(v.y >> 16);
+    t.co = h2f(v.z & 0xffffu);
+    t.cg = h2f(v.z >> 16);
+    t.x = h2f(v.w & 0xffffu);
+    typedef const __attribute__((address_space(3))) float LdsFloat;
+    t.ao = ((LdsFloat*)unorm8)[(v.w >> 16) & 0xffu];   // address-space cast: the table is in LDS
+    t.sky = ((LdsFloat*)unorm8)[v.w >> 24];
+    return t;
+}
+
+// Fills the table (a block of at least 256 threads; the caller then waits
+// at a barrier).
+__device__ inline void fill_unorm8_table(float* unorm8)
+{
+    if (threadIdx.x < 256) unorm8[threadIdx.x] = vpt_unorm8_decode((uint8_t)threadIdx.x);
+}
+
 __device__ inline void store_tx(vpt_svgf_texel* p, size_t i, const float sh[4], float co, float cg, float x, float ao,
                                 float sky)
 {
@@ -153,9 +179,8 @@ __device__ inline float bil(float t00, float t10, float t01, float t11, float a,
     return top * (1.0f - b) + bot * b;
 }
 
-__device__ inline Tx bil_tx(const vpt_svgf_texel* p, const Bil& f)
+__device__ inline Tx bil_of(const Tx& q00, const Tx& q10, const Tx& q01, const Tx& q11, const Bil& f)
 {
-    Tx q00 = load_tx(p, f.i00), q10 = load_tx(p, f.i10), q01 = load_tx(p, f.i01), q11 = load_tx(p, f.i11);
     Tx r;
     for (int c = 0; c < 4; c++) r.sh[c] = bil(q00.sh[c], q10.sh[c], q01.sh[c], q11.sh[c], f.a, f.b);
     r.co = bil(q00.co, q10.co, q01.co, q11.co, f.a, f.b);
@@ -164,6 +189,18 @@ __device__ inline Tx bil_tx(const vpt_svgf_texel* p, const Bil& f)
     r.ao = bil(q00.ao, q10.ao, q01.ao, q11.ao, f.a, f.b);
     r.sky = bil(q00.sky, q10.sky, q01.sky, q11.sky, f.a, f.b);
     return r;
+}
+
+__device__ inline Tx bil_tx(const vpt_svgf_texel* p, const Bil& f)
+{
+    return bil_of(load_tx(p, f.i00), load_tx(p, f.i10), load_tx(p, f.i01), load_tx(p, f.i11), f);
+}
+
+__device__ inline Tx bil_tx_lut(const vpt_svgf_texel* p, const Bil& f, const float* unorm8)
+{
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+    return bil_of(load_tx_word_lut(q[f.i00], unorm8), load_tx_word_lut(q[f.i10], unorm8),
+                  load_tx_word_lut(q[f.i01], unorm8), load_tx_word_lut(q[f.i11], unorm8), f);
 }
 
 // GL NEAREST + REPEAT texel index for coordinates inside (0, 1), where
@@ -267,6 +304,9 @@ __global__ void __launch_bounds__(256) temporal_kernel(CamView cam, Mat16 prev_v
                                                        const uint2* __restrict__ hist_util,
                                                        vpt_svgf_texel* __restrict__ out, uint2* __restrict__ out_util)
 {
+    __shared__ float s_unorm8[256];   // AO / sky bytes decoded through a table
+    fill_unorm8_table(s_unorm8);
+    __syncthreads();
     int px, py;
     if (!pixel_of(W, H, px, py)) return;
     const size_t i = (size_t)py * W + px;
@@ -279,7 +319,7 @@ __global__ void __launch_bounds__(256) temporal_kernel(CamView cam, Mat16 prev_v
     float base_w = cur_t[i];
     Vec3 base_pos = position_from(cam, uvx, uvy, base_w);
     Vec3 base_n = vpt::face_normal(cur_face[i], 1.0f);
-    Tx base = load_tx(cur, i);
+    Tx base = load_tx_word_lut(*reinterpret_cast<const uint4*>(cur + i), s_unorm8);
 
     float q0 = vpt::mat_row(prev_vp.m, 0, base_pos.x, base_pos.y, base_pos.z, 1.0f);
     float q1 = vpt::mat_row(prev_vp.m, 1, base_pos.x, base_pos.y, base_pos.z, 1.0f);
@@ -328,7 +368,7 @@ __global__ void __launch_bounds__(256) temporal_kernel(CamView cam, Mat16 prev_v
         }
         if (valid) {
             float cw = weights[k];
-            Tx h = bil_tx(hist, f);
+            Tx h = bil_tx_lut(hist, f, s_unorm8);
             uint2 u00 = hist_util[f.i00], u10 = hist_util[f.i10], u01 = hist_util[f.i01], u11 = hist_util[f.i11];
             float spp = bil(h2f(u00.x & 0xffffu), h2f(u10.x & 0xffffu), h2f(u01.x & 0xffffu), h2f(u11.x & 0xffffu),
                             f.a, f.b);
@@ -655,26 +695,12 @@ struct GlobalSrc {
     }
 };
 
-// The same source with the AO / sky bytes of a texel decoded through a
-// 256-entry table of vpt_unorm8_decode in LDS (one LDS read in place of the
-// three VALU operations of the exact decode; the table holds the same bits).
+// The same source with the AO / sky bytes decoded through the LDS table.
 struct GlobalSrcLut : GlobalSrc {
     const float* unorm8;   // LDS
     __device__ Tx tex_at(uint32_t j) const
     {
-        const uint4 v = *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(in) + j * 16u);
-        Tx t;
-        t.sh[0] = h2f(v.x & 0xffffu);
-        t.sh[1] = h2f(v.x >> 16);
-        t.sh[2] = h2f(v.y & 0xffffu);
-        t.sh[3] = h2f(v.y >> 16);
-        t.co = h2f(v.z & 0xffffu);
-        t.cg = h2f(v.z >> 16);
-        t.x = h2f(v.w & 0xffffu);
-        typedef const __attribute__((address_space(3))) float LdsFloat;
-        t.ao = ((LdsFloat*)unorm8)[(v.w >> 16) & 0xffu];   // address-space cast: the table is in LDS
-        t.sky = ((LdsFloat*)unorm8)[v.w >> 24];
-        return t;
+        return load_tx_word_lut(*reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(in) + j * 16u), unorm8);
     }
 };
 
@@ -853,7 +879,7 @@ __global__ void __launch_bounds__(256) atrous_kernel(int W, int H, const float* 
     } else {
         // the passes that filter AO or sky decode their bytes through a table
         __shared__ float s_unorm8[256];
-        s_unorm8[threadIdx.x] = vpt_unorm8_decode((uint8_t)threadIdx.x);   // 256 threads
+        fill_unorm8_table(s_unorm8);
         __syncthreads();
         if (!pixel_of(W, H, px, py)) return;
         GlobalSrcLut src;
