@@ -239,6 +239,9 @@ __global__ void __launch_bounds__(256) prespatial_kernel(CamView cam, int W, int
                                                          const vpt_svgf_texel* __restrict__ in,
                                                          vpt_svgf_texel* __restrict__ out)
 {
+    __shared__ float s_unorm8[256];   // AO / sky bytes decoded through a table
+    fill_unorm8_table(s_unorm8);
+    __syncthreads();
     int px, py;
     if (!pixel_of(W, H, px, py)) return;
     const size_t i = (size_t)py * W + px;
@@ -247,7 +250,8 @@ __global__ void __launch_bounds__(256) prespatial_kernel(CamView cam, int W, int
     float uvx = texc(px, W), uvy = texc(py, H);
     Vec3 base_pos = position_from(cam, uvx, uvy, gb_t[i]);
     Vec3 base_n = vpt::face_normal(gb_face[i], 1.0f);
-    Tx base = load_tx(in, i);
+    const uint4* in4 = reinterpret_cast<const uint4*>(in);
+    Tx base = load_tx_word_lut(in4[i], s_unorm8);
     float base_luma = sh_to_y(base.sh[3]);
     float tsh[4] = {base.sh[0], base.sh[1], base.sh[2], base.sh[3]};
     float tco = base.co, tcg = base.cg, tw = 1.0f;
@@ -264,7 +268,7 @@ __global__ void __launch_bounds__(256) prespatial_kernel(CamView cam, int W, int
             Vec3 pd = vpt::mk3(fabsf(sp.x - base_pos.x), fabsf(sp.y - base_pos.y), fabsf(sp.z - base_pos.z));
             float dist_sqr = vpt::dot3(pd, pd);
             if (dist_sqr < 1.0f) {
-                Tx s = load_tx(in, j);
+                Tx s = load_tx_word_lut(in4[j], s_unorm8);
                 Vec3 sn = vpt::face_normal(gb_face[j], 1.0f);
                 float sluma = sh_to_y(s.sh[3]);
                 float nw = vpt_powi(fmaxf(vpt::dot3(base_n, sn), 0.0f), 16);
