@@ -342,6 +342,30 @@ def main():
     s1.record(stream)
     torch.cuda.synchronize()
     svgf_ms = s0.elapsed_time(s1) / nf
+    den_eager = den.clone()
+
+    # the same sequence with each frame's pass chain replayed as one HIP graph
+    # (captured on the first frame, updated in place on the others); the last
+    # frame's output must equal the eager sequence's
+    svgf_graph = Graph(ctx)
+
+    def svgf_graph_sequence():
+        sv.reset()
+        for f, (c_f, vp_f, g_f, t_f) in enumerate(seq):
+            g_prev = seq[f - 1][2] if f else g_f
+            with svgf_graph.capture() as s_cap:
+                sv.frame(c_f, vp_f, g_f, g_prev, t_f, den, time=f / 60.0, stream=s_cap)
+
+    svgf_graph_sequence()
+    torch.cuda.synchronize()
+    s0.record(stream)
+    svgf_graph_sequence()
+    s1.record(stream)
+    torch.cuda.synchronize()
+    svgf_graph_ms = s0.elapsed_time(s1) / nf
+    svgf_graph_equal = bool(torch.equal(den, den_eager))
+    svgf_graph.close()
+    del den_eager
 
     # ---- sun-shadow denoise (ShadowTemporalFilter + ShadowFilter) over the
     # same sequence, inputs resident; and the shadow pass with its
@@ -434,6 +458,7 @@ def main():
         "df_build_ms": round(df_ms, 4),
         "df_update": df_update,
         "svgf_ms_per_frame": round(svgf_ms, 4),
+        "svgf_graph": {"ms_per_frame": round(svgf_graph_ms, 4), "matches_eager": svgf_graph_equal},
         "shadow_denoise": {"ms_per_frame": round(shadow_denoise_ms, 4), "shadow_pass_with_transversal_ms":
                            round(shadow2_ms, 4), "frames": nf},
         "reflect": reflect,
