@@ -487,15 +487,38 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
     const bool spatial = valid && frames < thresh;
     bool uniform = false, uniform_m = false;
     if (__syncthreads_or(spatial)) {
-        for (int c = (int)threadIdx.x; c < kVarCells; c += (int)blockDim.x) {
-            const int gx = x0 - kVarR + c % kVarSW, gy = y0 - kVarR + c / kVarSW;
-            if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;   // off screen: never read
-            const size_t j = (size_t)gy * W + gx;
-            const Tx t = load_tx(in, j);
-            s_sh[c] = make_float4(t.sh[0], t.sh[1], t.sh[2], t.sh[3]);
-            s_aux[c] = make_float4(t.co, t.cg, sh_to_y(t.sh[3]), h2f(util[j].x >> 16));
-            s_t[c] = gb_t[j];
-            s_n[c] = (uint8_t)normal_index(gb_face[j]);
+        // the block's cells, three per thread (256 threads), read together:
+        // an off-screen cell reads the nearest on-screen one and is never
+        // read back from LDS
+        constexpr int kPer = (kVarCells + 255) / 256;
+        const int tid = (int)threadIdx.x;
+        __builtin_assume(tid >= 0 && tid < 256);
+        uint4 tw4[kPer];
+        uint32_t um[kPer], fc[kPer];
+        float tt[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int c = tid + 256 * k;
+            if (c < kVarCells) {
+                const int gx = min(max(x0 - kVarR + c % kVarSW, 0), W - 1);
+                const int gy = min(max(y0 - kVarR + c / kVarSW, 0), H - 1);
+                const size_t j = (size_t)gy * W + gx;
+                tw4[k] = *reinterpret_cast<const uint4*>(in + j);
+                um[k] = util[j].x;
+                tt[k] = gb_t[j];
+                fc[k] = gb_face[j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int c = tid + 256 * k;
+            if (c < kVarCells) {
+                const Tx t = load_tx_word(tw4[k]);
+                s_sh[c] = make_float4(t.sh[0], t.sh[1], t.sh[2], t.sh[3]);
+                s_aux[c] = make_float4(t.co, t.cg, sh_to_y(t.sh[3]), h2f(um[k] >> 16));
+                s_t[c] = tt[k];
+                s_n[c] = (uint8_t)normal_index(fc[k]);
+            }
         }
         if (threadIdx.x < 49) {
             const int a = (int)threadIdx.x / 7, b = (int)threadIdx.x % 7;
