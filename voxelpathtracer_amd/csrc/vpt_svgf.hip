@@ -1136,12 +1136,34 @@ __global__ void __launch_bounds__(256) shadow_filter_kernel(int W, int H, const 
     const uint32_t tile = vpt::xcd_remap(blockIdx.x, tiles_x * ((uint32_t)(H + 7) / 8u));
     const int tx0 = (int)(tile % tiles_x) * 32, ty0 = (int)(tile / tiles_x) * 8;
     const int gx0 = tx0 - kSfA, gy0 = ty0 - kSfA;   // image position of staged cell (0, 0)
-    for (int c = (int)threadIdx.x; c < kSfCells; c += (int)blockDim.x) {
-        const int gx = gx0 + c % kSfW, gy = gy0 + c / kSfW;
-        if (gx < 0 || gx >= W || gy < 0 || gy >= H) continue;   // never read from LDS
-        const uint32_t j = (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
-        s_dc[c] = make_float2(gb_t[j], dec8(color[j]));
-        s_f[c] = (uint8_t)face_class(gb_face[j]);
+    {
+        // the staged cells, seven per thread (256 threads), read together: a
+        // cell off the image reads the nearest one on it and is never read
+        // back from LDS
+        constexpr int kPer = (kSfCells + 255) / 256;
+        const int tid = (int)threadIdx.x;
+        __builtin_assume(tid >= 0 && tid < 256);
+        float st[kPer];
+        uint32_t sc[kPer], sf[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int c = tid + 256 * k;
+            if (c < kSfCells) {
+                const int gx = min(max(gx0 + c % kSfW, 0), W - 1), gy = min(max(gy0 + c / kSfW, 0), H - 1);
+                const uint32_t j = (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
+                st[k] = gb_t[j];
+                sc[k] = color[j];
+                sf[k] = gb_face[j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int c = tid + 256 * k;
+            if (c < kSfCells) {
+                s_dc[c] = make_float2(st[k], dec8((uint8_t)sc[k]));
+                s_f[c] = (uint8_t)face_class((int)sf[k]);
+            }
+        }
     }
     if (threadIdx.x < 49) {
         const int a = (int)threadIdx.x / 7, b = (int)threadIdx.x % 7;
