@@ -999,16 +999,30 @@ __device__ inline float shadow_spatial_pre(int px, int py, int W, int H, float t
     float weight = 1.0f;
     const Vec3 bn = vpt::face_normal(cur_face[i], 1.0f);
     const float b = 0.03f;
+    // the eight neighbours are read first (clamped into the image; a tap off
+    // the screen margin is skipped below), so the reads are in flight together
+    float nd[9];
+    uint32_t nf[9], nm[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int x = k / 3 - 1, y = k % 3 - 1;
+        const size_t j = (size_t)min(max(py + y, 0), H - 1) * W + min(max(px + x, 0), W - 1);
+        nd[k] = cur_t[j];
+        nf[k] = cur_face[j];
+        nm[k] = mask[j];
+    }
+#pragma unroll
     for (int x = -1; x <= 1; x++) {
+#pragma unroll
         for (int y = -1; y <= 1; y++) {
             if (x == 0 && y == 0) continue;
             const float sx = uvx + (float)x * tex_w, sy = uvy + (float)y * tex_h;
             if (!(sx > b && sx < 1.0f - b && sy > b && sy < 1.0f - b)) continue;
-            const size_t j = (size_t)(py + y) * W + (px + x);
-            const float sd = cur_t[j];
-            const Vec3 sn = vpt::face_normal(cur_face[j], 1.0f);
+            const int k = (x + 1) * 3 + y + 1;
+            const float sd = nd[k];
+            const Vec3 sn = vpt::face_normal((uint8_t)nf[k], 1.0f);
             if (sn.x == bn.x && sn.y == bn.y && sn.z == bn.z && fabsf(sd - base_depth) < 1.0f) {
-                const float smp = (float)mask[j];
+                const float smp = (float)nm[k];
                 float wa = fclamp(1.0f - fclamp(fabsf(smp - base) / 3.0f, 0.0f, 1.0f), 0.0f, 1.0f);
                 wa = fclamp(vpt_powi(wa, 7), 0.000001f, 1.0f);
                 total = total + smp * wa;
