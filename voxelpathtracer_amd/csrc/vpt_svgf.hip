@@ -986,8 +986,8 @@ __device__ inline float dec8(uint8_t c) { return vpt_unorm8_decode(c); }
 
 // x / 3 for 0 <= x <= 2: q0 = x fl(1/3), then one correction with the exact
 // remainder x - 3 q0 (an fma).  Equal to the IEEE division for every float
-// in [0, 2], subnormals included (checked exhaustively against x / 3.0f on
-// the CPU); three VALU operations in place of the division's ten.
+// in [0, 2], subnormals included (tests/test_exact_division_cpu.py checks
+// all of them); three VALU operations in place of the division's ten.
 __device__ inline float div3_unit(float x)
 {
     const float y = 1.0f / 3.0f;
