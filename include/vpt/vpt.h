@@ -129,6 +129,15 @@ typedef struct vpt_hit {
 VPT_API const char* vpt_last_error(void);
 VPT_API const char* vpt_version(void);
 
+/* Device self-check, with no counterpart in the reference: the shadow
+ * filter's pow computes vpt_logf's (m - 1) / (m + 1) through the hardware
+ * reciprocal, one Newton step and one correction; this compares that
+ * sequence with the IEEE division for every float m in
+ * [sqrt(2) / 2, sqrt(2)] on the current device and stores the number of
+ * differing m in *mismatches (0 on a conforming device).  Synchronous on
+ * `stream`. */
+VPT_API int vpt_check_log_division(unsigned long long* mismatches, void* stream);
+
 /* ---- context ----------------------------------------------------------- */
 
 /* Binds `device` (HIP ordinal) for the lifetime of the context. */

@@ -73,6 +73,17 @@ def _run(ctx, w, g, df, W, H, frames, scale=1.0):
     return hist_f
 
 
+def test_log_division_on_device(ctx):
+    """The shadow filter's pow divides (m - 1) / (m + 1) through the hardware
+    reciprocal; on this device that equals the IEEE division for every float
+    m in [sqrt(2) / 2, sqrt(2)] (vpt_check_log_division compares them all)."""
+    import ctypes as C
+    from voxelpathtracer_amd._lib import call
+    n = C.c_ulonglong(123)
+    call("vpt_check_log_division", C.byref(n), None)
+    assert n.value == 0
+
+
 def test_shadow_denoise_sequence_small(ctx, terrain_world):
     g, df, w = terrain_world
     frames = _run(ctx, w, g, df, 320, 180, 6)

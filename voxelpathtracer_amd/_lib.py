@@ -132,6 +132,7 @@ _u32 = C.c_uint32
 _PROTOS = {
     "vpt_last_error": (C.c_char_p, []),
     "vpt_version": (C.c_char_p, []),
+    "vpt_check_log_division": (C.c_int, [C.POINTER(C.c_ulonglong), _vp]),
     "vpt_ctx_create": (_i, [_i, C.POINTER(_vp)]),
     "vpt_ctx_destroy": (_i, [_vp]),
     "vpt_ctx_device": (_i, [_vp, C.POINTER(_i)]),
