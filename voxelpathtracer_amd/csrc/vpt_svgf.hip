@@ -233,6 +233,22 @@ dim3 pass_grid(int W, int H) { return dim3((unsigned)(((W + 31) / 32) * ((H + 7)
 
 __device__ inline float texc(int p, int n) { return ((float)p + 0.5f) / (float)n; }
 
+// a / b given y = 1 / b (the correctly rounded reciprocal), for quotients
+// that are zero or normal with an exactly representable remainder, which is
+// the case for every weighted mean and weight ratio below: q0 = a y lies
+// within an ulp of a / b, the remainder a - b q0 is exact in an fma, and
+// q0 + (a - b q0) y rounded once is the correctly rounded quotient
+// (Markstein, IBM J. Res. Dev. 34(1), 1990).  A zero q0 is returned as is,
+// keeping the sign that the correction's addition would drop.  Three VALU
+// operations and a select per quotient, against about ten for a division,
+// where several quotients share a divisor.
+__device__ inline float div_rcp(float a, float b, float y)
+{
+    const float q0 = a * y;
+    const float q = fmaf(fmaf(-b, q0, a), y, q0);
+    return q0 == 0.0f ? q0 : q;
+}
+
 // ---- Spatial3x3Initial.main (Spatial3x3Initial.glsl:96-176) --------------
 __global__ void __launch_bounds__(256) prespatial_kernel(CamView cam, int W, int H, const float* __restrict__ gb_t,
                                                          const uint8_t* __restrict__ gb_face,
@@ -290,9 +306,12 @@ __global__ void __launch_bounds__(256) prespatial_kernel(CamView cam, int W, int
     }
     tw = fmaxf(tw, 0.01f);
     float osh[4];
-    for (int k = 0; k < 4; k++) osh[k] = tsh[k] / tw;
+    const float rtw = 1.0f / tw;
+    for (int k = 0; k < 4; k++) osh[k] = div_rcp(tsh[k], tw, rtw);
     float aw = fmaxf(taow, 0.01f);
-    store_tx(out, i, osh, tco / tw, tcg / tw, base.x, tao / aw, tsky / aw);
+    const float raw = 1.0f / aw;
+    store_tx(out, i, osh, div_rcp(tco, tw, rtw), div_rcp(tcg, tw, rtw), base.x, div_rcp(tao, aw, raw),
+             div_rcp(tsky, aw, raw));
 }
 
 // ---- TemporalFilter.main (SVGF/TemporalFilter.glsl:132-360) --------------
@@ -392,14 +411,15 @@ __global__ void __launch_bounds__(256) temporal_kernel(CamView cam, Mat16 prev_v
         }
     }
     if (tw > 0.001f) {
-        for (int c = 0; c < 4; c++) ssh[c] = ssh[c] / tw;
-        sco = sco / tw;
-        scg = scg / tw;
-        smom = smom / tw;
-        sspp = sspp / tw;
-        slum = slum / tw;
-        sao = sao / tw;
-        ssky = ssky / tw;
+        const float rtw = 1.0f / tw;
+        for (int c = 0; c < 4; c++) ssh[c] = div_rcp(ssh[c], tw, rtw);
+        sco = div_rcp(sco, tw, rtw);
+        scg = div_rcp(scg, tw, rtw);
+        smom = div_rcp(smom, tw, rtw);
+        sspp = div_rcp(sspp, tw, rtw);
+        slum = div_rcp(slum, tw, rtw);
+        sao = div_rcp(sao, tw, rtw);
+        ssky = div_rcp(ssky, tw, rtw);
     } else {
         successful = 0;
     }
@@ -651,11 +671,12 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
             }
         }
         if (tw > 0.0f) {
-            tmom = tmom / tw2;
-            tlum = tlum / tw2;
-            tco = tco / tw;
-            tcg = tcg / tw;
-            for (int k = 0; k < 4; k++) tsh[k] = tsh[k] / tw;
+            const float rtw = 1.0f / tw, rtw2 = 1.0f / tw2;
+            tmom = div_rcp(tmom, tw2, rtw2);
+            tlum = div_rcp(tlum, tw2, rtw2);
+            tco = div_rcp(tco, tw, rtw);
+            tcg = div_rcp(tcg, tw, rtw);
+            for (int k = 0; k < 4; k++) tsh[k] = div_rcp(tsh[k], tw, rtw);
         }
         for (int k = 0; k < 4; k++) osh[k] = tsh[k];
         oco = tco;
@@ -818,6 +839,7 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
     }
     float phi = sqrtf(fmaxf(0.0f, 0.000001f + tweaked));
     phi = phi / fmaxf(color_phi_bias, 0.1f);
+    const float rphi = 1.0f / phi;
 
     if (!(base_depth < 0.0f)) {
         const float jh = (float)jit * 0.5f;
@@ -852,7 +874,7 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
                 s.ao = bil(q00.ao, q10.ao, q01.ao, q11.ao, a, b);
                 s.sky = bil(q00.sky, q10.sky, q01.sky, q11.sky, a, b);
                 float slum = sh_to_y(s.sh[3]);
-                float lw = fabsf(slum - base_lum) / phi;
+                float lw = div_rcp(fabsf(slum - base_lum), phi, rphi);
                 // exp(-d)^2 < 1e-4 for every float d >= 4.7, so the depth
                 // weight sits on its lower clamp there without the exp
                 float dw = 0.0001f;
@@ -883,10 +905,11 @@ __device__ __attribute__((always_inline)) inline void atrous_px(const Src& src, 
         }
     }
     float osh[4];
-    for (int c = 0; c < 4; c++) osh[c] = fclamp(tsh[c] / tw, -100.0f, 100.0f);
-    float oao = tao / taow, osky = tsky / taow;
+    const float rtw = 1.0f / tw, rtaow = 1.0f / taow;
+    for (int c = 0; c < 4; c++) osh[c] = fclamp(div_rcp(tsh[c], tw, rtw), -100.0f, 100.0f);
+    float oao = div_rcp(tao, taow, rtaow), osky = div_rcp(tsky, taow, rtaow);
     if (!filter_ao) oao = base.ao;
-    store_tx(out, i, osh, fclamp(tco / tw, -10.0f, 100.0f), fclamp(tcg / tw, -10.0f, 100.0f),
+    store_tx(out, i, osh, fclamp(div_rcp(tco, tw, rtw), -10.0f, 100.0f), fclamp(div_rcp(tcg, tw, rtw), -10.0f, 100.0f),
              fclamp(tvar / (tw * tw), -1.0f, 50.0f), fclamp(oao, 0.0f, 1.0f), fclamp(osky, 0.0f, 1.0f));
 }
 
