@@ -140,11 +140,7 @@ __device__ inline float fclamp(float x, float lo, float hi) { return fminf(fmaxf
 __device__ inline float sh_to_y(float w) { return fmaxf(0.0f, 3.544905f * w); }
 __device__ inline bool in_screen(float x, float y) { return x > 0.0f && x < 1.0f && y > 0.0f && y < 1.0f; }
 
-__device__ inline Vec3 position_from(const CamView& c, float sx, float sy, float dist)
-{
-    Vec3 dir = vpt::normalize3(vpt::ray_dir_at(c.iv, c.ip, sx, sy));
-    return vpt::mk3(c.iv[12] + dir.x * dist, c.iv[13] + dir.y * dist, c.iv[14] + dir.z * dist);
-}
+
 
 // GL LINEAR + REPEAT footprint at normalised (s, t); valid for coordinates
 // inside (0, 1), where the taps lie in [-1, n].
@@ -247,6 +243,18 @@ __device__ inline float div_rcp(float a, float b, float y)
     const float q0 = a * y;
     const float q = fmaf(fmaf(-b, q0, a), y, q0);
     return q0 == 0.0f ? q0 : q;
+}
+
+// vpt::normalize3 of the camera ray through (sx, sy), its three components
+// divided by the length through one reciprocal (div_rcp), then the point at
+// `dist` along it.
+__device__ inline Vec3 position_from(const CamView& c, float sx, float sy, float dist)
+{
+    const Vec3 d = vpt::ray_dir_at(c.iv, c.ip, sx, sy);
+    const float len = sqrtf((d.x * d.x + d.y * d.y) + d.z * d.z);
+    const float rl = 1.0f / len;
+    const Vec3 dir = vpt::mk3(div_rcp(d.x, len, rl), div_rcp(d.y, len, rl), div_rcp(d.z, len, rl));
+    return vpt::mk3(c.iv[12] + dir.x * dist, c.iv[13] + dir.y * dist, c.iv[14] + dir.z * dist);
 }
 
 // ---- Spatial3x3Initial.main (Spatial3x3Initial.glsl:96-176) --------------
