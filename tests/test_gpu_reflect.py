@@ -97,8 +97,10 @@ def test_reflect_reference_defaults_1080p(ctx, terrain_world):
 def test_reflect_rough_sweep(ctx, terrain_world):
     g, df, w = terrain_world
     sun = np.float32(list(gi_params(50.0, 0).sun_dir))
+    # (rough, spp 2: a lane per sample, 16x8-pixel blocks, partial at these sizes)
     for spp, rough, roughness, div, frame in ((1, 1, 0.3, 1, 0), (2, 0, 0.5, 1, 0), (8, 1, 0.9, 2, 3),
-                                              (16, 1, 0.01, 4, 11), (5, 1, 1.7, 1, 2)):
+                                              (16, 1, 0.01, 4, 11), (5, 1, 1.7, 1, 2), (2, 1, 0.6, 1, 5),
+                                              (2, 1, 0.2, 3, 1)):
         _check(w, g, df, 320, 180, div, sun, spp=spp, rough=rough, roughness=roughness, frame=frame)
 
 

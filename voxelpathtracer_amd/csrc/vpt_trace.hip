@@ -1157,6 +1157,134 @@ __global__ void __launch_bounds__(256) reflect_kernel(WorldView w, CamView cam, 
     count_samples(counter, samples, rays);
 }
 
+// reflect_kernel<true, kMode> (kMode 0 or 2) for spp = 2 with a lane per
+// (pixel, sample): the reference's settings give one 480x270 target pixel
+// per 16 screen pixels, about two wavefronts per SIMD with a lane per pixel,
+// and each lane then traced both samples in turn.  Here a pair of adjacent
+// lanes traces the two samples of one pixel at once (the second lane first
+// advances the pixel's PCG state past the first sample's six draws), and
+// the pair exchanges what the sequential loop carries from sample 0 to
+// sample 1: whether sample 0 hit (then sample 1 casts no shadow ray),
+// sample 0's shadow result (the value sample 1's record holds when it casts
+// none), and, for the pixel's outputs, the hits' distances, faces, blocks,
+// reprojection flags and shadow results, which the sample-0 lane combines in
+// sample order.  Records, distances, hit records and ray / sample counts
+// equal reflect_kernel<true, kMode>'s.
+template <int kMode>
+__global__ void __launch_bounds__(256) reflect_rough2_kernel(WorldView w, CamView cam, int gW, int gH,
+                                                                      const float* __restrict__ gb_t,
+                                                                      const uint8_t* __restrict__ gb_face,
+                                                                      ReflectConst rc, uint16_t* __restrict__ dist_out,
+                                                                      uint32_t* __restrict__ hit_out,
+                                                                      unsigned long long* __restrict__ counter,
+                                                                      const ReflShade shd, uint4* __restrict__ records)
+{
+    // 16x8 pixels per block: each wavefront 8x4 pixels, two lanes per pixel
+    const uint32_t tiles_x = (uint32_t)(cam.W + 15) / 16u, tiles_y = (uint32_t)(cam.H + 7) / 8u;
+    const uint32_t tile = vpt::xcd_remap(blockIdx.x, tiles_x * tiles_y);
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int sample = lane & 1, pl = lane >> 1;
+    const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (pl & 7);
+    const int py = (int)(tile / tiles_x) * 8 + (wave >> 1) * 4 + (pl >> 3);
+    const bool valid = px < cam.W && py < cam.H;   // the same for both lanes of a pair
+    const size_t npx = (size_t)cam.W * cam.H;
+    const size_t i = valid ? (size_t)py * cam.W + px : 0;
+    int samples = 0, rays = 0;
+    bool sky = true, hit = false, reproj = false, traced = false;
+    int shadow_mine = 0;
+    float ht = 0.0f;
+    uint32_t fb = 255u;   // face | block << 8 of this sample's hit
+    Vec3 hp = vpt::mk3(0.0f, 0.0f, 0.0f);
+    Vec3 o = hp, R = hp, L = vpt::mk3(rc.lx, rc.ly, rc.lz);
+    vpt::Hit h{};
+    if (valid) {
+        float dist;
+        int face;
+        reflect_gbuffer(cam, gW, gH, gb_t, gb_face, rc.div, px, py, dist, face);
+        if (!(dist < 0.0f)) {
+            sky = false;
+            const Vec3 P = vpt::gbuffer_position(cam, px, py, dist);
+            const Vec3 N = vpt::face_normal(face, 1.0f);
+            const Vec3 eye = vpt::mk3(cam.iv[12], cam.iv[13], cam.iv[14]);
+            const Vec3 I = vpt::normalize3(vpt::mk3(P.x - eye.x, P.y - eye.y, P.z - eye.z));
+            o = vpt::mk3(P.x + N.x * 0.035f, P.y + N.y * 0.035f, P.z + N.z * 0.035f);
+            uint32_t state = vpt_pcg_hash(vpt_pcg_hash((uint32_t)px + (uint32_t)cam.W * (uint32_t)py +
+                                                       (uint32_t)cam.W * (uint32_t)cam.H * rc.frame) +
+                                          0x9E3779B9u);
+            if (sample == 1)
+                for (int k = 0; k < 6; k++) state = vpt_pcg_hash(state);   // sample 0's draws
+            const Vec3 Hn = reflection_normal(N, rc.roughness, state);
+            const float k = 2.0f * vpt::dot3(Hn, I);
+            R = vpt::mk3(I.x - k * Hn.x, I.y - k * Hn.y, I.z - k * Hn.z);
+            h = vpt::traverse(w, o, R, rc.max_iter);
+            samples += h.samples;
+            rays++;
+            hit = h.t > 0.0f;
+            if (hit) {
+                const Vec3 hn = vpt::face_normal(h.face, 0.0f);
+                hp = vpt::mk3(o.x + R.x * h.t, o.y + R.y * h.t, o.z + R.z * h.t);
+                if (kMode == 2) reproj = reproject(shd.vp, cam, gW, gH, gb_t, gb_face, eye, hp, hn).ok;
+                ht = h.t;
+                fb = (uint32_t)(h.face & 0xff) | ((uint32_t)(h.block & 0xff) << 8);
+            }
+        }
+    }
+    // sample 0's hit decides whether sample 1 casts the pixel's one shadow ray
+    const bool hit_other = __shfl_xor((int)hit, 1) != 0;
+    const bool hit0 = sample == 0 ? hit : hit_other;
+    if (hit && (sample == 0 || !hit0) && !reproj) {
+        shadow_mine = reflect_shadow(w, o, R, h, L, rc.shadow_iter, samples, rays);
+        traced = true;
+    }
+    const int shadow_other = __shfl_xor(shadow_mine, 1);
+    const float ht_other = __shfl_xor(ht, 1);
+    const uint32_t fb_other = (uint32_t)__shfl_xor((int)fb, 1);
+    const bool reproj_other = __shfl_xor((int)reproj, 1) != 0;
+    const bool traced_other = __shfl_xor((int)traced, 1) != 0;
+    if (kMode == 2 && valid && !sky) {
+        // the record's shadow field is the loop's running value: sample 1
+        // without a ray of its own keeps sample 0's
+        const int shadow_rec = (sample == 0 || traced) ? shadow_mine : shadow_other;
+        records[(size_t)sample * npx + i] = hit ? reflect_record(hp, (int)(fb & 0xffu), (int)(fb >> 8), true, traced,
+                                                                 shadow_rec)
+                                                : reflect_record(vpt::mk3(0.0f, 0.0f, 0.0f), 255, 0, false, false, 0);
+    }
+    if (valid && sample == 0) {
+        float hd = -1.0f;
+        uint32_t rec = 255u;
+        if (!sky) {
+            float sum = 0.001f;
+            int hits = 0;
+            bool first_reproj = false;
+            if (hit) {
+                rec = fb;
+                first_reproj = reproj;
+                sum = sum + ht;
+                hits = 1;
+            }
+            if (hit_other) {
+                if (hits == 0) {
+                    rec = fb_other;
+                    first_reproj = reproj_other;
+                }
+                sum = sum + ht_other;
+                hits++;
+            }
+            // the loop's last shadow value (kMode 0; kMode 2 leaves the bit to
+            // reflect_shade_kernel): at most one of the two samples casts a ray
+            const int shadow = kMode == 2 ? 0 : (traced ? shadow_mine : (traced_other ? shadow_other : 0));
+            if (hits > 0) {
+                hd = sum / fmaxf((float)hits, 0.01f);
+                rec |= ((uint32_t)shadow << 16) | ((uint32_t)hits << 24) | ((uint32_t)first_reproj << 17);
+            }
+            hd = vpt::fclamp(hd, -10.0f, 200.0f);
+        }
+        dist_out[i] = (uint16_t)vpt::hw_f2h(hd);
+        if (hit_out) hit_out[i] = rec;
+    }
+    count_samples(counter, samples, rays);
+}
+
 // The shading half of the deferred form: per target pixel, the samples of
 // reflect_kernel<., 2>'s records in order -- the same reprojection (from
 // the recorded hit position), the same choice of which samples take a
@@ -1341,12 +1469,19 @@ static int reflect_launch(const char* fn, vpt_world* w, const vpt_camera* cam, c
                            cam->width, cam->height, gb->d_t, gb->d_face, rc, d_dist, d_hit, w->d_counter, shd, col,
                            recs);
     };
-    if (mode == 1)
+    if (mode == 1) {
         rc.rough ? go(reflect_kernel<true, 1>) : go(reflect_kernel<false, 1>);
-    else if (mode == 2)
+    } else if (mode != 1 && rc.rough && rc.spp == 2) {
+        // the reference's setting: a lane per (pixel, sample), 16x8 pixels per block
+        const unsigned blocks = (unsigned)(((cv.W + 15) / 16) * ((cv.H + 7) / 8));
+        hipLaunchKernelGGL(mode == 2 ? reflect_rough2_kernel<2> : reflect_rough2_kernel<0>, dim3(blocks), dim3(256), 0,
+                           (hipStream_t)stream, vpt::make_view(w), cv, cam->width, cam->height, gb->d_t, gb->d_face, rc,
+                           d_dist, d_hit, w->d_counter, shd, recs);
+    } else if (mode == 2) {
         rc.rough ? go(reflect_kernel<true, 2>) : go(reflect_kernel<false, 2>);
-    else
+    } else {
         rc.rough ? go(reflect_kernel<true, 0>) : go(reflect_kernel<false, 0>);
+    }
     HIP_TRY(hipGetLastError());
     return VPT_OK;
 }
