@@ -526,7 +526,11 @@ __global__ void __launch_bounds__(256) variance_kernel(int W, int H, const float
     __shared__ float s_t[kVarCells];
     __shared__ uint8_t s_n[kVarCells];
     __shared__ float s_nw[7 * 8];         // normal weight by (base, sample) normal index
-    __shared__ float s_tab[9][82];        // uniform blocks: tw, tmom, tsh[4], tco, tcg, tlum after k taps
+    // uniform blocks: tw, tmom, tsh[4], tco, tcg, tlum after k taps, in the
+    // SH cells' storage (built after the last read of s_sh; a uniform block
+    // reads no SH cell afterwards), so that six blocks fit in a CU's LDS
+    static_assert(sizeof(float) * 9 * 82 <= sizeof(float4) * kVarCells, "s_tab fits in s_sh");
+    float (*const s_tab)[82] = reinterpret_cast<float (*)[82]>(s_sh);
     const uint32_t tiles_x = (uint32_t)(W + 31) / 32u;
     const uint32_t tile = vpt::xcd_remap(blockIdx.x, tiles_x * ((uint32_t)(H + 7) / 8u));
     const int x0 = (int)(tile % tiles_x) * 32, y0 = (int)(tile / tiles_x) * 8;
